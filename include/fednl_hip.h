@@ -1,0 +1,169 @@
+/*
+ * fednl_hip.h -- C ABI of the MI355X FedNL round engine (libfednl_hip.so).
+ *
+ * Plain pointers and sizes only. Host arrays are copied in/out by the
+ * library; all per-round state stays resident in device memory.
+ *
+ * Index conventions follow dopt (burlachenkok/ufednl):
+ *   - flat index of (i, j) in a d x d matrix is j*LDA + i with
+ *     LDA = roundUp(d, 4)  (dopt/dopt/linalg_matrices/include/MatrixNMD.h:44-81, 997-1000);
+ *   - the upper triangle (i <= j) is enumerated column by column; the
+ *     position of (i, j) in that list is j*(j+1)/2 + i
+ *     (dopt/dopt/math_routines/include/SpecialMathRoutinesForMatrix.h:35-64).
+ *     "Packed upper" arrays hold S = d*(d+1)/2 values in that order.
+ */
+#ifndef FEDNL_HIP_H
+#define FEDNL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compressor codes, as dopt's enum Compressor
+ * (dopt/bin_fednl_local/include/UsedTypes.h:17-25). */
+enum fednl_compressor {
+    FEDNL_IDENTICAL = 0,
+    FEDNL_RANDK = 1,
+    FEDNL_TOPK = 2,
+    FEDNL_RANDSEQK = 3,
+    FEDNL_TOPLEK = 4,
+    FEDNL_NATURAL = 5
+};
+
+/* Return codes. Non-negative values are success; the negative codes mirror
+ * dopt's TrainReturnCodes (dopt/bin_fednl_local/include/Args.h:53-64) and
+ * add device-side failures below -8. */
+enum fednl_status {
+    FEDNL_OK = 0,
+    FEDNL_E_MISSING_ARGUMENT = -1,
+    FEDNL_E_WRONG_ARGUMENT = -2,
+    FEDNL_E_DATASET_NOT_AVAILABLE = -3,
+    FEDNL_E_DATASET_PARSE = -4,
+    FEDNL_E_SPLIT = -5,
+    FEDNL_E_SAVE = -6,
+    FEDNL_E_INTERNAL = -7,
+    FEDNL_E_UNKNOWN_DEBUG_MODE = -8,
+    FEDNL_E_DEVICE = -20,       /* HIP runtime error (see fednl_last_error) */
+    FEDNL_E_UNSUPPORTED = -21   /* size or mode this build does not handle */
+};
+
+/* Options of one FedNL problem. Fill with fednl_default_opts(). */
+typedef struct fednl_opts {
+    double lambda;        /* L2 regulariser (--lambda) */
+    int compressor;       /* enum fednl_compressor (--compressor) */
+    double k_mult;        /* k = (int)(k_mult * d) (--k_compressor_as_d_mult, TrainLoop.h:362) */
+    int alpha_mode;       /* 0: explicit alpha; 1/2: theoretical option 1/2 (TrainLoop.h:409-518) */
+    double alpha;         /* used when alpha_mode == 0 (--alpha_step_size) */
+    int option_b;         /* 1: FedNL option b (shift = mean Lk); 0: option a (shift = lambda) */
+    int workers;          /* client worker count of the reference run (--workers); selects
+                             the TopLEK random stream of each client (TrainLoop.h:664-686) */
+    int tracking;         /* 1: compute f(x_r) every round (--tracking) */
+    int hessian_mode;     /* 0: FP64 MFMA (default) */
+    int reserved[7];
+} fednl_opts;
+
+void fednl_default_opts(fednl_opts *o);
+
+/* Last error message of the calling thread ("" if none). */
+const char *fednl_last_error(void);
+
+/* Creates a problem: n clients, m samples each, d features (the intercept,
+ * if wanted, is already a column of `features`). features: n*m*d doubles
+ * (client-major, then sample-major, row of d features); labels: n*m values
+ * in {-1, +1}. Client c owns samples [c*m, (c+1)*m) as in dopt's split
+ * without reshuffling (TrainLoop.h:212-250). Computes the initial learned
+ * Hessians H_c(x0 = 0) and their mean on the device (TrainLoop.h:603-643,732-735).
+ * Replaces the per-client setup of train() (TrainLoop.h:600-700). */
+int fednl_create(const fednl_opts *opts, int n, int m, int d, const double *features,
+                 const double *labels, void **handle);
+
+void fednl_destroy(void *handle);
+
+/* Sizes: k actually used, S = d(d+1)/2, LDA of flat indices. */
+int fednl_sizes(void *handle, int *k, int64_t *S, int *lda);
+
+/* Alpha actually used (explicit or from the theory formulas). */
+double fednl_alpha(void *handle);
+
+/* Enqueues `rounds` FedNL rounds on the engine's stream and waits for them.
+ * One round = dopt's client loop (LocalWorkerForFedNL_*.h) for every client
+ * plus the master update (TrainLoop.h:1054-1508). Returns FEDNL_OK, or
+ * FEDNL_E_INTERNAL if a Cholesky factorization failed (the round records it
+ * in the trace and the iterate is left unchanged). */
+int fednl_run(void *handle, int rounds);
+
+/* Enqueues rounds without waiting (graph replay when enabled). */
+int fednl_enqueue(void *handle, int rounds);
+int fednl_sync(void *handle);
+
+/* Per-round trace since creation: ||grad f(x_r)||, f(x_r) (0 without
+ * tracking), ||x_r||, seconds since the first round was enqueued (from
+ * device events), factorization flag (1 ok). Arrays sized >= *count. */
+int fednl_trace(void *handle, double *grad_norm, double *fval, double *x_norm, double *seconds,
+                int *chol_ok, int *count);
+
+int fednl_get_x(void *handle, double *x);
+
+/* Outputs of the last round, for parity checks: averaged gradient (d),
+ * per-client Lk (n), selected counts (n), selected list positions (n*k,
+ * ascending), values sent to the master (n*k), flat indices (n*k), Newton
+ * step (d). Any pointer may be NULL. */
+int fednl_last_round(void *handle, double *grad, double *lk, int *cnt, uint32_t *pos, double *vals,
+                     uint32_t *flat_idx, double *dx);
+
+/* Whole state for lockstep checks: x (d), learned Hessians (n*S packed),
+ * master Hessian estimate H (S packed), RandK/RandSeqK client generator
+ * states (n), TopLEK stream states (workers). Any pointer may be NULL. */
+int fednl_get_state(void *handle, double *x, double *learned, double *H, uint32_t *rng,
+                    uint32_t *toplek_rng);
+int fednl_set_state(void *handle, const double *x, const double *learned, const double *H,
+                    const uint32_t *rng, const uint32_t *toplek_rng);
+
+/* Per-kernel device time (milliseconds, summed over rounds since the last
+ * reset) measured with HIP events on the engine stream when enabled. */
+int fednl_kernel_timing(void *handle, int enable);
+int fednl_kernel_times(void *handle, double *ms, int *launches, int max_kernels, const char **names);
+
+/* Standalone compressor on identical inputs (parity of index selection).
+ * delta: nclients*S packed upper values (host). For TopK/TopLEK the
+ * selection is dopt's heap scan (SpecialMathRoutinesForMatrix.h:229-511);
+ * RandK/RandSeqK draw from rng[c] (one RandomGenIntegerLinear per client,
+ * updated in place); TopLEK draws from toplek_rng[c % workers] (updated in
+ * place), with delta_param = k/S as the TopLEK worker passes it
+ * (LocalWorkerForFedNL_TopLEK.h:19,58). out_pos: nclients*k list positions,
+ * ascending (RandSeqK: ascending too); out_cnt: counts. */
+int fednl_select(int compressor, int nclients, int d, int k, const double *delta, uint32_t *rng,
+                 uint32_t *toplek_rng, int workers, double delta_param, uint32_t *out_pos,
+                 int *out_cnt);
+
+/* Standalone Newton solve: Cholesky of (H_up + shift*I) and the two
+ * triangular solves (TrainLoop.h:1449-1466, CholeskyFactorization.h:153-272,
+ * ElementarySolvers.h:72-226). H_up: S packed upper. Returns 1 when the
+ * factorization succeeded, 0 otherwise, negative on error. */
+int fednl_newton_solve(int d, const double *H_up, double shift, const double *b, double *dx);
+
+/* Diagnostic: raw outputs of three v_mfma_f64_16x16x4f64 probes (768 doubles)
+ * used by the tests to pin the lane layout the Hessian kernel relies on.
+ * No counterpart in the reference. */
+int fednl_probe_mfma_f64(double *out768);
+
+/* Drop-in for dopt's shared-library entry point
+ * (dopt/bin_fednl_local/exported/FedNLLocalShared.h:8-17,
+ *  dopt/bin_fednl_local/src/mainShared.cpp:15-39): `command` holds the
+ * bin_fednl_local flags (without "train"), split on single spaces. The
+ * callback receives (d, ||grad f(x_last)||, ||grad||^2/(2 mu), 2||grad||^2/mu)
+ * (TrainLoop.h:1854-1859) once, on success. Returns dopt's TrainReturnCodes. */
+typedef void (*ResultCallbackLocal)(int xDimension, double discrWithOptSolution,
+                                    double discrWithOptValue, double L2NormInLastIterate);
+int doptExecuteLocal(const char *command, ResultCallbackLocal resultCallback);
+
+/* The CLI entry (bin_fednl_local main, dopt/bin_fednl_local/src/main.cpp:25-59). */
+int fednl_cli_main(int argc, char **argv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEDNL_HIP_H */

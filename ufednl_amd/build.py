@@ -1,0 +1,78 @@
+"""Builds the native parts in-tree.
+
+  ufednl_amd/libfednl_hip.so   HIP kernels + host engine + C ABI (hipcc, gfx950)
+  ufednl_amd/bin_fednl_local   drop-in CLI linked against it
+
+Run: python -m ufednl_amd.build  (or __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INC = os.path.join(ROOT, "include")
+LIB = os.path.join(HERE, "libfednl_hip.so")
+CLI = os.path.join(HERE, "bin_fednl_local")
+
+ARCH = os.environ.get("FEDNL_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
+    # exact-arithmetic kernels (message values, learned update, master sums)
+    # must not be contracted into FMA; kernels use fma() where they want one
+    "-ffp-contract=off",
+    # no packed FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+    f"-I{INC}",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib():
+    srcs = [os.path.join(CSRC, "fednl_kernels.hip"), os.path.join(CSRC, "fednl_cli.cpp")]
+    srcs = [s for s in srcs if os.path.exists(s)]
+    tmp = LIB + ".tmp"
+    run([hipcc(), *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
+    os.replace(tmp, LIB)
+
+
+def build_cli():
+    main = os.path.join(CSRC, "bin_fednl_local_main.cpp")
+    if not os.path.exists(main):
+        return
+    run(["g++", "-O2", "-std=c++17", f"-I{INC}", "-o", CLI, main, f"-L{HERE}", "-lfednl_hip",
+         f"-Wl,-rpath,$ORIGIN"])
+
+
+def build_oracle():
+    od = os.path.join(ROOT, "oracle")
+    run(["make", "-s", "-C", od, "-f", "Makefile"])
+    if os.path.isdir("/root/reference/dopt"):
+        run(["make", "-s", "-C", od, "-f", "Makefile.ref"])
+
+
+def main():
+    build_lib()
+    build_cli()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,223 @@
+"""ctypes binding of libfednl_hip.so (C ABI in include/fednl_hip.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfednl_hip.so")
+
+COMPRESSORS = {"identical": 0, "randk": 1, "topk": 2, "seqk": 3, "toplek": 4, "natural": 5}
+KERNEL_NAMES_MAX = 16
+
+_c_dp = ctypes.POINTER(ctypes.c_double)
+_c_up = ctypes.POINTER(ctypes.c_uint32)
+_c_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class FednlOpts(ctypes.Structure):
+    _fields_ = [
+        ("lambda_", ctypes.c_double),
+        ("compressor", ctypes.c_int),
+        ("k_mult", ctypes.c_double),
+        ("alpha_mode", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+        ("option_b", ctypes.c_int),
+        ("workers", ctypes.c_int),
+        ("tracking", ctypes.c_int),
+        ("hessian_mode", ctypes.c_int),
+        ("reserved", ctypes.c_int * 7),
+    ]
+
+
+class FednlError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """Loads libfednl_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FednlError(f"{LIB_PATH} is missing: build it with `python -m ufednl_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.fednl_last_error.restype = ctypes.c_char_p
+        L.fednl_default_opts.argtypes = [ctypes.POINTER(FednlOpts)]
+        L.fednl_create.argtypes = [ctypes.POINTER(FednlOpts), ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dp,
+                                   _c_dp, ctypes.POINTER(ctypes.c_void_p)]
+        L.fednl_destroy.argtypes = [ctypes.c_void_p]
+        L.fednl_sizes.argtypes = [ctypes.c_void_p, _c_ip, ctypes.POINTER(ctypes.c_int64), _c_ip]
+        L.fednl_alpha.restype = ctypes.c_double
+        L.fednl_alpha.argtypes = [ctypes.c_void_p]
+        L.fednl_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.fednl_enqueue.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.fednl_sync.argtypes = [ctypes.c_void_p]
+        L.fednl_trace.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_dp, _c_ip, _c_ip]
+        L.fednl_get_x.argtypes = [ctypes.c_void_p, _c_dp]
+        L.fednl_last_round.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_ip, _c_up, _c_dp, _c_up, _c_dp]
+        L.fednl_get_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
+        L.fednl_set_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
+        L.fednl_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.fednl_kernel_times.argtypes = [ctypes.c_void_p, _c_dp, _c_ip, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_char_p)]
+        L.fednl_select.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dp, _c_up, _c_up,
+                                   ctypes.c_int, ctypes.c_double, _c_up, _c_ip]
+        L.fednl_newton_solve.argtypes = [ctypes.c_int, _c_dp, ctypes.c_double, _c_dp, _c_dp]
+        _lib = L
+    return _lib
+
+
+def _err(rc, what):
+    if rc < 0:
+        msg = lib().fednl_last_error().decode(errors="replace")
+        raise FednlError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def dp(a):
+    return _p(a, _c_dp)
+
+
+def up(a):
+    return _p(a, _c_up)
+
+
+def ip(a):
+    return _p(a, _c_ip)
+
+
+class Engine:
+    """One FedNL problem resident on the GPU."""
+
+    def __init__(self, features, labels, lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0,
+                 option_b=True, workers=1, tracking=True):
+        L = lib()
+        feats = np.ascontiguousarray(features, dtype=np.float64)
+        labs = np.ascontiguousarray(labels, dtype=np.float64)
+        n, m, d = feats.shape
+        o = FednlOpts()
+        L.fednl_default_opts(ctypes.byref(o))
+        o.lambda_ = lam
+        o.compressor = COMPRESSORS[compressor]
+        o.k_mult = k_mult
+        o.alpha_mode = alpha_mode
+        o.alpha = alpha
+        o.option_b = 1 if option_b else 0
+        o.workers = workers
+        o.tracking = 1 if tracking else 0
+        h = ctypes.c_void_p()
+        _err(L.fednl_create(ctypes.byref(o), n, m, d, dp(feats), dp(labs), ctypes.byref(h)), "fednl_create")
+        self.h = h
+        self.n, self.m, self.d = n, m, d
+        k = ctypes.c_int(); S = ctypes.c_int64(); lda = ctypes.c_int()
+        L.fednl_sizes(self.h, ctypes.byref(k), ctypes.byref(S), ctypes.byref(lda))
+        self.k, self.S, self.lda = k.value, S.value, lda.value
+        self.workers = max(1, min(workers, n))
+        self.alpha = L.fednl_alpha(self.h)
+
+    @classmethod
+    def from_problem(cls, prob, **kw):
+        return cls(prob.features, prob.labels, lam=prob.lam, **kw)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fednl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, rounds: int, check=True):
+        rc = lib().fednl_run(self.h, rounds)
+        if check:
+            _err(rc, "fednl_run")
+        return rc
+
+    def enqueue(self, rounds: int):
+        _err(lib().fednl_enqueue(self.h, rounds), "fednl_enqueue")
+
+    def sync(self):
+        _err(lib().fednl_sync(self.h), "fednl_sync")
+
+    def trace(self):
+        cnt = ctypes.c_int()
+        _err(lib().fednl_trace(self.h, None, None, None, None, None, ctypes.byref(cnt)), "fednl_trace")
+        R = cnt.value
+        g = np.zeros(R); f = np.zeros(R); xn = np.zeros(R); sec = np.zeros(R); ok = np.zeros(R, dtype=np.int32)
+        _err(lib().fednl_trace(self.h, dp(g), dp(f), dp(xn), dp(sec), ip(ok), ctypes.byref(cnt)), "fednl_trace")
+        return dict(grad_norm=g, f=f, x_norm=xn, seconds=sec, chol_ok=ok)
+
+    def x(self):
+        out = np.zeros(self.d)
+        _err(lib().fednl_get_x(self.h, dp(out)), "fednl_get_x")
+        return out
+
+    def last_round(self):
+        n, k, d = self.n, self.k, self.d
+        g = np.zeros(d); lk = np.zeros(n); cnt = np.zeros(n, dtype=np.int32); pos = np.zeros(n * k, dtype=np.uint32)
+        vals = np.zeros(n * k); idx = np.zeros(n * k, dtype=np.uint32); dx = np.zeros(d)
+        _err(lib().fednl_last_round(self.h, dp(g), dp(lk), ip(cnt), up(pos), dp(vals), up(idx), dp(dx)),
+             "fednl_last_round")
+        return dict(grad=g, lk=lk, cnt=cnt, pos=pos.reshape(n, k), vals=vals.reshape(n, k), idx=idx.reshape(n, k),
+                    dx=dx)
+
+    def get_state(self):
+        x = np.zeros(self.d); L = np.zeros(self.n * self.S); H = np.zeros(self.S)
+        rng = np.zeros(self.n, dtype=np.uint32); lek = np.zeros(self.workers, dtype=np.uint32)
+        _err(lib().fednl_get_state(self.h, dp(x), dp(L), dp(H), up(rng), up(lek)), "fednl_get_state")
+        return dict(x=x, learned=L.reshape(self.n, self.S), H=H, rng=rng, lek=lek)
+
+    def set_state(self, st):
+        def c(a, dt):
+            return None if a is None else np.ascontiguousarray(a, dtype=dt).reshape(-1)
+        _err(lib().fednl_set_state(self.h, dp(c(st.get("x"), np.float64)), dp(c(st.get("learned"), np.float64)),
+                                   dp(c(st.get("H"), np.float64)), up(c(st.get("rng"), np.uint32)),
+                                   up(c(st.get("lek"), np.uint32))), "fednl_set_state")
+
+    def kernel_timing(self, enable=True):
+        _err(lib().fednl_kernel_timing(self.h, 1 if enable else 0), "fednl_kernel_timing")
+
+    def kernel_times(self):
+        ms = np.zeros(KERNEL_NAMES_MAX); lc = np.zeros(KERNEL_NAMES_MAX, dtype=np.int32)
+        names = (ctypes.c_char_p * KERNEL_NAMES_MAX)()
+        nk = _err(lib().fednl_kernel_times(self.h, dp(ms), ip(lc), KERNEL_NAMES_MAX, names), "fednl_kernel_times")
+        return {names[q].decode(): (float(ms[q]), int(lc[q])) for q in range(nk)}
+
+
+def select(compressor, delta, d, k, rng=None, toplek_rng=None, workers=1):
+    """Standalone selection on identical inputs. delta: (nclients, S) packed upper."""
+    delta = np.ascontiguousarray(delta, dtype=np.float64)
+    nc, S = delta.shape
+    assert S == d * (d + 1) // 2
+    rng_a = None if rng is None else np.ascontiguousarray(rng, dtype=np.uint32).copy()
+    lek_a = None if toplek_rng is None else np.ascontiguousarray(toplek_rng, dtype=np.uint32).copy()
+    out = np.zeros(nc * k, dtype=np.uint32)
+    cnt = np.zeros(nc, dtype=np.int32)
+    comp = COMPRESSORS[compressor]
+    _err(lib().fednl_select(comp, nc, d, k, dp(delta), up(rng_a), up(lek_a), workers, k * (1.0 / S), up(out),
+                            ip(cnt)), "fednl_select")
+    return out.reshape(nc, k), cnt, rng_a, lek_a
+
+
+def newton_solve(H_up, d, shift, b):
+    H_up = np.ascontiguousarray(H_up, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    dx = np.zeros(d)
+    ok = _err(lib().fednl_newton_solve(d, dp(H_up), shift, dp(b), dp(dx)), "fednl_newton_solve")
+    return bool(ok), dx
