@@ -1,0 +1,89 @@
+"""The oracle against the reference's own code compiled in oracle/_ref
+(skipped where the reference sources were not available at build time)."""
+import numpy as np
+import pytest
+
+from tests.tolerances import assert_grad_norms_close
+from ufednl_amd import datagen
+
+
+@pytest.fixture(scope="module")
+def ref(oracle):
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref/libdopt_ref.so not built (reference sources absent)")
+    return oracle.ref()
+
+
+def _ref_topk(oracle, ref, vals, d, k):
+    full = np.zeros(d * oracle.lda(d))
+    full[oracle.upper_list(d)] = vals
+    out = np.zeros(max(k, len(vals)), dtype=np.uint32)
+    c = ref.ref_topk(oracle.dp(full), d, k, oracle.up(out))
+    return out[:c]
+
+
+def test_topk_random(oracle, ref):
+    rng = np.random.default_rng(99)
+    for trial in range(300):
+        d = int(rng.integers(1, 45))
+        S = d * (d + 1) // 2
+        k = int(rng.integers(1, S + 2))
+        kind = trial % 5
+        if kind == 0:
+            v = rng.standard_normal(S)
+        elif kind == 1:
+            v = rng.integers(-2, 3, S).astype(float)
+        elif kind == 2:
+            v = np.where(rng.random(S) < 0.5, 0.0, -0.0)
+        elif kind == 3:
+            v = np.round(rng.standard_normal(S), 1)
+        else:
+            v = np.repeat(rng.standard_normal((S + 3) // 4), 4)[:S]
+        got = oracle.pos_to_flat(oracle.topk_packed(v, k), d)
+        np.testing.assert_array_equal(got, _ref_topk(oracle, ref, v, d, k), err_msg=f"trial {trial}")
+
+
+def test_client_eval_and_newton(oracle, ref):
+    prob = datagen.config3()
+    d, m = prob.d, prob.m
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(d) * 0.2
+    o = oracle.client_eval(prob.features[3], prob.labels[3], prob.lam, x)
+    t = np.zeros(m); s = np.zeros(m); g = np.zeros(d); h = np.zeros(d * oracle.lda(d)); f = np.zeros(1)
+    ref.ref_client_eval(oracle.dp(np.ascontiguousarray(prob.features[3])), oracle.dp(np.ascontiguousarray(prob.labels[3])),
+                        d, m, prob.lam, oracle.dp(x), oracle.dp(t), oracle.dp(s), oracle.dp(g), oracle.dp(h),
+                        oracle.dp(f))
+    np.testing.assert_allclose(o["grad"], g, rtol=1e-13, atol=1e-17)
+    # binary features: products are exact, so the Hessian sums agree bit for bit
+    # whenever the sigmoids do (they differ by at most an ulp of exp)
+    np.testing.assert_allclose(o["hess_up"], oracle.pack_upper(h, d), rtol=1e-13, atol=1e-18)
+    H = o["hess_up"]
+    b = rng.standard_normal(d)
+    ok, dx, _ = oracle.cholesky_solve(H, d, 1e-3, b)
+    dxr = np.zeros(d)
+    okr = ref.ref_newton_solve(oracle.dp(oracle.unpack_full(H, d)), d, 1e-3, oracle.dp(b), oracle.dp(dxr), None)
+    assert ok and okr
+    np.testing.assert_allclose(dx, dxr, rtol=1e-11)
+
+
+@pytest.mark.parametrize("comp,seed", [("randk", 21), ("seqk", 22), ("topk", 23), ("toplek", 24)])
+def test_fednl_trajectories(oracle, comp, seed):
+    prob = datagen.small_problem(n=7, m=50, d_feat=15, density=0.3, seed=seed)
+    R = 20
+    r = oracle.ref_fednl_run(prob, comp, 3.0, rounds=R, workers=3)
+    f = oracle.FedNL(prob, comp, k_mult=3.0, workers=3)
+    g = np.array([f.round()["grad_norm"] for _ in range(R)])
+    assert_grad_norms_close(g, r["grad_norm"])
+    np.testing.assert_allclose(f.get_state()["x"], r["x"], rtol=1e-7, atol=1e-9)
+
+
+def test_config1_topk_first_rounds(oracle, ref):
+    prob = datagen.config1()
+    R = 12
+    r = oracle.ref_fednl_run(prob, "topk", 8.0, rounds=R, workers=4, dump_rounds=2)
+    f = oracle.FedNL(prob, "topk", k_mult=8.0)
+    outs = [f.round() for _ in range(R)]
+    assert_grad_norms_close(np.array([o["grad_norm"] for o in outs]), r["grad_norm"])
+    for rr in range(2):
+        for c in range(prob.n):
+            np.testing.assert_array_equal(oracle.pos_to_flat(outs[rr]["pos"][c], prob.d), r["dump"]["idx"][rr, c])

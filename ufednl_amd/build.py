@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-result",
+    "-Wno-unused-result", "-Wno-unused-value",
     f"-I{INC}",
 ]
 

@@ -1,0 +1,709 @@
+// fednl_cli.cpp -- the bin_fednl_local command-line surface on top of the
+// engine's C ABI: flag parsing (dopt/bin_fednl_local/include/Args.h), LIBSVM
+// loading (dopt/dopt/optimization_problems/include/ml/dataset_loader_fs.h),
+// sample split (TrainLoop.h:212-250), the FedNL run on the GPU engine, the
+// per-round report and tracking file (TrainLoop.h:753-792, 1655-1862), and
+// dopt's shared-library entry doptExecuteLocal (src/mainShared.cpp:15-39).
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <sstream>
+#include <string>
+#include <string_view>
+#include <vector>
+
+#include "fednl_hip.h"
+
+namespace fednl_cli {
+
+// TrainReturnCodes (Args.h:53-64)
+enum Rc {
+    kOk = 0,
+    kMissingArgument = -1,
+    kWrongArgument = -2,
+    kDatasetNotAvailable = -3,
+    kDatasetParse = -4,
+    kSplitError = -5,
+    kSaveFailed = -6,
+    kInternal = -7,
+    kUnknownDebugMode = -8
+};
+
+// "--name value" / "--flag" lookup with one or two leading dashes ignored
+// (dopt/dopt/cmdline/src/CmdLineParser.cpp:129-154)
+class CmdLine {
+public:
+    CmdLine(int argc, char** argv) : argc_(argc), argv_(argv) {}
+
+    bool flag(std::string_view name) const { return find(name) >= 0; }
+
+    bool str(std::string_view name, std::string& out) const
+    {
+        int i = find(name);
+        if (i < 0 || i + 1 >= argc_)
+            return false;
+        out = argv_[i + 1];
+        return true;
+    }
+
+    template <class T>
+    bool num(std::string_view name, T& out) const
+    {
+        std::string s;
+        if (!str(name, s))
+            return false;
+        std::istringstream is(s);
+        T v;
+        if (!(is >> v))
+            return false;
+        out = v;
+        return true;
+    }
+
+private:
+    static std::string_view strip(std::string_view s)
+    {
+        if (!s.empty() && s[0] == '-') s.remove_prefix(1);
+        if (!s.empty() && s[0] == '-') s.remove_prefix(1);
+        return s;
+    }
+    int find(std::string_view name) const
+    {
+        for (int i = 1; i < argc_; ++i) {
+            if (argv_[i][0] != '-')
+                continue;
+            if (strip(argv_[i]) == strip(name))
+                return i;
+        }
+        return -1;
+    }
+    int argc_;
+    char** argv_;
+};
+
+struct Args {
+    // dataset (Args.h:135-170)
+    std::string path;
+    int clients = 1;
+    bool reshuffle = false;
+    uint32_t reshuffle_seed = 123;
+    bool add_intercept = false;
+    // tracking (Args.h:176-205)
+    bool tracking = false;
+    std::string out;
+    // problem
+    double lambda = 0.0;
+    // runtime (Args.h:226-262)
+    int workers = 1, rounds = 0, server_grad_workers = 0, server_hessian_workers = 0;
+    uint32_t x0_seed = 1234;
+    // algorithm (Args.h:268-395)
+    std::string algorithm, compressor;
+    double k_mult = 1.0;
+    double alpha = 0.0;
+    bool theoretical_alpha = false, alpha_opt1 = false, alpha_opt2 = false;
+    bool line_search = false;
+    double c_ls = 0.5, gamma_ls = 0.5;
+    uint32_t max_ls = 5;
+    bool option_a = false, option_b = false;
+    // debug (Args.h:74-129)
+    bool silent_print = false, meminfo = false, forceseq = false, crc_data = false, print_train = false;
+    bool silent = false;
+    bool transfer_idx = false;
+};
+
+static int parse_debug(const CmdLine& cl, Args& a)
+{
+    std::string s;
+    if (!cl.str("debug", s))
+        return kOk;
+    std::string cur;
+    auto take = [&](const std::string& mode) -> int {
+        if (mode.empty()) return kOk;
+        if (mode == "notracingprint") a.silent_print = true;
+        else if (mode == "meminfo") a.meminfo = true;
+        else if (mode == "forcesequpdate") a.forceseq = true;
+        else if (mode == "crc32fordata") a.crc_data = true;
+        else if (mode == "printtrain") a.print_train = true;
+        else {
+            std::cerr << "Unknown debug mode: " << mode << '\n';
+            return kUnknownDebugMode;
+        }
+        return kOk;
+    };
+    for (char ch : s) {
+        if (ch == ',' || ch == ';') {
+            int rc = take(cur);
+            if (rc) return rc;
+            cur.clear();
+        } else {
+            cur.push_back(ch);
+        }
+    }
+    return take(cur);
+}
+
+static bool file_exists(const std::string& p)
+{
+    std::ifstream f(p, std::ios::binary);
+    return (bool)f;
+}
+
+static int parse_args(const CmdLine& cl, Args& a)
+{
+    int rc = parse_debug(cl, a);
+    if (rc != kOk)
+        return rc;
+    a.tracking = cl.flag("tracking");
+    if (!a.tracking)
+        std::cout << "Tracking information [IGNORE]\n";
+    if (!cl.str("out", a.out)) {
+        std::cerr << "You did not specify argument \"out\" with path to results which optionally has tracking information\n";
+        return kMissingArgument;
+    }
+    cl.num("clients", a.clients);
+    a.reshuffle = cl.flag("reshuffle_train");
+    cl.num("reshuffle_train_seed", a.reshuffle_seed);
+    a.add_intercept = cl.flag("add_intercept");
+    if (!cl.str("train_dataset", a.path)) {
+        std::cerr << "Please specify argument \"train_dataset\" with path to train dataset\n";
+        return kMissingArgument;
+    }
+    if (!file_exists(a.path)) {
+        std::cerr << "Used train dataset: '" << a.path << "'. File exist: [NOT]\n";
+        return kDatasetNotAvailable;
+    }
+    if (!cl.num("lambda", a.lambda)) {
+        std::cerr << "Please specify argument \"lambda\" with value of L2 regulirization for logistic regression\n";
+        return kMissingArgument;
+    }
+    if (!cl.str("algorithm", a.algorithm)) {
+        std::cerr << "Please specify argument \"algorithm\" with name of algorithm\n";
+        return kMissingArgument;
+    }
+    if (a.algorithm != "gd") {
+        if (!cl.num("k_compressor_as_d_mult", a.k_mult)) {
+            std::cerr << "Please specify argument \"k_compressor_as_d_mult\" with value of multiplier for K=<mult>D for Rand-K\n";
+            return kMissingArgument;
+        }
+        if (!cl.str("compressor", a.compressor)) {
+            std::cerr << "Please specify argument \"compressor\" with value [identical,randk,topk,seqk]\n";
+            return kMissingArgument;
+        }
+        a.theoretical_alpha = cl.flag("theoretical_alpha");
+        a.alpha_opt1 = cl.flag("use_theoretical_alpha_option_1");
+        a.alpha_opt2 = cl.flag("use_theoretical_alpha_option_2");
+        if (!a.theoretical_alpha && !cl.num("alpha_step_size", a.alpha)) {
+            std::cerr << "Please specify argument \"alpha_step_size\" with value of step size for FedNL\n";
+            return kMissingArgument;
+        }
+        a.line_search = cl.flag("line_search");
+        if (a.line_search) {
+            if (!cl.num("c_line_search", a.c_ls)) {
+                std::cerr << "Please specify argument \"c_line_search\" for line search. Line search parameter c in (0, 0.5). \n";
+                return kMissingArgument;
+            }
+            if (!cl.num("gamma_line_search", a.gamma_ls)) {
+                std::cerr << "Please specify argument \"gamma_line_search\" for line search. Line search parameter c in (0, 1). \n";
+                return kMissingArgument;
+            }
+            cl.num("max_line_search_steps", a.max_ls);
+        }
+    }
+    cl.num("server_grad_workers", a.server_grad_workers);
+    cl.num("server_hessian_workers", a.server_hessian_workers);
+    cl.num("workers", a.workers);
+    if (!cl.num("rounds", a.rounds)) {
+        std::cerr << "Please specify argument \"rounds\" with number of rounds\n";
+        return kMissingArgument;
+    }
+    cl.num("x0_seed", a.x0_seed);
+    a.option_a = cl.flag("fednl-option-a");
+    a.option_b = cl.flag("fednl-option-b");
+    a.silent = cl.flag("silent");
+    a.transfer_idx = cl.flag("transfer_indicies_for_randk");
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// LIBSVM text: "label idx:val idx:val ..." per line, fields split on ' ' or '\t',
+// lines on '\n' or '\r'. d = maxIdx - minIdx + 1 (+1 intercept, last column);
+// labels must take exactly two values and map to -1 below their mean, +1 above
+// (dataset_loader_fs.h:381-604, 607-883).
+// ---------------------------------------------------------------------------
+struct Dataset {
+    int N = 0, d = 0;
+    std::vector<double> X;  // N x d, row-major
+    std::vector<double> y;  // N
+};
+
+static int load_libsvm(const std::string& path, bool add_intercept, Dataset& ds)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+        return kDatasetParse;
+    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    struct Row { double label; std::vector<std::pair<long, double>> feats; };
+    std::vector<Row> rows;
+    long lmin = 0, lmax = 0;
+    bool have_label = false;
+    int unique_labels = 0;
+    long fmin = 0, fmax = 0;
+    bool have_feat = false;
+    size_t pos = 0;
+    const size_t n = text.size();
+    while (pos < n) {
+        size_t e = pos;
+        while (e < n && text[e] != '\n' && text[e] != '\r') ++e;
+        std::string_view line(text.data() + pos, e - pos);
+        pos = e + 1;
+        // fields
+        std::vector<std::string_view> fields;
+        size_t a = 0;
+        while (a < line.size()) {
+            while (a < line.size() && (line[a] == ' ' || line[a] == '\t')) ++a;
+            size_t b = a;
+            while (b < line.size() && line[b] != ' ' && line[b] != '\t') ++b;
+            if (b > a) fields.emplace_back(line.substr(a, b - a));
+            a = b;
+        }
+        if (fields.empty())
+            continue;
+        Row r;
+        {
+            std::string lab(fields[0]);
+            char* endp = nullptr;
+            const double lv = strtod(lab.c_str(), &endp);
+            if (endp == lab.c_str()) {
+                std::cerr << "Error during parsing. Line: " << rows.size() + 1 << ". Can not parse label " << lab << '\n';
+                return kDatasetParse;
+            }
+            r.label = lv;
+            const long li = (long)lv;
+            if (!have_label) {
+                lmin = lmax = li;
+                unique_labels = 1;
+                have_label = true;
+            } else if (li < lmin || li > lmax) {
+                ++unique_labels;
+                if (li < lmin) lmin = li;
+                if (li > lmax) lmax = li;
+            }
+        }
+        for (size_t q = 1; q < fields.size(); ++q) {
+            const size_t colon = fields[q].find(':');
+            if (colon == std::string_view::npos) {
+                std::cerr << "Warning during parsing. Line: " << rows.size() + 1 << ". Can not parse " << fields[q]
+                          << " for purpose of obtaing feature_index:value of label.\n";
+                continue;
+            }
+            std::string is(fields[q].substr(0, colon)), vs(fields[q].substr(colon + 1));
+            char* endp = nullptr;
+            const long idx = strtol(is.c_str(), &endp, 10);
+            if (endp == is.c_str()) {
+                std::cerr << "Error during parsing. Line: " << rows.size() + 1 << ". Can not parse " << is << '\n';
+                return kDatasetParse;
+            }
+            const double v = strtod(vs.c_str(), nullptr);
+            if (!have_feat) {
+                fmin = fmax = idx;
+                have_feat = true;
+            } else if (idx < fmin) {
+                fmin = idx;
+            } else if (idx > fmax) {
+                fmax = idx;
+            }
+            r.feats.emplace_back(idx, v);
+        }
+        rows.push_back(std::move(r));
+    }
+    if (rows.empty() || unique_labels != 2) {
+        std::cout << "Specified dataset labels for " << unique_labels
+                  << " classes. And you need to have a dataset with 2 classes.\n";
+        return kDatasetParse;
+    }
+    const int dfeat = have_feat ? (int)(fmax - fmin + 1) : 0;
+    ds.N = (int)rows.size();
+    ds.d = dfeat + (add_intercept ? 1 : 0);
+    ds.X.assign((size_t)ds.N * ds.d, 0.0);
+    ds.y.resize(ds.N);
+    const double avg = ((double)lmin + (double)lmax) / 2.0;
+    for (int s = 0; s < ds.N; ++s) {
+        ds.y[s] = (rows[s].label < avg) ? -1.0 : 1.0;
+        for (auto& fv : rows[s].feats)
+            ds.X[(size_t)s * ds.d + (fv.first - fmin)] = fv.second;
+        if (add_intercept)
+            ds.X[(size_t)s * ds.d + ds.d - 1] = 1.0;
+    }
+    return kOk;
+}
+
+// Fisher-Yates of Shuffle.h:101-158 driven by RandomGenIntegerLinear
+// (RandomGenIntegerLinear.h:49-58): u = x / (2^32 - 1), offset rounded as
+// SimpleMathRoutines.h:407-414.
+static void shuffle_indices(std::vector<int32_t>& v, uint32_t seed)
+{
+    uint32_t x = seed;
+    size_t rem = v.size() - 1;
+    for (size_t i = 0; i < v.size(); ++i, --rem) {
+        x = 214013u * x + 2531011u;
+        const double u = (double)x * (1.0 / 4294967295.0);
+        const double p = u * (double)rem;
+        size_t off = (size_t)p;
+        if (p - (double)off > 0.5) off += 1;
+        if (off != 0) std::swap(v[i], v[i + off]);
+    }
+}
+
+// CRC-32 of dopt::crc32 (dopt/dopt/system/src/digest/Crc.cpp:74-90): reflected
+// polynomial 0xEDB88320, seed 0xFFFFFFFF, no final inversion.
+static uint32_t crc32_bytes(const void* buf, size_t size, uint32_t crc = 0xFFFFFFFFu)
+{
+    static uint32_t table[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t c = i;
+            for (int b = 0; b < 8; ++b) c = (c & 1) ? (0xEDB88320u ^ (c >> 1)) : (c >> 1);
+            table[i] = c;
+        }
+        init = true;
+    }
+    const unsigned char* p = (const unsigned char*)buf;
+    for (size_t i = 0; i < size; ++i) crc = (crc >> 8) ^ table[(crc ^ p[i]) & 0xFFu];
+    return crc;
+}
+
+static std::string base_name(const std::string& p)
+{
+    const size_t s = p.find_last_of("/\\");
+    return s == std::string::npos ? p : p.substr(s + 1);
+}
+
+struct Bytes {
+    std::vector<char> b;
+    void str(const std::string& s, bool zero) { b.insert(b.end(), s.begin(), s.end()); if (zero) b.push_back(0); }
+    void i32(int32_t v) { const char* p = (const char*)&v; b.insert(b.end(), p, p + 4); }
+    void f64(double v) { const char* p = (const char*)&v; b.insert(b.end(), p, p + 8); }
+};
+
+static int train(const CmdLine& cl, ResultCallbackLocal cb)
+{
+    Args a;
+    int rc = parse_args(cl, a);
+    if (rc != kOk)
+        return rc;
+    const bool print = !a.silent_print;
+
+    Dataset ds;
+    rc = load_libsvm(a.path, a.add_intercept, ds);
+    if (rc != kOk) {
+        if (print) std::cerr << "Train dataset: '" << a.path << "'. [Error during parsing]\n";
+        return kDatasetParse;
+    }
+    if (print) {
+        std::cout << "Train dataset: '" << a.path << "' was instantiated [OK].\n";
+        std::cout << "About data set: \"train dataset\"\n";
+        std::cout << "  Total number of samples in data set: " << ds.N << '\n';
+        std::cout << "  Total number of attributes in data set: " << ds.d << '\n';
+        std::cout << "  Design matrix includes intercept term: " << (a.add_intercept ? "[YES]" : "[NO]") << '\n';
+        if (a.crc_data) {
+            // the reference hashes its d x N column-major design (padded to LDA) and labels
+            const int lda = (ds.d + 3) / 4 * 4;
+            std::vector<double> col((size_t)lda * ds.N, 0.0);
+            for (int s = 0; s < ds.N; ++s)
+                for (int i = 0; i < ds.d; ++i) col[(size_t)s * lda + i] = ds.X[(size_t)s * ds.d + i];
+            std::cout << "  Dataset CRC-32 samples: " << crc32_bytes(col.data(), col.size() * 8) << '\n';
+            std::cout << "  Dataset CRC-32 labels: " << crc32_bytes(ds.y.data(), ds.y.size() * 8) << '\n';
+        }
+    }
+    const int n = a.clients;
+    if (n <= 0 || n > ds.N) {
+        std::cerr << "Number of clients must be in [1, number of samples]\n";
+        return kWrongArgument;
+    }
+    const int mpc = ds.N / n;  // TrainLoop.h:212-250: residual samples are dropped
+    std::vector<int32_t> order(ds.N);
+    for (int s = 0; s < ds.N; ++s) order[s] = s;
+    if (a.reshuffle)
+        shuffle_indices(order, a.reshuffle_seed);
+    if (print) {
+        std::cout << "  Train samples has been reshuffled: " << (a.reshuffle ? "[YES]" : "[NO]") << '\n';
+        std::cout << "  Train samples reshuffled seed: " << a.reshuffle_seed << '\n';
+        std::cout << "  Number of clients: " << n << '\n';
+        std::cout << "  Number of samples per client: " << mpc << '\n';
+        std::cout << "  Number of skipped samples during samples distribution: " << (ds.N - mpc * n) << '\n';
+        std::cout << "  Checking data split " << (cl.flag("check_split") ? "[OK]" : "[SKIP]") << '\n';
+    }
+
+    if (a.algorithm != "fednl1") {
+        std::cerr << "Algorithm '" << a.algorithm << "' is not provided by this engine (fednl1 only)\n";
+        return kWrongArgument;
+    }
+    if (!a.option_a && !a.option_b) {
+        if (print) std::cerr << "For FedNL you should specify fednl-option-a or fednl-option-b\n";
+        return kMissingArgument;
+    }
+    int comp;
+    if (a.compressor == "identical") comp = FEDNL_IDENTICAL;
+    else if (a.compressor == "natural") comp = FEDNL_NATURAL;
+    else if (a.compressor == "randk") comp = FEDNL_RANDK;
+    else if (a.compressor == "topk") comp = FEDNL_TOPK;
+    else if (a.compressor == "toplek") comp = FEDNL_TOPLEK;
+    else if (a.compressor == "seqk") comp = FEDNL_RANDSEQK;
+    else {
+        if (print) std::cerr << "You have specified unknown type of compressor: " << a.compressor << '\n';
+        return kWrongArgument;
+    }
+    if (comp == FEDNL_NATURAL || a.line_search) {
+        std::cerr << (a.line_search ? "--line_search" : "--compressor natural")
+                  << " is not provided by this engine\n";
+        return kWrongArgument;
+    }
+    const int d = ds.d;
+    const size_t S = (size_t)d * (d + 1) / 2;
+    const int k = (comp == FEDNL_IDENTICAL) ? (int)S : (int)(a.k_mult * d);
+    if (comp != FEDNL_IDENTICAL && (size_t)k > S) {
+        if (print)
+            std::cerr << " Multiplier for D is specifed as " << a.k_mult << ". For this problem with D=" << d
+                      << " maximum allowable muliplier is " << double(S) / double(d) << "!\n";
+        return kWrongArgument;
+    }
+    int alpha_mode = 0;
+    if (a.theoretical_alpha) {
+        if (comp == FEDNL_TOPK || comp == FEDNL_TOPLEK) {
+            if (a.alpha_opt1) alpha_mode = 1;
+            else if (a.alpha_opt2) alpha_mode = 2;
+            else {
+                if (print)
+                    std::cerr << "Please specify --use_theoretical_alpha_option_2 or --use_theoretical_alpha_option_1 in command line\n";
+                return kMissingArgument;
+            }
+        } else {
+            alpha_mode = 2;
+        }
+    }
+
+    // client-major design for the engine
+    std::vector<double> feats((size_t)n * mpc * d), labs((size_t)n * mpc);
+    for (int c = 0; c < n; ++c)
+        for (int s = 0; s < mpc; ++s) {
+            const int src = order[(size_t)c * mpc + s];
+            memcpy(&feats[((size_t)c * mpc + s) * d], &ds.X[(size_t)src * d], sizeof(double) * d);
+            labs[(size_t)c * mpc + s] = ds.y[src];
+        }
+
+    fednl_opts o;
+    fednl_default_opts(&o);
+    o.lambda = a.lambda;
+    o.compressor = comp;
+    o.k_mult = a.k_mult;
+    o.alpha_mode = alpha_mode;
+    o.alpha = a.alpha;
+    o.option_b = a.option_b ? 1 : 0;
+    o.workers = a.workers;
+    o.tracking = a.tracking ? 1 : 0;
+    void* eng = nullptr;
+    rc = fednl_create(&o, n, mpc, d, feats.data(), labs.data(), &eng);
+    if (rc != FEDNL_OK) {
+        std::cerr << "engine: " << fednl_last_error() << '\n';
+        return kInternal;
+    }
+    const double alpha = fednl_alpha(eng);
+    if (print) {
+        std::cout << "=========================================================================\n";
+        std::cout << "Main loop has been started: r:" << a.rounds << ",clients:" << n << ",d:" << d << ",ni:" << mpc << '\n';
+    }
+    rc = fednl_enqueue(eng, a.rounds);
+    if (rc == FEDNL_OK)
+        rc = fednl_sync(eng);
+    if (rc != FEDNL_OK) {
+        std::cerr << "engine: " << fednl_last_error() << '\n';
+        fednl_destroy(eng);
+        return kInternal;
+    }
+    int R = 0;
+    fednl_trace(eng, nullptr, nullptr, nullptr, nullptr, nullptr, &R);
+    std::vector<double> gn(R), fv(R), xn(R), sec(R);
+    std::vector<int> ok(R);
+    fednl_trace(eng, gn.data(), fv.data(), xn.data(), sec.data(), ok.data(), &R);
+    std::vector<int> cnt(n);
+    std::vector<double> x(d), glast(d);
+    fednl_get_x(eng, x.data());
+    fednl_last_round(eng, glast.data(), nullptr, cnt.data(), nullptr, nullptr, nullptr, nullptr);
+
+    // bytes received from the clients per round (TrainLoop.h:1104,1131-1132,1287,1533)
+    auto round_bytes = [&](double& scal, double& idxb) {
+        scal = 0.0;
+        idxb = 0.0;
+        for (int c = 0; c < n; ++c) {
+            const double items = (comp == FEDNL_TOPLEK) ? (double)cnt[c] : (double)k;
+            scal += 8.0 * d + 8.0 * items + (a.option_b ? 8.0 : 0.0);
+            if (a.tracking) scal += a.option_b ? 8.0 : 0.0;
+            if (comp == FEDNL_TOPK) idxb += 4.0 * k;
+            else if (comp == FEDNL_TOPLEK) idxb += 4.0 + 4.0 * cnt[c];
+            else if ((comp == FEDNL_RANDK || comp == FEDNL_RANDSEQK) && a.transfer_idx) idxb += 4.0 * k;
+        }
+    };
+    Bytes tr;
+    if (a.tracking) {
+        tr.str(base_name(a.path), true);
+        std::string alg = "FedNL";
+        if (a.line_search) alg += "-LS";
+        if (a.option_a) alg += " (a)";
+        if (a.option_b) alg += " (b)";
+        tr.str(alg, true);
+        tr.str(a.compressor, true);
+        tr.i32(mpc);
+        tr.i32(ds.N);
+        tr.i32(a.rounds);
+        tr.i32(n);
+        tr.i32(d);
+        tr.i32(k);
+        tr.f64(a.k_mult);
+        tr.f64(0.0);
+        tr.f64(alpha);
+        tr.f64(a.lambda);
+    }
+    double acc_s = 0.0, acc_i = 0.0;
+    for (int r = 0; r < R; ++r) {
+        if (!a.silent && print) {
+            const double t = sec[r] > 0 ? sec[r] : 1e-9;
+            std::cout << "  Round #" << r << ": norm of full gradient: " << gn[r];
+            if (a.tracking) std::cout << ": f(x): " << fv[r];
+            else std::cout << ": f(x): [N/A]";
+            std::cout << ": avg. time per round: " << 1e3 * t / (r + 1) << " msec"
+                      << ": rounds/second: " << (r + 1) / t << ": line-search num: 0\n";
+        }
+        if (!ok[r] && print)
+            std::cout << "SST FACTORIZATION FAILED\n";
+        if (a.tracking) {
+            tr.i32(r);
+            tr.f64(gn[r]);
+            tr.f64(fv[r]);
+            tr.f64(xn[r]);
+            tr.f64(acc_s);
+            tr.f64(acc_i);
+            tr.f64(sec[r]);
+        }
+        double bs, bi;
+        round_bytes(bs, bi);
+        acc_s += bs;
+        acc_i += bi;
+    }
+    const double elapsed_ms = R > 0 ? sec[R - 1] * 1e3 : 0.0;
+    double gg = 0.0, xx = 0.0;
+    for (int i = 0; i < d; ++i) {
+        gg += glast[i] * glast[i];
+        xx += x[i] * x[i];
+    }
+    const double mu = a.lambda;  // computeMuStrongConvexity (logistic_regression.h:702-705)
+    const double f_gap = gg / (2.0 * mu);   // ConvexOptimizationRoutines.h:12-27
+    const double x_gap = 2.0 * gg / mu;
+    if (print) {
+        std::cout << "=========================================================================\n";
+        std::cout << "Statistics\n\n";
+        std::cout << "  Average time per round: " << (R ? elapsed_ms / R : 0.0) << " milliseconds\n";
+        std::cout << "  Total number of rounds: " << a.rounds << '\n';
+        std::cout << "  Total number of clients: " << n << '\n';
+        std::cout << "  Algorithm: " << a.algorithm << '\n';
+        std::cout << "  Data set: " << a.path << '\n';
+        std::cout << "  d: " << d << '\n';
+        std::cout << "  Alpha step size: " << alpha << '\n';
+        std::cout << "  Lambda for regularization: " << a.lambda << '\n';
+        std::cout << "  Received MBytes from the clients: " << (acc_s + acc_i) / (1024.0 * 1024.0) << '\n';
+        std::cout << "  Clients compressor: " << a.compressor << '\n';
+        std::cout << "  Clients compressor K: " << k << " / Hessian shape is [d,d] where d: " << d
+                  << " / Maximum K ( {d(d+1)}/2 ): " << S << '\n';
+        std::cout << "=========================================================================\n";
+        std::cout << "Solution Properties\n\n";
+        std::cout << "  Last iterate CRC-32: " << crc32_bytes(x.data(), sizeof(double) * d) << '\n';
+        std::cout << "  Gradient in last iterate CRC-32: " << crc32_bytes(glast.data(), sizeof(double) * d) << '\n';
+        std::cout << "  Norm of last iterate: " << sqrt(xx) << '\n';
+        std::cout << "  Norm of full gradient in last iterate: " << sqrt(gg) << '\n';
+        std::cout << "  Norm square of full gradient in last iterate: " << gg << '\n';
+        std::cout << "  Suboptimality gap of Function, f(x) - f*: " << f_gap << '\n';
+        std::cout << "  Suboptimality distance to Optimal Solution |x - x*|: " << x_gap << '\n';
+        std::cout << "=========================================================================\n";
+        std::cout << "  Training has been finished successfully [OK]\n";
+    }
+    fednl_destroy(eng);
+
+    // {run_id},{algorithm},{dataset},{kmultiplier} (TrainLoop.h:1813-1831)
+    std::string outp = a.out;
+    {
+        srand((unsigned)time(nullptr));
+        const std::string rid = std::to_string((int)((double)rand() / RAND_MAX * 10000));
+        const std::string keys[4] = {"{run_id}", "{algorithm}", "{dataset}", "{kmultiplier}"};
+        const std::string vals[4] = {rid, a.algorithm, base_name(a.path), std::to_string(a.k_mult)};
+        for (int q = 0; q < 4; ++q) {
+            const size_t p = outp.find(keys[q]);
+            if (p != std::string::npos) outp.replace(p, keys[q].size(), vals[q]);
+        }
+    }
+    if (!outp.empty()) {
+        std::ofstream f(outp, std::ios::binary);
+        bool saved = (bool)f;
+        if (saved && !tr.b.empty()) {
+            f.write(tr.b.data(), (std::streamsize)tr.b.size());
+            saved = (bool)f;
+        }
+        if (print)
+            std::cout << "  Tracking results has been saved to: " << outp << (saved ? " [OK]" : " [FAILED]") << '\n';
+    }
+    if (cb)
+        cb(d, sqrt(gg), f_gap, x_gap);
+    return kOk;
+}
+
+}  // namespace fednl_cli
+
+extern "C" {
+
+int fednl_cli_main(int argc, char** argv)
+{
+    fednl_cli::CmdLine cl(argc, argv);
+    if (cl.flag("version")) {
+        std::cout << "fednl-mi355x engine (FedNL round engine for gfx950)\n";
+        if (!cl.flag("train"))
+            return 0;
+    }
+    if (!cl.flag("train")) {
+        std::cout << "Please specify one of the action [train]\n";
+        return -1;
+    }
+    return fednl_cli::train(cl, nullptr);
+}
+
+int doptExecuteLocal(const char* command, ResultCallbackLocal resultCallback)
+{
+    // "train " + command, split on single spaces, heap copies (src/mainShared.cpp:15-39)
+    std::string full = std::string("train ") + (command ? command : "");
+    std::vector<std::string> parts;
+    std::string cur;
+    for (char ch : full) {
+        if (ch == ' ') {
+            parts.push_back(cur);
+            cur.clear();
+        } else {
+            cur.push_back(ch);
+        }
+    }
+    parts.push_back(cur);
+    std::vector<char*> argv;
+    argv.push_back(const_cast<char*>("bin_fednl_local"));
+    for (auto& p : parts) {
+        if (p == "train") p = "--train";
+        argv.push_back(p.data());
+    }
+    fednl_cli::CmdLine cl((int)argv.size(), argv.data());
+    return fednl_cli::train(cl, resultCallback);
+}
+
+}  // extern "C"
