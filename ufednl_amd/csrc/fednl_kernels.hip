@@ -39,6 +39,21 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 namespace fednl {
 
+// Device-clock probes for phase timing in a diagnostic build (-DFEDNL_PROBE);
+// they compile to nothing otherwise.
+#ifdef FEDNL_PROBE
+__device__ unsigned long long g_probe[16384];
+#define PROBE_T(var) const unsigned long long var = wall_clock64()
+#define PROBE_ADD(slot, t0) \
+    do { if (threadIdx.x == 0) atomicAdd(&g_probe[(slot)], wall_clock64() - (t0)); } while (0)
+#define PROBE_SET(slot, v) \
+    do { if (threadIdx.x == 0) g_probe[(slot)] = (unsigned long long)(v); } while (0)
+#else
+#define PROBE_T(var) do {} while (0)
+#define PROBE_ADD(slot, t0) do {} while (0)
+#define PROBE_SET(slot, v) do {} while (0)
+#endif
+
 // ---------------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------------
@@ -68,9 +83,7 @@ static void set_err(const char* what, hipError_t e = hipSuccess)
 constexpr int kTile = 64;          // Hessian output tile (rows and columns)
 constexpr int kKC = 16;            // samples staged per LDS chunk
 constexpr int kPadLds = 4;         // LDS row padding (doubles)
-constexpr int kSeg = 1024;         // heap-scan staging segment (list positions)
 constexpr int kMaxHeapK = 8192;    // LDS-resident heap limit
-constexpr int kWin = 1024;         // master scatter window (list positions)
 constexpr int kMaxClients = 1024;  // master scatter client table
 constexpr int kNB = 32;            // Cholesky panel width
 constexpr int kMaxNewtonD = 512;   // single-workgroup Newton solve limit (LDS panel)
@@ -101,6 +114,15 @@ __device__ inline bool pos_is_diag(uint64_t p)
     while (r * r > v) --r;
     while ((r + 1) * (r + 1) <= v) ++r;
     return r * r == v;
+}
+
+// value of a double held by lane `src` (uniform), without an LDS round trip
+__device__ inline double bcast_lane(double v, int src)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)(u & 0xFFFFFFFFull), src);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
+    return __longlong_as_double(((unsigned long long)hi << 32) | lo);
 }
 
 // ---------------------------------------------------------------------------
@@ -141,10 +163,13 @@ __device__ inline uint64_t round_nearest(double v)
 __global__ __launch_bounds__(256) void k_margin(const double* __restrict__ A, const double* __restrict__ x,
                                                 int m, int m_pad, int d, int d_pad, double inv_m,
                                                 double* __restrict__ Dw, double* __restrict__ gw,
-                                                double* __restrict__ nlog)
+                                                double* __restrict__ nlog_part)
 {
     extern __shared__ double xs[];
+    __shared__ double nl[16];
     const int c = blockIdx.y;
+    if (threadIdx.x < 16)
+        nl[threadIdx.x] = 0.0;
     for (int i = threadIdx.x; i < d_pad; i += blockDim.x)
         xs[i] = (i < d) ? x[i] : 0.0;
     __syncthreads();
@@ -169,13 +194,20 @@ __global__ __launch_bounds__(256) void k_margin(const double* __restrict__ A, co
                 const double z2 = z1 * sg;
                 Dw[o] = inv_m * z2;
                 gw[o] = z1 * (-inv_m);
-                nlog[o] = -log(sg);
+                nl[wave * 4 + q] = -log(sg);
             } else {
                 Dw[o] = 0.0;
                 gw[o] = 0.0;
-                nlog[o] = 0.0;
+                nl[wave * 4 + q] = 0.0;
             }
         }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // per-block partial of sum_s -log sigma_s (samples in order)
+        double sacc = 0.0;
+        for (int q = 0; q < 16; ++q) sacc = sacc + nl[q];
+        nlog_part[(size_t)c * gridDim.x + blockIdx.x] = sacc;
     }
 }
 
@@ -327,32 +359,56 @@ __global__ __launch_bounds__(256) void k_hessian_f64(const double* __restrict__ 
 // dopt's MinHeap<4>::minHeapify (MinHeap.h:255-290), values and ids in LDS.
 // After a swap it keeps comparing against the value that moved up.
 // ---------------------------------------------------------------------------
-__device__ inline void heapify(double* hv, uint32_t* hid, int i, int n)
+__device__ inline double heapify_x(double* hv, uint32_t* hid, int i, int n, double xval, uint32_t xid)
 {
-    double lowv = hv[i];
+    // sifts element (xval, xid) that sits at slot i; returns the value left at slot i
+    double lowv = xval;
+    double first_moved = xval;
+    bool moved = false;
     for (;;) {
-        int lowest = i;
         const int first = 4 * i + 1;
-        const int bound = min(first + 4, n);
-        for (int ch = first; ch < bound; ++ch) {
-            const double cv = hv[ch];
-            if (lowv > cv) {
-                lowest = ch;
-                lowv = cv;
+        if (first >= n)
+            break;
+        // all four children (and ids) loaded unconditionally so the reads issue
+        // together (the heap arrays carry 4 slots of padding); absent children
+        // are masked to +inf and never win
+        double cv[4];
+        uint32_t ci[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            cv[q] = hv[first + q];
+            ci[q] = hid[first + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            cv[q] = (first + q < n) ? cv[q] : __builtin_inf();
+        int lowest = -1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (lowv > cv[q]) {
+                lowest = q;
+                lowv = cv[q];
             }
         }
-        if (lowest != i) {
-            const double tv = hv[i];
-            hv[i] = hv[lowest];
-            hv[lowest] = tv;
-            const uint32_t ti = hid[i];
-            hid[i] = hid[lowest];
-            hid[lowest] = ti;
-            i = lowest;
-        } else {
+        if (lowest < 0)
             break;
+        // the chosen child moves up into slot i; comparison continues against its value
+        hv[i] = lowv;
+        hid[i] = ci[lowest];
+        if (!moved) {
+            first_moved = lowv;
+            moved = true;
         }
+        i = first + lowest;
     }
+    hv[i] = xval;
+    hid[i] = xid;
+    return moved ? first_moved : xval;
+}
+
+__device__ inline double heapify(double* hv, uint32_t* hid, int i, int n)
+{
+    return heapify_x(hv, hid, i, n, hv[i], hid[i]);
 }
 
 __device__ inline int heap_level_first(int L)
@@ -411,6 +467,72 @@ __host__ __device__ inline int pow2_at_least(int v)
     return p;
 }
 
+// |v| of `len` consecutive doubles into LDS by threads [t0, t0 + nt), eight
+// loads per thread in flight before the stores
+__device__ inline void stage_abs(double* dst, const double* __restrict__ src, int len, int t0, int nt)
+{
+    for (int e0 = t0; e0 < len; e0 += nt * 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = e0 + q * nt;
+            v[q] = (e < len) ? src[e] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = e0 + q * nt;
+            if (e < len)
+                dst[e] = fabs(v[q]);
+        }
+    }
+}
+
+// exclusive prefix sum of one value per thread (256 threads)
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
+{
+    __shared__ uint32_t wsum[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off);
+        if (lane >= off)
+            incl += o;
+    }
+    if (lane == 63)
+        wsum[wave] = incl;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (int w = 0; w < 4; ++w) {
+        if (w < wave)
+            base += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    if (total)
+        *total = tot;
+    return base + incl - v;
+}
+
+// writes the set bits of bm (positions 32*w + b) in ascending order
+__device__ inline void compact_bitmap(const uint32_t* bm, size_t words, uint32_t* out)
+{
+    const size_t per = (words + blockDim.x - 1) / blockDim.x;
+    const size_t w0 = (size_t)threadIdx.x * per;
+    const size_t w1 = min(w0 + per, words);
+    uint32_t local = 0;
+    for (size_t w = w0; w < w1; ++w)
+        local += __popc(bm[w]);
+    uint32_t o = block_exclusive_scan(local, nullptr);
+    for (size_t w = w0; w < w1; ++w) {
+        uint32_t bits = bm[w];
+        while (bits) {
+            const int bpos = __builtin_ctz(bits);
+            out[o++] = (uint32_t)(w * 32 + bpos);
+            bits &= bits - 1;
+        }
+    }
+}
+
 // Lk (TopK/TopLEK worker: frobeniusNormForSymmetricMatrixFromUpPart of Delta,
 // LocalWorkerForFedNL_TopK.h:74-78) from the Hessian kernel's tile partials,
 // summed in tile order.
@@ -436,7 +558,7 @@ __device__ inline double fro2_from_parts(const double* part, int c, int tiles_up
 // ---------------------------------------------------------------------------
 template <bool kLek>
 __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ delta, size_t S, int k,
-                                                     const double* __restrict__ part, int tiles_up,
+                                                     int seg_len, const double* __restrict__ part, int tiles_up,
                                                      double lek_delta, double* __restrict__ lk_out,
                                                      uint32_t* __restrict__ sel, int* __restrict__ sel_cnt,
                                                      double* __restrict__ lek_p, int* __restrict__ lek_hit,
@@ -447,6 +569,7 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const double* dc = delta + (size_t)c * S;
     __shared__ double s_fro2;
+    PROBE_T(t_sel0);
     if (tid == 0) {
         const double f2 = part ? fro2_from_parts(part, c, tiles_up) : 0.0;
         s_fro2 = f2;
@@ -464,9 +587,9 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
         return;
     }
     const int P = pow2_at_least(k);
-    double* hv = (double*)smem;
-    uint32_t* hid = (uint32_t*)(hv + k);
-    double* seg = (double*)(hid + round_up(P, 2));
+    double* hv = (double*)smem;                                // k + 4 (padding read by heapify)
+    uint32_t* hid = (uint32_t*)(hv + k + 4);                   // P + 4
+    double* seg = (double*)(hid + round_up(P + 4, 2));
 
     for (int q = tid; q < k; q += blockDim.x) {
         hv[q] = fabs(dc[q]);
@@ -489,47 +612,65 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
         }
     }
 
-    // scan of list positions k .. S-1
+    PROBE_ADD(1000 + 8 * c + 0, t_sel0);
+    PROBE_T(t_sel1);
+    // scan of list positions k .. S-1 in segments of seg_len: wave 0 scans the
+    // current segment while waves 1-3 load the next one (all loads in flight)
     const size_t rest = S - (size_t)k;
-    const int nseg = (int)((rest + kSeg - 1) / kSeg);
-    for (int e = tid; e < kSeg && (size_t)e < rest; e += blockDim.x)
-        seg[e] = fabs(dc[(size_t)k + e]);
+    const int nseg = (int)((rest + seg_len - 1) / seg_len);
+    {
+        const int len0 = (int)min((size_t)seg_len, rest);
+        stage_abs(seg, dc + k, len0, tid, blockDim.x);
+    }
     __syncthreads();
     double rv = hv[0];
     double rv0 = rv;
     for (int sgi = 0; sgi < nseg; ++sgi) {
-        double* cur = seg + (sgi & 1) * kSeg;
-        const size_t sbase = (size_t)k + (size_t)sgi * kSeg;
-        const int slen = (int)min((size_t)kSeg, S - sbase);
+        const double* cur = seg + (size_t)(sgi & 1) * seg_len;
+        const size_t sbase = (size_t)k + (size_t)sgi * seg_len;
+        const int slen = (int)min((size_t)seg_len, S - sbase);
         if (wave == 0) {
-            for (int e0 = 0; e0 < slen; e0 += 64) {
-                const int e = e0 + lane;
-                const double v = (e < slen) ? cur[e] : -1.0;
-                uint64_t mask = __ballot(v > rv);
-                while (mask) {
-                    const int f = __builtin_ctzll(mask);
-                    const double key = __shfl(v, f);
-                    if (lane == 0) {
-                        hv[0] = key;  // increaseMinimumInMinHeap (MinHeap.h:78-85)
-                        hid[0] = (uint32_t)(sbase + e0 + f);
-                        heapify(hv, hid, 0, k);
-                        rv0 = hv[0];
+            for (int e4 = 0; e4 < slen; e4 += 256) {
+                double v4[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int e = e4 + 64 * j + lane;
+                    v4[j] = (e < slen) ? cur[e] : -1.0;
+                }
+                // one ballot for 256 positions: the root only grows, so a position
+                // not above it now is never admitted in this pass
+                const bool any = (v4[0] > rv) || (v4[1] > rv) || (v4[2] > rv) || (v4[3] > rv);
+                if (__ballot(any) == 0ull)
+                    continue;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const double v = v4[j];
+                    const int e0 = e4 + 64 * j;
+                    uint64_t mask = __ballot(v > rv);
+                    while (mask) {
+                        const int f = __builtin_ctzll(mask);
+                        const double key = bcast_lane(v, f);
+                        if (lane == 0) {
+                            // increaseMinimumInMinHeap (MinHeap.h:78-85)
+                            rv0 = heapify_x(hv, hid, 0, k, key, (uint32_t)(sbase + e0 + f));
+                        }
+                        rv = bcast_lane(rv0, 0);
+                        const uint64_t above = (f == 63) ? 0ull : (~0ull << (f + 1));
+                        mask = __ballot(v > rv) & above;
                     }
-                    rv = __shfl(rv0, 0);
-                    const uint64_t above = (f == 63) ? 0ull : (~0ull << (f + 1));
-                    mask = __ballot(v > rv) & above;
                 }
             }
         } else if (sgi + 1 < nseg) {
-            double* nxt = seg + ((sgi + 1) & 1) * kSeg;
-            const size_t nbase = sbase + kSeg;
-            const int nlen = (int)min((size_t)kSeg, S - nbase);
-            for (int e = tid - 64; e < nlen; e += blockDim.x - 64)
-                nxt[e] = fabs(dc[nbase + e]);
+            double* nxt = seg + (size_t)((sgi + 1) & 1) * seg_len;
+            const size_t nbase = sbase + seg_len;
+            const int nlen = (int)min((size_t)seg_len, S - nbase);
+            stage_abs(nxt, dc + nbase, nlen, tid - 64, blockDim.x - 64);
         }
         __syncthreads();
     }
 
+    PROBE_ADD(1000 + 8 * c + 1, t_sel1);
+    PROBE_T(t_sel2);
     int cnt = k;
     if (kLek) {
         __shared__ int s_cnt;
@@ -582,14 +723,28 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
         cnt = s_cnt;
     }
 
-    for (int q = cnt + tid; q < P; q += blockDim.x)
-        hid[q] = 0xFFFFFFFFu;
-    __syncthreads();
-    bitonic_sort_u32(hid, P);
-    for (int q = tid; q < cnt; q += blockDim.x)
-        sel[(size_t)c * k + q] = hid[q];
+    const size_t words = (S + 31) / 32;
+    if (words * 4 <= (size_t)seg_len * 16) {
+        // ascending order through a bitmap of the selected list positions
+        uint32_t* bm = (uint32_t*)seg;
+        for (size_t w = tid; w < words; w += blockDim.x)
+            bm[w] = 0u;
+        __syncthreads();
+        for (int q = tid; q < cnt; q += blockDim.x)
+            atomicOr(&bm[hid[q] >> 5], 1u << (hid[q] & 31));
+        __syncthreads();
+        compact_bitmap(bm, words, sel + (size_t)c * k);
+    } else {
+        for (int q = cnt + tid; q < P; q += blockDim.x)
+            hid[q] = 0xFFFFFFFFu;
+        __syncthreads();
+        bitonic_sort_u32(hid, P);
+        for (int q = tid; q < cnt; q += blockDim.x)
+            sel[(size_t)c * k + q] = hid[q];
+    }
     if (tid == 0)
         sel_cnt[c] = cnt;
+    PROBE_ADD(1000 + 8 * c + 2, t_sel2);
 }
 
 // ---------------------------------------------------------------------------
@@ -733,9 +888,13 @@ __global__ __launch_bounds__(256) void k_emit(int comp, const double* __restrict
                                               const uint32_t* __restrict__ lek_state, int workers, double alpha,
                                               double prescale, double mult, double* __restrict__ learned,
                                               uint32_t* __restrict__ pos_out, uint32_t* __restrict__ idx_out,
-                                              double* __restrict__ vals_out, int* __restrict__ cnt_out)
+                                              double* __restrict__ vals_out, int* __restrict__ cnt_out,
+                                              uint32_t* __restrict__ sel_bm, int bm_words)
 {
+    extern __shared__ __align__(16) uint32_t bml[];  // bm_words
     const int c = blockIdx.x;
+    for (int w = threadIdx.x; w < bm_words; w += blockDim.x)
+        bml[w] = 0u;
     __shared__ int s_cnt, s_ins;
     __shared__ uint32_t s_pop;
     if (threadIdx.x == 0) {
@@ -785,53 +944,96 @@ __global__ __launch_bounds__(256) void k_emit(int comp, const double* __restrict
         lc[p] = lc[p] + upd;
         pos_out[(size_t)c * k + t] = p;
         idx_out[(size_t)c * k + t] = pos_flat(p, lda);
+        atomicOr(&bml[p >> 5], 1u << (p & 31));
     }
     for (int t = cnt + threadIdx.x; t < k; t += blockDim.x) {
         pos_out[(size_t)c * k + t] = 0xFFFFFFFFu;
         idx_out[(size_t)c * k + t] = 0xFFFFFFFFu;
         vals_out[(size_t)c * k + t] = 0.0;
     }
+    __syncthreads();
+    for (int w = threadIdx.x; w < bm_words; w += blockDim.x)
+        sel_bm[(size_t)c * bm_words + w] = bml[w];
 }
 
 // ---------------------------------------------------------------------------
 // k_master_h: H[idx] += val for clients 0..n-1 in order (TrainLoop.h:1197-1274).
-// One workgroup per window of list positions; client ranges by binary search.
+// Each thread owns one list position and walks the clients in order; a
+// client contributes where its selection bitmap has the bit set, with the
+// value it sent recomputed from Delta by the same operations as k_emit
+// (fl(Delta * prescale), or fl(fl(Delta * S/k) * prescale) for RandK/RandSeqK).
+// The bitmap words of the workgroup's 256 positions are staged in LDS; the
+// Delta loads of 16 clients are issued together.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_master_h(double* __restrict__ H, size_t S, int n, int k,
-                                                  const uint32_t* __restrict__ pos, const double* __restrict__ vals,
-                                                  const int* __restrict__ cnt)
+constexpr int kMasterPos = 256;
+
+__global__ __launch_bounds__(256) void k_master_h(double* __restrict__ H, size_t S, int n,
+                                                  const double* __restrict__ delta,
+                                                  const uint32_t* __restrict__ sel_bm, int bm_words, int scaled,
+                                                  double mult, double prescale)
 {
-    __shared__ double hw[kWin];
-    __shared__ int lo_s[kMaxClients], hi_s[kMaxClients];
-    const size_t p0 = (size_t)blockIdx.x * kWin;
-    const size_t p1 = min(p0 + (size_t)kWin, S);
-    for (size_t i = threadIdx.x; i < p1 - p0; i += blockDim.x)
-        hw[i] = H[p0 + i];
-    for (int cc = threadIdx.x; cc < n; cc += blockDim.x) {
-        const uint32_t* pc = pos + (size_t)cc * k;
-        const int m = cnt[cc];
-        int a = 0, b = m;
-        while (a < b) { const int mid = (a + b) >> 1; if (pc[mid] < p0) a = mid + 1; else b = mid; }
-        int a2 = a, b2 = m;
-        while (a2 < b2) { const int mid = (a2 + b2) >> 1; if (pc[mid] < p1) a2 = mid + 1; else b2 = mid; }
-        lo_s[cc] = a;
-        hi_s[cc] = a2;
+    __shared__ uint32_t bw[kMaxClients][kMasterPos / 32];
+    const size_t p0 = (size_t)blockIdx.x * kMasterPos;
+    const int w0 = (int)(p0 / 32);
+    const int nw = min(kMasterPos / 32, bm_words - w0);
+    for (int e = threadIdx.x; e < n * (kMasterPos / 32); e += blockDim.x) {
+        const int cc = e / (kMasterPos / 32), q = e % (kMasterPos / 32);
+        bw[cc][q] = (q < nw) ? sel_bm[(size_t)cc * bm_words + w0 + q] : 0u;
     }
     __syncthreads();
-    for (int cc = 0; cc < n; ++cc) {
-        const int a = lo_s[cc], b = hi_s[cc];
-        if (a == b)
-            continue;
-        const uint32_t* pc = pos + (size_t)cc * k;
-        const double* vc = vals + (size_t)cc * k;
-        for (int e = a + threadIdx.x; e < b; e += blockDim.x) {
-            const size_t o = pc[e] - p0;
-            hw[o] = hw[o] + vc[e];
+    const size_t p = p0 + threadIdx.x;
+    if (p >= S)
+        return;
+    const int q = threadIdx.x >> 5;
+    const uint32_t bit = 1u << (threadIdx.x & 31);
+    double h = H[p];
+    constexpr int kBatch = 16;
+    for (int c0 = 0; c0 < n; c0 += kBatch) {
+        double v[kBatch];
+        bool on[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int cc = c0 + u;
+            on[u] = (cc < n) && (bw[cc < n ? cc : 0][q] & bit);
+            v[u] = on[u] ? delta[(size_t)cc * S + p] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (on[u]) {
+                const double item = scaled ? v[u] * mult : v[u];
+                h = h + item * prescale;
+            }
+        }
+    }
+    H[p] = h;
+}
+
+// averaged gradient (TrainLoop.h:1100-1124,1292-1300): clients in order, 64
+// coordinates per workgroup, client gradients staged through LDS
+constexpr int kGradCols = 64;
+constexpr int kGradClients = 64;
+
+__global__ __launch_bounds__(256) void k_master_grad(int n, int d, int d_pad, const double* __restrict__ grad_c,
+                                                     double inv_n, double* __restrict__ g_avg)
+{
+    __shared__ double tile[kGradClients][kGradCols + 1];
+    const int i0 = blockIdx.x * kGradCols;
+    const int col = threadIdx.x & (kGradCols - 1);
+    double s = 0.0;
+    for (int c0 = 0; c0 < n; c0 += kGradClients) {
+        const int cn = min(kGradClients, n - c0);
+        for (int e = threadIdx.x; e < cn * kGradCols; e += blockDim.x) {
+            const int cc = e / kGradCols, ii = e % kGradCols;
+            const int i = i0 + ii;
+            tile[cc][ii] = (i < d) ? grad_c[(size_t)(c0 + cc) * d_pad + i] : 0.0;
         }
         __syncthreads();
+        if (threadIdx.x < kGradCols)
+            for (int cc = 0; cc < cn; ++cc) s = s + tile[cc][col];
+        __syncthreads();
     }
-    for (size_t i = threadIdx.x; i < p1 - p0; i += blockDim.x)
-        H[p0 + i] = hw[i];
+    if (threadIdx.x < kGradCols && i0 + col < d)
+        g_avg[i0 + col] = s * inv_n;
 }
 
 // H = (sum_c learned_c) / n, clients in order (TrainLoop.h:639-643, 732-735)
@@ -862,57 +1064,48 @@ __device__ inline double block_sum(double v, double* scratch)
 }
 
 // ---------------------------------------------------------------------------
-// k_master_vec: averaged gradient, Lk, f (TrainLoop.h:1100-1124,1278-1333,
-// 1512-1544), norms for the record (TrainLoop.h:1679-1691), TopLEK stream
+// k_master_vec: Lk and f averages (TrainLoop.h:1278-1333, 1512-1544), the
+// Newton shift, norms for the record (TrainLoop.h:1679-1691), TopLEK stream
 // advance. One workgroup.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_master_vec(int comp, int n, int d, int d_pad, int m, int m_pad,
-                                                    const double* __restrict__ grad_c, const double* __restrict__ part,
+__global__ __launch_bounds__(256) void k_master_vec(int comp, int n, int d, int m, int nlog_blocks,
+                                                    const double* __restrict__ g_avg, const double* __restrict__ part,
                                                     int tiles_up, double* __restrict__ lk,
-                                                    const double* __restrict__ nlog, const double* __restrict__ x,
+                                                    const double* __restrict__ nlog_part, const double* __restrict__ x,
                                                     double lambda, double inv_n, int option_b, int tracking,
                                                     const int* __restrict__ lek_hit, uint32_t* __restrict__ lek_state,
-                                                    int workers, double* __restrict__ g_avg,
-                                                    double* __restrict__ shift_out, double* __restrict__ trace,
-                                                    const int* __restrict__ round_ctr, int trace_cap)
+                                                    int workers, double* __restrict__ shift_out,
+                                                    double* __restrict__ trace, const int* __restrict__ round_ctr,
+                                                    int trace_cap)
 {
     __shared__ double scratch[16];
+    __shared__ double fc_s[kMaxClients];
     double gg = 0.0, xx = 0.0;
     for (int i = threadIdx.x; i < d; i += blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < n; ++c)
-            s = s + grad_c[(size_t)c * d_pad + i];
-        const double g = s * inv_n;
-        g_avg[i] = g;
+        const double g = g_avg[i];
         gg = fma(g, g, gg);
         xx = fma(x[i], x[i], xx);
     }
     gg = block_sum(gg, scratch);
     xx = block_sum(xx, scratch);
-    for (int c = threadIdx.x; c < n; c += blockDim.x)
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
         lk[c] = sqrt(fro2_from_parts(part, c, tiles_up));
-    __syncthreads();
-    double fsum = 0.0;
-    if (tracking) {
-        // f_c = mean_s(-log sigma_s) + lambda/2 ||x||^2 (logistic_regression.h:119-123)
-        __shared__ double fc_s[kMaxClients];
-        for (int c = threadIdx.x; c < n; c += blockDim.x) {
-            double s = 0.0;
-            for (int q = 0; q < m; ++q)
-                s = s + nlog[(size_t)c * m_pad + q];
-            fc_s[c] = s / (double)m + (lambda / 2.0) * xx;
+        if (tracking) {
+            // f_c = mean_s(-log sigma_s) + lambda/2 ||x||^2 (logistic_regression.h:119-123)
+            double sacc = 0.0;
+            for (int q = 0; q < nlog_blocks; ++q) sacc = sacc + nlog_part[(size_t)c * nlog_blocks + q];
+            fc_s[c] = sacc / (double)m + (lambda / 2.0) * xx;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double l = 0.0, fsum = 0.0;
+        for (int c = 0; c < n; ++c) l = l + lk[c];
+        l = l * inv_n;
+        if (tracking) {
             for (int c = 0; c < n; ++c) fsum = fsum + fc_s[c];
             fsum = fsum * inv_n;
         }
-    }
-    if (threadIdx.x == 0) {
-        double l = 0.0;
-        if (lk)
-            for (int c = 0; c < n; ++c) l = l + lk[c];
-        l = l * inv_n;
         *shift_out = option_b ? l : lambda;  // option a: mu(f) = lambda (logistic_regression.h:702-705)
         const int r = *round_ctr;
         if (r < trace_cap) {
@@ -934,200 +1127,299 @@ __global__ __launch_bounds__(256) void k_master_vec(int comp, int n, int d, int 
 // ---------------------------------------------------------------------------
 // k_newton: Cholesky of (H + shift I) and the two triangular solves of the
 // master (TrainLoop.h:1449-1466; CholeskyFactorization.h:153-272;
-// ElementarySolvers.h:72-226). Right-looking blocked factorisation in one
-// workgroup: each 32-column panel is factored in LDS, the trailing lower
-// triangle is updated in L2-resident scratch W (column-major lower, ld = dp).
-// On success x -= dx. The factorisation fails, as dopt's, when a pivot is < 0.
+// ElementarySolvers.h:72-226) in one workgroup of 16 waves.
+// Right-looking, 32-column panels, the right-hand side carried along so that
+// the forward substitution happens inside the factorisation:
+//   1. wave 0 factors the 32 x 32 diagonal block in LDS and solves its part
+//      of L y = g;
+//   2. every thread takes one row below the block: L21 = A21 L11^-T, and
+//      folds L21 y into the remaining right-hand side;
+//   3. the trailing lower triangle is updated with v_mfma_f64_16x16x4f64
+//      (W22 -= L21 L21^T) in L2-resident scratch W (column-major, ld = dp,
+//      dp = d rounded up to 32, padding set to the identity).
+// Then L^T dx = y by 32-row blocks from the bottom, and x -= dx.
+// The factorisation fails, as dopt's, when a pivot is < 0.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_newton(int d, int dp, const double* __restrict__ Hup,
-                                                 const double* __restrict__ shift_p, const double* __restrict__ b,
-                                                 double* __restrict__ W, double* __restrict__ x,
-                                                 double* __restrict__ dx_out, double* __restrict__ trace,
-                                                 int* __restrict__ round_ctr, int trace_cap, int update_x)
+constexpr int kNewtonThreads = 512;
+
+__global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const double* __restrict__ Hup,
+                                                           const double* __restrict__ shift_p,
+                                                           const double* __restrict__ b, double* __restrict__ W,
+                                                           double* __restrict__ x, double* __restrict__ dx_out,
+                                                           double* __restrict__ trace, int* __restrict__ round_ctr,
+                                                           int trace_cap, int update_x)
 {
     extern __shared__ __align__(16) double lds[];
-    double* Pn = lds;                // kNB x dp (column cc at Pn + cc*dp, rows relative to jb)
-    __shared__ double yv[kMaxNewtonD];
+    constexpr int kPS = kNB + 1;                    // row stride of the panel in LDS
+    double* P21 = lds;                              // rows below the diagonal block: P21[r * kPS + c]
+    __shared__ double Dg[kNB][kNB + 1];             // L11 of the current panel (lower)
+    __shared__ double invd[kNB];                    // 1 / L11[c][c]
+    __shared__ double colbuf[64];                   // broadcast of one column of L11 (lanes 32..63 scratch)
+    __shared__ double yv[kMaxNewtonD];              // right-hand side -> y -> dx
     __shared__ int s_fail;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    const int wave = tid >> 6, lane = tid & 63, nwaves = nthr >> 6;
     const double shift = *shift_p;
     if (tid == 0)
         s_fail = 0;
-    // W(i,j) = H(j,i) (+shift on the diagonal) for i >= j
-    for (size_t e = tid; e < (size_t)d * d; e += nthr) {
-        const int j = (int)(e / d), i = (int)(e % d);
-        if (i >= j) {
-            double v = Hup[(size_t)i * (i + 1) / 2 + j];
-            if (i == j)
-                v = v + shift;
-            W[(size_t)j * dp + i] = v;
+    PROBE_T(t_n0);
+    // W(i,j) = H(j,i) (+shift on the diagonal) for i >= j < d, from the packed
+    // upper H in order (eight loads in flight per thread). The padding of W
+    // (identity) is set once when W is allocated and survives the factorisation.
+    {
+        const size_t S = (size_t)d * (d + 1) / 2;
+        for (size_t p0 = tid; p0 < S; p0 += (size_t)nthr * 8) {
+            double hvv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const size_t p = p0 + (size_t)q * nthr;
+                hvv[q] = (p < S) ? Hup[p] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const size_t p = p0 + (size_t)q * nthr;
+                if (p < S) {
+                    const uint32_t j = pos_col(p);
+                    const uint32_t i = (uint32_t)(p - (size_t)j * (j + 1) / 2);  // i <= j
+                    W[(size_t)i * dp + j] = (i == j) ? hvv[q] + shift : hvv[q];
+                }
+            }
         }
     }
+    for (int i = tid; i < dp; i += nthr)
+        yv[i] = (i < d) ? b[i] : 0.0;
     __syncthreads();
 
-    for (int jb = 0; jb < d; jb += kNB) {
-        const int nb = min(kNB, d - jb);
-        const int R = d - jb;
-        for (int e = tid; e < nb * R; e += nthr) {
-            const int cc = e / R, r = e % R;
-            if (r >= cc)
-                Pn[cc * dp + r] = W[(size_t)(jb + cc) * dp + jb + r];
-        }
-        __syncthreads();
-        for (int cc = 0; cc < nb; ++cc) {
-            if (tid == 0) {
-                const double v = Pn[cc * dp + cc];
-                if (v < 0.0)
-                    s_fail = 1;
-                Pn[cc * dp + cc] = sqrt(v);
-            }
-            __syncthreads();
-            if (s_fail)
-                break;
-            const double inv = 1.0 / Pn[cc * dp + cc];
-            for (int r = cc + 1 + tid; r < R; r += nthr)
-                Pn[cc * dp + r] = inv * Pn[cc * dp + r];
-            __syncthreads();
-            // rank-1 update of the remaining panel columns
-            const int w = nb - cc - 1;
-            if (w > 0) {
-                for (int e = tid; e < w * R; e += nthr) {
-                    const int c2 = cc + 1 + e / R, r = e % R;
-                    if (r >= c2)
-                        Pn[c2 * dp + r] = fma(-Pn[cc * dp + r], Pn[cc * dp + c2], Pn[c2 * dp + r]);
+    PROBE_ADD(0, t_n0);
+    for (int jb = 0; jb < dp; jb += kNB) {
+        PROBE_T(t_p0);
+        const int T = dp - jb - kNB;
+        // 0. panel into LDS: A11 (lower) and A21
+        for (int e0 = tid; e0 < kNB * (kNB + T); e0 += nthr * 8) {
+            double pv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = e0 + q * nthr;
+                if (e < kNB * (kNB + T)) {
+                    const int c = e / (kNB + T), r = e % (kNB + T);
+                    pv[q] = W[(size_t)(jb + c) * dp + jb + r];
                 }
             }
-            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = e0 + q * nthr;
+                if (e < kNB * (kNB + T)) {
+                    const int c = e / (kNB + T), r = e % (kNB + T);
+                    if (r < kNB) {
+                        if (r >= c)
+                            Dg[r][c] = pv[q];
+                    } else {
+                        P21[(r - kNB) * kPS + c] = pv[q];
+                    }
+                }
+            }
         }
+        __syncthreads();
+        PROBE_ADD(1, t_p0);
+        PROBE_T(t_p1);
+        // 1. diagonal block (wave 0): lane r < 32 keeps row r of the block in
+        // registers; each step broadcasts one column of L through LDS.
+        if (wave == 0) {
+            double row[kNB];
+            const bool act = lane < kNB;
+            const int lr = act ? lane : 0;
+            // entries above the diagonal are carried along but never feed the
+            // lower triangle, so the updates below need no per-lane predicates
+#pragma unroll
+            for (int c = 0; c < kNB; ++c)
+                row[c] = Dg[lr][c];
+            double yr = act ? yv[jb + lane] : 0.0;
+            double mydiag = 1.0;
+            bool fail = false;
+#pragma unroll
+            for (int cc = 0; cc < kNB; ++cc) {
+                const double piv = bcast_lane(row[cc], cc);
+                fail = fail || (piv < 0.0);  // finish the (now meaningless) block, report after
+                const double l = sqrt(piv);
+                const double inv = 1.0 / l;
+                const double ycc = bcast_lane(yr, cc) * inv;
+                const double scaled = inv * row[cc];
+                const bool below = lane > cc;
+                const bool me = lane == cc;
+                row[cc] = me ? l : (below ? scaled : row[cc]);
+                mydiag = me ? l : mydiag;
+                yr = me ? ycc : (below ? fma(-scaled, ycc, yr) : yr);
+                colbuf[lane] = row[cc];
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int c2 = cc + 1; c2 < kNB; ++c2)
+                    row[c2] = fma(-row[cc], colbuf[c2], row[c2]);
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (fail) {
+                if (lane == 0)
+                    s_fail = 1;
+            } else if (act) {
+#pragma unroll
+                for (int c = 0; c < kNB; ++c)
+                    Dg[lane][c] = row[c];
+                invd[lane] = 1.0 / mydiag;
+                yv[jb + lane] = yr;
+            }
+        }
+        __syncthreads();
+        PROBE_ADD(2, t_p1);
+        PROBE_T(t_p2);
         if (s_fail)
             break;
-        for (int e = tid; e < nb * R; e += nthr) {
-            const int cc = e / R, r = e % R;
-            if (r >= cc)
-                W[(size_t)(jb + cc) * dp + jb + r] = Pn[cc * dp + r];
+        // 2. rows below the block: L21 = A21 L11^-T (column-oriented for
+        // independent FMAs); the right-hand side takes its L21 y share
+        for (int rr = tid; rr < T; rr += nthr) {
+            double a[kNB];
+            double* pr = P21 + rr * kPS;
+#pragma unroll
+            for (int cc = 0; cc < kNB; ++cc)
+                a[cc] = pr[cc];
+            const int i = jb + kNB + rr;
+            double bi = yv[i];
+#pragma unroll
+            for (int cc = 0; cc < kNB; ++cc) {
+                a[cc] = a[cc] * invd[cc];
+                bi = fma(-a[cc], yv[jb + cc], bi);
+#pragma unroll
+                for (int c2 = cc + 1; c2 < kNB; ++c2)
+                    a[c2] = fma(-a[cc], Dg[c2][cc], a[c2]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            yv[i] = bi;
+#pragma unroll
+            for (int cc = 0; cc < kNB; ++cc)
+                pr[cc] = a[cc];
         }
-        // trailing update of the lower triangle, 4 x 4 register tiles
-        const int T = R - nb;
+        __syncthreads();
+        PROBE_ADD(3, t_p2);
+        PROBE_T(t_p3);
+        // 3. L11, L21 back to W; trailing update W22 -= L21 L21^T on lower
+        // 16 x 16 tiles with FP64 MFMA (lane (l16, kq), item r: column 4r + kq,
+        // row l16 of the tile)
+        for (int e = tid; e < kNB * (kNB + T); e += nthr) {
+            const int c = e / (kNB + T), r = e % (kNB + T);
+            if (r < kNB) {
+                if (r >= c)
+                    W[(size_t)(jb + c) * dp + jb + r] = Dg[r][c];
+            } else {
+                W[(size_t)(jb + c) * dp + jb + r] = P21[(r - kNB) * kPS + c];
+            }
+        }
         if (T > 0) {
-            const int tb = (T + 3) / 4;
+            const int tb = T / 16;
             const int ntiles = tb * (tb + 1) / 2;
-            for (int tt = tid; tt < ntiles; tt += nthr) {
-                int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-                while (ti * (ti + 1) / 2 > tt) --ti;
-                while ((ti + 1) * (ti + 2) / 2 <= tt) ++ti;
-                const int tj = tt - ti * (ti + 1) / 2;  // tj <= ti
-                const int rb = nb + ti * 4, cb = nb + tj * 4;  // rows i, cols j (relative to jb)
-                double a4[4][4];
+            const int l16 = lane & 15, kq = lane >> 4;
+            constexpr int kGroup = 4;  // tiles per wave whose W loads are in flight together
+            for (int t0 = wave * kGroup; t0 < ntiles; t0 += nwaves * kGroup) {
+                int rbs[kGroup], cbs[kGroup];
+                double wv[kGroup][4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int g = 0; g < kGroup; ++g) {
+                    const int tt = t0 + g;
+                    rbs[g] = -1;
+                    cbs[g] = 0;
+                    if (tt < ntiles) {
+                        int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
+                        while (ti * (ti + 1) / 2 > tt) --ti;
+                        while ((ti + 1) * (ti + 2) / 2 <= tt) ++ti;
+                        rbs[g] = ti * 16;
+                        cbs[g] = (tt - ti * (ti + 1) / 2) * 16;  // column tile <= row tile
+                        const double* wp = W + (size_t)(jb + kNB + cbs[g] + kq) * dp + jb + kNB + rbs[g] + l16;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) a4[u][v] = 0.0;
-                for (int cc = 0; cc < nb; ++cc) {
-                    double ri[4], cj[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        ri[u] = (rb + u < R) ? Pn[cc * dp + rb + u] : 0.0;
-                        cj[u] = (cb + u < R) ? Pn[cc * dp + cb + u] : 0.0;
+                        for (int r = 0; r < 4; ++r)
+                            wv[g][r] = wp[(size_t)4 * r * dp];
                     }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) a4[u][v] = fma(ri[u], cj[v], a4[u][v]);
                 }
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int j = cb + v;
-                    if (j >= R)
+                for (int g = 0; g < kGroup; ++g) {
+                    if (rbs[g] < 0)
                         continue;
+                    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int i = rb + u;
-                        if (i < R && i >= j) {
-                            double* wp = &W[(size_t)(jb + j) * dp + jb + i];
-                            *wp = *wp - a4[u][v];
-                        }
+                    for (int k0 = 0; k0 < kNB; k0 += 4) {
+                        const double av = P21[(cbs[g] + l16) * kPS + k0 + kq];  // A[m = col][k]
+                        const double bv = P21[(rbs[g] + l16) * kPS + k0 + kq];  // B[k][n = row]
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
                     }
+                    double* wp = W + (size_t)(jb + kNB + cbs[g] + kq) * dp + jb + kNB + rbs[g] + l16;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        wp[(size_t)4 * r * dp] = wv[g][r] - acc[r];
                 }
             }
         }
         __syncthreads();
+        PROBE_ADD(4, t_p3);
     }
+    PROBE_T(t_b0);
 
-    const int r = *round_ctr;
+    const int rnd = *round_ctr;
     if (s_fail) {
         if (tid == 0) {
-            if (r < trace_cap)
-                trace[(size_t)r * 4 + 3] = 0.0;
-            *round_ctr = r + 1;
+            if (rnd < trace_cap)
+                trace[(size_t)rnd * 4 + 3] = 0.0;
+            *round_ctr = rnd + 1;
         }
         for (int i = tid; i < d; i += nthr)
             dx_out[i] = 0.0;
         return;
     }
 
-    // forward: L y = b
-    for (int i = tid; i < d; i += nthr)
-        yv[i] = b[i];
-    __syncthreads();
-    for (int jb = 0; jb < d; jb += kNB) {
-        const int nb = min(kNB, d - jb);
-        if (tid < 64) {
-            for (int q = 0; q < nb; ++q) {
-                const int i = jb + q;
-                if (tid == 0)
-                    yv[i] = yv[i] / W[(size_t)i * dp + i];
-                __builtin_amdgcn_wave_barrier();
-                const double yi = yv[i];
-                const int rr = i + 1 + tid;
-                if (rr < jb + nb)
-                    yv[rr] = fma(-W[(size_t)i * dp + rr], yi, yv[rr]);
-                __builtin_amdgcn_wave_barrier();
+    // L^T dx = y, 32-row blocks from the bottom. In a block, lane r < 32 keeps
+    // column r of L11 (rows jb..jb+31) in registers.
+    for (int jb = dp - kNB; jb >= 0; jb -= kNB) {
+        if (wave == 0) {
+            const bool act = lane < kNB;
+            const int lr = act ? lane : 0;
+            double colv[kNB];
+            // column lr of L11 from rows jb..jb+31 (entries above the diagonal unused)
+            const double* cp = W + (size_t)(jb + lr) * dp + jb;
+#pragma unroll
+            for (int c = 0; c < kNB; ++c)
+                colv[c] = cp[c];
+            double yr = act ? yv[jb + lane] : 0.0;
+#pragma unroll
+            for (int cc = kNB - 1; cc >= 0; --cc) {
+                const double q = yr / colv[cc];
+                const double xc = bcast_lane(q, cc);
+                const double upd = fma(-colv[cc], xc, yr);
+                yr = (lane == cc) ? xc : ((lane < cc) ? upd : yr);
+                __builtin_amdgcn_sched_barrier(0);
             }
-        }
-        __syncthreads();
-        for (int rr = jb + nb + tid; rr < d; rr += nthr) {
-            double s = yv[rr];
-            for (int q = 0; q < nb; ++q)
-                s = fma(-W[(size_t)(jb + q) * dp + rr], yv[jb + q], s);
-            yv[rr] = s;
-        }
-        __syncthreads();
-    }
-    // backward: L^T x = y
-    const int last_jb = ((d - 1) / kNB) * kNB;
-    for (int jb = last_jb; jb >= 0; jb -= kNB) {
-        const int nb = min(kNB, d - jb);
-        if (tid < 64) {
-            for (int q = nb - 1; q >= 0; --q) {
-                const int i = jb + q;
-                if (tid == 0)
-                    yv[i] = yv[i] / W[(size_t)i * dp + i];
-                __builtin_amdgcn_wave_barrier();
-                const double xi = yv[i];
-                const int rr = jb + tid;
-                if (rr < i)
-                    yv[rr] = fma(-W[(size_t)rr * dp + i], xi, yv[rr]);
-                __builtin_amdgcn_wave_barrier();
-            }
+            if (act)
+                yv[jb + lane] = yr;
         }
         __syncthreads();
         for (int rr = tid; rr < jb; rr += nthr) {
+            double wl[kNB];
+#pragma unroll
+            for (int cc = 0; cc < kNB; ++cc)
+                wl[cc] = W[(size_t)rr * dp + jb + cc];
             double s = yv[rr];
-            for (int q = 0; q < nb; ++q)
-                s = fma(-W[(size_t)rr * dp + jb + q], yv[jb + q], s);
+#pragma unroll
+            for (int cc = 0; cc < kNB; ++cc)
+                s = fma(-wl[cc], yv[jb + cc], s);
             yv[rr] = s;
         }
         __syncthreads();
     }
+    PROBE_ADD(5, t_b0);
     for (int i = tid; i < d; i += nthr) {
         dx_out[i] = yv[i];
         if (update_x)
             x[i] = x[i] - yv[i];
     }
     if (tid == 0) {
-        if (r < trace_cap)
-            trace[(size_t)r * 4 + 3] = 1.0;
-        *round_ctr = r + 1;
+        if (rnd < trace_cap)
+            trace[(size_t)rnd * 4 + 3] = 1.0;
+        *round_ctr = rnd + 1;
     }
 }
 
@@ -1167,7 +1459,8 @@ struct Engine {
     double *A = nullptr, *x = nullptr, *Dw = nullptr, *gw = nullptr, *nlog = nullptr;
     double *learned = nullptr, *delta = nullptr, *grad_c = nullptr, *part = nullptr, *lk = nullptr;
     uint32_t *sel = nullptr, *pos = nullptr, *idx = nullptr, *rng = nullptr, *lek_state = nullptr, *lek_popped = nullptr;
-    uint32_t *seq_start = nullptr;
+    uint32_t *seq_start = nullptr, *sel_bm = nullptr;
+    int bm_words = 0;
     int *sel_cnt = nullptr, *cnt = nullptr, *lek_hit = nullptr, *round_ctr = nullptr;
     double *lek_p = nullptr, *vals = nullptr, *H = nullptr, *g_avg = nullptr, *shift = nullptr, *W = nullptr;
     double *dx = nullptr, *trace = nullptr;
@@ -1183,6 +1476,7 @@ struct Engine {
     double kms[KID_COUNT] = {0};
     int klaunch[KID_COUNT] = {0};
     size_t lds_select = 0, lds_newton = 0;
+    int seg_len = 256;
 };
 
 static hipEvent_t pool_event(Engine* e)
@@ -1274,11 +1568,11 @@ static int launch_round(Engine* e)
         KTime kt(e, KID_SELECT);
         if (comp == FEDNL_TOPK) {
             hipLaunchKernelGGL(k_select_heap<false>, dim3(n), dim3(256), e->lds_select, e->st, e->delta, e->S, e->k,
-                               e->part, e->tiles_up, e->lek_delta, (double*)nullptr, e->sel, e->sel_cnt, e->lek_p, e->lek_hit,
+                               e->seg_len, e->part, e->tiles_up, e->lek_delta, (double*)nullptr, e->sel, e->sel_cnt, e->lek_p, e->lek_hit,
                                e->lek_popped);
         } else if (comp == FEDNL_TOPLEK) {
             hipLaunchKernelGGL(k_select_heap<true>, dim3(n), dim3(256), e->lds_select, e->st, e->delta, e->S, e->k,
-                               e->part, e->tiles_up, e->lek_delta, (double*)nullptr, e->sel, e->sel_cnt, e->lek_p, e->lek_hit,
+                               e->seg_len, e->part, e->tiles_up, e->lek_delta, (double*)nullptr, e->sel, e->sel_cnt, e->lek_p, e->lek_hit,
                                e->lek_popped);
         } else {
             if (comp == FEDNL_RANDK)
@@ -1291,34 +1585,52 @@ static int launch_round(Engine* e)
     }
     {
         KTime kt(e, KID_EMIT);
-        hipLaunchKernelGGL(k_emit, dim3(n), dim3(256), 0, e->st, comp, e->delta, e->S, e->k, e->lda, e->sel,
-                           e->sel_cnt, e->lek_p, e->lek_hit, e->lek_popped, e->lek_state, e->workers, e->alpha,
-                           e->prescale, e->mult, e->learned, e->pos, e->idx, e->vals, e->cnt);
+        hipLaunchKernelGGL(k_emit, dim3(n), dim3(256), sizeof(uint32_t) * e->bm_words, e->st, comp, e->delta, e->S,
+                           e->k, e->lda, e->sel, e->sel_cnt, e->lek_p, e->lek_hit, e->lek_popped, e->lek_state,
+                           e->workers, e->alpha, e->prescale, e->mult, e->learned, e->pos, e->idx, e->vals, e->cnt,
+                           e->sel_bm, e->bm_words);
     }
     {
         KTime kt(e, KID_MASTER_H);
-        const int nblk = (int)((e->S + kWin - 1) / kWin);
-        hipLaunchKernelGGL(k_master_h, dim3(nblk), dim3(256), 0, e->st, e->H, e->S, n, e->k, e->pos, e->vals, e->cnt);
+        const int nblk = (int)((e->S + kMasterPos - 1) / kMasterPos);
+        const int scaled = (comp == FEDNL_RANDK || comp == FEDNL_RANDSEQK) ? 1 : 0;
+        hipLaunchKernelGGL(k_master_h, dim3(nblk), dim3(256), 0, e->st, e->H, e->S, n, e->delta, e->sel_bm,
+                           e->bm_words, scaled, e->mult, e->prescale);
     }
     {
         KTime kt(e, KID_MASTER_VEC);
-        hipLaunchKernelGGL(k_master_vec, dim3(1), dim3(256), 0, e->st, comp, n, e->d, e->d_pad, e->m, e->m_pad,
-                           e->grad_c, e->part, e->tiles_up, e->lk, e->nlog, e->x, e->o.lambda, e->inv_n, e->o.option_b, e->o.tracking,
-                           e->lek_hit, e->lek_state, e->workers, e->g_avg, e->shift, e->trace, e->round_ctr,
+        hipLaunchKernelGGL(k_master_grad, dim3((e->d + kGradCols - 1) / kGradCols), dim3(256), 0, e->st, n, e->d,
+                           e->d_pad, e->grad_c, e->inv_n, e->g_avg);
+        hipLaunchKernelGGL(k_master_vec, dim3(1), dim3(256), 0, e->st, comp, n, e->d, e->m, e->m_pad / 16, e->g_avg,
+                           e->part, e->tiles_up, e->lk, e->nlog, e->x, e->o.lambda, e->inv_n, e->o.option_b,
+                           e->o.tracking, e->lek_hit, e->lek_state, e->workers, e->shift, e->trace, e->round_ctr,
                            e->trace_cap);
     }
     {
         KTime kt(e, KID_NEWTON);
-        hipLaunchKernelGGL(k_newton, dim3(1), dim3(1024), e->lds_newton, e->st, e->d, e->dp, e->H, e->shift,
+        hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), e->lds_newton, e->st, e->d, e->dp, e->H, e->shift,
                            e->g_avg, e->W, e->x, e->dx, e->trace, e->round_ctr, e->trace_cap, 1);
     }
     return FEDNL_OK;
 }
 
+constexpr size_t kLdsBudget = 160 * 1024 - 4096;  // dynamic LDS of the selection kernels
+
+// staging segment of the heap scan: what is left of the LDS budget after the
+// heap, two buffers, a multiple of 256 list positions, at most 8192
+static int select_seg_len(int k)
+{
+    const int P = pow2_at_least(k);
+    const size_t heap = (size_t)8 * (k + 4) + (size_t)4 * round_up(P + 4, 2);
+    const size_t left = kLdsBudget > heap ? kLdsBudget - heap : 0;
+    int seg = (int)std::min<size_t>(8192, left / 16 / 256 * 256);
+    return std::max(seg, 256);
+}
+
 static size_t select_lds_bytes(int k)
 {
     const int P = pow2_at_least(k);
-    const size_t heap = (size_t)8 * k + (size_t)4 * round_up(P, 2) + (size_t)16 * kSeg;
+    const size_t heap = (size_t)8 * (k + 4) + (size_t)4 * round_up(P + 4, 2) + (size_t)16 * select_seg_len(k);
     const size_t rk = (size_t)8 * P + (size_t)8 * k;
     return std::max(heap, rk);
 }
@@ -1327,7 +1639,7 @@ static void free_engine(Engine* e)
 {
     if (!e)
         return;
-    void* ptrs[] = {e->A, e->x, e->Dw, e->gw, e->nlog, e->learned, e->delta, e->grad_c, e->part, e->lk,
+    void* ptrs[] = {e->A, e->x, e->Dw, e->gw, e->nlog, e->learned, e->delta, e->grad_c, e->part, e->lk, e->sel_bm,
                     e->sel, e->pos, e->idx, e->rng, e->lek_state, e->lek_popped, e->seq_start, e->sel_cnt,
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
                     e->dx, e->trace};
@@ -1420,7 +1732,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     e->d_pad = round_up(d, kTile);
     e->nt = e->d_pad / kTile;
     e->tiles_up = e->nt * (e->nt + 1) / 2;
-    e->dp = round_up(d, 4);
+    e->dp = round_up(d, kNB);
     e->workers = std::max(1, std::min(opts->workers, n));
     e->alpha = theory_alpha(comp, opts->alpha_mode, opts->alpha, e->k, d, e->S);
     e->inv_n = 1.0 / (double)n;
@@ -1428,7 +1740,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     e->mult = (double)e->S / (double)e->k;
     e->lek_delta = (double)e->k * (1.0 / (double)e->S);
     e->lds_select = select_lds_bytes(std::min<int>(e->k, kMaxHeapK));
-    e->lds_newton = sizeof(double) * (size_t)kNB * e->dp;
+    e->seg_len = select_seg_len(std::min<int>(e->k, kMaxHeapK));
+    e->lds_newton = sizeof(double) * (size_t)(kNB + 1) * e->dp;
 
     int rc = FEDNL_OK;
 #define ALLOC(ptr, cnt)                                          \
@@ -1464,6 +1777,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     ALLOC(e->lek_state, e->workers);
     ALLOC(e->lek_popped, n);
     ALLOC(e->seq_start, n);
+    e->bm_words = (int)((e->S + 31) / 32);
+    ALLOC(e->sel_bm, (size_t)n * e->bm_words);
     ALLOC(e->sel_cnt, n);
     ALLOC(e->cnt, n);
     ALLOC(e->lek_hit, n);
@@ -1473,6 +1788,17 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     ALLOC(e->g_avg, e->d_pad);
     ALLOC(e->shift, 1);
     ALLOC(e->W, (size_t)e->dp * e->dp);
+    {
+        // W padding: identity (rows/columns d..dp-1), zero elsewhere; the lower
+        // d x d part is rewritten every round
+        std::vector<double> w0((size_t)e->dp * e->dp, 0.0);
+        for (int i = d; i < e->dp; ++i) w0[(size_t)i * e->dp + i] = 1.0;
+        if (hipMemcpy(e->W, w0.data(), sizeof(double) * w0.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            set_err("hipMemcpy W");
+            free_engine(e);
+            return FEDNL_E_DEVICE;
+        }
+    }
     ALLOC(e->dx, e->d_pad);
 #undef ALLOC
     if ((rc = ensure_trace(e, 1024)) != FEDNL_OK) {
@@ -1828,6 +2154,7 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     if (rng) hipMemcpyAsync(drng, rng, sizeof(uint32_t) * nclients, hipMemcpyHostToDevice, st);
     if (toplek_rng) hipMemcpyAsync(dlek, toplek_rng, sizeof(uint32_t) * workers, hipMemcpyHostToDevice, st);
     const size_t lds = select_lds_bytes(std::min(k, kMaxHeapK));
+    const int seg_len = select_seg_len(std::min(k, kMaxHeapK));
     if (lds > 64 * 1024) {
         hipFuncSetAttribute((const void*)k_select_heap<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipFuncSetAttribute((const void*)k_select_heap<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1886,10 +2213,10 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
         hipStreamSynchronize(st);
     }
     if (compressor == FEDNL_TOPK)
-        hipLaunchKernelGGL(k_select_heap<false>, dim3(nclients), dim3(256), lds, st, ddelta, S, k, (const double*)nullptr,
+        hipLaunchKernelGGL(k_select_heap<false>, dim3(nclients), dim3(256), lds, st, ddelta, S, k, seg_len, (const double*)nullptr,
                            0, delta_param, (double*)nullptr, sel, sel_cnt, lek_p, hit, popped);
     else if (compressor == FEDNL_TOPLEK)
-        hipLaunchKernelGGL(k_select_heap<true>, dim3(nclients), dim3(256), lds, st, ddelta, S, k, parts, 1,
+        hipLaunchKernelGGL(k_select_heap<true>, dim3(nclients), dim3(256), lds, st, ddelta, S, k, seg_len, parts, 1,
                            delta_param, (double*)nullptr, sel, sel_cnt, lek_p, hit, popped);
     else if (compressor == FEDNL_RANDK)
         hipLaunchKernelGGL(k_select_randk, dim3(nclients), dim3(256), lds, st, S, k, drng, sel, sel_cnt);
@@ -1900,8 +2227,15 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
         cleanup();
         return FEDNL_E_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(k_emit, dim3(nclients), dim3(256), 0, st, compressor, ddelta, S, k, round_up(d, 4), sel, sel_cnt,
-                       lek_p, hit, popped, dlek, workers, 1.0, 1.0, 1.0, learned, pos, idx, vals, cnt);
+    const int bm_words = (int)((S + 31) / 32);
+    uint32_t* bm = nullptr;
+    if (dalloc(&bm, (size_t)nclients * bm_words) != hipSuccess) {
+        cleanup();
+        return FEDNL_E_DEVICE;
+    }
+    hipLaunchKernelGGL(k_emit, dim3(nclients), dim3(256), sizeof(uint32_t) * bm_words, st, compressor, ddelta, S, k,
+                       round_up(d, 4), sel, sel_cnt, lek_p, hit, popped, dlek, workers, 1.0, 1.0, 1.0, learned, pos,
+                       idx, vals, cnt, bm, bm_words);
     if (compressor == FEDNL_TOPLEK) {
         // advance the streams as the master kernel does
         std::vector<int> hh(nclients);
@@ -1929,9 +2263,24 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
         if (rng) hipMemcpy(rng, drng, sizeof(uint32_t) * nclients, hipMemcpyDeviceToHost);
     }
     if (parts) hipFree(parts);
+    hipFree(bm);
     cleanup();
     return rc;
 }
+
+#ifdef FEDNL_PROBE
+int fednl_probe_read(unsigned long long* out, int count, int reset)
+{
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_probe), sizeof(unsigned long long) * count) != hipSuccess)
+        return -1;
+    if (reset) {
+        std::vector<unsigned long long> z(16384, 0ull);
+        if (hipMemcpyToSymbol(HIP_SYMBOL(g_probe), z.data(), sizeof(unsigned long long) * z.size()) != hipSuccess)
+            return -1;
+    }
+    return 0;
+}
+#endif
 
 int fednl_probe_mfma_f64(double* out768)
 {
@@ -1956,7 +2305,7 @@ int fednl_newton_solve(int d, const double* H_up, double shift, const double* b,
         return FEDNL_E_UNSUPPORTED;
     }
     const size_t S = (size_t)d * (d + 1) / 2;
-    const int dp = round_up(d, 4);
+    const int dp = round_up(d, kNB);
     double *dH = nullptr, *dsh = nullptr, *db = nullptr, *dW = nullptr, *dx_d = nullptr, *dtr = nullptr;
     int* dctr = nullptr;
     HIP_OK(dalloc(&dH, S));
@@ -1971,10 +2320,15 @@ int fednl_newton_solve(int d, const double* H_up, double shift, const double* b,
     HIP_OK(hipMemcpy(db, b, sizeof(double) * d, hipMemcpyHostToDevice));
     HIP_OK(hipMemset(dctr, 0, sizeof(int)));
     HIP_OK(hipMemset(dtr, 0, sizeof(double) * 4));
-    const size_t lds = sizeof(double) * (size_t)kNB * dp;
+    {
+        std::vector<double> w0((size_t)dp * dp, 0.0);
+        for (int i = d; i < dp; ++i) w0[(size_t)i * dp + i] = 1.0;
+        HIP_OK(hipMemcpy(dW, w0.data(), sizeof(double) * w0.size(), hipMemcpyHostToDevice));
+    }
+    const size_t lds = sizeof(double) * (size_t)(kNB + 1) * dp;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k_newton, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_newton, dim3(1), dim3(1024), lds, 0, d, dp, dH, dsh, db, dW, (double*)nullptr, dx_d, dtr, dctr,
+    hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), lds, 0, d, dp, dH, dsh, db, dW, (double*)nullptr, dx_d, dtr, dctr,
                        1, 0);
     HIP_OK(hipGetLastError());
     HIP_OK(hipDeviceSynchronize());
