@@ -98,7 +98,20 @@ def main():
     ap.add_argument("--tol-rounds", type=int, default=400, help="round budget of the time-to-tolerance solve")
     ap.add_argument("--cpu-rounds", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe_mb:
+        import torch
+        n = args.copy_probe_mb * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the FedNL simulation runs every client "

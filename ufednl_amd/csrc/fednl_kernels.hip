@@ -1141,6 +1141,7 @@ __global__ __launch_bounds__(256) void k_master_vec(int comp, int n, int d, int 
 // The factorisation fails, as dopt's, when a pivot is < 0.
 // ---------------------------------------------------------------------------
 constexpr int kNewtonThreads = 512;
+static_assert(kMaxNewtonD - 32 < (kNewtonThreads / 64) * 64, "k_newton TRSM assumes one 64-row chunk per wave");
 
 __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const double* __restrict__ Hup,
                                                            const double* __restrict__ shift_p,
@@ -1154,7 +1155,6 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
     double* P21 = lds;                              // rows below the diagonal block: P21[r * kPS + c]
     __shared__ double Dg[kNB][kNB + 1];             // L11 of the current panel (lower)
     __shared__ double invd[kNB];                    // 1 / L11[c][c]
-    __shared__ double colbuf[64];                   // broadcast of one column of L11 (lanes 32..63 scratch)
     __shared__ double yv[kMaxNewtonD];              // right-hand side -> y -> dx
     __shared__ int s_fail;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -1249,12 +1249,10 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
                 row[cc] = me ? l : (below ? scaled : row[cc]);
                 mydiag = me ? l : mydiag;
                 yr = me ? ycc : (below ? fma(-scaled, ycc, yr) : yr);
-                colbuf[lane] = row[cc];
-                __builtin_amdgcn_wave_barrier();
+                // L[c2][cc] lives in lane c2: broadcast by v_readlane
 #pragma unroll
                 for (int c2 = cc + 1; c2 < kNB; ++c2)
-                    row[c2] = fma(-row[cc], colbuf[c2], row[c2]);
-                __builtin_amdgcn_wave_barrier();
+                    row[c2] = fma(-row[cc], bcast_lane(row[cc], c2), row[c2]);
                 __builtin_amdgcn_sched_barrier(0);
             }
             if (fail) {
@@ -1274,28 +1272,49 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
         if (s_fail)
             break;
         // 2. rows below the block: L21 = A21 L11^-T (column-oriented for
-        // independent FMAs); the right-hand side takes its L21 y share
-        for (int rr = tid; rr < T; rr += nthr) {
-            double a[kNB];
-            double* pr = P21 + rr * kPS;
+        // independent FMAs); the right-hand side takes its L21 y share. Every
+        // wave holds L11 in lanes 0..31 (row r in lane r) and broadcasts its
+        // entries by v_readlane; lane l solves row (wave row base) + l.
+        // T <= kMaxNewtonD - kNB < nwaves * 64: one chunk of rows per wave.
+        if (wave * 64 < T) {
+            const int rb = wave * 64;
+            double lrow[kNB];
+            const int lr = lane & (kNB - 1);
 #pragma unroll
-            for (int cc = 0; cc < kNB; ++cc)
-                a[cc] = pr[cc];
-            const int i = jb + kNB + rr;
-            double bi = yv[i];
+            for (int c = 0; c < kNB; ++c)
+                lrow[c] = Dg[lr][c];
+            const double invl = invd[lr];
+            const double ypan = yv[jb + lr];
+            {
+                const int rr = rb + lane;
+                const bool live = rr < T;
+                double a[kNB];
+                double* pr = P21 + (live ? rr : 0) * kPS;
 #pragma unroll
-            for (int cc = 0; cc < kNB; ++cc) {
-                a[cc] = a[cc] * invd[cc];
-                bi = fma(-a[cc], yv[jb + cc], bi);
+                for (int cc = 0; cc < kNB; ++cc)
+                    a[cc] = pr[cc];
+                const int i = jb + kNB + (live ? rr : 0);
+                double bi = yv[i];
 #pragma unroll
-                for (int c2 = cc + 1; c2 < kNB; ++c2)
-                    a[c2] = fma(-a[cc], Dg[c2][cc], a[c2]);
-                __builtin_amdgcn_sched_barrier(0);
+                for (int cc = 0; cc < kNB; ++cc) {
+                    a[cc] = a[cc] * bcast_lane(invl, cc);
+                    bi = fma(-a[cc], bcast_lane(ypan, cc), bi);
+                    // ties this column's broadcasts to a[cc] so that they are
+                    // not all hoisted ahead of the solve (SGPR pressure)
+                    double lc = lrow[cc];
+                    asm volatile("" : "+v"(lc) : "v"(a[cc]));
+#pragma unroll
+                    for (int c2 = cc + 1; c2 < kNB; ++c2)
+                        a[c2] = fma(-a[cc], bcast_lane(lc, c2), a[c2]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (live) {
+                    yv[i] = bi;
+#pragma unroll
+                    for (int cc = 0; cc < kNB; ++cc)
+                        pr[cc] = a[cc];
+                }
             }
-            yv[i] = bi;
-#pragma unroll
-            for (int cc = 0; cc < kNB; ++cc)
-                pr[cc] = a[cc];
         }
         __syncthreads();
         PROBE_ADD(3, t_p2);
