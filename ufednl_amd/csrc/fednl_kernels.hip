@@ -125,6 +125,24 @@ __device__ inline double bcast_lane(double v, int src)
     return __longlong_as_double(((unsigned long long)hi << 32) | lo);
 }
 
+// quad_perm DPP move of a double (both halves with the same pattern)
+template <int kPerm>
+__device__ inline double dpp_quad(double v)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u & 0xFFFFFFFFull), kPerm, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), kPerm, 0xF, 0xF, false);
+    return __longlong_as_double(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// minimum over each aligned group of four lanes, in every lane of the group
+__device__ inline double quad_min(double v)
+{
+    v = fmin(v, dpp_quad<0xB1>(v));  // lanes (1,0,3,2)
+    v = fmin(v, dpp_quad<0x4E>(v));  // lanes (2,3,0,1)
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // LCGs of dopt (RandomGenIntegerLinear.h:49-123, RandomGenRealLinear.h:49-54)
 // ---------------------------------------------------------------------------
@@ -411,6 +429,192 @@ __device__ inline double heapify(double* hv, uint32_t* hid, int i, int n)
     return heapify_x(hv, hid, i, n, hv[i], hid[i]);
 }
 
+// minHeapify from slot i after an element moved up out of it: comparisons run
+// against cmpv (the value that moved up), the sifted element (xval, xid) lands
+// in the final slot. Returns nothing; the heap arrays are updated in place.
+__device__ inline void heapify_cont(double* hv, uint32_t* hid, int i, int n, double cmpv, double xval,
+                                    uint32_t xid)
+{
+    double lowv = cmpv;
+    for (;;) {
+        const int first = 4 * i + 1;
+        if (first >= n)
+            break;
+        double cv[4];
+        uint32_t ci[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            cv[q] = hv[first + q];
+            ci[q] = hid[first + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            cv[q] = (first + q < n) ? cv[q] : __builtin_inf();
+        int lowest = -1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (lowv > cv[q]) {
+                lowest = q;
+                lowv = cv[q];
+            }
+        }
+        if (lowest < 0)
+            break;
+        hv[i] = lowv;
+        hid[i] = ci[lowest];
+        i = first + lowest;
+    }
+    hv[i] = xval;
+    hid[i] = xid;
+}
+
+// ---------------------------------------------------------------------------
+// The top four levels of the 4-ary heap (slots 0..84) held in the registers
+// of one wave: slot 0 in every lane (r0v, r0i), level 1 (slots 1..4) in lanes
+// 0..3 of (v1, i1), level 2 (slots 5..20) in lanes 0..15 of (v2, i2), level 3
+// (slots 21..84) in lanes 0..63 of (v3, i3). The children of the level-L slot
+// with offset o are the lanes 4o..4o+3 of level L+1, one DPP quad. Deeper
+// slots stay in LDS. admit() is dopt's increaseMinimumInMinHeap + minHeapify
+// (MinHeap.h:78-85, 255-290) with the same comparison sequence: the child that
+// moves up is the first of its group holding the group minimum, when that
+// minimum is below the running comparison value.
+// ---------------------------------------------------------------------------
+struct RegHeap {
+    double r0v, v1, v2, v3;
+    uint32_t r0i, i1, i2, i3;
+    int n, lane;
+
+    __device__ void load(const double* hv, const uint32_t* hid, int n_, int lane_)
+    {
+        n = n_;
+        lane = lane_;
+        const double inf = __builtin_inf();
+        r0v = hv[0];
+        r0i = hid[0];
+        v1 = (lane < 4 && 1 + lane < n) ? hv[1 + lane] : inf;
+        i1 = (lane < 4 && 1 + lane < n) ? hid[1 + lane] : 0u;
+        v2 = (lane < 16 && 5 + lane < n) ? hv[5 + lane] : inf;
+        i2 = (lane < 16 && 5 + lane < n) ? hid[5 + lane] : 0u;
+        v3 = (21 + lane < n) ? hv[21 + lane] : inf;
+        i3 = (21 + lane < n) ? hid[21 + lane] : 0u;
+    }
+
+    __device__ void store(double* hv, uint32_t* hid) const
+    {
+        if (lane == 0) {
+            hv[0] = r0v;
+            hid[0] = r0i;
+        }
+        if (lane < 4 && 1 + lane < n) {
+            hv[1 + lane] = v1;
+            hid[1 + lane] = i1;
+        }
+        if (lane < 16 && 5 + lane < n) {
+            hv[5 + lane] = v2;
+            hid[5 + lane] = i2;
+        }
+        if (21 + lane < n) {
+            hv[21 + lane] = v3;
+            hid[21 + lane] = i3;
+        }
+    }
+
+    // one level: children of offset o held in (cv_reg, ci_reg); on a move the
+    // minimum and its lane come back in (mu, q)
+    __device__ bool step(double cv_reg, int o, double lowv, double& mu, int& q) const
+    {
+        const bool mine = (lane >> 2) == o;
+        const double cv = mine ? cv_reg : __builtin_inf();
+        const double mn = quad_min(cv);
+        mu = bcast_lane(mn, 4 * o);
+        if (!(lowv > mu))
+            return false;
+        const uint64_t eq = __ballot(mine && cv == mu);
+        q = __builtin_ctzll(eq);
+        return true;
+    }
+
+    // slot 0 receives (key, id) and sifts down
+    __device__ void admit(double* hv, uint32_t* hid, double key, uint32_t id)
+    {
+        double mu;
+        int q;
+        // level 0 -> 1
+        if (!step(v1, 0, key, mu, q)) {
+            r0v = key;
+            r0i = id;
+            return;
+        }
+        r0v = mu;
+        r0i = __builtin_amdgcn_readlane(i1, q);
+        int o = q;
+        double lowv = mu;
+        // level 1 -> 2
+        if (!step(v2, o, lowv, mu, q)) {
+            v1 = (lane == o) ? key : v1;
+            i1 = (lane == o) ? id : i1;
+            return;
+        }
+        {
+            const uint32_t cid = __builtin_amdgcn_readlane(i2, q);
+            v1 = (lane == o) ? mu : v1;
+            i1 = (lane == o) ? cid : i1;
+        }
+        o = q;
+        lowv = mu;
+        // level 2 -> 3
+        if (!step(v3, o, lowv, mu, q)) {
+            v2 = (lane == o) ? key : v2;
+            i2 = (lane == o) ? id : i2;
+            return;
+        }
+        {
+            const uint32_t cid = __builtin_amdgcn_readlane(i3, q);
+            v2 = (lane == o) ? mu : v2;
+            i2 = (lane == o) ? cid : i2;
+        }
+        o = q;
+        lowv = mu;
+        // level 3 -> 4 and below, in LDS (all lanes run the same uniform steps)
+        const int i = 21 + o;
+        const int first = 4 * i + 1;
+        double cv[4];
+        uint32_t ci[4];
+        if (first < n) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                cv[c] = hv[first + c];
+                ci[c] = hid[first + c];
+            }
+        }
+        int lowest = -1;
+        if (first < n) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double x = (first + c < n) ? cv[c] : __builtin_inf();
+                if (lowv > x) {
+                    lowest = c;
+                    lowv = x;
+                }
+            }
+        }
+        if (lowest < 0) {
+            v3 = (lane == o) ? key : v3;
+            i3 = (lane == o) ? id : i3;
+            return;
+        }
+        uint32_t cid = ci[0];
+#pragma unroll
+        for (int c = 1; c < 4; ++c)
+            cid = (lowest == c) ? ci[c] : cid;
+        v3 = (lane == o) ? lowv : v3;
+        i3 = (lane == o) ? cid : i3;
+        __builtin_amdgcn_wave_barrier();
+        heapify_cont(hv, hid, first + lowest, n, lowv, key, id);
+        __builtin_amdgcn_wave_barrier();
+    }
+};
+
 __device__ inline int heap_level_first(int L)
 {
     int f = 0, w = 1;
@@ -472,11 +676,13 @@ __host__ __device__ inline int pow2_at_least(int v)
 __device__ inline void stage_abs(double* dst, const double* __restrict__ src, int len, int t0, int nt)
 {
     for (int e0 = t0; e0 < len; e0 += nt * 8) {
+        // clamped, unconditional loads: a guarded load would be issued and
+        // waited for one at a time
         double v[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int e = e0 + q * nt;
-            v[q] = (e < len) ? src[e] : 0.0;
+            v[q] = src[e < len ? e : len - 1];
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -591,10 +797,9 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
     uint32_t* hid = (uint32_t*)(hv + k + 4);                   // P + 4
     double* seg = (double*)(hid + round_up(P + 4, 2));
 
-    for (int q = tid; q < k; q += blockDim.x) {
-        hv[q] = fabs(dc[q]);
+    stage_abs(hv, dc, k, tid, blockDim.x);
+    for (int q = tid; q < k; q += blockDim.x)
         hid[q] = (uint32_t)q;
-    }
     __syncthreads();
 
     // buildMinHeap (MinHeap.h:147-191): nodes (k+2)/4 .. 0, one level at a time
@@ -623,42 +828,54 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
         stage_abs(seg, dc + k, len0, tid, blockDim.x);
     }
     __syncthreads();
-    double rv = hv[0];
-    double rv0 = rv;
+    RegHeap rh;
+    if (wave == 0)
+        rh.load(hv, hid, k, lane);
     for (int sgi = 0; sgi < nseg; ++sgi) {
         const double* cur = seg + (size_t)(sgi & 1) * seg_len;
         const size_t sbase = (size_t)k + (size_t)sgi * seg_len;
         const int slen = (int)min((size_t)seg_len, S - sbase);
         if (wave == 0) {
+            // chunks of 256 positions, the next chunk's loads in flight while
+            // the current one is tested
+            double v4[4], nx[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int e = 64 * j + lane;
+                const double t = cur[e < slen ? e : slen - 1];
+                v4[j] = (e < slen) ? t : -1.0;
+            }
             for (int e4 = 0; e4 < slen; e4 += 256) {
-                double v4[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int e = e4 + 64 * j + lane;
-                    v4[j] = (e < slen) ? cur[e] : -1.0;
+                    const int e = e4 + 256 + 64 * j + lane;
+                    const double t = cur[e < slen ? e : slen - 1];
+                    nx[j] = (e < slen) ? t : -1.0;
                 }
                 // one ballot for 256 positions: the root only grows, so a position
                 // not above it now is never admitted in this pass
+                double rv = rh.r0v;
                 const bool any = (v4[0] > rv) || (v4[1] > rv) || (v4[2] > rv) || (v4[3] > rv);
-                if (__ballot(any) == 0ull)
-                    continue;
+                if (__ballot(any) != 0ull) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const double v = v4[j];
-                    const int e0 = e4 + 64 * j;
-                    uint64_t mask = __ballot(v > rv);
-                    while (mask) {
-                        const int f = __builtin_ctzll(mask);
-                        const double key = bcast_lane(v, f);
-                        if (lane == 0) {
+                    for (int j = 0; j < 4; ++j) {
+                        const double v = v4[j];
+                        const int e0 = e4 + 64 * j;
+                        uint64_t mask = __ballot(v > rv);
+                        while (mask) {
+                            const int f = __builtin_ctzll(mask);
+                            const double key = bcast_lane(v, f);
                             // increaseMinimumInMinHeap (MinHeap.h:78-85)
-                            rv0 = heapify_x(hv, hid, 0, k, key, (uint32_t)(sbase + e0 + f));
+                            rh.admit(hv, hid, key, (uint32_t)(sbase + e0 + f));
+                            rv = rh.r0v;
+                            const uint64_t above = (f == 63) ? 0ull : (~0ull << (f + 1));
+                            mask = __ballot(v > rv) & above;
                         }
-                        rv = bcast_lane(rv0, 0);
-                        const uint64_t above = (f == 63) ? 0ull : (~0ull << (f + 1));
-                        mask = __ballot(v > rv) & above;
                     }
                 }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    v4[j] = nx[j];
             }
         } else if (sgi + 1 < nseg) {
             double* nxt = seg + (size_t)((sgi + 1) & 1) * seg_len;
@@ -668,6 +885,9 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
         }
         __syncthreads();
     }
+    if (wave == 0)
+        rh.store(hv, hid);
+    __syncthreads();
 
     PROBE_ADD(1000 + 8 * c + 1, t_sel1);
     PROBE_T(t_sel2);
