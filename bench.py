@@ -32,6 +32,9 @@ METRIC = "time-to-‖∇f‖²≤1e-17 & rounds/s vs plain FP64 GPU; solves/s + 
 PEAK_FP64_MFMA_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 TOL = 1e-17
+# HBM bytes per launch of each kernel on this workload (rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes scaled by a plain-copy calibration)
+TRAFFIC_FILE = "fednl_round_topk_n142_m350_d300_hbm_traffic.txt"
 
 
 def _sync_all(eng):
@@ -44,27 +47,62 @@ def _sync_all(eng):
         pass
 
 
-def kernel_roofline(eng, times, prob):
+def algorithmic_work(name, n, m, d, k, S):
+    """Algorithmic work of one launch of each round kernel: ("mfma", flops)
+    for the FP64 arithmetic kernels, ("hbm", bytes) for the data-movement
+    ones. Bytes count each input read once and each output written once."""
+    if name == "k_hessian_f64":
+        # upper triangle incl. diagonal of A_c^T D_c A_c: m FMAs per entry
+        return "mfma", float(n) * m * d * (d + 1)
+    if name == "k_newton":
+        # Cholesky d^3/3, two triangular solves 2 d^2
+        return "mfma", d ** 3 / 3.0 + 2.0 * d * d
+    if name == "k_select":
+        # |Delta_c| read once, k positions + count written
+        return "hbm", float(n) * (8.0 * S + 4.0 * k + 4.0)
+    if name == "k_margin":
+        # features once, labels, x; sigmoid weights, gradient weights, partials
+        return "hbm", float(n) * m * (8.0 * d + 8.0 + 16.0) + 8.0 * d
+    if name == "k_emit":
+        # Delta and learned at the k positions, learned written back, message
+        # (value, flat index, position) and the selection bitmap written
+        return "hbm", float(n) * (k * (8.0 + 16.0 + 8.0 + 4.0 + 4.0) + S / 8.0)
+    if name == "k_master_h":
+        # selection bitmaps, the selected Delta values, H read and written
+        return "hbm", float(n) * (S / 8.0 + 8.0 * k) + 16.0 * S
+    if name == "k_master_grad":
+        return "hbm", float(n) * d * 8.0 + 8.0 * d
+    return "hbm", 8.0 * S
+
+
+def load_traffic(path):
+    """HBM bytes per launch from the committed counter summary: lines
+    `kernel fetch_kib write_kib fetch_factor write_factor`."""
+    out = {}
+    try:
+        with open(path) as fh:
+            for ln in fh:
+                f = ln.split()
+                if len(f) == 5 and not ln.startswith("#"):
+                    kib_r, kib_w, fr, fw = (float(x) for x in f[1:])
+                    out[f[0]] = 1024.0 * (kib_r * fr + kib_w * fw)
+    except OSError:
+        pass
+    return out
+
+
+def kernel_roofline(eng, times, traffic):
     """Roofline record of the kernel with the largest device time."""
     n, m, d, k, S = eng.n, eng.m, eng.d, eng.k, eng.S
     name, (ms, launches) = max(times.items(), key=lambda kv: kv[1][0])
     avg_s = ms * 1e-3 / max(launches, 1)
-    if name == "k_hessian_f64":
-        flops = float(n) * m * d * (d + 1)  # upper triangle incl. diagonal, per round
-        ach = flops / avg_s / 1e12
-        return name, dict(bound="mfma", achieved=ach, peak=PEAK_FP64_MFMA_TFLOPS, unit="TFLOP/s",
-                          frac=ach / PEAK_FP64_MFMA_TFLOPS, traffic=None), avg_s
-    if name in ("k_select", "k_emit"):
-        by = float(n) * (8 * S + 12 * k + 16 * k)
-    elif name == "k_margin":
-        by = float(n) * (m * d * 8 + m * 8 * 3 + 8 * d)
-    elif name == "k_master_h":
-        by = float(n) * k * 12 + 16.0 * n * k
+    bound, work = algorithmic_work(name, n, m, d, k, S)
+    if bound == "mfma":
+        ach, peak, unit = work / avg_s / 1e12, PEAK_FP64_MFMA_TFLOPS, "TFLOP/s"
     else:
-        by = float(S) * 8
-    ach = by / avg_s / 1e9
-    return name, dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
-                      traffic=None), avg_s
+        ach, peak, unit = work / avg_s / 1e9, PEAK_HBM_GBS, "GB/s"
+    return name, dict(bound=bound, achieved=ach, peak=peak, unit=unit, frac=ach / peak,
+                      traffic=traffic.get(name)), avg_s
 
 
 def cpu_baseline(prob, rounds, cores):
@@ -149,7 +187,14 @@ def main():
     eng.kernel_timing(False)
     elapsed = t1 - t0
     rps = args.steps / elapsed
-    kname, roof, kavg = kernel_roofline(eng, times, prob)
+    traffic = load_traffic(os.path.join(ROOT, "profiles", TRAFFIC_FILE))
+    kname, roof, kavg = kernel_roofline(eng, times, traffic)
+    all_roofs = {}
+    for name_, (ms_, nl_) in times.items():
+        b_, w_ = algorithmic_work(name_, eng.n, eng.m, eng.d, eng.k, eng.S)
+        sec_ = ms_ * 1e-3 / max(nl_, 1)
+        pk_ = PEAK_FP64_MFMA_TFLOPS * 1e12 if b_ == "mfma" else PEAK_HBM_GBS * 1e9
+        all_roofs[name_] = dict(bound=b_, us=sec_ * 1e6, frac=w_ / sec_ / pk_, traffic=traffic.get(name_))
 
     cpu = None
     if not args.no_cpu:
@@ -180,6 +225,7 @@ def main():
         "tol": TOL,
         "final_grad_norm_sq": final_g2,
         "kernel_ms_per_round": {k_: v[0] / max(v[1], 1) for k_, v in times.items()},
+        "kernel_roofline_all": all_roofs,
         "roofline_kernel": kname,
         "roofline": roof,
         "cpu_baseline": cpu,
