@@ -1190,7 +1190,8 @@ constexpr int kMasterPos = 256;
 __global__ __launch_bounds__(256) void k_master_h(double* __restrict__ H, size_t S, int n,
                                                   const double* __restrict__ delta,
                                                   const uint32_t* __restrict__ sel_bm, int bm_words, int scaled,
-                                                  double mult, double prescale)
+                                                  double mult, double prescale, double* __restrict__ W, int dp,
+                                                  const double* __restrict__ shift_p)
 {
     __shared__ uint32_t bw[kMaxClients][kMasterPos / 32];
     const size_t p0 = (size_t)blockIdx.x * kMasterPos;
@@ -1226,6 +1227,13 @@ __global__ __launch_bounds__(256) void k_master_h(double* __restrict__ H, size_t
         }
     }
     H[p] = h;
+    if (W) {
+        // the Newton workspace: W(j, i) = H(i, j) (+shift on the diagonal),
+        // column-major lower, ld = dp (TrainLoop.h:1449-1454)
+        const uint32_t j = pos_col(p);
+        const uint32_t i = (uint32_t)(p - (size_t)j * (j + 1) / 2);
+        W[(size_t)i * dp + j] = (i == j) ? h + *shift_p : h;
+    }
 }
 
 // averaged gradient (TrainLoop.h:1100-1124,1292-1300): clients in order, 64
@@ -1384,16 +1392,17 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
         s_fail = 0;
     PROBE_T(t_n0);
     // W(i,j) = H(j,i) (+shift on the diagonal) for i >= j < d, from the packed
-    // upper H in order (eight loads in flight per thread). The padding of W
-    // (identity) is set once when W is allocated and survives the factorisation.
-    {
+    // upper H in order (eight loads in flight per thread), unless the caller
+    // has written W already (Hup == nullptr). The padding of W (identity) is
+    // set once when W is allocated and survives the factorisation.
+    if (Hup) {
         const size_t S = (size_t)d * (d + 1) / 2;
         for (size_t p0 = tid; p0 < S; p0 += (size_t)nthr * 8) {
             double hvv[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const size_t p = p0 + (size_t)q * nthr;
-                hvv[q] = (p < S) ? Hup[p] : 0.0;
+                hvv[q] = Hup[p < S ? p : S - 1];
             }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
@@ -1417,13 +1426,12 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
         // 0. panel into LDS: A11 (lower) and A21
         for (int e0 = tid; e0 < kNB * (kNB + T); e0 += nthr * 8) {
             double pv[8];
+            const int etot = kNB * (kNB + T);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int e = e0 + q * nthr;
-                if (e < kNB * (kNB + T)) {
-                    const int c = e / (kNB + T), r = e % (kNB + T);
-                    pv[q] = W[(size_t)(jb + c) * dp + jb + r];
-                }
+                const int e = min(e0 + q * nthr, etot - 1);
+                const int c = e / (kNB + T), r = e % (kNB + T);
+                pv[q] = W[(size_t)(jb + c) * dp + jb + r];
             }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
@@ -1830,13 +1838,6 @@ static int launch_round(Engine* e)
                            e->sel_bm, e->bm_words);
     }
     {
-        KTime kt(e, KID_MASTER_H);
-        const int nblk = (int)((e->S + kMasterPos - 1) / kMasterPos);
-        const int scaled = (comp == FEDNL_RANDK || comp == FEDNL_RANDSEQK) ? 1 : 0;
-        hipLaunchKernelGGL(k_master_h, dim3(nblk), dim3(256), 0, e->st, e->H, e->S, n, e->delta, e->sel_bm,
-                           e->bm_words, scaled, e->mult, e->prescale);
-    }
-    {
         KTime kt(e, KID_MASTER_VEC);
         hipLaunchKernelGGL(k_master_grad, dim3((e->d + kGradCols - 1) / kGradCols), dim3(256), 0, e->st, n, e->d,
                            e->d_pad, e->grad_c, e->inv_n, e->g_avg);
@@ -1846,9 +1847,18 @@ static int launch_round(Engine* e)
                            e->trace_cap);
     }
     {
+        // after the shift is known: H update and the Newton workspace in one pass
+        KTime kt(e, KID_MASTER_H);
+        const int nblk = (int)((e->S + kMasterPos - 1) / kMasterPos);
+        const int scaled = (comp == FEDNL_RANDK || comp == FEDNL_RANDSEQK) ? 1 : 0;
+        hipLaunchKernelGGL(k_master_h, dim3(nblk), dim3(256), 0, e->st, e->H, e->S, n, e->delta, e->sel_bm,
+                           e->bm_words, scaled, e->mult, e->prescale, e->W, e->dp, (const double*)e->shift);
+    }
+    {
         KTime kt(e, KID_NEWTON);
-        hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), e->lds_newton, e->st, e->d, e->dp, e->H, e->shift,
-                           e->g_avg, e->W, e->x, e->dx, e->trace, e->round_ctr, e->trace_cap, 1);
+        hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), e->lds_newton, e->st, e->d, e->dp,
+                           (const double*)nullptr, e->shift, e->g_avg, e->W, e->x, e->dx, e->trace, e->round_ctr,
+                           e->trace_cap, 1);
     }
     return FEDNL_OK;
 }
