@@ -48,7 +48,10 @@ __device__ unsigned long long g_probe[16384];
     do { if (threadIdx.x == 0) atomicAdd(&g_probe[(slot)], wall_clock64() - (t0)); } while (0)
 #define PROBE_SET(slot, v) \
     do { if (threadIdx.x == 0) g_probe[(slot)] = (unsigned long long)(v); } while (0)
+#define PROBE_ADD_AT(slot, t0, thread) \
+    do { if (threadIdx.x == (thread)) atomicAdd(&g_probe[(slot)], wall_clock64() - (t0)); } while (0)
 #else
+#define PROBE_ADD_AT(slot, t0, thread) do {} while (0)
 #define PROBE_T(var) do {} while (0)
 #define PROBE_ADD(slot, t0) do {} while (0)
 #define PROBE_SET(slot, v) do {} while (0)
@@ -1355,21 +1358,106 @@ __global__ __launch_bounds__(256) void k_master_vec(int comp, int n, int d, int 
 // ---------------------------------------------------------------------------
 // k_newton: Cholesky of (H + shift I) and the two triangular solves of the
 // master (TrainLoop.h:1449-1466; CholeskyFactorization.h:153-272;
-// ElementarySolvers.h:72-226) in one workgroup of 16 waves.
-// Right-looking, 32-column panels, the right-hand side carried along so that
-// the forward substitution happens inside the factorisation:
-//   1. wave 0 factors the 32 x 32 diagonal block in LDS and solves its part
-//      of L y = g;
-//   2. every thread takes one row below the block: L21 = A21 L11^-T, and
-//      folds L21 y into the remaining right-hand side;
-//   3. the trailing lower triangle is updated with v_mfma_f64_16x16x4f64
-//      (W22 -= L21 L21^T) in L2-resident scratch W (column-major, ld = dp,
-//      dp = d rounded up to 32, padding set to the identity).
+// ElementarySolvers.h:72-226) in one workgroup of 8 waves. W holds the lower
+// triangle of H + shift I (column-major, ld = dp, dp = d rounded up to 32,
+// padding set to the identity). Right-looking with 32-column panels and a
+// look-ahead of one panel; the right-hand side is carried along, so the
+// forward substitution happens inside the factorisation. Per panel j:
+//   A. every wave solves its 64 rows of L21 = A21 L11^-T (L11 broadcast by
+//      v_readlane), folds L21 y into the right-hand side and writes L21 to W;
+//   B. wave 0 updates the next diagonal block with FP64 MFMA and factors it
+//      (rows in lanes, columns broadcast by v_readlane) while waves 1..7
+//      apply the rest of W22 -= L21 L21^T on 16 x 16 tiles with FP64 MFMA,
+//      the next tile group's loads in flight during the current one;
+//   C. the next panel's rows below its diagonal block are staged in LDS.
 // Then L^T dx = y by 32-row blocks from the bottom, and x -= dx.
 // The factorisation fails, as dopt's, when a pivot is < 0.
 // ---------------------------------------------------------------------------
 constexpr int kNewtonThreads = 512;
 static_assert(kMaxNewtonD - 32 < (kNewtonThreads / 64) * 64, "k_newton TRSM assumes one 64-row chunk per wave");
+static_assert(kMaxNewtonD <= kNewtonThreads, "k_newton back solve assumes one row per thread");
+
+__host__ __device__ constexpr size_t newton_lds_bytes(int dp)
+{
+    // rows below the diagonal block, row stride kNB + 1 (at least one row)
+    return sizeof(double) * (size_t)(kNB + 1) * (size_t)(dp > kNB ? dp - kNB : 1);
+}
+
+// 1/sqrt(a) and sqrt(a) for a > 0: v_rsq_f64 and two Newton steps, then one
+// correction of the square root (within an ulp or two of the rounded values)
+__device__ inline void sqrt_and_inv(double a, double& root, double& inv)
+{
+    double y = __builtin_amdgcn_rsq(a);
+    double h = a * y;
+    y = fma(0.5 * y, fma(-h, y, 1.0), y);
+    h = a * y;
+    y = fma(0.5 * y, fma(-h, y, 1.0), y);
+    double r = a * y;
+    r = fma(fma(-r, r, a), 0.5 * y, r);
+    root = r;
+    inv = y;
+}
+
+// Factors the 32 x 32 block held in Dg (row-major, lower part meaningful) in
+// wave 0 and solves its part of L y = b: on return Dg holds L11, invd /
+// invall the reciprocal pivots, yv[jb..jb+31] the solution.
+__device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double* invd, double* invall, double* yv,
+                                                   int jb, int lane, int* s_fail)
+{
+    double row[kNB];
+    const bool act = lane < kNB;
+    const int lr = act ? lane : 0;
+    // entries above the diagonal are carried along but never feed the lower
+    // triangle, so the updates below need no per-lane predicates
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+        row[c] = Dg[lr][c];
+    double yr = yv[jb + lr];
+    double myinv = 1.0;
+    bool fail = false;
+#pragma unroll
+    for (int cc = 0; cc < kNB; ++cc) {
+        const double piv = bcast_lane(row[cc], cc);
+        fail = fail || (piv < 0.0);  // finish the (now meaningless) block, report after
+        // piv < 0 is reported after the block; piv == 0 gives non-finite
+        // entries as a division by a zero pivot would
+        double l, inv;
+        sqrt_and_inv(piv, l, inv);
+        const double ycc = bcast_lane(yr, cc) * inv;
+        const double scaled = inv * row[cc];
+        const bool below = lane > cc;
+        const bool me = lane == cc;
+        row[cc] = me ? l : (below ? scaled : row[cc]);
+        myinv = me ? inv : myinv;
+        yr = me ? ycc : (below ? fma(-scaled, ycc, yr) : yr);
+        // L[c2][cc] lives in lane c2: broadcast by v_readlane
+#pragma unroll
+        for (int c2 = cc + 1; c2 < kNB; ++c2)
+            row[c2] = fma(-row[cc], bcast_lane(row[cc], c2), row[c2]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (fail) {
+        if (lane == 0)
+            *s_fail = 1;
+    } else if (act) {
+#pragma unroll
+        for (int c = 0; c < kNB; ++c)
+            Dg[lane][c] = row[c];
+        invd[lane] = myinv;
+        invall[jb + lane] = myinv;
+        yv[jb + lane] = yr;
+    }
+}
+
+// lower 16 x 16 tile index tt -> (row tile, column tile), column <= row
+__device__ inline void tri_tile(int tt, int& ti, int& tj)
+{
+    int t = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
+    while (t * (t + 1) / 2 > tt) --t;
+    while ((t + 1) * (t + 2) / 2 <= tt) ++t;
+    ti = t;
+    tj = tt - t * (t + 1) / 2;
+}
 
 __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const double* __restrict__ Hup,
                                                            const double* __restrict__ shift_p,
@@ -1381,12 +1469,14 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
     extern __shared__ __align__(16) double lds[];
     constexpr int kPS = kNB + 1;                    // row stride of the panel in LDS
     double* P21 = lds;                              // rows below the diagonal block: P21[r * kPS + c]
-    __shared__ double Dg[kNB][kNB + 1];             // L11 of the current panel (lower)
+    __shared__ double Dg[kNB][kNB + 1];             // L11 of the current panel, row-major (lower)
     __shared__ double invd[kNB];                    // 1 / L11[c][c]
+    __shared__ double invall[kMaxNewtonD];          // 1 / L[i][i] for the back solve
     __shared__ double yv[kMaxNewtonD];              // right-hand side -> y -> dx
     __shared__ int s_fail;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63, nwaves = nthr >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
     const double shift = *shift_p;
     if (tid == 0)
         s_fail = 0;
@@ -1414,108 +1504,34 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
                 }
             }
         }
+        __threadfence_block();
     }
     for (int i = tid; i < dp; i += nthr)
         yv[i] = (i < d) ? b[i] : 0.0;
     __syncthreads();
-
     PROBE_ADD(0, t_n0);
-    for (int jb = 0; jb < dp; jb += kNB) {
-        PROBE_T(t_p0);
-        const int T = dp - jb - kNB;
-        // 0. panel into LDS: A11 (lower) and A21
-        for (int e0 = tid; e0 < kNB * (kNB + T); e0 += nthr * 8) {
-            double pv[8];
-            const int etot = kNB * (kNB + T);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int e = min(e0 + q * nthr, etot - 1);
-                const int c = e / (kNB + T), r = e % (kNB + T);
-                pv[q] = W[(size_t)(jb + c) * dp + jb + r];
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int e = e0 + q * nthr;
-                if (e < kNB * (kNB + T)) {
-                    const int c = e / (kNB + T), r = e % (kNB + T);
-                    if (r < kNB) {
-                        if (r >= c)
-                            Dg[r][c] = pv[q];
-                    } else {
-                        P21[(r - kNB) * kPS + c] = pv[q];
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        PROBE_ADD(1, t_p0);
-        PROBE_T(t_p1);
-        // 1. diagonal block (wave 0): lane r < 32 keeps row r of the block in
-        // registers; each step broadcasts one column of L through LDS.
-        if (wave == 0) {
-            double row[kNB];
-            const bool act = lane < kNB;
-            const int lr = act ? lane : 0;
-            // entries above the diagonal are carried along but never feed the
-            // lower triangle, so the updates below need no per-lane predicates
-#pragma unroll
-            for (int c = 0; c < kNB; ++c)
-                row[c] = Dg[lr][c];
-            double yr = act ? yv[jb + lane] : 0.0;
-            double mydiag = 1.0;
-            bool fail = false;
-#pragma unroll
-            for (int cc = 0; cc < kNB; ++cc) {
-                const double piv = bcast_lane(row[cc], cc);
-                fail = fail || (piv < 0.0);  // finish the (now meaningless) block, report after
-                const double l = sqrt(piv);
-                const double inv = 1.0 / l;
-                const double ycc = bcast_lane(yr, cc) * inv;
-                const double scaled = inv * row[cc];
-                const bool below = lane > cc;
-                const bool me = lane == cc;
-                row[cc] = me ? l : (below ? scaled : row[cc]);
-                mydiag = me ? l : mydiag;
-                yr = me ? ycc : (below ? fma(-scaled, ycc, yr) : yr);
-                // L[c2][cc] lives in lane c2: broadcast by v_readlane
-#pragma unroll
-                for (int c2 = cc + 1; c2 < kNB; ++c2)
-                    row[c2] = fma(-row[cc], bcast_lane(row[cc], c2), row[c2]);
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            if (fail) {
-                if (lane == 0)
-                    s_fail = 1;
-            } else if (act) {
-#pragma unroll
-                for (int c = 0; c < kNB; ++c)
-                    Dg[lane][c] = row[c];
-                invd[lane] = 1.0 / mydiag;
-                yv[jb + lane] = yr;
-            }
-        }
-        __syncthreads();
-        PROBE_ADD(2, t_p1);
-        PROBE_T(t_p2);
+
+    // jb = -kNB is a lead-in: it only stages and factors panel 0's diagonal
+    // block and stages panel 0's rows below it (one copy of each phase)
+    for (int jb = -kNB; jb < dp; jb += kNB) {
         if (s_fail)
             break;
-        // 2. rows below the block: L21 = A21 L11^-T (column-oriented for
-        // independent FMAs); the right-hand side takes its L21 y share. Every
-        // wave holds L11 in lanes 0..31 (row r in lane r) and broadcasts its
-        // entries by v_readlane; lane l solves row (wave row base) + l.
-        // T <= kMaxNewtonD - kNB < nwaves * 64: one chunk of rows per wave.
-        if (wave * 64 < T) {
-            const int rb = wave * 64;
-            double lrow[kNB];
-            const int lr = lane & (kNB - 1);
-#pragma unroll
-            for (int c = 0; c < kNB; ++c)
-                lrow[c] = Dg[lr][c];
-            const double invl = invd[lr];
-            const double ypan = yv[jb + lr];
-            {
-                const int rr = rb + lane;
+        const int T = dp - jb - kNB;
+        PROBE_T(t_a);
+        if (jb >= 0) {
+            // A. L21 = A21 L11^-T, one 64-row chunk per wave (T < nwaves * 64).
+            // Every wave holds L11 in lanes 0..31 (row r in lane r) and
+            // broadcasts its entries by v_readlane; lane l solves row rb + l.
+            if (wave * 64 < T) {
+                const int rr = wave * 64 + lane;
                 const bool live = rr < T;
+                double lrow[kNB];
+                const int lr = lane & (kNB - 1);
+#pragma unroll
+                for (int c = 0; c < kNB; ++c)
+                    lrow[c] = Dg[lr][c];
+                const double invl = invd[lr];
+                const double ypan = yv[jb + lr];
                 double a[kNB];
                 double* pr = P21 + (live ? rr : 0) * kPS;
 #pragma unroll
@@ -1539,71 +1555,158 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
                 if (live) {
                     yv[i] = bi;
 #pragma unroll
-                    for (int cc = 0; cc < kNB; ++cc)
+                    for (int cc = 0; cc < kNB; ++cc) {
                         pr[cc] = a[cc];
+                        W[(size_t)(jb + cc) * dp + jb + kNB + rr] = a[cc];
+                    }
                 }
+            }
+            // L11 back to W (the back solve reads it there)
+            for (int e = tid; e < kNB * kNB; e += nthr) {
+                const int c = e >> 5, r = e & (kNB - 1);
+                if (r >= c)
+                    W[(size_t)(jb + c) * dp + jb + r] = Dg[r][c];
             }
         }
         __syncthreads();
-        PROBE_ADD(3, t_p2);
-        PROBE_T(t_p3);
-        // 3. L11, L21 back to W; trailing update W22 -= L21 L21^T on lower
-        // 16 x 16 tiles with FP64 MFMA (lane (l16, kq), item r: column 4r + kq,
-        // row l16 of the tile)
-        for (int e = tid; e < kNB * (kNB + T); e += nthr) {
-            const int c = e / (kNB + T), r = e % (kNB + T);
-            if (r < kNB) {
-                if (r >= c)
-                    W[(size_t)(jb + c) * dp + jb + r] = Dg[r][c];
+        PROBE_ADD(3, t_a);
+        if (T == 0)
+            break;
+        PROBE_T(t_b);
+        // B. the next diagonal block (row/column tiles 0 and 1 of the trailing
+        // matrix) goes to wave 0, which updates and factors it; waves 1..7
+        // apply the rest of the trailing update
+        if (wave == 0) {
+            if (jb >= 0) {
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const int ti = (t == 0) ? 0 : 1, tj = (t == 2) ? 1 : 0;
+                    const double* wp = W + (size_t)(jb + kNB + tj * 16 + kq) * dp + jb + kNB + ti * 16 + l16;
+                    double wold[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        wold[r] = wp[(size_t)4 * r * dp];
+                    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int k0 = 0; k0 < kNB; k0 += 4) {
+                        const double av = P21[(tj * 16 + l16) * kPS + k0 + kq];  // A[m = col][k]
+                        const double bv = P21[(ti * 16 + l16) * kPS + k0 + kq];  // B[k][n = row]
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        Dg[ti * 16 + l16][tj * 16 + kq + 4 * r] = wold[r] - acc[r];
+                }
             } else {
-                W[(size_t)(jb + c) * dp + jb + r] = P21[(r - kNB) * kPS + c];
+                // panel 0's diagonal block as it stands in W
+                for (int e = lane; e < kNB * kNB; e += 64) {
+                    const int c = e >> 5, r = e & (kNB - 1);
+                    Dg[r][c] = W[(size_t)c * dp + r];
+                }
             }
-        }
-        if (T > 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            PROBE_T(t_dg);
+            newton_diag_factor(Dg, invd, invall, yv, jb + kNB, lane, &s_fail);
+            PROBE_ADD(2, t_dg);
+        } else if (jb >= 0) {
+            PROBE_T(t_sy);
             const int tb = T / 16;
             const int ntiles = tb * (tb + 1) / 2;
-            const int l16 = lane & 15, kq = lane >> 4;
-            constexpr int kGroup = 4;  // tiles per wave whose W loads are in flight together
-            for (int t0 = wave * kGroup; t0 < ntiles; t0 += nwaves * kGroup) {
-                int rbs[kGroup], cbs[kGroup];
-                double wv[kGroup][4];
+            constexpr int kGroup = 4;  // tiles per wave per step
+            const int nw = nwaves - 1, wid = wave - 1;
+            const int stride = nw * kGroup;
+            int tcur = 3 + wid * kGroup;
+            int rb[kGroup], cb[kGroup];
+            double wv[kGroup][4];
+            // first group's coordinates and loads
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                const int tt = tcur + g;
+                int ti = 0, tj = 0;
+                if (tt < ntiles)
+                    tri_tile(tt, ti, tj);
+                rb[g] = (tt < ntiles) ? ti * 16 : -1;
+                cb[g] = tj * 16;
+                const double* wp = W + (size_t)(jb + kNB + cb[g] + kq) * dp + jb + kNB + max(rb[g], 0) + l16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    wv[g][r] = wp[(size_t)4 * r * dp];
+            }
+            for (; tcur < ntiles; tcur += stride) {
+                // next group's loads in flight during this group's MFMAs
+                int rbn[kGroup], cbn[kGroup];
+                double wn[kGroup][4];
 #pragma unroll
                 for (int g = 0; g < kGroup; ++g) {
-                    const int tt = t0 + g;
-                    rbs[g] = -1;
-                    cbs[g] = 0;
-                    if (tt < ntiles) {
-                        int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-                        while (ti * (ti + 1) / 2 > tt) --ti;
-                        while ((ti + 1) * (ti + 2) / 2 <= tt) ++ti;
-                        rbs[g] = ti * 16;
-                        cbs[g] = (tt - ti * (ti + 1) / 2) * 16;  // column tile <= row tile
-                        const double* wp = W + (size_t)(jb + kNB + cbs[g] + kq) * dp + jb + kNB + rbs[g] + l16;
+                    const int tt = tcur + stride + g;
+                    int ti = 0, tj = 0;
+                    if (tt < ntiles)
+                        tri_tile(tt, ti, tj);
+                    rbn[g] = (tt < ntiles) ? ti * 16 : -1;
+                    cbn[g] = tj * 16;
+                    const double* wp =
+                        W + (size_t)(jb + kNB + cbn[g] + kq) * dp + jb + kNB + max(rbn[g], 0) + l16;
 #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            wv[g][r] = wp[(size_t)4 * r * dp];
-                    }
+                    for (int r = 0; r < 4; ++r)
+                        wn[g][r] = wp[(size_t)4 * r * dp];
                 }
 #pragma unroll
                 for (int g = 0; g < kGroup; ++g) {
-                    if (rbs[g] < 0)
+                    if (rb[g] < 0)
                         continue;
                     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                     for (int k0 = 0; k0 < kNB; k0 += 4) {
-                        const double av = P21[(cbs[g] + l16) * kPS + k0 + kq];  // A[m = col][k]
-                        const double bv = P21[(rbs[g] + l16) * kPS + k0 + kq];  // B[k][n = row]
+                        const double av = P21[(cb[g] + l16) * kPS + k0 + kq];  // A[m = col][k]
+                        const double bv = P21[(rb[g] + l16) * kPS + k0 + kq];  // B[k][n = row]
                         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
                     }
-                    double* wp = W + (size_t)(jb + kNB + cbs[g] + kq) * dp + jb + kNB + rbs[g] + l16;
+                    double* wp = W + (size_t)(jb + kNB + cb[g] + kq) * dp + jb + kNB + rb[g] + l16;
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
                         wp[(size_t)4 * r * dp] = wv[g][r] - acc[r];
                 }
+#pragma unroll
+                for (int g = 0; g < kGroup; ++g) {
+                    rb[g] = rbn[g];
+                    cb[g] = cbn[g];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        wv[g][r] = wn[g][r];
+                }
+            }
+            PROBE_ADD_AT(6, t_sy, 64);
+        }
+        __syncthreads();
+        PROBE_ADD(4, t_b);
+        PROBE_T(t_c);
+        // C. the next panel's rows below its diagonal block
+        {
+            const int jn = jb + kNB;
+            const int rows = T - kNB;
+            const int etot = kNB * rows;
+            for (int e0 = tid; e0 < etot; e0 += nthr * 8) {
+                double pv[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int e = min(e0 + q * nthr, etot - 1);
+                    const int c = e / rows, r = e % rows;
+                    pv[q] = W[(size_t)(jn + c) * dp + jn + kNB + r];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int e = e0 + q * nthr;
+                    if (e < etot) {
+                        const int c = e / rows, r = e % rows;
+                        P21[r * kPS + c] = pv[q];
+                    }
+                }
             }
         }
         __syncthreads();
-        PROBE_ADD(4, t_p3);
+        PROBE_ADD(1, t_c);
     }
     PROBE_T(t_b0);
 
@@ -1619,9 +1722,18 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
         return;
     }
 
-    // L^T dx = y, 32-row blocks from the bottom. In a block, lane r < 32 keeps
-    // column r of L11 (rows jb..jb+31) in registers.
+    // L^T dx = y, 32-row blocks from the bottom. Thread rr < jb keeps row rr
+    // of the block's L^T columns (loaded before the block is solved); in the
+    // block, lane r < 32 keeps column r of L11.
     for (int jb = dp - kNB; jb >= 0; jb -= kNB) {
+        const int rr = tid;
+        const bool has = rr < jb;
+        double wl[kNB];
+        if (has) {
+#pragma unroll
+            for (int cc = 0; cc < kNB; ++cc)
+                wl[cc] = W[(size_t)rr * dp + jb + cc];
+        }
         if (wave == 0) {
             const bool act = lane < kNB;
             const int lr = act ? lane : 0;
@@ -1631,10 +1743,10 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
 #pragma unroll
             for (int c = 0; c < kNB; ++c)
                 colv[c] = cp[c];
-            double yr = act ? yv[jb + lane] : 0.0;
+            double yr = yv[jb + lr];
 #pragma unroll
             for (int cc = kNB - 1; cc >= 0; --cc) {
-                const double q = yr / colv[cc];
+                const double q = yr * invall[jb + cc];
                 const double xc = bcast_lane(q, cc);
                 const double upd = fma(-colv[cc], xc, yr);
                 yr = (lane == cc) ? xc : ((lane < cc) ? upd : yr);
@@ -1644,11 +1756,7 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const 
                 yv[jb + lane] = yr;
         }
         __syncthreads();
-        for (int rr = tid; rr < jb; rr += nthr) {
-            double wl[kNB];
-#pragma unroll
-            for (int cc = 0; cc < kNB; ++cc)
-                wl[cc] = W[(size_t)rr * dp + jb + cc];
+        if (has) {
             double s = yv[rr];
 #pragma unroll
             for (int cc = 0; cc < kNB; ++cc)
@@ -1990,7 +2098,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     e->lek_delta = (double)e->k * (1.0 / (double)e->S);
     e->lds_select = select_lds_bytes(std::min<int>(e->k, kMaxHeapK));
     e->seg_len = select_seg_len(std::min<int>(e->k, kMaxHeapK));
-    e->lds_newton = sizeof(double) * (size_t)(kNB + 1) * e->dp;
+    e->lds_newton = newton_lds_bytes(e->dp);
 
     int rc = FEDNL_OK;
 #define ALLOC(ptr, cnt)                                          \
@@ -2574,7 +2682,7 @@ int fednl_newton_solve(int d, const double* H_up, double shift, const double* b,
         for (int i = d; i < dp; ++i) w0[(size_t)i * dp + i] = 1.0;
         HIP_OK(hipMemcpy(dW, w0.data(), sizeof(double) * w0.size(), hipMemcpyHostToDevice));
     }
-    const size_t lds = sizeof(double) * (size_t)(kNB + 1) * dp;
+    const size_t lds = newton_lds_bytes(dp);
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k_newton, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), lds, 0, d, dp, dH, dsh, db, dW, (double*)nullptr, dx_d, dtr, dctr,
