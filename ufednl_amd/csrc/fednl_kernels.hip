@@ -258,10 +258,12 @@ __global__ __launch_bounds__(256) void k_hessian_f64(const double* __restrict__ 
     const int i0 = bi * kTile, j0 = bj * kTile;
     const bool dg = (bi == bj);
 
-    __shared__ double sI[kKC][kTile + kPadLds];
-    __shared__ double sJ[kKC][kTile + kPadLds];
-    __shared__ double sD[kKC];
-    __shared__ double sG[kKC];
+    // two LDS buffers: the next chunk's rows are loaded into registers while
+    // the current chunk is multiplied, then stored (scaled by D) into the
+    // other buffer; one barrier per chunk
+    __shared__ __align__(16) double sI[2][kKC][kTile + kPadLds];
+    __shared__ __align__(16) double sJ[2][kKC][kTile + kPadLds];
+    __shared__ double sG[2][kKC];
     __shared__ double red[2][4];
     __shared__ double gred[4][kTile];
 
@@ -280,28 +282,52 @@ __global__ __launch_bounds__(256) void k_hessian_f64(const double* __restrict__ 
     const double* Dc = Dw + (size_t)c * m_pad;
     const double* Gc = gw + (size_t)c * m_pad;
 
-    for (int s0 = 0; s0 < m_pad; s0 += kKC) {
-        if (tid < kKC) {
-            sD[tid] = Dc[s0 + tid];
-            sG[tid] = Gc[s0 + tid];
-        }
-        __syncthreads();
-        for (int e = tid; e < kKC * kTile; e += 256) {
-            const int kk = e >> 6, ii = e & 63;
+    // load assignment: rows lr and lr + 8 of a chunk, columns lc, lc + 1 of
+    // both column blocks (diagonal tiles read the same block twice, from cache)
+    const int lr = tid >> 5, lc = (tid & 31) * 2;
+    double2 ri[2], rj[2];
+    double rd[2], rg[2];
+    auto gload = [&](int s0) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int kk = lr + 8 * u;
             const double* row = Ac + (size_t)(s0 + kk) * d_pad;
-            const double a = row[i0 + ii];
-            sI[kk][ii] = a;
-            const double bv = dg ? a : row[j0 + ii];
-            sJ[kk][ii] = sD[kk] * bv;
+            ri[u] = *reinterpret_cast<const double2*>(row + i0 + lc);
+            rj[u] = *reinterpret_cast<const double2*>(row + j0 + lc);
+            rd[u] = Dc[s0 + kk];
+            rg[u] = Gc[s0 + kk];
         }
-        __syncthreads();
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int kk = lr + 8 * u;
+            *reinterpret_cast<double2*>(&sI[buf][kk][lc]) = ri[u];
+            double2 sj;
+            sj.x = rd[u] * rj[u].x;
+            sj.y = rd[u] * rj[u].y;
+            *reinterpret_cast<double2*>(&sJ[buf][kk][lc]) = sj;
+            if (lc == 0)
+                sG[buf][kk] = rg[u];
+        }
+    };
+
+    const int nch = m_pad / kKC;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        const int buf = ch & 1;
+        const bool more = ch + 1 < nch;
+        if (more)
+            gload((ch + 1) * kKC);
 #pragma unroll
         for (int ks = 0; ks < kKC; ks += 4) {
             const int kk = ks + kq;
-            const double a0 = sJ[kk][wj * 32 + l16];
-            const double a1 = sJ[kk][wj * 32 + 16 + l16];
-            const double b0 = sI[kk][wi * 32 + l16];
-            const double b1 = sI[kk][wi * 32 + 16 + l16];
+            const double a0 = sJ[buf][kk][wj * 32 + l16];
+            const double a1 = sJ[buf][kk][wj * 32 + 16 + l16];
+            const double b0 = sI[buf][kk][wi * 32 + l16];
+            const double b1 = sI[buf][kk][wi * 32 + 16 + l16];
             acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
             acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
             acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
@@ -311,15 +337,35 @@ __global__ __launch_bounds__(256) void k_hessian_f64(const double* __restrict__ 
             const int ii = tid & 63, g4 = tid >> 6;
 #pragma unroll
             for (int kk = g4 * 4; kk < g4 * 4 + 4; ++kk)
-                gacc = fma(sI[kk][ii], sG[kk], gacc);
+                gacc = fma(sI[buf][kk][ii], sG[buf][kk], gacc);
         }
+        if (more)
+            lstore(buf ^ 1);
         __syncthreads();
     }
 
     // epilogue: lane (l16, kq), item r of v_mfma_f64_16x16x4f64 holds
     // D[m = 4r + kq][n = l16] (pinned by the layout probe test), i.e.
     // rows i = .. + l16 and columns j = .. + kq + 4r of the sub-tile (jt, it)
+    // all sixteen learned values of this thread are loaded before use
+    // (clamped addresses for entries outside the upper triangle)
     double soff = 0.0, sdiag = 0.0;
+    double lv[2][2][4];
+    if (!kInit) {
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+            for (int it = 0; it < 2; ++it) {
+                const int i = i0 + wi * 32 + it * 16 + l16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = j0 + wj * 32 + jt * 16 + kq + 4 * r;
+                    const bool ok = (j < d && i <= j);
+                    const size_t o = (size_t)c * S + (ok ? (size_t)j * (j + 1) / 2 + i : 0);
+                    lv[jt][it][r] = learned[o];
+                }
+            }
+    }
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt) {
 #pragma unroll
@@ -337,7 +383,7 @@ __global__ __launch_bounds__(256) void k_hessian_f64(const double* __restrict__ 
                     if (kInit) {
                         learned[o] = h;
                     } else {
-                        const double dv = h - learned[o];
+                        const double dv = h - lv[jt][it][r];
                         delta[o] = dv;
                         if (i == j)
                             sdiag = fma(dv, dv, sdiag);
@@ -834,6 +880,10 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
     RegHeap rh;
     if (wave == 0)
         rh.load(hv, hid, k, lane);
+#ifdef FEDNL_PROBE
+    const unsigned long long tscan = clock64();
+    unsigned long long adm_cnt = 0, adm_cyc = 0;
+#endif
     for (int sgi = 0; sgi < nseg; ++sgi) {
         const double* cur = seg + (size_t)(sgi & 1) * seg_len;
         const size_t sbase = (size_t)k + (size_t)sgi * seg_len;
@@ -869,7 +919,14 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
                             const int f = __builtin_ctzll(mask);
                             const double key = bcast_lane(v, f);
                             // increaseMinimumInMinHeap (MinHeap.h:78-85)
+#ifdef FEDNL_PROBE
+                            const unsigned long long ta = clock64();
+#endif
                             rh.admit(hv, hid, key, (uint32_t)(sbase + e0 + f));
+#ifdef FEDNL_PROBE
+                            adm_cnt += 1;
+                            adm_cyc += clock64() - ta;
+#endif
                             rv = rh.r0v;
                             const uint64_t above = (f == 63) ? 0ull : (~0ull << (f + 1));
                             mask = __ballot(v > rv) & above;
@@ -888,6 +945,13 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
         }
         __syncthreads();
     }
+#ifdef FEDNL_PROBE
+    if (tid == 0) {
+        g_probe[1000 + 8 * c + 5] += clock64() - tscan;
+        g_probe[1000 + 8 * c + 3] += adm_cnt;
+        g_probe[1000 + 8 * c + 4] += adm_cyc;
+    }
+#endif
     if (wave == 0)
         rh.store(hv, hid);
     __syncthreads();
