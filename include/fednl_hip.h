@@ -60,7 +60,9 @@ typedef struct fednl_opts {
     int workers;          /* client worker count of the reference run (--workers); selects
                              the TopLEK random stream of each client (TrainLoop.h:664-686) */
     int tracking;         /* 1: compute f(x_r) every round (--tracking) */
-    int hessian_mode;     /* 0: FP64 MFMA (default) */
+    int hessian_mode;     /* A^T D A contraction: 0 FP64 MFMA (default); 1 FP32 MFMA;
+                             2 BF16 hi/lo split with FP32 accumulation. Gradient, Delta,
+                             Lk, aggregation and the Newton solve stay FP64. */
     int reserved[7];
 } fednl_opts;
 
@@ -148,6 +150,20 @@ int fednl_newton_solve(int d, const double *H_up, double shift, const double *b,
  * used by the tests to pin the lane layout the Hessian kernel relies on.
  * No counterpart in the reference. */
 int fednl_probe_mfma_f64(double *out768);
+
+/* Client Hessians H_c(x) for all clients (n*S packed upper, host), computed
+ * by the engine's Hessian mode at the given x (d values); the engine's
+ * iterate and learned Hessians are unchanged. Used to state the error of the
+ * reduced-precision modes against FP64 (logistic_regression.h:365-547 is the
+ * FP64 reference of this quantity). */
+int fednl_hessians_at(void *handle, const double *x, double *out);
+
+/* Diagnostic: one v_mfma_f32_16x16x4f32 (a32, b32: 64 per-lane values) and
+ * one v_mfma_f32_16x16x32_bf16 (abf, bbf: 64 x 8 per-lane values, rounded to
+ * BF16), 64 x 4 outputs each; pins the lane layouts the reduced-precision
+ * Hessian kernels rely on. No counterpart in the reference. */
+int fednl_probe_mfma_mixed(const float *a32, const float *b32, float *out32, const float *abf,
+                           const float *bbf, float *outbf);
 
 /* Drop-in for dopt's shared-library entry point
  * (dopt/bin_fednl_local/exported/FedNLLocalShared.h:8-17,

@@ -233,6 +233,380 @@ __global__ __launch_bounds__(256) void k_margin(const double* __restrict__ A, co
 }
 
 // ---------------------------------------------------------------------------
+// Epilogue shared by the Hessian kernels: each thread holds sixteen entries
+// H(i, j) of its wave's 32 x 32 sub-tile (hval, with row iv and column jv).
+// Upper entries (i <= j < d) get +lambda on the diagonal; at creation they
+// become the learned Hessian, in a round they give Delta = H - learned and the
+// tile's partial sums of Delta^2 (off-diagonal, diagonal) for Lk
+// (frobeniusNormForSymmetricMatrixFromUpPart, MatrixNMD_SIMD_double.h:1067-1160).
+// With with_grad, the diagonal tile also writes the client gradient
+// lambda x + (per-column sums gathered in gacc).
+// ---------------------------------------------------------------------------
+template <bool kInit>
+__device__ __forceinline__ void hess_epilogue(const double (&hval)[16], const int (&iv)[16], const int (&jv)[16],
+                                              int c, int d, int d_pad, double lambda, size_t S, int tiles_up, int t,
+                                              const double* __restrict__ x, double* __restrict__ learned,
+                                              double* __restrict__ delta, double* __restrict__ part,
+                                              double* __restrict__ grad_c, bool with_grad, double gacc, int i0,
+                                              double (*red)[4], double (*gred)[kTile])
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // all sixteen learned values are loaded before use (clamped addresses for
+    // entries outside the upper triangle)
+    double soff = 0.0, sdiag = 0.0;
+    double lv[16];
+    if (!kInit) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int i = iv[q], j = jv[q];
+            const bool ok = (j < d && i <= j);
+            const size_t o = (size_t)c * S + (ok ? (size_t)j * (j + 1) / 2 + i : 0);
+            lv[q] = learned[o];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = iv[q], j = jv[q];
+        if (j < d && i <= j) {
+            double h = hval[q];
+            if (i == j)
+                h = h + lambda;
+            const size_t o = (size_t)c * S + (size_t)j * (j + 1) / 2 + i;
+            if (kInit) {
+                learned[o] = h;
+            } else {
+                const double dv = h - lv[q];
+                delta[o] = dv;
+                if (i == j)
+                    sdiag = fma(dv, dv, sdiag);
+                else
+                    soff = fma(dv, dv, soff);
+            }
+        }
+    }
+    if (kInit)
+        return;
+    for (int off = 32; off > 0; off >>= 1) {
+        soff += __shfl_xor(soff, off);
+        sdiag += __shfl_xor(sdiag, off);
+    }
+    if (lane == 0) {
+        red[0][wave] = soff;
+        red[1][wave] = sdiag;
+    }
+    if (with_grad)
+        gred[tid >> 6][tid & 63] = gacc;
+    __syncthreads();
+    if (tid == 0) {
+        const double o = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        const double dd = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        part[((size_t)c * tiles_up + t) * 2 + 0] = o;
+        part[((size_t)c * tiles_up + t) * 2 + 1] = dd;
+    }
+    if (with_grad && tid < kTile) {
+        const int i = i0 + tid;
+        if (i < d) {
+            const double sum = (gred[0][tid] + gred[1][tid]) + (gred[2][tid] + gred[3][tid]);
+            grad_c[(size_t)c * d_pad + i] = lambda * x[i] + sum;
+        }
+    }
+}
+
+// block -> (client, upper tile) with all tiles of one client on one XCD
+__device__ inline bool hess_tile_of_block(int n, int tiles_up, int& c, int& t, int& i0, int& j0)
+{
+    const int b = blockIdx.x;
+    const int xcd = b & 7, slot = b >> 3;
+    const int q = slot / tiles_up;
+    t = slot - q * tiles_up;
+    c = xcd + 8 * q;
+    if (c >= n)
+        return false;
+    int bj = 0;
+    while ((bj + 1) * (bj + 2) / 2 <= t) ++bj;
+    const int bi = t - bj * (bj + 1) / 2;
+    i0 = bi * kTile;
+    j0 = bj * kTile;
+    return true;
+}
+
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------------------
+// k_hessian_f32: the same contraction with v_mfma_f32_16x16x4f32 on an FP32
+// copy of A (fp32 D * A, FP32 accumulation). Only the contraction is FP32:
+// the result is widened to FP64 before lambda, Delta and Lk. The gradient is
+// computed in FP64 by k_grad_f64. Lane layout of the output: lane (l16, kq),
+// item r holds D[m = 4 kq + r][n = l16] (pinned by the layout probe test).
+// ---------------------------------------------------------------------------
+template <bool kInit>
+__global__ __launch_bounds__(256) void k_hessian_f32(const float* __restrict__ A32, const double* __restrict__ Dw,
+                                                     const double* __restrict__ x, int n, int m_pad, int d,
+                                                     int d_pad, int tiles_up, double lambda, size_t S,
+                                                     double* __restrict__ learned, double* __restrict__ delta,
+                                                     double* __restrict__ part)
+{
+    int c, t, i0, j0;
+    if (!hess_tile_of_block(n, tiles_up, c, t, i0, j0))
+        return;
+    __shared__ __align__(16) float sI[2][kKC][kTile + kPadLds];
+    __shared__ __align__(16) float sJ[2][kKC][kTile + kPadLds];
+    __shared__ double red[2][4];
+    __shared__ double gred[4][kTile];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wi = wave & 1, wj = wave >> 1;
+    const int l16 = lane & 15, kq = lane >> 4;
+
+    flt4 acc[2][2];
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+        for (int it = 0; it < 2; ++it) acc[jt][it] = flt4{0.f, 0.f, 0.f, 0.f};
+
+    const float* Ac = A32 + (size_t)c * m_pad * d_pad;
+    const double* Dc = Dw + (size_t)c * m_pad;
+    // rows lr of a chunk, columns lc .. lc + 3 of both column blocks
+    const int lr = tid >> 4, lc = (tid & 15) * 4;
+    float4 ri, rj;
+    float rd;
+    auto gload = [&](int s0) {
+        const float* row = Ac + (size_t)(s0 + lr) * d_pad;
+        ri = *reinterpret_cast<const float4*>(row + i0 + lc);
+        rj = *reinterpret_cast<const float4*>(row + j0 + lc);
+        rd = (float)Dc[s0 + lr];
+    };
+    auto lstore = [&](int buf) {
+        *reinterpret_cast<float4*>(&sI[buf][lr][lc]) = ri;
+        float4 sj;
+        sj.x = rd * rj.x;
+        sj.y = rd * rj.y;
+        sj.z = rd * rj.z;
+        sj.w = rd * rj.w;
+        *reinterpret_cast<float4*>(&sJ[buf][lr][lc]) = sj;
+    };
+    const int nch = m_pad / kKC;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        const int buf = ch & 1;
+        const bool more = ch + 1 < nch;
+        if (more)
+            gload((ch + 1) * kKC);
+#pragma unroll
+        for (int ks = 0; ks < kKC; ks += 4) {
+            const int kk = ks + kq;
+            const float a0 = sJ[buf][kk][wj * 32 + l16];
+            const float a1 = sJ[buf][kk][wj * 32 + 16 + l16];
+            const float b0 = sI[buf][kk][wi * 32 + l16];
+            const float b1 = sI[buf][kk][wi * 32 + 16 + l16];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        if (more)
+            lstore(buf ^ 1);
+        __syncthreads();
+    }
+    double hval[16];
+    int iv[16], jv[16];
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+        for (int it = 0; it < 2; ++it)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = (jt * 2 + it) * 4 + r;
+                hval[q] = (double)acc[jt][it][r];
+                iv[q] = i0 + wi * 32 + it * 16 + l16;
+                jv[q] = j0 + wj * 32 + jt * 16 + 4 * kq + r;
+            }
+    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, part, nullptr,
+                         false, 0.0, i0, red, gred);
+}
+
+// ---------------------------------------------------------------------------
+// k_hessian_bf16: split-precision contraction with v_mfma_f32_16x16x32_bf16.
+// A is held transposed as two BF16 arrays (hi = bf16(a), lo = bf16(a - hi));
+// each chunk of 32 samples forms fp32 D * A for the column block J and splits
+// it the same way. H ~= (DA)_hi A_hi + (DA)_lo A_hi + (DA)_hi A_lo, FP32
+// accumulation; the lo * lo term is dropped. Output layout as k_hessian_f32.
+// ---------------------------------------------------------------------------
+constexpr int kKB = 32;        // samples per chunk (K of one bf16 MFMA)
+constexpr int kKBPad = 40;     // LDS row length in bf16 (80 B rows)
+
+__device__ inline void split_bf16(float v, __bf16& hi, __bf16& lo)
+{
+    hi = (__bf16)v;
+    lo = (__bf16)(v - (float)hi);
+}
+
+template <bool kInit>
+__global__ __launch_bounds__(256) void k_hessian_bf16(const __bf16* __restrict__ AThi,
+                                                      const __bf16* __restrict__ ATlo,
+                                                      const double* __restrict__ Dw, const double* __restrict__ x,
+                                                      int n, int m_pad, int d, int d_pad, int tiles_up,
+                                                      double lambda, size_t S, double* __restrict__ learned,
+                                                      double* __restrict__ delta, double* __restrict__ part)
+{
+    int c, t, i0, j0;
+    if (!hess_tile_of_block(n, tiles_up, c, t, i0, j0))
+        return;
+    __shared__ __align__(16) __bf16 tIh[2][kTile][kKBPad];
+    __shared__ __align__(16) __bf16 tIl[2][kTile][kKBPad];
+    __shared__ __align__(16) __bf16 tJh[2][kTile][kKBPad];
+    __shared__ __align__(16) __bf16 tJl[2][kTile][kKBPad];
+    __shared__ double red[2][4];
+    __shared__ double gred[4][kTile];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wi = wave & 1, wj = wave >> 1;
+    const int l16 = lane & 15, kq = lane >> 4;
+
+    flt4 acc[2][2];
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+        for (int it = 0; it < 2; ++it) acc[jt][it] = flt4{0.f, 0.f, 0.f, 0.f};
+
+    const size_t cbase = (size_t)c * d_pad * m_pad;
+    const double* Dc = Dw + (size_t)c * m_pad;
+    // column col of each block, samples ks .. ks + 7 of the chunk
+    const int col = tid >> 2, ks = (tid & 3) * 8;
+    uint4 rih, ril, rjh, rjl;
+    double rdv[8];
+    auto gload = [&](int s0) {
+        rih = *reinterpret_cast<const uint4*>(AThi + cbase + (size_t)(i0 + col) * m_pad + s0 + ks);
+        ril = *reinterpret_cast<const uint4*>(ATlo + cbase + (size_t)(i0 + col) * m_pad + s0 + ks);
+        rjh = *reinterpret_cast<const uint4*>(AThi + cbase + (size_t)(j0 + col) * m_pad + s0 + ks);
+        rjl = *reinterpret_cast<const uint4*>(ATlo + cbase + (size_t)(j0 + col) * m_pad + s0 + ks);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            rdv[u] = Dc[s0 + ks + u];
+    };
+    auto lstore = [&](int buf) {
+        *reinterpret_cast<uint4*>(&tIh[buf][col][ks]) = rih;
+        *reinterpret_cast<uint4*>(&tIl[buf][col][ks]) = ril;
+        const bf16x8 jh = *reinterpret_cast<const bf16x8*>(&rjh);
+        const bf16x8 jl = *reinterpret_cast<const bf16x8*>(&rjl);
+        bf16x8 oh, ol;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const float a = (float)jh[u] + (float)jl[u];
+            __bf16 h, l;
+            split_bf16((float)rdv[u] * a, h, l);
+            oh[u] = h;
+            ol[u] = l;
+        }
+        *reinterpret_cast<bf16x8*>(&tJh[buf][col][ks]) = oh;
+        *reinterpret_cast<bf16x8*>(&tJl[buf][col][ks]) = ol;
+    };
+    const int nch = m_pad / kKB;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        const int buf = ch & 1;
+        const bool more = ch + 1 < nch;
+        if (more)
+            gload((ch + 1) * kKB);
+        bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            ah[u] = *reinterpret_cast<const bf16x8*>(&tJh[buf][wj * 32 + u * 16 + l16][8 * kq]);
+            al[u] = *reinterpret_cast<const bf16x8*>(&tJl[buf][wj * 32 + u * 16 + l16][8 * kq]);
+            bh[u] = *reinterpret_cast<const bf16x8*>(&tIh[buf][wi * 32 + u * 16 + l16][8 * kq]);
+            bl[u] = *reinterpret_cast<const bf16x8*>(&tIl[buf][wi * 32 + u * 16 + l16][8 * kq]);
+        }
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+            for (int it = 0; it < 2; ++it) {
+                acc[jt][it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[jt], bh[it], acc[jt][it], 0, 0, 0);
+                acc[jt][it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[jt], bl[it], acc[jt][it], 0, 0, 0);
+                acc[jt][it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[jt], bh[it], acc[jt][it], 0, 0, 0);
+            }
+        if (more)
+            lstore(buf ^ 1);
+        __syncthreads();
+    }
+    double hval[16];
+    int iv[16], jv[16];
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+        for (int it = 0; it < 2; ++it)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = (jt * 2 + it) * 4 + r;
+                hval[q] = (double)acc[jt][it][r];
+                iv[q] = i0 + wi * 32 + it * 16 + l16;
+                jv[q] = j0 + wj * 32 + jt * 16 + 4 * kq + r;
+            }
+    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, part, nullptr,
+                         false, 0.0, i0, red, gred);
+}
+
+// ---------------------------------------------------------------------------
+// k_grad_f64: client gradients lambda x + sum_s A_c[s] g_s in FP64
+// (logistic_regression.h:148-161) for the reduced-precision Hessian modes.
+// Block (column group of 64, client); four sample phases per column, summed
+// in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grad_f64(const double* __restrict__ A, const double* __restrict__ gw,
+                                                  const double* __restrict__ x, int m_pad, int d, int d_pad,
+                                                  double lambda, double* __restrict__ grad_c)
+{
+    __shared__ double gp[4][kTile];
+    const int c = blockIdx.y, i0 = blockIdx.x * kTile;
+    const int ii = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const double* Ac = A + (size_t)c * m_pad * d_pad;
+    const double* Gc = gw + (size_t)c * m_pad;
+    double acc = 0.0;
+    for (int s0 = ph * 8; s0 < m_pad; s0 += 32) {
+        double av[8], gv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            av[u] = Ac[(size_t)(s0 + u) * d_pad + i0 + ii];
+            gv[u] = Gc[s0 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            acc = fma(av[u], gv[u], acc);
+    }
+    gp[ph][ii] = acc;
+    __syncthreads();
+    if (threadIdx.x < kTile) {
+        const int i = i0 + ii;
+        if (i < d)
+            grad_c[(size_t)c * d_pad + i] = lambda * x[i] + ((gp[0][ii] + gp[1][ii]) + (gp[2][ii] + gp[3][ii]));
+    }
+}
+
+// FP32 copy of A (same layout) and the transposed BF16 hi/lo pair
+__global__ void k_convert_a(const double* __restrict__ A, int n, int m_pad, int d_pad, float* __restrict__ A32,
+                            __bf16* __restrict__ AThi, __bf16* __restrict__ ATlo)
+{
+    const size_t total = (size_t)n * m_pad * d_pad;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const float v = (float)A[e];
+        if (A32)
+            A32[e] = v;
+        if (AThi) {
+            const size_t c = e / ((size_t)m_pad * d_pad);
+            const size_t rem = e - c * (size_t)m_pad * d_pad;
+            const size_t s = rem / d_pad, i = rem - s * d_pad;
+            __bf16 h, l;
+            split_bf16(v, h, l);
+            const size_t o = c * (size_t)d_pad * m_pad + i * m_pad + s;
+            AThi[o] = h;
+            ATlo[o] = l;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_hessian_f64: H_c = A_c^T diag(D) A_c on the upper tiles, FP64 MFMA
 // block = 4 waves (2 x 2), each wave a 32 x 32 sub-tile of 2 x 2 MFMA 16x16 blocks.
 // Blocks are mapped so that all tiles of one client run on one XCD
@@ -246,17 +620,10 @@ __global__ __launch_bounds__(256) void k_hessian_f64(const double* __restrict__ 
                                                      double* __restrict__ delta, double* __restrict__ grad_c,
                                                      double* __restrict__ part)
 {
-    const int b = blockIdx.x;
-    const int xcd = b & 7, slot = b >> 3;
-    const int q = slot / tiles_up, t = slot - q * tiles_up;
-    const int c = xcd + 8 * q;
-    if (c >= n)
+    int c, t, i0, j0;
+    if (!hess_tile_of_block(n, tiles_up, c, t, i0, j0))
         return;
-    int bj = 0;
-    while ((bj + 1) * (bj + 2) / 2 <= t) ++bj;
-    const int bi = t - bj * (bj + 1) / 2;
-    const int i0 = bi * kTile, j0 = bj * kTile;
-    const bool dg = (bi == bj);
+    const bool dg = (i0 == j0);
 
     // two LDS buffers: the next chunk's rows are loaded into registers while
     // the current chunk is multiplied, then stored (scaled by D) into the
@@ -347,79 +714,21 @@ __global__ __launch_bounds__(256) void k_hessian_f64(const double* __restrict__ 
     // epilogue: lane (l16, kq), item r of v_mfma_f64_16x16x4f64 holds
     // D[m = 4r + kq][n = l16] (pinned by the layout probe test), i.e.
     // rows i = .. + l16 and columns j = .. + kq + 4r of the sub-tile (jt, it)
-    // all sixteen learned values of this thread are loaded before use
-    // (clamped addresses for entries outside the upper triangle)
-    double soff = 0.0, sdiag = 0.0;
-    double lv[2][2][4];
-    if (!kInit) {
+    double hval[16];
+    int iv[16], jv[16];
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt)
+    for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
-            for (int it = 0; it < 2; ++it) {
-                const int i = i0 + wi * 32 + it * 16 + l16;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int j = j0 + wj * 32 + jt * 16 + kq + 4 * r;
-                    const bool ok = (j < d && i <= j);
-                    const size_t o = (size_t)c * S + (ok ? (size_t)j * (j + 1) / 2 + i : 0);
-                    lv[jt][it][r] = learned[o];
-                }
-            }
-    }
-#pragma unroll
-    for (int jt = 0; jt < 2; ++jt) {
-#pragma unroll
-        for (int it = 0; it < 2; ++it) {
-            const dbl4 v = acc[jt][it];
-            const int i = i0 + wi * 32 + it * 16 + l16;
+        for (int it = 0; it < 2; ++it)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int j = j0 + wj * 32 + jt * 16 + kq + 4 * r;
-                if (j < d && i <= j) {
-                    double h = v[r];
-                    if (i == j)
-                        h = h + lambda;
-                    const size_t o = (size_t)c * S + (size_t)j * (j + 1) / 2 + i;
-                    if (kInit) {
-                        learned[o] = h;
-                    } else {
-                        const double dv = h - lv[jt][it][r];
-                        delta[o] = dv;
-                        if (i == j)
-                            sdiag = fma(dv, dv, sdiag);
-                        else
-                            soff = fma(dv, dv, soff);
-                    }
-                }
+                const int q = (jt * 2 + it) * 4 + r;
+                hval[q] = acc[jt][it][r];
+                iv[q] = i0 + wi * 32 + it * 16 + l16;
+                jv[q] = j0 + wj * 32 + jt * 16 + kq + 4 * r;
             }
-        }
-    }
-    if (kInit)
-        return;
-    for (int off = 32; off > 0; off >>= 1) {
-        soff += __shfl_xor(soff, off);
-        sdiag += __shfl_xor(sdiag, off);
-    }
-    if (lane == 0) {
-        red[0][wave] = soff;
-        red[1][wave] = sdiag;
-    }
-    if (dg)
-        gred[tid >> 6][tid & 63] = gacc;
-    __syncthreads();
-    if (tid == 0) {
-        const double o = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
-        const double dd = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
-        part[((size_t)c * tiles_up + t) * 2 + 0] = o;
-        part[((size_t)c * tiles_up + t) * 2 + 1] = dd;
-    }
-    if (dg && tid < kTile) {
-        const int i = i0 + tid;
-        if (i < d) {
-            const double s = (gred[0][tid] + gred[1][tid]) + (gred[2][tid] + gred[3][tid]);
-            grad_c[(size_t)c * d_pad + i] = lambda * x[i] + s;
-        }
-    }
+    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, part, grad_c,
+                         !kInit && dg, gacc, i0, red, gred);
 }
 
 // ---------------------------------------------------------------------------
@@ -1861,6 +2170,28 @@ __global__ __launch_bounds__(64) void k_mfma_probe(double* __restrict__ out)
     }
 }
 
+// Lane layout checks of v_mfma_f32_16x16x4f32 and v_mfma_f32_16x16x32_bf16:
+// in[0..63] / in[64..127] are the per-lane f32 operands, in[128..639] /
+// in[640..1151] eight bf16 operands per lane (given as floats).
+__global__ __launch_bounds__(64) void k_mfma_probe_mixed(const float* __restrict__ in, float* __restrict__ out)
+{
+    const int lane = threadIdx.x;
+    const flt4 z = {0.f, 0.f, 0.f, 0.f};
+    const flt4 r1 = __builtin_amdgcn_mfma_f32_16x16x4f32(in[lane], in[64 + lane], z, 0, 0, 0);
+    bf16x8 a, b;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        a[u] = (__bf16)in[128 + lane * 8 + u];
+        b[u] = (__bf16)in[640 + lane * 8 + u];
+    }
+    const flt4 r2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, z, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        out[lane * 4 + q] = r1[q];
+        out[256 + lane * 4 + q] = r2[q];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host engine
 // ---------------------------------------------------------------------------
@@ -1896,6 +2227,10 @@ struct Engine {
     int klaunch[KID_COUNT] = {0};
     size_t lds_select = 0, lds_newton = 0;
     int seg_len = 256;
+    int hmode = 0;                 // 0 FP64, 1 FP32, 2 BF16 split (A^T D A only)
+    float* A32 = nullptr;          // FP32 copy of A (mode 1)
+    __bf16* AThi = nullptr;        // transposed BF16 hi / lo of A (mode 2)
+    __bf16* ATlo = nullptr;
 };
 
 static hipEvent_t pool_event(Engine* e)
@@ -1966,6 +2301,45 @@ static int ensure_trace(Engine* e, int need)
     return FEDNL_OK;
 }
 
+// client Hessians (and, for the reduced-precision modes, the FP64 client
+// gradients) at the current margin weights; init: write learned instead of Delta
+static void launch_hessian(Engine* e, bool init, double* out_learned)
+{
+    const int n = e->n;
+    const int nblk = 8 * ((n + 7) / 8) * e->tiles_up;
+    double* lrn = out_learned ? out_learned : e->learned;
+    if (e->hmode == 0) {
+        if (init)
+            hipLaunchKernelGGL(k_hessian_f64<true>, dim3(nblk), dim3(256), 0, e->st, e->A, e->Dw, e->gw, e->x, n,
+                               e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta,
+                               e->grad_c, e->part);
+        else
+            hipLaunchKernelGGL(k_hessian_f64<false>, dim3(nblk), dim3(256), 0, e->st, e->A, e->Dw, e->gw, e->x, n,
+                               e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta,
+                               e->grad_c, e->part);
+        return;
+    }
+    if (!init)
+        hipLaunchKernelGGL(k_grad_f64, dim3(e->d_pad / kTile, n), dim3(256), 0, e->st, e->A, e->gw, e->x, e->m_pad,
+                           e->d, e->d_pad, e->o.lambda, e->grad_c);
+    if (e->hmode == 1) {
+        if (init)
+            hipLaunchKernelGGL(k_hessian_f32<true>, dim3(nblk), dim3(256), 0, e->st, e->A32, e->Dw, e->x, n,
+                               e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta, e->part);
+        else
+            hipLaunchKernelGGL(k_hessian_f32<false>, dim3(nblk), dim3(256), 0, e->st, e->A32, e->Dw, e->x, n,
+                               e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta, e->part);
+    } else {
+        if (init)
+            hipLaunchKernelGGL(k_hessian_bf16<true>, dim3(nblk), dim3(256), 0, e->st, e->AThi, e->ATlo, e->Dw, e->x,
+                               n, e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta, e->part);
+        else
+            hipLaunchKernelGGL(k_hessian_bf16<false>, dim3(nblk), dim3(256), 0, e->st, e->AThi, e->ATlo, e->Dw,
+                               e->x, n, e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta,
+                               e->part);
+    }
+}
+
 static int launch_round(Engine* e)
 {
     const int n = e->n;
@@ -1977,10 +2351,7 @@ static int launch_round(Engine* e)
     }
     {
         KTime kt(e, KID_HESS);
-        const int nblk = 8 * ((n + 7) / 8) * e->tiles_up;
-        hipLaunchKernelGGL(k_hessian_f64<false>, dim3(nblk), dim3(256), 0, e->st, e->A, e->Dw, e->gw, e->x, n,
-                           e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, e->learned, e->delta,
-                           e->grad_c, e->part);
+        launch_hessian(e, false, nullptr);
     }
     const int comp = e->o.compressor;
     {
@@ -2063,7 +2434,7 @@ static void free_engine(Engine* e)
     void* ptrs[] = {e->A, e->x, e->Dw, e->gw, e->nlog, e->learned, e->delta, e->grad_c, e->part, e->lk, e->sel_bm,
                     e->sel, e->pos, e->idx, e->rng, e->lek_state, e->lek_popped, e->seq_start, e->sel_cnt,
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
-                    e->dx, e->trace};
+                    e->dx, e->trace, e->A32, e->AThi, e->ATlo};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -2120,7 +2491,12 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         set_err("fednl_create: d > 512 needs the multi-workgroup Newton solve");
         return FEDNL_E_UNSUPPORTED;
     }
+    if (opts->hessian_mode < 0 || opts->hessian_mode > 2) {
+        set_err("fednl_create: hessian_mode must be 0 (FP64), 1 (FP32) or 2 (BF16 split)");
+        return FEDNL_E_WRONG_ARGUMENT;
+    }
     Engine* e = new Engine();
+    e->hmode = opts->hessian_mode;
     e->o = *opts;
     e->n = n;
     e->m = m;
@@ -2149,7 +2525,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         delete e;
         return FEDNL_E_UNSUPPORTED;
     }
-    e->m_pad = round_up(m, kKC);
+    e->m_pad = round_up(m, kKB);  // a multiple of every kernel's sample chunk
     e->d_pad = round_up(d, kTile);
     e->nt = e->d_pad / kTile;
     e->tiles_up = e->nt * (e->nt + 1) / 2;
@@ -2257,6 +2633,25 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
             }
         }
     }
+    // reduced-precision copies of A for the Hessian contraction
+    if (e->hmode != 0) {
+        const size_t nA = (size_t)n * e->m_pad * e->d_pad;
+        hipError_t ae = hipSuccess;
+        if (e->hmode == 1)
+            ae = hipMalloc((void**)&e->A32, sizeof(float) * nA);
+        else {
+            ae = hipMalloc((void**)&e->AThi, sizeof(__bf16) * nA);
+            if (ae == hipSuccess)
+                ae = hipMalloc((void**)&e->ATlo, sizeof(__bf16) * nA);
+        }
+        if (ae != hipSuccess) {
+            set_err("hipMalloc (reduced-precision A)", ae);
+            free_engine(e);
+            return FEDNL_E_DEVICE;
+        }
+        hipLaunchKernelGGL(k_convert_a, dim3(2048), dim3(256), 0, e->st, e->A, n, e->m_pad, e->d_pad, e->A32,
+                           e->AThi, e->ATlo);
+    }
     std::vector<uint32_t> seeds(n);
     for (int c = 0; c < n; ++c) seeds[c] = (uint32_t)(123 + c);  // TrainLoop.h:632
     std::vector<uint32_t> lek(e->workers, 1234u);                   // RandomGenRealLinear default seed
@@ -2272,10 +2667,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         dim3 grid(e->m_pad / 16, n);
         hipLaunchKernelGGL(k_margin, grid, dim3(256), sizeof(double) * e->d_pad, e->st, e->A, e->x, e->m, e->m_pad,
                            e->d, e->d_pad, 1.0 / (double)e->m, e->Dw, e->gw, e->nlog);
-        const int nblk = 8 * ((n + 7) / 8) * e->tiles_up;
-        hipLaunchKernelGGL(k_hessian_f64<true>, dim3(nblk), dim3(256), 0, e->st, e->A, e->Dw, e->gw, e->x, n,
-                           e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, e->learned, e->delta,
-                           e->grad_c, e->part);
+        launch_hessian(e, true, nullptr);
         hipLaunchKernelGGL(k_init_H, dim3((unsigned)((e->S + 255) / 256)), dim3(256), 0, e->st, e->H, e->learned,
                            e->S, n, e->inv_n);
     }
@@ -2714,6 +3106,76 @@ int fednl_probe_mfma_f64(double* out768)
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(hipMemcpy(out768, d_out, sizeof(double) * 768, hipMemcpyDeviceToHost));
     HIP_OK(hipFree(d_out));
+    return FEDNL_OK;
+}
+
+int fednl_probe_mfma_mixed(const float* a32, const float* b32, float* out32, const float* abf, const float* bbf,
+                           float* outbf)
+{
+    float *d_a = nullptr, *d_o = nullptr;
+    // inputs: a32[64], b32[64], abf[512], bbf[512]; outputs: 256 + 256 floats
+    HIP_OK(hipMalloc((void**)&d_a, sizeof(float) * (64 + 64 + 512 + 512)));
+    HIP_OK(hipMalloc((void**)&d_o, sizeof(float) * 512));
+    hipMemcpy(d_a, a32, sizeof(float) * 64, hipMemcpyHostToDevice);
+    hipMemcpy(d_a + 64, b32, sizeof(float) * 64, hipMemcpyHostToDevice);
+    hipMemcpy(d_a + 128, abf, sizeof(float) * 512, hipMemcpyHostToDevice);
+    hipMemcpy(d_a + 640, bbf, sizeof(float) * 512, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_mfma_probe_mixed, dim3(1), dim3(64), 0, 0, d_a, d_o);
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess)
+        le = hipDeviceSynchronize();
+    if (le == hipSuccess) {
+        hipMemcpy(out32, d_o, sizeof(float) * 256, hipMemcpyDeviceToHost);
+        hipMemcpy(outbf, d_o + 256, sizeof(float) * 256, hipMemcpyDeviceToHost);
+    }
+    hipFree(d_a);
+    hipFree(d_o);
+    if (le != hipSuccess) {
+        set_err("mfma probe", le);
+        return FEDNL_E_DEVICE;
+    }
+    return FEDNL_OK;
+}
+
+int fednl_hessians_at(void* handle, const double* x, double* out)
+{
+    Engine* e = (Engine*)handle;
+    if (!e || !x || !out) {
+        set_err("fednl_hessians_at: null argument");
+        return FEDNL_E_WRONG_ARGUMENT;
+    }
+    double *dxv = nullptr, *dout = nullptr;
+    HIP_OK(hipMalloc((void**)&dxv, sizeof(double) * e->d_pad));
+    if (hipMalloc((void**)&dout, sizeof(double) * (size_t)e->n * e->S) != hipSuccess) {
+        hipFree(dxv);
+        set_err("hipMalloc (hessians_at)");
+        return FEDNL_E_DEVICE;
+    }
+    std::vector<double> hx(e->d_pad, 0.0);
+    std::copy(x, x + e->d, hx.begin());
+    hipError_t err = hipStreamSynchronize(e->st);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(dxv, hx.data(), sizeof(double) * e->d_pad, hipMemcpyHostToDevice, e->st);
+    if (err == hipSuccess) {
+        // the margin weights of x, then the Hessians in the engine's mode
+        double* keep_x = e->x;
+        e->x = dxv;
+        hipLaunchKernelGGL(k_margin, dim3(e->m_pad / 16, e->n), dim3(256), sizeof(double) * e->d_pad, e->st, e->A,
+                           e->x, e->m, e->m_pad, e->d, e->d_pad, 1.0 / (double)e->m, e->Dw, e->gw, e->nlog);
+        launch_hessian(e, true, dout);
+        e->x = keep_x;
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess)
+        err = hipStreamSynchronize(e->st);
+    if (err == hipSuccess)
+        err = hipMemcpy(out, dout, sizeof(double) * (size_t)e->n * e->S, hipMemcpyDeviceToHost);
+    hipFree(dxv);
+    hipFree(dout);
+    if (err != hipSuccess) {
+        set_err("fednl_hessians_at", err);
+        return FEDNL_E_DEVICE;
+    }
     return FEDNL_OK;
 }
 

@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfednl_hip.so")
 
 COMPRESSORS = {"identical": 0, "randk": 1, "topk": 2, "seqk": 3, "toplek": 4, "natural": 5}
+HESSIAN_MODES = {"fp64": 0, "fp32": 1, "bf16x2": 2}
 KERNEL_NAMES_MAX = 16
 
 _c_dp = ctypes.POINTER(ctypes.c_double)
@@ -72,6 +73,9 @@ def lib():
         L.fednl_select.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dp, _c_up, _c_up,
                                    ctypes.c_int, ctypes.c_double, _c_up, _c_ip]
         L.fednl_newton_solve.argtypes = [ctypes.c_int, _c_dp, ctypes.c_double, _c_dp, _c_dp]
+        L.fednl_hessians_at.argtypes = [ctypes.c_void_p, _c_dp, _c_dp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.fednl_probe_mfma_mixed.argtypes = [fp, fp, fp, fp, fp, fp]
         _lib = L
     return _lib
 
@@ -103,7 +107,7 @@ class Engine:
     """One FedNL problem resident on the GPU."""
 
     def __init__(self, features, labels, lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0,
-                 option_b=True, workers=1, tracking=True):
+                 option_b=True, workers=1, tracking=True, hessian="fp64"):
         L = lib()
         feats = np.ascontiguousarray(features, dtype=np.float64)
         labs = np.ascontiguousarray(labels, dtype=np.float64)
@@ -118,6 +122,7 @@ class Engine:
         o.option_b = 1 if option_b else 0
         o.workers = workers
         o.tracking = 1 if tracking else 0
+        o.hessian_mode = HESSIAN_MODES[hessian]
         h = ctypes.c_void_p()
         _err(L.fednl_create(ctypes.byref(o), n, m, d, dp(feats), dp(labs), ctypes.byref(h)), "fednl_create")
         self.h = h
@@ -127,6 +132,14 @@ class Engine:
         self.k, self.S, self.lda = k.value, S.value, lda.value
         self.workers = max(1, min(workers, n))
         self.alpha = L.fednl_alpha(self.h)
+
+    def hessians_at(self, x):
+        """Client Hessians H_c(x) (n x S packed upper) as the engine's
+        Hessian mode computes them."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.n * self.S)
+        _err(lib().fednl_hessians_at(self.h, dp(x), dp(out)), "fednl_hessians_at")
+        return out.reshape(self.n, self.S)
 
     @classmethod
     def from_problem(cls, prob, **kw):
@@ -221,3 +234,16 @@ def newton_solve(H_up, d, shift, b):
     dx = np.zeros(d)
     ok = _err(lib().fednl_newton_solve(d, dp(H_up), shift, dp(b), dp(dx)), "fednl_newton_solve")
     return bool(ok), dx
+
+
+def probe_mfma_mixed(a32, b32, abf, bbf):
+    """Raw outputs of one v_mfma_f32_16x16x4f32 (a32, b32: 64 lane values) and
+    one v_mfma_f32_16x16x32_bf16 (abf, bbf: 64 x 8 lane values); 64 x 4 each."""
+    f = ctypes.POINTER(ctypes.c_float)
+    arrs = [np.ascontiguousarray(v, dtype=np.float32).reshape(-1) for v in (a32, b32, abf, bbf)]
+    o1 = np.zeros(256, dtype=np.float32)
+    o2 = np.zeros(256, dtype=np.float32)
+    a32_, b32_, abf_, bbf_ = (a.ctypes.data_as(f) for a in arrs)
+    _err(lib().fednl_probe_mfma_mixed(a32_, b32_, o1.ctypes.data_as(f), abf_, bbf_, o2.ctypes.data_as(f)),
+         "fednl_probe_mfma_mixed")
+    return o1.reshape(64, 4), o2.reshape(64, 4)
