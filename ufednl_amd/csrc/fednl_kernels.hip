@@ -179,23 +179,63 @@ __device__ inline uint64_t round_nearest(double v)
 }
 
 // ---------------------------------------------------------------------------
+// Per-problem device descriptor. Every round kernel takes an array of these
+// and a block map: entry b of the map is (problem, local block) for global
+// block b, so one launch can serve many independent problems (a batch); a
+// null map means one problem and local block = blockIdx.x. Map entries with
+// problem < 0 are padding (the Hessian kernels align each problem's blocks to
+// 8 so that blockIdx % 8 still names the XCD).
+// ---------------------------------------------------------------------------
+struct ProblemDev {
+    int n, m, d, k, lda, m_pad, d_pad, tiles_up, dp, workers, comp, hmode, option_b, tracking, seg_len, bm_words,
+        trace_cap, update_x;
+    size_t S;
+    double lambda, alpha, prescale, inv_n, mult, lek_delta;
+    const double* A;
+    const float* A32;
+    const __bf16* AThi;
+    const __bf16* ATlo;
+    double *x, *Dw, *gw, *nlog, *learned, *delta, *grad_c, *part, *lk, *vals, *H, *g_avg, *shift, *W, *dx, *trace,
+        *lek_p;
+    const double* hup;  // Newton: packed H to fill W from (standalone solve), or null
+    const double* nb;   // Newton: right-hand side
+    uint32_t *sel, *pos, *idx, *rng, *lek_state, *lek_popped, *seq_start, *sel_bm;
+    int *sel_cnt, *cnt, *lek_hit, *round_ctr;
+};
+
+#define FEDNL_BLOCK()                                                                     \
+    const int2 bmv_ = bmap ? bmap[blockIdx.x] : make_int2(0, (int)blockIdx.x);            \
+    if (bmv_.x < 0)                                                                       \
+        return;                                                                           \
+    const ProblemDev& PD = probs[bmv_.x];                                                 \
+    const int blk = bmv_.y
+
+#define FEDNL_KARGS const ProblemDev* __restrict__ probs, const int2* __restrict__ bmap
+
+// ---------------------------------------------------------------------------
 // k_margin: one wave per sample row
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_margin(const double* __restrict__ A, const double* __restrict__ x,
-                                                int m, int m_pad, int d, int d_pad, double inv_m,
-                                                double* __restrict__ Dw, double* __restrict__ gw,
-                                                double* __restrict__ nlog_part)
+__global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
 {
+    FEDNL_BLOCK();
+    const double* __restrict__ A = PD.A;
+    const double* __restrict__ x = PD.x;
+    const int m = PD.m, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad;
+    const double inv_m = 1.0 / (double)m;
+    double* __restrict__ Dw = PD.Dw;
+    double* __restrict__ gw = PD.gw;
+    double* __restrict__ nlog_part = PD.nlog;
+    const int gx = m_pad / 16, bx = blk % gx, by = blk / gx;
     extern __shared__ double xs[];
     __shared__ double nl[16];
-    const int c = blockIdx.y;
+    const int c = by;
     if (threadIdx.x < 16)
         nl[threadIdx.x] = 0.0;
     for (int i = threadIdx.x; i < d_pad; i += blockDim.x)
         xs[i] = (i < d) ? x[i] : 0.0;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int s0 = blockIdx.x * 16 + wave * 4;
+    const int s0 = bx * 16 + wave * 4;
     for (int q = 0; q < 4; ++q) {
         const int s = s0 + q;
         if (s >= m_pad)
@@ -228,7 +268,7 @@ __global__ __launch_bounds__(256) void k_margin(const double* __restrict__ A, co
         // per-block partial of sum_s -log sigma_s (samples in order)
         double sacc = 0.0;
         for (int q = 0; q < 16; ++q) sacc = sacc + nl[q];
-        nlog_part[(size_t)c * gridDim.x + blockIdx.x] = sacc;
+        nlog_part[(size_t)c * gx + bx] = sacc;
     }
 }
 
@@ -313,9 +353,8 @@ __device__ __forceinline__ void hess_epilogue(const double (&hval)[16], const in
 }
 
 // block -> (client, upper tile) with all tiles of one client on one XCD
-__device__ inline bool hess_tile_of_block(int n, int tiles_up, int& c, int& t, int& i0, int& j0)
+__device__ inline bool hess_tile_of_block(int b, int n, int tiles_up, int& c, int& t, int& i0, int& j0)
 {
-    const int b = blockIdx.x;
     const int xcd = b & 7, slot = b >> 3;
     const int q = slot / tiles_up;
     t = slot - q * tiles_up;
@@ -341,14 +380,20 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // item r holds D[m = 4 kq + r][n = l16] (pinned by the layout probe test).
 // ---------------------------------------------------------------------------
 template <bool kInit>
-__global__ __launch_bounds__(256) void k_hessian_f32(const float* __restrict__ A32, const double* __restrict__ Dw,
-                                                     const double* __restrict__ x, int n, int m_pad, int d,
-                                                     int d_pad, int tiles_up, double lambda, size_t S,
-                                                     double* __restrict__ learned, double* __restrict__ delta,
-                                                     double* __restrict__ part)
+__global__ __launch_bounds__(256) void k_hessian_f32(FEDNL_KARGS, double* __restrict__ learned_out)
 {
+    FEDNL_BLOCK();
+    const float* __restrict__ A32 = PD.A32;
+    const double* __restrict__ Dw = PD.Dw;
+    const double* __restrict__ x = PD.x;
+    const int n = PD.n, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad, tiles_up = PD.tiles_up;
+    const double lambda = PD.lambda;
+    const size_t S = PD.S;
+    double* __restrict__ learned = learned_out ? learned_out : PD.learned;
+    double* __restrict__ delta = PD.delta;
+    double* __restrict__ part = PD.part;
     int c, t, i0, j0;
-    if (!hess_tile_of_block(n, tiles_up, c, t, i0, j0))
+    if (!hess_tile_of_block(blk, n, tiles_up, c, t, i0, j0))
         return;
     __shared__ __align__(16) float sI[2][kKC][kTile + kPadLds];
     __shared__ __align__(16) float sJ[2][kKC][kTile + kPadLds];
@@ -444,15 +489,21 @@ __device__ inline void split_bf16(float v, __bf16& hi, __bf16& lo)
 }
 
 template <bool kInit>
-__global__ __launch_bounds__(256) void k_hessian_bf16(const __bf16* __restrict__ AThi,
-                                                      const __bf16* __restrict__ ATlo,
-                                                      const double* __restrict__ Dw, const double* __restrict__ x,
-                                                      int n, int m_pad, int d, int d_pad, int tiles_up,
-                                                      double lambda, size_t S, double* __restrict__ learned,
-                                                      double* __restrict__ delta, double* __restrict__ part)
+__global__ __launch_bounds__(256) void k_hessian_bf16(FEDNL_KARGS, double* __restrict__ learned_out)
 {
+    FEDNL_BLOCK();
+    const __bf16* __restrict__ AThi = PD.AThi;
+    const __bf16* __restrict__ ATlo = PD.ATlo;
+    const double* __restrict__ Dw = PD.Dw;
+    const double* __restrict__ x = PD.x;
+    const int n = PD.n, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad, tiles_up = PD.tiles_up;
+    const double lambda = PD.lambda;
+    const size_t S = PD.S;
+    double* __restrict__ learned = learned_out ? learned_out : PD.learned;
+    double* __restrict__ delta = PD.delta;
+    double* __restrict__ part = PD.part;
     int c, t, i0, j0;
-    if (!hess_tile_of_block(n, tiles_up, c, t, i0, j0))
+    if (!hess_tile_of_block(blk, n, tiles_up, c, t, i0, j0))
         return;
     __shared__ __align__(16) __bf16 tIh[2][kTile][kKBPad];
     __shared__ __align__(16) __bf16 tIl[2][kTile][kKBPad];
@@ -554,12 +605,18 @@ __global__ __launch_bounds__(256) void k_hessian_bf16(const __bf16* __restrict__
 // Block (column group of 64, client); four sample phases per column, summed
 // in a fixed order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_grad_f64(const double* __restrict__ A, const double* __restrict__ gw,
-                                                  const double* __restrict__ x, int m_pad, int d, int d_pad,
-                                                  double lambda, double* __restrict__ grad_c)
+__global__ __launch_bounds__(256) void k_grad_f64(FEDNL_KARGS)
 {
+    FEDNL_BLOCK();
+    const double* __restrict__ A = PD.A;
+    const double* __restrict__ gw = PD.gw;
+    const double* __restrict__ x = PD.x;
+    const int m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad;
+    const double lambda = PD.lambda;
+    double* __restrict__ grad_c = PD.grad_c;
     __shared__ double gp[4][kTile];
-    const int c = blockIdx.y, i0 = blockIdx.x * kTile;
+    const int gx = d_pad / kTile;
+    const int c = blk / gx, i0 = (blk % gx) * kTile;
     const int ii = threadIdx.x & 63, ph = threadIdx.x >> 6;
     const double* Ac = A + (size_t)c * m_pad * d_pad;
     const double* Gc = gw + (size_t)c * m_pad;
@@ -613,15 +670,22 @@ __global__ void k_convert_a(const double* __restrict__ A, int n, int m_pad, int 
 // (blockIdx % 8 selects the XCD), keeping the client's rows of A in that L2.
 // ---------------------------------------------------------------------------
 template <bool kInit>
-__global__ __launch_bounds__(256) void k_hessian_f64(const double* __restrict__ A, const double* __restrict__ Dw,
-                                                     const double* __restrict__ gw, const double* __restrict__ x,
-                                                     int n, int m_pad, int d, int d_pad, int tiles_up,
-                                                     double lambda, size_t S, double* __restrict__ learned,
-                                                     double* __restrict__ delta, double* __restrict__ grad_c,
-                                                     double* __restrict__ part)
+__global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __restrict__ learned_out)
 {
+    FEDNL_BLOCK();
+    const double* __restrict__ A = PD.A;
+    const double* __restrict__ Dw = PD.Dw;
+    const double* __restrict__ gw = PD.gw;
+    const double* __restrict__ x = PD.x;
+    const int n = PD.n, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad, tiles_up = PD.tiles_up;
+    const double lambda = PD.lambda;
+    const size_t S = PD.S;
+    double* __restrict__ learned = learned_out ? learned_out : PD.learned;
+    double* __restrict__ delta = PD.delta;
+    double* __restrict__ grad_c = PD.grad_c;
+    double* __restrict__ part = PD.part;
     int c, t, i0, j0;
-    if (!hess_tile_of_block(n, tiles_up, c, t, i0, j0))
+    if (!hess_tile_of_block(blk, n, tiles_up, c, t, i0, j0))
         return;
     const bool dg = (i0 == j0);
 
@@ -1121,15 +1185,22 @@ __device__ inline double fro2_from_parts(const double* part, int c, int tiles_up
 // Output: sorted selected list positions, count; TopLEK: hit flag, p, popped.
 // ---------------------------------------------------------------------------
 template <bool kLek>
-__global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ delta, size_t S, int k,
-                                                     int seg_len, const double* __restrict__ part, int tiles_up,
-                                                     double lek_delta, double* __restrict__ lk_out,
-                                                     uint32_t* __restrict__ sel, int* __restrict__ sel_cnt,
-                                                     double* __restrict__ lek_p, int* __restrict__ lek_hit,
-                                                     uint32_t* __restrict__ lek_popped)
+__global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
 {
+    FEDNL_BLOCK();
+    const double* __restrict__ delta = PD.delta;
+    const size_t S = PD.S;
+    const int k = PD.k, seg_len = PD.seg_len, tiles_up = PD.tiles_up;
+    const double* __restrict__ part = PD.part;
+    const double lek_delta = PD.lek_delta;
+    double* __restrict__ lk_out = nullptr;
+    uint32_t* __restrict__ sel = PD.sel;
+    int* __restrict__ sel_cnt = PD.sel_cnt;
+    double* __restrict__ lek_p = PD.lek_p;
+    int* __restrict__ lek_hit = PD.lek_hit;
+    uint32_t* __restrict__ lek_popped = PD.lek_popped;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int c = blockIdx.x;
+    const int c = blk;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const double* dc = delta + (size_t)c * S;
     __shared__ double s_fro2;
@@ -1350,11 +1421,16 @@ __global__ __launch_bounds__(256) void k_select_heap(const double* __restrict__ 
 // target t_i = i + round(u_i (S-1-i)); the value left at slot i is found by
 // following the chain of earlier swaps that touched its source slot.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_select_randk(size_t S, int k, uint32_t* __restrict__ rng,
-                                                      uint32_t* __restrict__ sel, int* __restrict__ sel_cnt)
+__global__ __launch_bounds__(256) void k_select_randk(FEDNL_KARGS)
 {
+    FEDNL_BLOCK();
+    const size_t S = PD.S;
+    const int k = PD.k;
+    uint32_t* __restrict__ rng = PD.rng;
+    uint32_t* __restrict__ sel = PD.sel;
+    int* __restrict__ sel_cnt = PD.sel_cnt;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int c = blockIdx.x;
+    const int c = blk;
     const int tid = threadIdx.x;
     const int P = pow2_at_least(k);
     uint64_t* keys = (uint64_t*)smem;            // P
@@ -1438,11 +1514,17 @@ __global__ __launch_bounds__(256) void k_select_randk(size_t S, int k, uint32_t*
 
 // RandSeqK (SpecialMathRoutinesForMatrix.h:137-169): window of k list positions
 // from generateInteger() % S, cyclic; stored ascending. Identical: all positions.
-__global__ __launch_bounds__(256) void k_select_window(int comp, size_t S, int k, uint32_t* __restrict__ rng,
-                                                       uint32_t* __restrict__ sel, int* __restrict__ sel_cnt,
-                                                       uint32_t* __restrict__ seq_start)
+__global__ __launch_bounds__(256) void k_select_window(FEDNL_KARGS)
 {
-    const int c = blockIdx.x;
+    FEDNL_BLOCK();
+    const int comp = PD.comp;
+    const size_t S = PD.S;
+    const int k = PD.k;
+    uint32_t* __restrict__ rng = PD.rng;
+    uint32_t* __restrict__ sel = PD.sel;
+    int* __restrict__ sel_cnt = PD.sel_cnt;
+    uint32_t* __restrict__ seq_start = PD.seq_start;
+    const int c = blk;
     __shared__ uint64_t s_start;
     if (threadIdx.x == 0) {
         uint64_t st = 0;
@@ -1477,18 +1559,28 @@ __global__ __launch_bounds__(256) void k_select_window(int comp, size_t S, int k
 // draw from their worker's RealLinear stream: the draw index is the number of
 // earlier hits among the clients of the same worker this round.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_emit(int comp, const double* __restrict__ delta, size_t S, int k, int lda,
-                                              const uint32_t* __restrict__ sel, const int* __restrict__ sel_cnt,
-                                              const double* __restrict__ lek_p, const int* __restrict__ lek_hit,
-                                              const uint32_t* __restrict__ lek_popped,
-                                              const uint32_t* __restrict__ lek_state, int workers, double alpha,
-                                              double prescale, double mult, double* __restrict__ learned,
-                                              uint32_t* __restrict__ pos_out, uint32_t* __restrict__ idx_out,
-                                              double* __restrict__ vals_out, int* __restrict__ cnt_out,
-                                              uint32_t* __restrict__ sel_bm, int bm_words)
+__global__ __launch_bounds__(256) void k_emit(FEDNL_KARGS)
 {
+    FEDNL_BLOCK();
+    const int comp = PD.comp;
+    const double* __restrict__ delta = PD.delta;
+    const size_t S = PD.S;
+    const int k = PD.k, lda = PD.lda, workers = PD.workers, bm_words = PD.bm_words;
+    const uint32_t* __restrict__ sel = PD.sel;
+    const int* __restrict__ sel_cnt = PD.sel_cnt;
+    const double* __restrict__ lek_p = PD.lek_p;
+    const int* __restrict__ lek_hit = PD.lek_hit;
+    const uint32_t* __restrict__ lek_popped = PD.lek_popped;
+    const uint32_t* __restrict__ lek_state = PD.lek_state;
+    const double alpha = PD.alpha, prescale = PD.prescale, mult = PD.mult;
+    double* __restrict__ learned = PD.learned;
+    uint32_t* __restrict__ pos_out = PD.pos;
+    uint32_t* __restrict__ idx_out = PD.idx;
+    double* __restrict__ vals_out = PD.vals;
+    int* __restrict__ cnt_out = PD.cnt;
+    uint32_t* __restrict__ sel_bm = PD.sel_bm;
     extern __shared__ __align__(16) uint32_t bml[];  // bm_words
-    const int c = blockIdx.x;
+    const int c = blk;
     for (int w = threadIdx.x; w < bm_words; w += blockDim.x)
         bml[w] = 0u;
     __shared__ int s_cnt, s_ins;
@@ -1563,14 +1655,20 @@ __global__ __launch_bounds__(256) void k_emit(int comp, const double* __restrict
 // ---------------------------------------------------------------------------
 constexpr int kMasterPos = 256;
 
-__global__ __launch_bounds__(256) void k_master_h(double* __restrict__ H, size_t S, int n,
-                                                  const double* __restrict__ delta,
-                                                  const uint32_t* __restrict__ sel_bm, int bm_words, int scaled,
-                                                  double mult, double prescale, double* __restrict__ W, int dp,
-                                                  const double* __restrict__ shift_p)
+__global__ __launch_bounds__(256) void k_master_h(FEDNL_KARGS)
 {
+    FEDNL_BLOCK();
+    double* __restrict__ H = PD.H;
+    const size_t S = PD.S;
+    const int n = PD.n, bm_words = PD.bm_words, dp = PD.dp;
+    const int scaled = (PD.comp == FEDNL_RANDK || PD.comp == FEDNL_RANDSEQK) ? 1 : 0;
+    const double* __restrict__ delta = PD.delta;
+    const uint32_t* __restrict__ sel_bm = PD.sel_bm;
+    const double mult = PD.mult, prescale = PD.prescale;
+    double* __restrict__ W = PD.W;
+    const double* __restrict__ shift_p = PD.shift;
     __shared__ uint32_t bw[kMaxClients][kMasterPos / 32];
-    const size_t p0 = (size_t)blockIdx.x * kMasterPos;
+    const size_t p0 = (size_t)blk * kMasterPos;
     const int w0 = (int)(p0 / 32);
     const int nw = min(kMasterPos / 32, bm_words - w0);
     for (int e = threadIdx.x; e < n * (kMasterPos / 32); e += blockDim.x) {
@@ -1617,11 +1715,15 @@ __global__ __launch_bounds__(256) void k_master_h(double* __restrict__ H, size_t
 constexpr int kGradCols = 64;
 constexpr int kGradClients = 64;
 
-__global__ __launch_bounds__(256) void k_master_grad(int n, int d, int d_pad, const double* __restrict__ grad_c,
-                                                     double inv_n, double* __restrict__ g_avg)
+__global__ __launch_bounds__(256) void k_master_grad(FEDNL_KARGS)
 {
+    FEDNL_BLOCK();
+    const int n = PD.n, d = PD.d, d_pad = PD.d_pad;
+    const double* __restrict__ grad_c = PD.grad_c;
+    const double inv_n = PD.inv_n;
+    double* __restrict__ g_avg = PD.g_avg;
     __shared__ double tile[kGradClients][kGradCols + 1];
-    const int i0 = blockIdx.x * kGradCols;
+    const int i0 = blk * kGradCols;
     const int col = threadIdx.x & (kGradCols - 1);
     double s = 0.0;
     for (int c0 = 0; c0 < n; c0 += kGradClients) {
@@ -1672,16 +1774,23 @@ __device__ inline double block_sum(double v, double* scratch)
 // Newton shift, norms for the record (TrainLoop.h:1679-1691), TopLEK stream
 // advance. One workgroup.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_master_vec(int comp, int n, int d, int m, int nlog_blocks,
-                                                    const double* __restrict__ g_avg, const double* __restrict__ part,
-                                                    int tiles_up, double* __restrict__ lk,
-                                                    const double* __restrict__ nlog_part, const double* __restrict__ x,
-                                                    double lambda, double inv_n, int option_b, int tracking,
-                                                    const int* __restrict__ lek_hit, uint32_t* __restrict__ lek_state,
-                                                    int workers, double* __restrict__ shift_out,
-                                                    double* __restrict__ trace, const int* __restrict__ round_ctr,
-                                                    int trace_cap)
+__global__ __launch_bounds__(256) void k_master_vec(FEDNL_KARGS)
 {
+    FEDNL_BLOCK();
+    (void)blk;
+    const int comp = PD.comp, n = PD.n, d = PD.d, m = PD.m, nlog_blocks = PD.m_pad / 16, tiles_up = PD.tiles_up;
+    const double* __restrict__ g_avg = PD.g_avg;
+    const double* __restrict__ part = PD.part;
+    double* __restrict__ lk = PD.lk;
+    const double* __restrict__ nlog_part = PD.nlog;
+    const double* __restrict__ x = PD.x;
+    const double lambda = PD.lambda, inv_n = PD.inv_n;
+    const int option_b = PD.option_b, tracking = PD.tracking, workers = PD.workers, trace_cap = PD.trace_cap;
+    const int* __restrict__ lek_hit = PD.lek_hit;
+    uint32_t* __restrict__ lek_state = PD.lek_state;
+    double* __restrict__ shift_out = PD.shift;
+    double* __restrict__ trace = PD.trace;
+    const int* __restrict__ round_ctr = PD.round_ctr;
     __shared__ double scratch[16];
     __shared__ double fc_s[kMaxClients];
     double gg = 0.0, xx = 0.0;
@@ -1832,13 +1941,19 @@ __device__ inline void tri_tile(int tt, int& ti, int& tj)
     tj = tt - t * (t + 1) / 2;
 }
 
-__global__ __launch_bounds__(kNewtonThreads) void k_newton(int d, int dp, const double* __restrict__ Hup,
-                                                           const double* __restrict__ shift_p,
-                                                           const double* __restrict__ b, double* __restrict__ W,
-                                                           double* __restrict__ x, double* __restrict__ dx_out,
-                                                           double* __restrict__ trace, int* __restrict__ round_ctr,
-                                                           int trace_cap, int update_x)
+__global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
 {
+    FEDNL_BLOCK();
+    (void)blk;
+    const int d = PD.d, dp = PD.dp, trace_cap = PD.trace_cap, update_x = PD.update_x;
+    const double* __restrict__ Hup = PD.hup;
+    const double* __restrict__ shift_p = PD.shift;
+    const double* __restrict__ b = PD.nb;
+    double* __restrict__ W = PD.W;
+    double* __restrict__ x = PD.x;
+    double* __restrict__ dx_out = PD.dx;
+    double* __restrict__ trace = PD.trace;
+    int* __restrict__ round_ctr = PD.round_ctr;
     extern __shared__ __align__(16) double lds[];
     constexpr int kPS = kNB + 1;                    // row stride of the panel in LDS
     double* P21 = lds;                              // rows below the diagonal block: P21[r * kPS + c]
@@ -2228,6 +2343,8 @@ struct Engine {
     size_t lds_select = 0, lds_newton = 0;
     int seg_len = 256;
     int hmode = 0;                 // 0 FP64, 1 FP32, 2 BF16 split (A^T D A only)
+    ProblemDev hdesc{};            // descriptor as last uploaded
+    ProblemDev* pdev = nullptr;    // device copy read by every round kernel
     float* A32 = nullptr;          // FP32 copy of A (mode 1)
     __bf16* AThi = nullptr;        // transposed BF16 hi / lo of A (mode 2)
     __bf16* ATlo = nullptr;
@@ -2301,53 +2418,92 @@ static int ensure_trace(Engine* e, int need)
     return FEDNL_OK;
 }
 
-// client Hessians (and, for the reduced-precision modes, the FP64 client
-// gradients) at the current margin weights; init: write learned instead of Delta
-static void launch_hessian(Engine* e, bool init, double* out_learned)
+// the engine's descriptor as the kernels see it
+static ProblemDev make_desc(const Engine* e)
 {
-    const int n = e->n;
-    const int nblk = 8 * ((n + 7) / 8) * e->tiles_up;
-    double* lrn = out_learned ? out_learned : e->learned;
-    if (e->hmode == 0) {
+    ProblemDev P{};
+    P.n = e->n; P.m = e->m; P.d = e->d; P.k = e->k; P.lda = e->lda; P.m_pad = e->m_pad; P.d_pad = e->d_pad;
+    P.tiles_up = e->tiles_up; P.dp = e->dp; P.workers = e->workers; P.comp = e->o.compressor; P.hmode = e->hmode;
+    P.option_b = e->o.option_b; P.tracking = e->o.tracking; P.seg_len = e->seg_len; P.bm_words = e->bm_words;
+    P.trace_cap = e->trace_cap; P.update_x = 1; P.S = e->S;
+    P.lambda = e->o.lambda; P.alpha = e->alpha; P.prescale = e->prescale; P.inv_n = e->inv_n; P.mult = e->mult;
+    P.lek_delta = e->lek_delta;
+    P.A = e->A; P.A32 = e->A32; P.AThi = e->AThi; P.ATlo = e->ATlo;
+    P.x = e->x; P.Dw = e->Dw; P.gw = e->gw; P.nlog = e->nlog; P.learned = e->learned; P.delta = e->delta;
+    P.grad_c = e->grad_c; P.part = e->part; P.lk = e->lk; P.vals = e->vals; P.H = e->H; P.g_avg = e->g_avg;
+    P.shift = e->shift; P.W = e->W; P.dx = e->dx; P.trace = e->trace; P.lek_p = e->lek_p;
+    P.hup = nullptr; P.nb = e->g_avg;
+    P.sel = e->sel; P.pos = e->pos; P.idx = e->idx; P.rng = e->rng; P.lek_state = e->lek_state;
+    P.lek_popped = e->lek_popped; P.seq_start = e->seq_start; P.sel_bm = e->sel_bm;
+    P.sel_cnt = e->sel_cnt; P.cnt = e->cnt; P.lek_hit = e->lek_hit; P.round_ctr = e->round_ctr;
+    return P;
+}
+
+// uploads the descriptor when a field changed (e.g. the trace buffer grew)
+static int sync_desc(Engine* e)
+{
+    const ProblemDev P = make_desc(e);
+    if (memcmp(&P, &e->hdesc, sizeof(P)) == 0)
+        return FEDNL_OK;
+    e->hdesc = P;
+    HIP_OK(hipMemcpyAsync(e->pdev, &e->hdesc, sizeof(ProblemDev), hipMemcpyHostToDevice, e->st));
+    // the source is a member that the next call may change: finish the copy now
+    HIP_OK(hipStreamSynchronize(e->st));
+    return FEDNL_OK;
+}
+
+// number of blocks each round kernel uses for one problem
+static int blocks_margin(const ProblemDev& P) { return (P.m_pad / 16) * P.n; }
+static int blocks_hessian(const ProblemDev& P) { return 8 * ((P.n + 7) / 8) * P.tiles_up; }
+static int blocks_grad(const ProblemDev& P) { return (P.d_pad / kTile) * P.n; }
+static int blocks_master_grad(const ProblemDev& P) { return (P.d + kGradCols - 1) / kGradCols; }
+static int blocks_master_h(const ProblemDev& P) { return (int)((P.S + kMasterPos - 1) / kMasterPos); }
+
+// client Hessians (and, for the reduced-precision modes, the FP64 client
+// gradients) at the current margin weights; init: write learned (or
+// out_learned) instead of Delta. pd / map / nblk: a descriptor array and
+// block map (batches); null pd means the engine itself.
+static void launch_hessian_on(hipStream_t st, int hmode, bool init, const ProblemDev* pd, const int2* map,
+                              int nblk_h, int nblk_g, double* out_learned)
+{
+    if (hmode == 0) {
         if (init)
-            hipLaunchKernelGGL(k_hessian_f64<true>, dim3(nblk), dim3(256), 0, e->st, e->A, e->Dw, e->gw, e->x, n,
-                               e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta,
-                               e->grad_c, e->part);
+            hipLaunchKernelGGL(k_hessian_f64<true>, dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
         else
-            hipLaunchKernelGGL(k_hessian_f64<false>, dim3(nblk), dim3(256), 0, e->st, e->A, e->Dw, e->gw, e->x, n,
-                               e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta,
-                               e->grad_c, e->part);
+            hipLaunchKernelGGL(k_hessian_f64<false>, dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
         return;
     }
     if (!init)
-        hipLaunchKernelGGL(k_grad_f64, dim3(e->d_pad / kTile, n), dim3(256), 0, e->st, e->A, e->gw, e->x, e->m_pad,
-                           e->d, e->d_pad, e->o.lambda, e->grad_c);
-    if (e->hmode == 1) {
+        hipLaunchKernelGGL(k_grad_f64, dim3(nblk_g), dim3(256), 0, st, pd, map == nullptr ? nullptr : map + nblk_h);
+    if (hmode == 1) {
         if (init)
-            hipLaunchKernelGGL(k_hessian_f32<true>, dim3(nblk), dim3(256), 0, e->st, e->A32, e->Dw, e->x, n,
-                               e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta, e->part);
+            hipLaunchKernelGGL(k_hessian_f32<true>, dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
         else
-            hipLaunchKernelGGL(k_hessian_f32<false>, dim3(nblk), dim3(256), 0, e->st, e->A32, e->Dw, e->x, n,
-                               e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta, e->part);
+            hipLaunchKernelGGL(k_hessian_f32<false>, dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
     } else {
         if (init)
-            hipLaunchKernelGGL(k_hessian_bf16<true>, dim3(nblk), dim3(256), 0, e->st, e->AThi, e->ATlo, e->Dw, e->x,
-                               n, e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta, e->part);
+            hipLaunchKernelGGL(k_hessian_bf16<true>, dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
         else
-            hipLaunchKernelGGL(k_hessian_bf16<false>, dim3(nblk), dim3(256), 0, e->st, e->AThi, e->ATlo, e->Dw,
-                               e->x, n, e->m_pad, e->d, e->d_pad, e->tiles_up, e->o.lambda, e->S, lrn, e->delta,
-                               e->part);
+            hipLaunchKernelGGL(k_hessian_bf16<false>, dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
     }
+}
+
+static void launch_hessian(Engine* e, bool init, double* out_learned, const ProblemDev* pd = nullptr)
+{
+    const ProblemDev& P = e->hdesc;
+    launch_hessian_on(e->st, e->hmode, init, pd ? pd : e->pdev, nullptr, blocks_hessian(P), blocks_grad(P),
+                      out_learned);
 }
 
 static int launch_round(Engine* e)
 {
+    const ProblemDev& P = e->hdesc;
+    const ProblemDev* pd = e->pdev;
+    const int2* nomap = nullptr;
     const int n = e->n;
     {
         KTime kt(e, KID_MARGIN);
-        dim3 grid(e->m_pad / 16, n);
-        hipLaunchKernelGGL(k_margin, grid, dim3(256), sizeof(double) * e->d_pad, e->st, e->A, e->x, e->m, e->m_pad,
-                           e->d, e->d_pad, 1.0 / (double)e->m, e->Dw, e->gw, e->nlog);
+        hipLaunchKernelGGL(k_margin, dim3(blocks_margin(P)), dim3(256), sizeof(double) * e->d_pad, e->st, pd, nomap);
     }
     {
         KTime kt(e, KID_HESS);
@@ -2356,52 +2512,32 @@ static int launch_round(Engine* e)
     const int comp = e->o.compressor;
     {
         KTime kt(e, KID_SELECT);
-        if (comp == FEDNL_TOPK) {
-            hipLaunchKernelGGL(k_select_heap<false>, dim3(n), dim3(256), e->lds_select, e->st, e->delta, e->S, e->k,
-                               e->seg_len, e->part, e->tiles_up, e->lek_delta, (double*)nullptr, e->sel, e->sel_cnt, e->lek_p, e->lek_hit,
-                               e->lek_popped);
-        } else if (comp == FEDNL_TOPLEK) {
-            hipLaunchKernelGGL(k_select_heap<true>, dim3(n), dim3(256), e->lds_select, e->st, e->delta, e->S, e->k,
-                               e->seg_len, e->part, e->tiles_up, e->lek_delta, (double*)nullptr, e->sel, e->sel_cnt, e->lek_p, e->lek_hit,
-                               e->lek_popped);
-        } else {
-            if (comp == FEDNL_RANDK)
-                hipLaunchKernelGGL(k_select_randk, dim3(n), dim3(256), e->lds_select, e->st, e->S, e->k, e->rng,
-                                   e->sel, e->sel_cnt);
-            else
-                hipLaunchKernelGGL(k_select_window, dim3(n), dim3(256), 0, e->st, comp, e->S, e->k, e->rng, e->sel,
-                                   e->sel_cnt, e->seq_start);
-        }
+        if (comp == FEDNL_TOPK)
+            hipLaunchKernelGGL(k_select_heap<false>, dim3(n), dim3(256), e->lds_select, e->st, pd, nomap);
+        else if (comp == FEDNL_TOPLEK)
+            hipLaunchKernelGGL(k_select_heap<true>, dim3(n), dim3(256), e->lds_select, e->st, pd, nomap);
+        else if (comp == FEDNL_RANDK)
+            hipLaunchKernelGGL(k_select_randk, dim3(n), dim3(256), e->lds_select, e->st, pd, nomap);
+        else
+            hipLaunchKernelGGL(k_select_window, dim3(n), dim3(256), 0, e->st, pd, nomap);
     }
     {
         KTime kt(e, KID_EMIT);
-        hipLaunchKernelGGL(k_emit, dim3(n), dim3(256), sizeof(uint32_t) * e->bm_words, e->st, comp, e->delta, e->S,
-                           e->k, e->lda, e->sel, e->sel_cnt, e->lek_p, e->lek_hit, e->lek_popped, e->lek_state,
-                           e->workers, e->alpha, e->prescale, e->mult, e->learned, e->pos, e->idx, e->vals, e->cnt,
-                           e->sel_bm, e->bm_words);
+        hipLaunchKernelGGL(k_emit, dim3(n), dim3(256), sizeof(uint32_t) * e->bm_words, e->st, pd, nomap);
     }
     {
         KTime kt(e, KID_MASTER_VEC);
-        hipLaunchKernelGGL(k_master_grad, dim3((e->d + kGradCols - 1) / kGradCols), dim3(256), 0, e->st, n, e->d,
-                           e->d_pad, e->grad_c, e->inv_n, e->g_avg);
-        hipLaunchKernelGGL(k_master_vec, dim3(1), dim3(256), 0, e->st, comp, n, e->d, e->m, e->m_pad / 16, e->g_avg,
-                           e->part, e->tiles_up, e->lk, e->nlog, e->x, e->o.lambda, e->inv_n, e->o.option_b,
-                           e->o.tracking, e->lek_hit, e->lek_state, e->workers, e->shift, e->trace, e->round_ctr,
-                           e->trace_cap);
+        hipLaunchKernelGGL(k_master_grad, dim3(blocks_master_grad(P)), dim3(256), 0, e->st, pd, nomap);
+        hipLaunchKernelGGL(k_master_vec, dim3(1), dim3(256), 0, e->st, pd, nomap);
     }
     {
         // after the shift is known: H update and the Newton workspace in one pass
         KTime kt(e, KID_MASTER_H);
-        const int nblk = (int)((e->S + kMasterPos - 1) / kMasterPos);
-        const int scaled = (comp == FEDNL_RANDK || comp == FEDNL_RANDSEQK) ? 1 : 0;
-        hipLaunchKernelGGL(k_master_h, dim3(nblk), dim3(256), 0, e->st, e->H, e->S, n, e->delta, e->sel_bm,
-                           e->bm_words, scaled, e->mult, e->prescale, e->W, e->dp, (const double*)e->shift);
+        hipLaunchKernelGGL(k_master_h, dim3(blocks_master_h(P)), dim3(256), 0, e->st, pd, nomap);
     }
     {
         KTime kt(e, KID_NEWTON);
-        hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), e->lds_newton, e->st, e->d, e->dp,
-                           (const double*)nullptr, e->shift, e->g_avg, e->W, e->x, e->dx, e->trace, e->round_ctr,
-                           e->trace_cap, 1);
+        hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), e->lds_newton, e->st, pd, nomap);
     }
     return FEDNL_OK;
 }
@@ -2434,7 +2570,7 @@ static void free_engine(Engine* e)
     void* ptrs[] = {e->A, e->x, e->Dw, e->gw, e->nlog, e->learned, e->delta, e->grad_c, e->part, e->lk, e->sel_bm,
                     e->sel, e->pos, e->idx, e->rng, e->lek_state, e->lek_popped, e->seq_start, e->sel_cnt,
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
-                    e->dx, e->trace, e->A32, e->AThi, e->ATlo};
+                    e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -2663,10 +2799,14 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     hipMemsetAsync(e->grad_c, 0, sizeof(double) * (size_t)n * e->d_pad, e->st);
 
     // learned_c = H_c(x0), H = mean_c learned_c
+    if (hipMalloc((void**)&e->pdev, sizeof(ProblemDev)) != hipSuccess || sync_desc(e) != FEDNL_OK) {
+        set_err("descriptor upload (create)");
+        free_engine(e);
+        return FEDNL_E_DEVICE;
+    }
     {
-        dim3 grid(e->m_pad / 16, n);
-        hipLaunchKernelGGL(k_margin, grid, dim3(256), sizeof(double) * e->d_pad, e->st, e->A, e->x, e->m, e->m_pad,
-                           e->d, e->d_pad, 1.0 / (double)e->m, e->Dw, e->gw, e->nlog);
+        hipLaunchKernelGGL(k_margin, dim3(blocks_margin(e->hdesc)), dim3(256), sizeof(double) * e->d_pad, e->st,
+                           (const ProblemDev*)e->pdev, (const int2*)nullptr);
         launch_hessian(e, true, nullptr);
         hipLaunchKernelGGL(k_init_H, dim3((unsigned)((e->S + 255) / 256)), dim3(256), 0, e->st, e->H, e->learned,
                            e->S, n, e->inv_n);
@@ -2692,6 +2832,8 @@ static int enqueue_rounds(Engine* e, int rounds)
         return FEDNL_OK;
     int rc = ensure_trace(e, e->rounds_enqueued + rounds + 1);
     if (rc != FEDNL_OK)
+        return rc;
+    if ((rc = sync_desc(e)) != FEDNL_OK)
         return rc;
     if (!e->ev_start) {
         HIP_OK(hipEventCreate(&e->ev_start));
@@ -3025,30 +3167,70 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
         hipMemcpyAsync(parts, hp.data(), sizeof(double) * hp.size(), hipMemcpyHostToDevice, st);
         hipStreamSynchronize(st);
     }
-    if (compressor == FEDNL_TOPK)
-        hipLaunchKernelGGL(k_select_heap<false>, dim3(nclients), dim3(256), lds, st, ddelta, S, k, seg_len, (const double*)nullptr,
-                           0, delta_param, (double*)nullptr, sel, sel_cnt, lek_p, hit, popped);
-    else if (compressor == FEDNL_TOPLEK)
-        hipLaunchKernelGGL(k_select_heap<true>, dim3(nclients), dim3(256), lds, st, ddelta, S, k, seg_len, parts, 1,
-                           delta_param, (double*)nullptr, sel, sel_cnt, lek_p, hit, popped);
-    else if (compressor == FEDNL_RANDK)
-        hipLaunchKernelGGL(k_select_randk, dim3(nclients), dim3(256), lds, st, S, k, drng, sel, sel_cnt);
-    else if (compressor == FEDNL_RANDSEQK || compressor == FEDNL_IDENTICAL)
-        hipLaunchKernelGGL(k_select_window, dim3(nclients), dim3(256), 0, st, compressor, S, k, drng, sel, sel_cnt,
-                           (uint32_t*)nullptr);
-    else {
-        cleanup();
-        return FEDNL_E_UNSUPPORTED;
-    }
     const int bm_words = (int)((S + 31) / 32);
     uint32_t* bm = nullptr;
     if (dalloc(&bm, (size_t)nclients * bm_words) != hipSuccess) {
         cleanup();
         return FEDNL_E_DEVICE;
     }
-    hipLaunchKernelGGL(k_emit, dim3(nclients), dim3(256), sizeof(uint32_t) * bm_words, st, compressor, ddelta, S, k,
-                       round_up(d, 4), sel, sel_cnt, lek_p, hit, popped, dlek, workers, 1.0, 1.0, 1.0, learned, pos,
-                       idx, vals, cnt, bm, bm_words);
+    // descriptor of the standalone call: delta as given, unit scales
+    ProblemDev hp{};
+    hp.n = nclients;
+    hp.d = d;
+    hp.k = k;
+    hp.S = S;
+    hp.lda = round_up(d, 4);
+    hp.seg_len = seg_len;
+    hp.tiles_up = (compressor == FEDNL_TOPLEK) ? 1 : 0;
+    hp.part = parts;
+    hp.lek_delta = delta_param;
+    hp.comp = compressor;
+    hp.workers = workers;
+    hp.alpha = 1.0;
+    hp.prescale = 1.0;
+    hp.mult = 1.0;
+    hp.delta = ddelta;
+    hp.learned = learned;
+    hp.sel = sel;
+    hp.sel_cnt = sel_cnt;
+    hp.lek_p = lek_p;
+    hp.lek_hit = hit;
+    hp.lek_popped = popped;
+    hp.lek_state = dlek;
+    hp.rng = drng;
+    hp.seq_start = nullptr;
+    hp.pos = pos;
+    hp.idx = idx;
+    hp.vals = vals;
+    hp.cnt = cnt;
+    hp.sel_bm = bm;
+    hp.bm_words = bm_words;
+    ProblemDev* dpd = nullptr;
+    if (hipMalloc((void**)&dpd, sizeof(ProblemDev)) != hipSuccess ||
+        hipMemcpyAsync(dpd, &hp, sizeof(ProblemDev), hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_err("descriptor upload (select)");
+        if (dpd) hipFree(dpd);
+        hipFree(bm);
+        cleanup();
+        return FEDNL_E_DEVICE;
+    }
+    const ProblemDev* pd = dpd;
+    const int2* nomap = nullptr;
+    if (compressor == FEDNL_TOPK)
+        hipLaunchKernelGGL(k_select_heap<false>, dim3(nclients), dim3(256), lds, st, pd, nomap);
+    else if (compressor == FEDNL_TOPLEK)
+        hipLaunchKernelGGL(k_select_heap<true>, dim3(nclients), dim3(256), lds, st, pd, nomap);
+    else if (compressor == FEDNL_RANDK)
+        hipLaunchKernelGGL(k_select_randk, dim3(nclients), dim3(256), lds, st, pd, nomap);
+    else if (compressor == FEDNL_RANDSEQK || compressor == FEDNL_IDENTICAL)
+        hipLaunchKernelGGL(k_select_window, dim3(nclients), dim3(256), 0, st, pd, nomap);
+    else {
+        hipFree(dpd);
+        hipFree(bm);
+        cleanup();
+        return FEDNL_E_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(k_emit, dim3(nclients), dim3(256), sizeof(uint32_t) * bm_words, st, pd, nomap);
     if (compressor == FEDNL_TOPLEK) {
         // advance the streams as the master kernel does
         std::vector<int> hh(nclients);
@@ -3077,6 +3259,7 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     }
     if (parts) hipFree(parts);
     hipFree(bm);
+    hipFree(dpd);
     cleanup();
     return rc;
 }
@@ -3156,14 +3339,18 @@ int fednl_hessians_at(void* handle, const double* x, double* out)
     hipError_t err = hipStreamSynchronize(e->st);
     if (err == hipSuccess)
         err = hipMemcpyAsync(dxv, hx.data(), sizeof(double) * e->d_pad, hipMemcpyHostToDevice, e->st);
+    ProblemDev* dpd = nullptr;
+    ProblemDev hp = make_desc(e);
+    hp.x = dxv;
+    if (err == hipSuccess)
+        err = hipMalloc((void**)&dpd, sizeof(ProblemDev));
+    if (err == hipSuccess)
+        err = hipMemcpy(dpd, &hp, sizeof(ProblemDev), hipMemcpyHostToDevice);
     if (err == hipSuccess) {
         // the margin weights of x, then the Hessians in the engine's mode
-        double* keep_x = e->x;
-        e->x = dxv;
-        hipLaunchKernelGGL(k_margin, dim3(e->m_pad / 16, e->n), dim3(256), sizeof(double) * e->d_pad, e->st, e->A,
-                           e->x, e->m, e->m_pad, e->d, e->d_pad, 1.0 / (double)e->m, e->Dw, e->gw, e->nlog);
-        launch_hessian(e, true, dout);
-        e->x = keep_x;
+        hipLaunchKernelGGL(k_margin, dim3(blocks_margin(hp)), dim3(256), sizeof(double) * e->d_pad, e->st,
+                           (const ProblemDev*)dpd, (const int2*)nullptr);
+        launch_hessian(e, true, dout, dpd);
         err = hipGetLastError();
     }
     if (err == hipSuccess)
@@ -3172,6 +3359,8 @@ int fednl_hessians_at(void* handle, const double* x, double* out)
         err = hipMemcpy(out, dout, sizeof(double) * (size_t)e->n * e->S, hipMemcpyDeviceToHost);
     hipFree(dxv);
     hipFree(dout);
+    if (dpd)
+        hipFree(dpd);
     if (err != hipSuccess) {
         set_err("fednl_hessians_at", err);
         return FEDNL_E_DEVICE;
@@ -3211,14 +3400,29 @@ int fednl_newton_solve(int d, const double* H_up, double shift, const double* b,
     const size_t lds = newton_lds_bytes(dp);
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k_newton, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), lds, 0, d, dp, dH, dsh, db, dW, (double*)nullptr, dx_d, dtr, dctr,
-                       1, 0);
+    ProblemDev hp{};
+    hp.d = d;
+    hp.dp = dp;
+    hp.trace_cap = 1;
+    hp.update_x = 0;
+    hp.hup = dH;
+    hp.shift = dsh;
+    hp.nb = db;
+    hp.W = dW;
+    hp.x = nullptr;
+    hp.dx = dx_d;
+    hp.trace = dtr;
+    hp.round_ctr = dctr;
+    ProblemDev* dpd = nullptr;
+    HIP_OK(hipMalloc((void**)&dpd, sizeof(ProblemDev)));
+    HIP_OK(hipMemcpy(dpd, &hp, sizeof(ProblemDev), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), lds, 0, (const ProblemDev*)dpd, (const int2*)nullptr);
     HIP_OK(hipGetLastError());
     HIP_OK(hipDeviceSynchronize());
     double tr[4];
     HIP_OK(hipMemcpy(tr, dtr, sizeof(tr), hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(dx, dx_d, sizeof(double) * d, hipMemcpyDeviceToHost));
-    hipFree(dH); hipFree(dsh); hipFree(db); hipFree(dW); hipFree(dx_d); hipFree(dtr); hipFree(dctr);
+    hipFree(dH); hipFree(dsh); hipFree(db); hipFree(dW); hipFree(dx_d); hipFree(dtr); hipFree(dctr); hipFree(dpd);
     return tr[3] == 1.0 ? 1 : 0;
 }
 
