@@ -151,6 +151,20 @@ int fednl_newton_solve(int d, const double *H_up, double shift, const double *b,
  * No counterpart in the reference. */
 int fednl_probe_mfma_f64(double *out768);
 
+/* Batches: independent problems stepped together, each round kernel launched
+ * once for all members (one workgroup grid over a member/block map), the
+ * round optionally captured once into a HIP graph (use_graph) and replayed.
+ * Members keep their own state, trace and results (fednl_trace, fednl_get_x
+ * ... on each engine); while in a batch an engine must not be stepped on its
+ * own. fednl_batch_sync / fednl_batch_run return after all enqueued rounds.
+ * No single counterpart in the reference: dopt runs one problem per process
+ * (TrainLoop.h:103), this replaces running many bin_fednl_local instances. */
+int fednl_batch_create(void *const *engines, int count, int use_graph, void **batch);
+void fednl_batch_destroy(void *batch);
+int fednl_batch_enqueue(void *batch, int rounds);
+int fednl_batch_sync(void *batch);
+int fednl_batch_run(void *batch, int rounds);
+
 /* Client Hessians H_c(x) for all clients (n*S packed upper, host), computed
  * by the engine's Hessian mode at the given x (d values); the engine's
  * iterate and learned Hessians are unchanged. Used to state the error of the

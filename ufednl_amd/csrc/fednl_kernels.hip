@@ -2851,6 +2851,207 @@ static int enqueue_rounds(Engine* e, int rounds)
     return FEDNL_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batch: many independent problems (engines) stepped together. Every kernel
+// of a round is launched once for all members, over a block map that names
+// (member, local block) for each workgroup, so small problems share the chip:
+// e.g. the members' Newton solves run as concurrent workgroups. A round can be
+// captured once into a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+enum BatchKernel {
+    BK_MARGIN, BK_HESS64, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
+    BK_EMIT, BK_MGRAD, BK_MVEC, BK_MH, BK_NEWTON, BK_COUNT
+};
+
+struct Batch {
+    std::vector<Engine*> m;
+    hipStream_t st = nullptr;
+    ProblemDev* d_probs = nullptr;
+    std::vector<ProblemDev> h_probs;
+    int2* d_maps = nullptr;
+    int off[BK_COUNT] = {0}, cnt[BK_COUNT] = {0};
+    size_t lds_margin = 0, lds_select = 0, lds_emit = 0, lds_newton = 0;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int rounds = 0;
+};
+
+static void free_batch(Batch* b)
+{
+    if (!b)
+        return;
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->d_probs) hipFree(b->d_probs);
+    if (b->d_maps) hipFree(b->d_maps);
+    if (b->st) hipStreamDestroy(b->st);
+    delete b;
+}
+
+static int batch_build(Batch* b)
+{
+    std::vector<int2> maps[BK_COUNT];
+    auto add = [&](int kind, int p, int nb, bool align8) {
+        auto& v = maps[kind];
+        if (align8)
+            while (v.size() % 8) v.push_back(make_int2(-1, 0));
+        for (int q = 0; q < nb; ++q) v.push_back(make_int2(p, q));
+    };
+    b->h_probs.clear();
+    for (size_t i = 0; i < b->m.size(); ++i) {
+        Engine* e = b->m[i];
+        const ProblemDev P = make_desc(e);
+        b->h_probs.push_back(P);
+        const int p = (int)i;
+        add(BK_MARGIN, p, blocks_margin(P), false);
+        add(e->hmode == 0 ? BK_HESS64 : (e->hmode == 1 ? BK_HESS32 : BK_HESS16), p, blocks_hessian(P), true);
+        if (e->hmode != 0)
+            add(BK_GRAD, p, blocks_grad(P), false);
+        const int comp = e->o.compressor;
+        add(comp == FEDNL_TOPK ? BK_SEL_TOPK
+                               : comp == FEDNL_TOPLEK ? BK_SEL_TOPLEK : comp == FEDNL_RANDK ? BK_SEL_RANDK : BK_SEL_WINDOW,
+            p, e->n, false);
+        add(BK_EMIT, p, e->n, false);
+        add(BK_MGRAD, p, blocks_master_grad(P), false);
+        add(BK_MVEC, p, 1, false);
+        add(BK_MH, p, blocks_master_h(P), false);
+        add(BK_NEWTON, p, 1, false);
+        b->lds_margin = std::max(b->lds_margin, sizeof(double) * (size_t)e->d_pad);
+        b->lds_select = std::max(b->lds_select, e->lds_select);
+        b->lds_emit = std::max(b->lds_emit, sizeof(uint32_t) * (size_t)e->bm_words);
+        b->lds_newton = std::max(b->lds_newton, e->lds_newton);
+    }
+    std::vector<int2> all;
+    for (int k = 0; k < BK_COUNT; ++k) {
+        // every segment starts at a multiple of 8 entries
+        while (all.size() % 8) all.push_back(make_int2(-1, 0));
+        b->off[k] = (int)all.size();
+        b->cnt[k] = (int)maps[k].size();
+        all.insert(all.end(), maps[k].begin(), maps[k].end());
+    }
+    HIP_OK(hipMalloc((void**)&b->d_probs, sizeof(ProblemDev) * std::max<size_t>(1, b->h_probs.size())));
+    HIP_OK(hipMalloc((void**)&b->d_maps, sizeof(int2) * std::max<size_t>(1, all.size())));
+    HIP_OK(hipMemcpy(b->d_probs, b->h_probs.data(), sizeof(ProblemDev) * b->h_probs.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(b->d_maps, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice));
+    return FEDNL_OK;
+}
+
+// the members' descriptors again (trace buffers may have grown)
+static int batch_sync_desc(Batch* b)
+{
+    bool changed = false;
+    for (size_t i = 0; i < b->m.size(); ++i) {
+        const ProblemDev P = make_desc(b->m[i]);
+        if (memcmp(&P, &b->h_probs[i], sizeof(P)) != 0) {
+            b->h_probs[i] = P;
+            changed = true;
+        }
+    }
+    if (changed) {
+        HIP_OK(hipMemcpyAsync(b->d_probs, b->h_probs.data(), sizeof(ProblemDev) * b->h_probs.size(),
+                              hipMemcpyHostToDevice, b->st));
+        HIP_OK(hipStreamSynchronize(b->st));
+    }
+    return FEDNL_OK;
+}
+
+static void batch_launch_round(Batch* b)
+{
+    const ProblemDev* pd = b->d_probs;
+    const int2* mp = b->d_maps;
+    hipStream_t st = b->st;
+    auto map = [&](int k) { return mp + b->off[k]; };
+    if (b->cnt[BK_MARGIN])
+        hipLaunchKernelGGL(k_margin, dim3(b->cnt[BK_MARGIN]), dim3(256), b->lds_margin, st, pd, map(BK_MARGIN));
+    if (b->cnt[BK_HESS64])
+        hipLaunchKernelGGL(k_hessian_f64<false>, dim3(b->cnt[BK_HESS64]), dim3(256), 0, st, pd, map(BK_HESS64),
+                           (double*)nullptr);
+    if (b->cnt[BK_GRAD])
+        hipLaunchKernelGGL(k_grad_f64, dim3(b->cnt[BK_GRAD]), dim3(256), 0, st, pd, map(BK_GRAD));
+    if (b->cnt[BK_HESS32])
+        hipLaunchKernelGGL(k_hessian_f32<false>, dim3(b->cnt[BK_HESS32]), dim3(256), 0, st, pd, map(BK_HESS32),
+                           (double*)nullptr);
+    if (b->cnt[BK_HESS16])
+        hipLaunchKernelGGL(k_hessian_bf16<false>, dim3(b->cnt[BK_HESS16]), dim3(256), 0, st, pd, map(BK_HESS16),
+                           (double*)nullptr);
+    if (b->cnt[BK_SEL_TOPK])
+        hipLaunchKernelGGL(k_select_heap<false>, dim3(b->cnt[BK_SEL_TOPK]), dim3(256), b->lds_select, st, pd,
+                           map(BK_SEL_TOPK));
+    if (b->cnt[BK_SEL_TOPLEK])
+        hipLaunchKernelGGL(k_select_heap<true>, dim3(b->cnt[BK_SEL_TOPLEK]), dim3(256), b->lds_select, st, pd,
+                           map(BK_SEL_TOPLEK));
+    if (b->cnt[BK_SEL_RANDK])
+        hipLaunchKernelGGL(k_select_randk, dim3(b->cnt[BK_SEL_RANDK]), dim3(256), b->lds_select, st, pd,
+                           map(BK_SEL_RANDK));
+    if (b->cnt[BK_SEL_WINDOW])
+        hipLaunchKernelGGL(k_select_window, dim3(b->cnt[BK_SEL_WINDOW]), dim3(256), 0, st, pd, map(BK_SEL_WINDOW));
+    hipLaunchKernelGGL(k_emit, dim3(b->cnt[BK_EMIT]), dim3(256), b->lds_emit, st, pd, map(BK_EMIT));
+    hipLaunchKernelGGL(k_master_grad, dim3(b->cnt[BK_MGRAD]), dim3(256), 0, st, pd, map(BK_MGRAD));
+    hipLaunchKernelGGL(k_master_vec, dim3(b->cnt[BK_MVEC]), dim3(256), 0, st, pd, map(BK_MVEC));
+    hipLaunchKernelGGL(k_master_h, dim3(b->cnt[BK_MH]), dim3(256), 0, st, pd, map(BK_MH));
+    hipLaunchKernelGGL(k_newton, dim3(b->cnt[BK_NEWTON]), dim3(kNewtonThreads), b->lds_newton, st, pd,
+                       map(BK_NEWTON));
+}
+
+static int batch_enqueue(Batch* b, int rounds)
+{
+    if (rounds <= 0 || b->m.empty())
+        return FEDNL_OK;
+    // a trace buffer that grows is copied on the member's stream: the batch
+    // stream must not be writing the old one meanwhile
+    for (Engine* e : b->m)
+        if (e->rounds_enqueued + rounds + 1 > e->trace_cap) {
+            HIP_OK(hipStreamSynchronize(b->st));
+            break;
+        }
+    for (Engine* e : b->m) {
+        int rc = ensure_trace(e, e->rounds_enqueued + rounds + 1);
+        if (rc != FEDNL_OK)
+            return rc;
+        if (!e->ev_start) {
+            HIP_OK(hipEventCreate(&e->ev_start));
+            HIP_OK(hipEventRecord(e->ev_start, b->st));
+        }
+    }
+    int rc = batch_sync_desc(b);
+    if (rc != FEDNL_OK)
+        return rc;
+    if (b->use_graph && !b->gexec) {
+        HIP_OK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        batch_launch_round(b);
+        hipError_t ce = hipStreamEndCapture(b->st, &b->graph);
+        if (ce == hipSuccess)
+            ce = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) {
+            // fall back to plain launches on this stream
+            b->use_graph = false;
+            if (b->graph) {
+                hipGraphDestroy(b->graph);
+                b->graph = nullptr;
+            }
+            b->gexec = nullptr;
+            hipGetLastError();
+        }
+    }
+    for (int r = 0; r < rounds; ++r) {
+        if (b->gexec)
+            HIP_OK(hipGraphLaunch(b->gexec, b->st));
+        else
+            batch_launch_round(b);
+        for (Engine* e : b->m) {
+            hipEvent_t ev = nullptr;
+            HIP_OK(hipEventCreate(&ev));
+            HIP_OK(hipEventRecord(ev, b->st));
+            e->round_ev.push_back(ev);
+            e->rounds_enqueued += 1;
+        }
+        b->rounds += 1;
+    }
+    HIP_OK(hipGetLastError());
+    return FEDNL_OK;
+}
+
 }  // namespace fednl
 
 using namespace fednl;
@@ -2941,6 +3142,76 @@ int fednl_run(void* handle, int rounds)
             return FEDNL_E_INTERNAL;
         }
     return FEDNL_OK;
+}
+
+int fednl_batch_create(void* const* engines, int count, int use_graph, void** batch)
+{
+    if (!engines || count <= 0 || !batch) {
+        set_err("fednl_batch_create: empty batch");
+        return FEDNL_E_MISSING_ARGUMENT;
+    }
+    Batch* b = new Batch();
+    b->use_graph = use_graph != 0;
+    for (int i = 0; i < count; ++i) {
+        Engine* e = (Engine*)engines[i];
+        if (!e) {
+            delete b;
+            set_err("fednl_batch_create: null engine");
+            return FEDNL_E_MISSING_ARGUMENT;
+        }
+        // the member's own queue (creation, earlier rounds) must be done
+        if (hipStreamSynchronize(e->st) != hipSuccess || collect_timing(e) != FEDNL_OK) {
+            delete b;
+            set_err("fednl_batch_create: member stream");
+            return FEDNL_E_DEVICE;
+        }
+        b->m.push_back(e);
+    }
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        set_err("hipStreamCreate (batch)");
+        return FEDNL_E_DEVICE;
+    }
+    int rc = batch_build(b);
+    if (rc != FEDNL_OK) {
+        free_batch(b);
+        return rc;
+    }
+    *batch = b;
+    return FEDNL_OK;
+}
+
+void fednl_batch_destroy(void* batch)
+{
+    Batch* b = (Batch*)batch;
+    if (b && b->st)
+        hipStreamSynchronize(b->st);
+    free_batch(b);
+}
+
+int fednl_batch_enqueue(void* batch, int rounds)
+{
+    Batch* b = (Batch*)batch;
+    if (!b)
+        return FEDNL_E_MISSING_ARGUMENT;
+    return batch_enqueue(b, rounds);
+}
+
+int fednl_batch_sync(void* batch)
+{
+    Batch* b = (Batch*)batch;
+    if (!b)
+        return FEDNL_E_MISSING_ARGUMENT;
+    HIP_OK(hipStreamSynchronize(b->st));
+    return FEDNL_OK;
+}
+
+int fednl_batch_run(void* batch, int rounds)
+{
+    int rc = fednl_batch_enqueue(batch, rounds);
+    if (rc != FEDNL_OK)
+        return rc;
+    return fednl_batch_sync(batch);
 }
 
 int fednl_trace(void* handle, double* grad_norm, double* fval, double* x_norm, double* seconds, int* chol_ok,

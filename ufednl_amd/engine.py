@@ -74,6 +74,12 @@ def lib():
                                    ctypes.c_int, ctypes.c_double, _c_up, _c_ip]
         L.fednl_newton_solve.argtypes = [ctypes.c_int, _c_dp, ctypes.c_double, _c_dp, _c_dp]
         L.fednl_hessians_at.argtypes = [ctypes.c_void_p, _c_dp, _c_dp]
+        L.fednl_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]
+        L.fednl_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.fednl_batch_enqueue.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.fednl_batch_sync.argtypes = [ctypes.c_void_p]
+        L.fednl_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
         fp = ctypes.POINTER(ctypes.c_float)
         L.fednl_probe_mfma_mixed.argtypes = [fp, fp, fp, fp, fp, fp]
         _lib = L
@@ -234,6 +240,39 @@ def newton_solve(H_up, d, shift, b):
     dx = np.zeros(d)
     ok = _err(lib().fednl_newton_solve(d, dp(H_up), shift, dp(b), dp(dx)), "fednl_newton_solve")
     return bool(ok), dx
+
+
+class Batch:
+    """Independent engines stepped together: each round kernel is launched
+    once for every member (optionally one captured HIP graph per round)."""
+
+    def __init__(self, engines, use_graph=True):
+        self.engines = list(engines)
+        arr = (ctypes.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        h = ctypes.c_void_p()
+        _err(lib().fednl_batch_create(arr, len(self.engines), 1 if use_graph else 0, ctypes.byref(h)),
+             "fednl_batch_create")
+        self.h = h
+
+    def run(self, rounds: int):
+        _err(lib().fednl_batch_run(self.h, rounds), "fednl_batch_run")
+
+    def enqueue(self, rounds: int):
+        _err(lib().fednl_batch_enqueue(self.h, rounds), "fednl_batch_enqueue")
+
+    def sync(self):
+        _err(lib().fednl_batch_sync(self.h), "fednl_batch_sync")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().fednl_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def probe_mfma_mixed(a32, b32, abf, bbf):
