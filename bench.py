@@ -105,6 +105,144 @@ def kernel_roofline(eng, times, traffic):
                       traffic=traffic.get(name)), avg_s
 
 
+def time_to_tol(eng, rounds):
+    """Runs `rounds` rounds; returns (round index, seconds) of the first
+    ||grad f||^2 <= TOL (None if not reached) and the round rate."""
+    eng.run(rounds, check=False)
+    tr = eng.trace()
+    g2 = tr["grad_norm"] ** 2
+    hit = np.nonzero(g2 <= TOL)[0]
+    r = int(hit[0]) if hit.size else None
+    sec = tr["seconds"]
+    rate = (len(sec) - 1) / (sec[-1] - sec[0]) if len(sec) > 1 and sec[-1] > sec[0] else None
+    return r, (float(sec[r]) if r is not None else None), rate
+
+
+def hessian_rel_error(eng_mode, eng_ref, x):
+    """Max over clients of ||H_mode(x) - H_fp64(x)||_F / ||H_fp64(x)||_F."""
+    a = eng_mode.hessians_at(x)
+    b = eng_ref.hessians_at(x)
+    d = eng_ref.d
+    diag = np.array([j * (j + 3) // 2 for j in range(d)])
+
+    def fro(v):
+        return np.sqrt(2.0 * np.sum(v * v, axis=1) - np.sum(v[:, diag] ** 2, axis=1))
+
+    return float(np.max(fro(a - b) / fro(b)))
+
+
+def mixed_precision_leg(rounds_c2, rounds_c3):
+    """FP64 / FP32 / BF16-split A^T D A on the 142 x 350, d=300 TopK workload
+    and on the 80 x 407, d=123 RandSeqK / TopLEK workload."""
+    from ufednl_amd import datagen
+    from ufednl_amd.engine import Engine
+    out = {}
+    for wname, prob, comp, rounds in (("n142_m350_d300_topk", datagen.config2(), "topk", rounds_c2),
+                                      ("n80_m407_d123_seqk", datagen.config3(), "seqk", rounds_c3),
+                                      ("n80_m407_d123_toplek", datagen.config3(), "toplek", rounds_c3)):
+        res = {}
+        ref = None
+        for mode in ("fp64", "fp32", "bf16x2"):
+            eng = Engine.from_problem(prob, compressor=comp, k_mult=8.0, hessian=mode, tracking=False)
+            r, t, rate = time_to_tol(eng, rounds)
+            eng.kernel_timing(True)
+            eng.run(5, check=False)
+            kt = eng.kernel_times()
+            eng.kernel_timing(False)
+            # the Hessian stage: the contraction of the mode (plus, below FP64, the FP64 gradient pass)
+            entry = dict(rounds_to_tol=r, time_to_tol_s=t, rounds_per_s=rate,
+                         hessian_stage_us=1e3 * kt["k_hessian_f64"][0] / max(kt["k_hessian_f64"][1], 1))
+            if mode == "fp64":
+                ref = eng
+                entry["x_final"] = eng.x()
+            else:
+                xs = [np.zeros(prob.features.shape[2]), res["fp64"]["x_final"]]
+                entry["hessian_rel_err_max"] = max(hessian_rel_error(eng, ref, x) for x in xs)
+                entry["final_x_max_abs_diff_vs_fp64"] = float(np.max(np.abs(eng.x() - res["fp64"]["x_final"])))
+                eng.close()
+            res[mode] = entry
+        ref.close()
+        for v in res.values():
+            v.pop("x_final", None)
+        base = res["fp64"]["time_to_tol_s"]
+        for mode in ("fp32", "bf16x2"):
+            t = res[mode]["time_to_tol_s"]
+            res[mode]["time_to_tol_speedup_vs_fp64"] = (base / t) if (base and t) else None
+        out[wname] = res
+    return out
+
+
+def batched_leg(count, rounds):
+    """count independent dense-Gaussian problems (d~U[32,512], n~U[16,256],
+    m~U[100,600], TopK k=8d) stepped as one batch (one launch per kernel,
+    graph replay) against the same problems stepped one after another."""
+    from ufednl_amd import datagen
+    from ufednl_amd.engine import Batch, Engine
+    shapes = datagen.batched_shapes(count)
+    t0 = time.perf_counter()
+    engs = []
+    for i, (d, n, m) in enumerate(shapes):
+        prob = datagen.gaussian_problem(f"b{i}", n, m, d, seed=100 + i)
+        engs.append(Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False))
+        del prob
+    setup = time.perf_counter() - t0
+    # one after another (each engine's own stream, waited for in turn)
+    t0 = time.perf_counter()
+    for e in engs:
+        e.run(2, check=False)
+    t_seq = (time.perf_counter() - t0) / 2
+    b = Batch(engs, use_graph=True)
+    b.run(2)  # warm-up (graph capture)
+    t0 = time.perf_counter()
+    b.run(rounds)
+    t_batch = (time.perf_counter() - t0) / rounds
+    b.close()
+    ttol = []
+    for e in engs:
+        tr = e.trace()
+        g2 = tr["grad_norm"] ** 2
+        hit = np.nonzero(g2 <= TOL)[0]  # round index from x0 = 0
+        ttol.append(int(hit[0]) if hit.size else None)
+        e.close()
+    conv = [r for r in ttol if r is not None]
+    return dict(problems=count, shapes="d~U[32,512], n~U[16,256], m~U[100,600], N(0,1) features, TopK k=8d, FP64",
+                setup_s=setup, batch_round_ms=1e3 * t_batch, sequential_round_ms=1e3 * t_seq,
+                batch_speedup=t_seq / t_batch, problem_rounds_per_s=count / t_batch,
+                converged=len(conv), rounds_to_tol_p50=float(np.percentile(conv, 50)) if conv else None,
+                rounds_to_tol_max=max(conv) if conv else None,
+                solves_per_s=(len(conv) / (max(conv) * t_batch)) if conv else None)
+
+
+def serving_leg(count, rate, budget_gb, max_rounds, seed=5):
+    """Poisson arrivals (rate per second, fixed seed) of FedNL solves with
+    dense N(0,1) features, n in [32,512], m in [200,1000], d log-uniform in
+    [64,2048] clipped to the engine's d <= 512, TopK k=8d, each run to
+    ||grad f||^2 <= 1e-17; continuous batching under a device-memory budget."""
+    from ufednl_amd import datagen
+    from ufednl_amd import engine as E
+    rng = np.random.default_rng(seed)
+    shapes = datagen.serving_shapes(count, seed=seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    reqs = []
+    t0 = time.perf_counter()
+    for i, ((d, n, m), t) in enumerate(zip(shapes, arrivals)):
+        d = min(d, 512)
+        prob = datagen.gaussian_problem(f"r{i}", n, m, d, seed=1000 + i)
+        reqs.append(dict(arrival_s=float(t), problem=prob, max_rounds=max_rounds, tol=TOL,
+                         opts=dict(compressor="topk", k_mult=8.0)))
+    gen_s = time.perf_counter() - t0
+    res, _ = E.serve(reqs, hbm_budget_bytes=budget_gb * 1e9, rounds_per_step=4)
+    lat = np.array([r["finished_s"] - rq["arrival_s"] for r, rq in zip(res, reqs)])
+    ok = [r["status"] == 0 for r in res]
+    span = max(r["finished_s"] for r in res) - float(arrivals[0])
+    return dict(requests=count, arrival_rate_per_s=rate, budget_gb=budget_gb, data_gen_s=gen_s,
+                shapes="n~U[32,512], m~U[200,1000], d log-U[64,2048] clipped to 512, N(0,1), TopK k=8d, FP64",
+                converged=int(sum(ok)), solves_per_s=sum(ok) / span if span > 0 else None,
+                latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
+                rounds_to_tol_p50=float(np.median([r["rounds_to_tol"] for r in res if r["status"] == 0]))
+                if any(ok) else None)
+
+
 def cpu_baseline(prob, rounds, cores):
     import oracle as O
     kind = "reference" if O.ref_available() else "port"
@@ -136,6 +274,11 @@ def main():
     ap.add_argument("--tol-rounds", type=int, default=400, help="round budget of the time-to-tolerance solve")
     ap.add_argument("--cpu-rounds", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and batched legs")
+    ap.add_argument("--batch-problems", type=int, default=64)
+    ap.add_argument("--batch-rounds", type=int, default=60)
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -196,6 +339,12 @@ def main():
         pk_ = PEAK_FP64_MFMA_TFLOPS * 1e12 if b_ == "mfma" else PEAK_HBM_GBS * 1e9
         all_roofs[name_] = dict(bound=b_, us=sec_ * 1e6, frac=w_ / sec_ / pk_, traffic=traffic.get(name_))
 
+    extra = {}
+    if not args.no_extra:
+        extra["mixed_precision"] = mixed_precision_leg(args.tol_rounds, 500)
+        extra["batched"] = batched_leg(args.batch_problems, args.batch_rounds)
+        extra["serving"] = serving_leg(args.serve_requests, args.serve_rate, 64.0, 400)
+
     cpu = None
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
@@ -230,6 +379,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

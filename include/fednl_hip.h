@@ -165,6 +165,37 @@ int fednl_batch_enqueue(void *batch, int rounds);
 int fednl_batch_sync(void *batch);
 int fednl_batch_run(void *batch, int rounds);
 
+/* Serving: solves that arrive over time, each a whole FedNL problem run to
+ * ||grad f||^2 <= tol (or max_rounds). fednl_serve runs them in real time:
+ * a request is admitted at or after arrival_s (seconds from the call), in
+ * arrival order, while the device bytes of the admitted problems stay within
+ * hbm_budget_bytes (fednl_footprint); admitted problems are stepped together
+ * as one batch, rounds_per_step rounds at a time, members joining and leaving
+ * between steps (continuous batching). Results per request, in input order;
+ * x_out (d doubles, optional) receives the final iterate. Returns FEDNL_OK or
+ * the first device error. No counterpart in the reference. */
+typedef struct fednl_request {
+    double arrival_s;
+    int n, m, d;
+    const double *features;  /* n*m*d, as for fednl_create */
+    const double *labels;    /* n*m */
+    fednl_opts opts;
+    int max_rounds;
+    double tol;
+    double *x_out;
+} fednl_request;
+
+typedef struct fednl_result {
+    int status;         /* 0 converged, 2 round budget spent, < 0 error, 1 not run */
+    int rounds;         /* rounds run */
+    int rounds_to_tol;  /* first round with ||grad f||^2 <= tol, -1 if none */
+    double admitted_s, finished_s, grad_norm;
+} fednl_result;
+
+size_t fednl_footprint(const fednl_opts *opts, int n, int m, int d);
+int fednl_serve(const fednl_request *reqs, int count, double hbm_budget_bytes, int rounds_per_step, int use_graph,
+                fednl_result *out);
+
 /* Client Hessians H_c(x) for all clients (n*S packed upper, host), computed
  * by the engine's Hessian mode at the given x (d values); the engine's
  * iterate and learned Hessians are unchanged. Used to state the error of the

@@ -32,6 +32,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <thread>
 
 #include "fednl_hip.h"
 
@@ -3052,6 +3053,23 @@ static int batch_enqueue(Batch* b, int rounds)
     return FEDNL_OK;
 }
 
+// device bytes one engine holds for a problem of this size (what the engine
+// allocates; the admission test of the serving loop)
+static size_t engine_footprint(const fednl_opts* o, int n, int m, int d)
+{
+    const size_t m_pad = (size_t)round_up(m, kKB), d_pad = (size_t)round_up(d, kTile), dp = (size_t)round_up(d, kNB);
+    const size_t S = (size_t)d * (d + 1) / 2;
+    size_t k = (o->compressor == FEDNL_IDENTICAL) ? S : (size_t)(o->k_mult * d);
+    size_t b = 8 * (size_t)n * m_pad * d_pad;                 // A
+    if (o->hessian_mode == 1) b += 4 * (size_t)n * m_pad * d_pad;
+    if (o->hessian_mode == 2) b += 4 * (size_t)n * m_pad * d_pad;
+    b += 8 * 2 * (size_t)n * S;                                 // learned, delta
+    b += 8 * 3 * (size_t)n * m_pad + 8 * (size_t)n * d_pad;     // Dw, gw, nlog, grad_c
+    b += (size_t)n * k * (4 + 4 + 4 + 8) + 4 * (size_t)n * ((S + 31) / 32);
+    b += 8 * (S + dp * dp) + (1 << 20);
+    return b;
+}
+
 }  // namespace fednl
 
 using namespace fednl;
@@ -3212,6 +3230,150 @@ int fednl_batch_run(void* batch, int rounds)
     if (rc != FEDNL_OK)
         return rc;
     return fednl_batch_sync(batch);
+}
+
+size_t fednl_footprint(const fednl_opts* opts, int n, int m, int d)
+{
+    fednl_opts o;
+    if (opts)
+        o = *opts;
+    else
+        fednl_default_opts(&o);
+    return engine_footprint(&o, n, m, d);
+}
+
+int fednl_serve(const fednl_request* reqs, int count, double hbm_budget_bytes, int rounds_per_step, int use_graph,
+                fednl_result* out)
+{
+    if (!reqs || count <= 0 || !out || rounds_per_step <= 0) {
+        set_err("fednl_serve: empty request list");
+        return FEDNL_E_MISSING_ARGUMENT;
+    }
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    struct Slot {
+        int req;
+        Engine* e;
+        size_t bytes;
+        int rounds;
+    };
+    std::vector<Slot> active;
+    std::vector<int> waiting;  // arrived, not admitted (arrival order)
+    int next = 0, done = 0;
+    size_t used = 0;
+    for (int i = 0; i < count; ++i) {
+        memset(&out[i], 0, sizeof(fednl_result));
+        out[i].status = 1;  // not finished
+    }
+    // requests are taken in arrival order
+    std::vector<int> order(count);
+    for (int i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+    Batch* batch = nullptr;
+    bool dirty = true;
+    int rc = FEDNL_OK;
+    while (done < count && rc == FEDNL_OK) {
+        const double t = now_s();
+        while (next < count && reqs[order[next]].arrival_s <= t)
+            waiting.push_back(order[next++]);
+        // admission by device-memory footprint, arrival order (no overtaking)
+        while (!waiting.empty()) {
+            const int q = waiting.front();
+            const fednl_request& R = reqs[q];
+            const size_t need = engine_footprint(&R.opts, R.n, R.m, R.d);
+            if (!active.empty() && (double)(used + need) > hbm_budget_bytes)
+                break;
+            Engine* e = nullptr;
+            const int crc = create_engine(&R.opts, R.n, R.m, R.d, R.features, R.labels, &e);
+            out[q].admitted_s = now_s();
+            if (crc != FEDNL_OK) {
+                out[q].status = crc;
+                out[q].finished_s = out[q].admitted_s;
+                ++done;
+                waiting.erase(waiting.begin());
+                continue;
+            }
+            active.push_back({q, e, need, 0});
+            used += need;
+            waiting.erase(waiting.begin());
+            dirty = true;
+        }
+        if (active.empty()) {
+            if (next < count) {
+                const double wait = reqs[order[next]].arrival_s - now_s();
+                if (wait > 0)
+                    std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+            }
+            continue;
+        }
+        if (dirty) {
+            if (batch) {
+                hipStreamSynchronize(batch->st);
+                free_batch(batch);
+                batch = nullptr;
+            }
+            std::vector<void*> hs;
+            for (auto& sl : active) hs.push_back(sl.e);
+            if ((rc = fednl_batch_create(hs.data(), (int)hs.size(), use_graph, (void**)&batch)) != FEDNL_OK)
+                break;
+            dirty = false;
+        }
+        if ((rc = batch_enqueue(batch, rounds_per_step)) != FEDNL_OK)
+            break;
+        if (hipStreamSynchronize(batch->st) != hipSuccess) {
+            set_err("hipStreamSynchronize (serve)");
+            rc = FEDNL_E_DEVICE;
+            break;
+        }
+        const double tnow = now_s();
+        // retire members that reached the tolerance or their round budget
+        for (size_t i = 0; i < active.size();) {
+            Slot& sl = active[i];
+            const fednl_request& R = reqs[sl.req];
+            const int first = sl.rounds;
+            sl.rounds += rounds_per_step;
+            std::vector<double> tr((size_t)4 * rounds_per_step);
+            hipMemcpy(tr.data(), sl.e->trace + (size_t)4 * first, sizeof(double) * tr.size(), hipMemcpyDeviceToHost);
+            int hit = -1;
+            for (int r = 0; r < rounds_per_step; ++r) {
+                const double g = tr[(size_t)4 * r];
+                if (g * g <= R.tol) {
+                    hit = first + r;
+                    break;
+                }
+            }
+            const bool finished = (hit >= 0) || sl.rounds >= R.max_rounds;
+            if (!finished) {
+                ++i;
+                continue;
+            }
+            fednl_result& o = out[sl.req];
+            o.finished_s = tnow;
+            o.rounds = sl.rounds;
+            o.rounds_to_tol = hit;
+            o.status = (hit >= 0) ? 0 : 2;
+            o.grad_norm = tr[(size_t)4 * (rounds_per_step - 1)];
+            if (R.x_out)
+                hipMemcpy(R.x_out, sl.e->x, sizeof(double) * R.d, hipMemcpyDeviceToHost);
+            used -= sl.bytes;
+            free_engine(sl.e);
+            active.erase(active.begin() + i);
+            ++done;
+            dirty = true;
+        }
+        if (dirty && batch) {
+            free_batch(batch);
+            batch = nullptr;
+        }
+    }
+    if (batch) {
+        hipStreamSynchronize(batch->st);
+        free_batch(batch);
+    }
+    for (auto& sl : active) free_engine(sl.e);
+    return rc;
 }
 
 int fednl_trace(void* handle, double* grad_norm, double* fval, double* x_norm, double* seconds, int* chol_ok,

@@ -33,6 +33,26 @@ class FednlOpts(ctypes.Structure):
     ]
 
 
+class FednlRequest(ctypes.Structure):
+    _fields_ = [
+        ("arrival_s", ctypes.c_double),
+        ("n", ctypes.c_int), ("m", ctypes.c_int), ("d", ctypes.c_int),
+        ("features", ctypes.POINTER(ctypes.c_double)),
+        ("labels", ctypes.POINTER(ctypes.c_double)),
+        ("opts", FednlOpts),
+        ("max_rounds", ctypes.c_int),
+        ("tol", ctypes.c_double),
+        ("x_out", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+class FednlResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int), ("rounds", ctypes.c_int), ("rounds_to_tol", ctypes.c_int),
+        ("admitted_s", ctypes.c_double), ("finished_s", ctypes.c_double), ("grad_norm", ctypes.c_double),
+    ]
+
+
 class FednlError(RuntimeError):
     pass
 
@@ -80,6 +100,10 @@ def lib():
         L.fednl_batch_enqueue.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.fednl_batch_sync.argtypes = [ctypes.c_void_p]
         L.fednl_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.fednl_footprint.restype = ctypes.c_size_t
+        L.fednl_footprint.argtypes = [ctypes.POINTER(FednlOpts), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.fednl_serve.argtypes = [ctypes.POINTER(FednlRequest), ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  ctypes.c_int, ctypes.POINTER(FednlResult)]
         fp = ctypes.POINTER(ctypes.c_float)
         L.fednl_probe_mfma_mixed.argtypes = [fp, fp, fp, fp, fp, fp]
         _lib = L
@@ -273,6 +297,61 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def make_opts(lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0, option_b=True, workers=1,
+              tracking=False, hessian="fp64"):
+    o = FednlOpts()
+    lib().fednl_default_opts(ctypes.byref(o))
+    o.lambda_ = lam
+    o.compressor = COMPRESSORS[compressor]
+    o.k_mult = k_mult
+    o.alpha_mode = alpha_mode
+    o.alpha = alpha
+    o.option_b = 1 if option_b else 0
+    o.workers = workers
+    o.tracking = 1 if tracking else 0
+    o.hessian_mode = HESSIAN_MODES[hessian]
+    return o
+
+
+def serve(requests, hbm_budget_bytes, rounds_per_step=4, use_graph=True):
+    """Runs fednl_serve. requests: list of dicts with arrival_s, problem
+    (features n x m x d, labels n x m, lam), max_rounds, tol and the engine
+    options (compressor, k_mult, hessian ...). Returns (results, final x
+    per request)."""
+    L = lib()
+    keep = []
+    arr = (FednlRequest * len(requests))()
+    xs = []
+    for i, rq in enumerate(requests):
+        prob = rq["problem"]
+        feats = np.ascontiguousarray(prob.features, dtype=np.float64)
+        labs = np.ascontiguousarray(prob.labels, dtype=np.float64)
+        n, m, d = feats.shape
+        x = np.zeros(d)
+        keep += [feats, labs]
+        xs.append(x)
+        r = arr[i]
+        r.arrival_s = rq.get("arrival_s", 0.0)
+        r.n, r.m, r.d = n, m, d
+        r.features = dp(feats)
+        r.labels = dp(labs)
+        r.opts = make_opts(lam=prob.lam, **rq.get("opts", {}))
+        r.max_rounds = rq.get("max_rounds", 500)
+        r.tol = rq.get("tol", 1e-17)
+        r.x_out = dp(x)
+    res = (FednlResult * len(requests))()
+    _err(L.fednl_serve(arr, len(requests), float(hbm_budget_bytes), rounds_per_step, 1 if use_graph else 0, res),
+         "fednl_serve")
+    out = [dict(status=r.status, rounds=r.rounds, rounds_to_tol=r.rounds_to_tol, admitted_s=r.admitted_s,
+                finished_s=r.finished_s, grad_norm=r.grad_norm) for r in res]
+    return out, xs
+
+
+def footprint(n, m, d, **opts):
+    o = make_opts(**opts)
+    return lib().fednl_footprint(ctypes.byref(o), n, m, d)
 
 
 def probe_mfma_mixed(a32, b32, abf, bbf):
