@@ -1,0 +1,164 @@
+"""The bin_fednl_local drop-in: LIBSVM in, FedNL on the GPU engine, dopt's
+tracking file out (TrainLoop.h:753-792, 1679-1691), and the shared-library
+entry doptExecuteLocal (src/mainShared.cpp:15-39).
+
+Checked against the oracle on the same split: client c holds samples
+[c*m, (c+1)*m) of the (optionally reshuffled) order, the residual is dropped
+(TrainLoop.h:212-250); the reshuffle permutation comes from the reference's
+own dopt::shuffle (fixture tests/golden/reshuffle.npz). Labels are mapped to
+-1 below / +1 above the mean of the two label values (dataset_loader_fs.h).
+RandK is used so that trajectories are value independent.
+"""
+import ctypes
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.tolerances import assert_grad_norms_close
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "ufednl_amd", "bin_fednl_local")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "reshuffle.npz")
+
+N_SAMPLES, D_FEAT, N_CLIENTS = 273, 14, 6   # m = 45 per client, 3 samples dropped
+
+
+def _dataset(seed=91):
+    rng = np.random.default_rng(seed)
+    X = (rng.random((N_SAMPLES, D_FEAT)) < 0.3).astype(np.float64) * rng.integers(1, 4, (N_SAMPLES, D_FEAT))
+    w = rng.standard_normal(D_FEAT)
+    y = np.where(rng.random(N_SAMPLES) < 1.0 / (1.0 + np.exp(-(X @ w - 0.3))), 1.0, -1.0)
+    return X, y
+
+
+def _write_libsvm(path, X, y):
+    # labels 2 / 4 (mapped to -1 / +1); indices 1-based; the last index is
+    # written explicitly on the first line so that d = max - min + 1 holds
+    with open(path, "w") as f:
+        for s in range(X.shape[0]):
+            toks = [str(4 if y[s] > 0 else 2)]
+            for i in range(X.shape[1]):
+                if X[s, i] != 0 or (s == 0 and i in (0, X.shape[1] - 1)):
+                    toks.append(f"{i + 1}:{X[s, i]:g}")
+            f.write(" ".join(toks) + "\n")
+
+
+def _read_tracking(path):
+    b = open(path, "rb").read()
+    pos = 0
+    strs = []
+    for _ in range(3):
+        e = b.index(b"\0", pos)
+        strs.append(b[pos:e].decode())
+        pos = e + 1
+    ints = struct.unpack_from("<6i", b, pos)
+    pos += 24
+    dbl = struct.unpack_from("<4d", b, pos)
+    pos += 32
+    rec = []
+    while pos + 52 <= len(b):
+        rec.append(struct.unpack_from("<i6d", b, pos))
+        pos += 52
+    assert pos == len(b)
+    return strs, ints, dbl, rec
+
+
+def _oracle_grad_norms(oracle, X, y, order, rounds, k_mult):
+    from ufednl_amd.datagen import Problem
+    m = N_SAMPLES // N_CLIENTS
+    Xi = np.hstack([X, np.ones((N_SAMPLES, 1))])  # --add_intercept: last column
+    idx = np.asarray(order[: m * N_CLIENTS])
+    feats = Xi[idx].reshape(N_CLIENTS, m, D_FEAT + 1)
+    labs = y[idx].reshape(N_CLIENTS, m)
+    prob = Problem("cli", np.ascontiguousarray(feats), np.ascontiguousarray(labs), 1e-3, np.zeros(D_FEAT + 1))
+    f = oracle.FedNL(prob, "randk", k_mult=k_mult)
+    return np.array([f.round(tracking=False)["grad_norm"] for _ in range(rounds)])
+
+
+def _args(data, out, rounds, extra=()):
+    return ["--train", "--train_dataset", data, "--clients", str(N_CLIENTS), "--add_intercept", "--lambda", "0.001",
+            "--algorithm", "fednl1", "--compressor", "randk", "--k_compressor_as_d_mult", "2",
+            "--theoretical_alpha", "--use_theoretical_alpha_option_2", "--fednl-option-b", "--rounds", str(rounds),
+            "--tracking", "--out", out, "--silent", *extra]
+
+
+@pytest.mark.parametrize("reshuffle_seed", [None, 7])
+def test_cli_tracking_file_matches_oracle(oracle, tmp_path, reshuffle_seed):
+    X, y = _dataset()
+    data = str(tmp_path / "train.libsvm")
+    _write_libsvm(data, X, y)
+    out = str(tmp_path / "run_{run_id}.bin")
+    rounds = 30
+    extra = () if reshuffle_seed is None else ("--reshuffle_train", "--reshuffle_train_seed", str(reshuffle_seed))
+    p = subprocess.run([BIN, *_args(data, out, rounds, extra)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    files = [f for f in os.listdir(tmp_path) if f.startswith("run_") and f.endswith(".bin")]
+    assert len(files) == 1, files
+    strs, ints, dbl, rec = _read_tracking(str(tmp_path / files[0]))
+    m = N_SAMPLES // N_CLIENTS
+    assert strs == ["train.libsvm", "FedNL (b)", "randk"]
+    assert ints == (m, N_SAMPLES, rounds, N_CLIENTS, D_FEAT + 1, 2 * (D_FEAT + 1))
+    assert dbl[0] == 2.0 and dbl[3] == 0.001
+    assert [r[0] for r in rec] == list(range(rounds))
+    if reshuffle_seed is None:
+        order = np.arange(N_SAMPLES)
+    else:
+        order = np.load(GOLDEN)[f"perm_{reshuffle_seed}_{N_SAMPLES}"]
+    ref = _oracle_grad_norms(oracle, X, y, order, rounds, 2.0)
+    assert_grad_norms_close(np.array([r[1] for r in rec]), ref)
+    # f(x) is tracked and decreases; ||x|| is finite
+    fv = np.array([r[2] for r in rec])
+    assert np.all(np.isfinite(fv)) and fv[-1] < fv[0]
+
+
+def test_dopt_execute_local_callback(oracle, tmp_path):
+    X, y = _dataset()
+    data = str(tmp_path / "train.libsvm")
+    _write_libsvm(data, X, y)
+    out = str(tmp_path / "r.bin")
+    lib = ctypes.CDLL(os.path.join(ROOT, "ufednl_amd", "libfednl_hip.so"))
+    CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double)
+    lib.doptExecuteLocal.argtypes = [ctypes.c_char_p, CB]
+    lib.doptExecuteLocal.restype = ctypes.c_int
+    got = []
+    cb = CB(lambda d, g, gap, l2: got.append((d, g, gap, l2)))
+    rounds = 20
+    cmd = " ".join(_args(data, out, rounds)[1:])  # the library adds "train"
+    rc = lib.doptExecuteLocal(cmd.encode(), cb)
+    assert rc == 0
+    assert len(got) == 1
+    d, g, gap, l2 = got[0]
+    assert d == D_FEAT + 1
+    # the callback values are (d, ||grad f(x_last)||, ||g||^2/(2 lambda), 2||g||^2/lambda)
+    np.testing.assert_allclose(gap, g * g / (2 * 1e-3), rtol=1e-12)
+    np.testing.assert_allclose(l2, 2 * g * g / 1e-3, rtol=1e-12)
+    g0 = _oracle_grad_norms(oracle, X, y, np.arange(N_SAMPLES), 1, 2.0)[0]
+    assert np.isfinite(g) and 0 < g < g0
+
+
+@pytest.mark.parametrize("drop,code", [("--out", -1), ("--lambda", -1)])
+def test_cli_missing_arguments(tmp_path, drop, code):
+    X, y = _dataset()
+    data = str(tmp_path / "train.libsvm")
+    _write_libsvm(data, X, y)
+    args = _args(data, str(tmp_path / "o.bin"), 3)
+    i = args.index(drop)
+    del args[i:i + 2]
+    p = subprocess.run([BIN, *args], capture_output=True, text=True, timeout=60)
+    assert p.returncode == (code & 0xFF)
+
+
+def test_cli_dataset_errors(tmp_path):
+    args = _args(str(tmp_path / "missing.libsvm"), str(tmp_path / "o.bin"), 3)
+    p = subprocess.run([BIN, *args], capture_output=True, text=True, timeout=60)
+    assert p.returncode == (-3 & 0xFF)
+    bad = tmp_path / "three.libsvm"
+    bad.write_text("1 1:1\n2 1:0\n3 1:1\n")
+    p = subprocess.run([BIN, *_args(str(bad), str(tmp_path / "o.bin"), 3)], capture_output=True, text=True,
+                       timeout=60)
+    assert p.returncode == (-4 & 0xFF)

@@ -159,3 +159,19 @@ def test_cholesky_known_factor(oracle):
         Bm = np.array(bad, dtype=np.float64)
         ok, _, _ = oracle.cholesky_solve(np.array([Bm[0, 0], Bm[0, 1], Bm[1, 1]]), 2, 0.0, np.ones(2))
         assert not ok
+
+
+def test_reshuffle_fixture_matches_oracle_shuffle(oracle):
+    """dopt's --reshuffle_train permutation (reference dopt::shuffle over all
+    samples, fixture written from oracle/_ref) equals the oracle's partial
+    Fisher-Yates run to k = n with the same LCG seed."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "reshuffle.npz"))
+    for key in z.files:
+        _, seed, n = key.split("_")
+        seed, n = int(seed), int(n)
+        v = np.arange(n, dtype=np.uint32)
+        st = np.array([seed], dtype=np.uint32)
+        oracle.lib().orc_shuffle_partial(oracle.up(v), n, n, oracle.up(st))
+        np.testing.assert_array_equal(v, z[key], err_msg=key)
+        assert sorted(z[key].tolist()) == list(range(n))
