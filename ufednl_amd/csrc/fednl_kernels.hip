@@ -942,6 +942,40 @@ struct RegHeap {
         }
     }
 
+    // value and id of slot sidx (uniform), wherever it is held
+    __device__ void get(const double* hv, const uint32_t* hid, int sidx, double& v, uint32_t& id) const
+    {
+        if (sidx == 0) {
+            v = r0v;
+            id = r0i;
+        } else if (sidx < 5) {
+            v = bcast_lane(v1, sidx - 1);
+            id = __builtin_amdgcn_readlane(i1, sidx - 1);
+        } else if (sidx < 21) {
+            v = bcast_lane(v2, sidx - 5);
+            id = __builtin_amdgcn_readlane(i2, sidx - 5);
+        } else if (sidx < 85) {
+            v = bcast_lane(v3, sidx - 21);
+            id = __builtin_amdgcn_readlane(i3, sidx - 21);
+        } else {
+            v = hv[sidx];
+            id = hid[sidx];
+        }
+    }
+
+    // slot sidx left the heap: a register-held slot becomes +inf so that it
+    // never wins as a child (LDS slots are bounded by n)
+    __device__ void kill(int sidx)
+    {
+        const double inf = __builtin_inf();
+        if (sidx >= 1 && sidx < 5)
+            v1 = (lane == sidx - 1) ? inf : v1;
+        else if (sidx >= 5 && sidx < 21)
+            v2 = (lane == sidx - 5) ? inf : v2;
+        else if (sidx >= 21 && sidx < 85)
+            v3 = (lane == sidx - 21) ? inf : v3;
+    }
+
     // one level: children of offset o held in (cv_reg, ci_reg); on a move the
     // minimum and its lane come back in (mu, q)
     __device__ bool step(double cv_reg, int o, double lowv, double& mu, int& q) const
@@ -1341,19 +1375,26 @@ __global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
     PROBE_T(t_sel2);
     int cnt = k;
     if (kLek) {
+        // SpecialMathRoutinesForMatrix.h:383-511, in wave 0: the energy terms
+        // of the heap (array order, one chunk of 64 at a time, subtracted in
+        // order), then extracFromMinHeap (MinHeap.h:125-141) with the top
+        // levels still in registers
         __shared__ int s_cnt;
-        if (tid == 0) {
-            // SpecialMathRoutinesForMatrix.h:383-511
+        if (wave == 0) {
             const double fro2 = s_fro2;
             const double one_minus_delta_xf = (1.0 - lek_delta) * fro2;
             const double eps = 1e-1 * fro2;
             double numerator = fro2;
-            for (int q = 0; q < k; ++q) {
-                const double it = hv[q];
-                if (pos_is_diag(hid[q]))
-                    numerator -= it * it;
-                else
-                    numerator -= 2.0 * it * it;
+            for (int q0 = 0; q0 < k; q0 += 64) {
+                const int q = q0 + lane;
+                double tq = 0.0;
+                if (q < k) {
+                    const double it = hv[q];
+                    tq = pos_is_diag(hid[q]) ? it * it : 2.0 * it * it;
+                }
+                const int nq = min(64, k - q0);
+                for (int j = 0; j < nq; ++j)
+                    numerator -= bcast_lane(tq, j);
             }
             int nh = k;
             int hit = 0;
@@ -1361,15 +1402,21 @@ __global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
             uint32_t popped = 0;
             do {
                 const double prev = numerator;
-                const uint32_t pid = hid[0];
-                const double pv = hv[0];
-                // extracFromMinHeap (MinHeap.h:125-141)
-                hv[0] = hv[nh - 1];
-                hid[0] = hid[nh - 1];
-                hv[nh - 1] = pv;
-                hid[nh - 1] = pid;
-                heapify(hv, hid, 0, nh - 1);
-                nh -= 1;
+                const uint32_t pid = rh.r0i;
+                const double pv = rh.r0v;
+                const int last = nh - 1;
+                if (last > 0) {
+                    // the last element goes to the root and sifts down a heap of last items
+                    double lv;
+                    uint32_t lid;
+                    rh.get(hv, hid, last, lv, lid);
+                    rh.kill(last);
+                    rh.n = last;
+                    rh.admit(hv, hid, lv, lid);
+                } else {
+                    rh.n = 0;
+                }
+                nh = last;
                 if (pos_is_diag(pid))
                     numerator += pv * pv;
                 else
@@ -1382,10 +1429,13 @@ __global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
                     break;
                 }
             } while (nh != 0);
-            s_cnt = nh;
-            lek_hit[c] = hit;
-            lek_p[c] = pp;
-            lek_popped[c] = popped;
+            rh.store(hv, hid);
+            if (lane == 0) {
+                s_cnt = nh;
+                lek_hit[c] = hit;
+                lek_p[c] = pp;
+                lek_popped[c] = popped;
+            }
         }
         __syncthreads();
         cnt = s_cnt;
