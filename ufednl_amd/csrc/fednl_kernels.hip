@@ -717,30 +717,35 @@ __global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __rest
     // load assignment: rows lr and lr + 8 of a chunk, columns lc, lc + 1 of
     // both column blocks (diagonal tiles read the same block twice, from cache)
     const int lr = tid >> 5, lc = (tid & 31) * 2;
-    double2 ri[2], rj[2];
-    double rd[2], rg[2];
+    // named registers (no arrays: an array captured by the lambdas below
+    // would be placed in scratch memory)
+    double2 ri0, ri1, rj0, rj1;
+    double rd0, rd1, rg0, rg1;
     auto gload = [&](int s0) {
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int kk = lr + 8 * u;
-            const double* row = Ac + (size_t)(s0 + kk) * d_pad;
-            ri[u] = *reinterpret_cast<const double2*>(row + i0 + lc);
-            rj[u] = *reinterpret_cast<const double2*>(row + j0 + lc);
-            rd[u] = Dc[s0 + kk];
-            rg[u] = Gc[s0 + kk];
-        }
+        const double* row0 = Ac + (size_t)(s0 + lr) * d_pad;
+        const double* row1 = Ac + (size_t)(s0 + lr + 8) * d_pad;
+        ri0 = *reinterpret_cast<const double2*>(row0 + i0 + lc);
+        rj0 = *reinterpret_cast<const double2*>(row0 + j0 + lc);
+        ri1 = *reinterpret_cast<const double2*>(row1 + i0 + lc);
+        rj1 = *reinterpret_cast<const double2*>(row1 + j0 + lc);
+        rd0 = Dc[s0 + lr];
+        rd1 = Dc[s0 + lr + 8];
+        rg0 = Gc[s0 + lr];
+        rg1 = Gc[s0 + lr + 8];
     };
     auto lstore = [&](int buf) {
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int kk = lr + 8 * u;
-            *reinterpret_cast<double2*>(&sI[buf][kk][lc]) = ri[u];
-            double2 sj;
-            sj.x = rd[u] * rj[u].x;
-            sj.y = rd[u] * rj[u].y;
-            *reinterpret_cast<double2*>(&sJ[buf][kk][lc]) = sj;
-            if (lc == 0)
-                sG[buf][kk] = rg[u];
+        *reinterpret_cast<double2*>(&sI[buf][lr][lc]) = ri0;
+        *reinterpret_cast<double2*>(&sI[buf][lr + 8][lc]) = ri1;
+        double2 sj0, sj1;
+        sj0.x = rd0 * rj0.x;
+        sj0.y = rd0 * rj0.y;
+        sj1.x = rd1 * rj1.x;
+        sj1.y = rd1 * rj1.y;
+        *reinterpret_cast<double2*>(&sJ[buf][lr][lc]) = sj0;
+        *reinterpret_cast<double2*>(&sJ[buf][lr + 8][lc]) = sj1;
+        if (lc == 0) {
+            sG[buf][lr] = rg0;
+            sG[buf][lr + 8] = rg1;
         }
     };
 
@@ -1297,7 +1302,7 @@ __global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
         rh.load(hv, hid, k, lane);
 #ifdef FEDNL_PROBE
     const unsigned long long tscan = clock64();
-    unsigned long long adm_cnt = 0, adm_cyc = 0;
+    unsigned long long adm_cnt = 0, adm_cyc = 0, adm_wait = 0;
 #endif
     for (int sgi = 0; sgi < nseg; ++sgi) {
         const double* cur = seg + (size_t)(sgi & 1) * seg_len;
@@ -1358,13 +1363,21 @@ __global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
             const int nlen = (int)min((size_t)seg_len, S - nbase);
             stage_abs(nxt, dc + nbase, nlen, tid - 64, blockDim.x - 64);
         }
+#ifdef FEDNL_PROBE
+        const unsigned long long tw0 = clock64();
+#endif
         __syncthreads();
+#ifdef FEDNL_PROBE
+        if (tid == 0)
+            adm_wait += clock64() - tw0;
+#endif
     }
 #ifdef FEDNL_PROBE
     if (tid == 0) {
         g_probe[1000 + 8 * c + 5] += clock64() - tscan;
         g_probe[1000 + 8 * c + 3] += adm_cnt;
         g_probe[1000 + 8 * c + 4] += adm_cyc;
+        g_probe[1000 + 8 * c + 6] += adm_wait;
     }
 #endif
     if (wave == 0)
