@@ -22,7 +22,7 @@ _c_dp = ctypes.POINTER(ctypes.c_double)
 _c_up = ctypes.POINTER(ctypes.c_uint32)
 _c_ip = ctypes.POINTER(ctypes.c_int)
 
-COMPRESSORS = {"identical": 0, "randk": 1, "topk": 2, "seqk": 3, "toplek": 4}
+COMPRESSORS = {"identical": 0, "randk": 1, "topk": 2, "seqk": 3, "toplek": 4, "natural": 5}
 
 
 def _p(a, t):
@@ -203,7 +203,8 @@ class FedNL:
         n, m, d = prob.features.shape
         self.n, self.m, self.d = n, m, d
         self.S = d * (d + 1) // 2
-        self.k = int(k_mult * d)
+        # Identical and Natural send the whole upper triangle (TrainLoop.h:354-377)
+        self.k = self.S if compressor in ("identical", "natural") else int(k_mult * d)
         self.comp = COMPRESSORS[compressor]
         if alpha is None:
             alpha = theoretical_alpha(compressor, self.k, d, option=2)
@@ -221,7 +222,7 @@ class FedNL:
 
     def round(self, tracking=True):
         n, d, k = self.n, self.d, self.k
-        kk = max(k, self.S) if self.comp == 0 else k
+        kk = max(k, self.S) if self.comp in (0, 5) else k
         g = np.zeros(d); lk = np.zeros(n); cnt = np.zeros(n, dtype=np.int32)
         pos = np.zeros(n * kk, dtype=np.uint32); vals = np.zeros(n * kk); dx = np.zeros(d); st = np.zeros(3)
         ok = lib().orc_fednl_round(self.h, 1 if tracking else 0, dp(g), dp(lk), ip(cnt), up(pos), dp(vals),
@@ -262,7 +263,7 @@ def ref_fednl_run(prob, compressor="topk", k_mult=8.0, alpha=None, option_b=True
                   tracking=True, dump_rounds=0):
     """Run the reference building blocks (oracle/_ref) for `rounds` rounds."""
     n, m, d = prob.features.shape
-    k = int(k_mult * d)
+    k = d * (d + 1) // 2 if compressor in ("identical", "natural") else int(k_mult * d)
     if alpha is None:
         alpha = theoretical_alpha(compressor, k, d, option=2)
     feat = np.ascontiguousarray(prob.features, dtype=np.float64)

@@ -569,7 +569,19 @@ ORC_API void orc_tri_solve_packed(const double *U, int d, const double *b, doubl
 /* FedNL round engine (TrainLoop.h:1054-1698, forcesequpdate semantics)      */
 /* ------------------------------------------------------------------------ */
 
-enum { ORC_IDENTICAL = 0, ORC_RANDK = 1, ORC_TOPK = 2, ORC_RANDSEQK = 3, ORC_TOPLEK = 4 };
+enum { ORC_IDENTICAL = 0, ORC_RANDK = 1, ORC_TOPK = 2, ORC_RANDSEQK = 3, ORC_TOPLEK = 4, ORC_NATURAL = 5 };
+
+/* Natural compressor of dopt (MatrixNMD.h:2009-2029, applyNaturalCompressorNaive):
+ * the mantissa bits are cleared, sign and exponent kept (the value the
+ * 24-bit pair packing of LocalWorkerForFedNL_Natural.h:80-126 carries). */
+ORC_API double orc_natural(double v)
+{
+    uint64_t b;
+    memcpy(&b, &v, 8);
+    b &= 0xFFF0000000000000ull;
+    memcpy(&v, &b, 8);
+    return v;
+}
 
 typedef struct
 {
@@ -729,9 +741,12 @@ ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int
             break;
         }
         f->cnt[c] = ck;
+        const int nat = (f->compressor == ORC_NATURAL);
         for (int q = 0; q < ck; ++q)
         {
             double item = f->delta[pc[q]];
+            if (nat)
+                item = orc_natural(item);
             if (scale != 1.0)
                 item = item * scale;
             vc[q] = item * f->prescale;
@@ -739,6 +754,8 @@ ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int
         for (int q = 0; q < ck; ++q)
         {
             double item = f->delta[pc[q]];
+            if (nat)
+                item = orc_natural(item);
             if (scale != 1.0)
                 item = item * scale;
             double upd = f->alpha * item;

@@ -310,7 +310,7 @@ extern "C"
     }
 
     // compressor codes follow UsedTypes.h:17-25
-    enum { C_IDENTICAL = 0, C_RANDK = 1, C_TOPK = 2, C_RANDSEQK = 3, C_TOPLEK = 4 };
+    enum { C_IDENTICAL = 0, C_RANDK = 1, C_TOPK = 2, C_RANDSEQK = 3, C_TOPLEK = 4, C_NATURAL = 5 };
 
     // FedNL (option b when option_b != 0, else option a) with
     // forcesequpdate semantics. Clients are assigned round-robin to
@@ -405,6 +405,10 @@ extern "C"
                         }
                         else
                             kc = up;
+                        // Natural: the reference's own masking of the whole difference
+                        // (LocalWorkerForFedNL_Natural.h:73-75)
+                        if (compressor == C_NATURAL)
+                            diff.applyNaturalCompressor();
                         std::vector<double> vc(kc.size());
                         for (size_t q = 0; q < kc.size(); ++q)
                         {

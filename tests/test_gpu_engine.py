@@ -22,7 +22,8 @@ def _engine(prob, comp, **kw):
     return Engine.from_problem(prob, compressor=comp, **kw)
 
 
-@pytest.mark.parametrize("comp,kmult", [("randk", 2.0), ("seqk", 2.0), ("randk", 5.0)])
+@pytest.mark.parametrize("comp,kmult", [("randk", 2.0), ("seqk", 2.0), ("randk", 5.0), ("natural", 0.0),
+                                       ("identical", 0.0)])
 def test_trajectory_value_independent(oracle, comp, kmult):
     prob = datagen.small_problem(n=5, m=41, d_feat=11, density=0.35, seed=11)
     R = 30
@@ -36,7 +37,7 @@ def test_trajectory_value_independent(oracle, comp, kmult):
     assert np.all(tr["chol_ok"] == 1)
 
 
-@pytest.mark.parametrize("comp", ["topk", "toplek", "randk", "seqk"])
+@pytest.mark.parametrize("comp", ["topk", "toplek", "randk", "seqk", "natural", "identical"])
 def test_lockstep_rounds(oracle, comp):
     prob = datagen.small_problem(n=6, m=45, d_feat=13, density=0.3, seed=5)
     kmult = 2.0

@@ -175,3 +175,18 @@ def test_reshuffle_fixture_matches_oracle_shuffle(oracle):
         oracle.lib().orc_shuffle_partial(oracle.up(v), n, n, oracle.up(st))
         np.testing.assert_array_equal(v, z[key], err_msg=key)
         assert sorted(z[key].tolist()) == list(range(n))
+
+
+def test_natural_compressor_known_answers(oracle):
+    """dopt's Natural compressor keeps sign and exponent and clears the
+    mantissa (MatrixNMD.h:2009-2029): values round toward zero to a power of
+    two; subnormals become zero of the same sign."""
+    f = oracle.lib().orc_natural
+    import ctypes
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double]
+    cases = [(3.0, 2.0), (-5.5, -4.0), (0.75, 0.5), (1.0, 1.0), (-1.9999, -1.0), (0.0, 0.0),
+             (1e-310, 0.0), (-1e-310, -0.0), (np.inf, np.inf), (6.02e23, 2.0 ** 78)]
+    for v, want in cases:
+        got = f(v)
+        assert got == want and np.signbit(got) == np.signbit(want), (v, got, want)

@@ -66,7 +66,8 @@ def test_client_eval_and_newton(oracle, ref):
     np.testing.assert_allclose(dx, dxr, rtol=1e-11)
 
 
-@pytest.mark.parametrize("comp,seed", [("randk", 21), ("seqk", 22), ("topk", 23), ("toplek", 24)])
+@pytest.mark.parametrize("comp,seed", [("randk", 21), ("seqk", 22), ("topk", 23), ("toplek", 24),
+                                       ("natural", 25), ("identical", 26)])
 def test_fednl_trajectories(oracle, comp, seed):
     prob = datagen.small_problem(n=7, m=50, d_feat=15, density=0.3, seed=seed)
     R = 20

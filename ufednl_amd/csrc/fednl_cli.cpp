@@ -461,15 +461,15 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
         if (print) std::cerr << "You have specified unknown type of compressor: " << a.compressor << '\n';
         return kWrongArgument;
     }
-    if (comp == FEDNL_NATURAL || a.line_search) {
-        std::cerr << (a.line_search ? "--line_search" : "--compressor natural")
-                  << " is not provided by this engine\n";
+    if (a.line_search) {
+        std::cerr << "--line_search is not provided by this engine\n";
         return kWrongArgument;
     }
     const int d = ds.d;
     const size_t S = (size_t)d * (d + 1) / 2;
-    const int k = (comp == FEDNL_IDENTICAL) ? (int)S : (int)(a.k_mult * d);
-    if (comp != FEDNL_IDENTICAL && (size_t)k > S) {
+    const bool whole = (comp == FEDNL_IDENTICAL || comp == FEDNL_NATURAL);  // TrainLoop.h:354-377
+    const int k = whole ? (int)S : (int)(a.k_mult * d);
+    if (!whole && (size_t)k > S) {
         if (print)
             std::cerr << " Multiplier for D is specifed as " << a.k_mult << ". For this problem with D=" << d
                       << " maximum allowable muliplier is " << double(S) / double(d) << "!\n";
@@ -544,7 +544,10 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
         idxb = 0.0;
         for (int c = 0; c < n; ++c) {
             const double items = (comp == FEDNL_TOPLEK) ? (double)cnt[c] : (double)k;
-            scal += 8.0 * d + 8.0 * items + (a.option_b ? 8.0 : 0.0);
+            // Natural: sign + exponent of two values per 3 bytes, one trailing pair
+            // (LocalWorkerForFedNL_Natural.h:80-120)
+            const double item_bytes = (comp == FEDNL_NATURAL) ? 3.0 * (double)(S / 2 + 1) : 8.0 * items;
+            scal += 8.0 * d + item_bytes + (a.option_b ? 8.0 : 0.0);
             if (a.tracking) scal += a.option_b ? 8.0 : 0.0;
             if (comp == FEDNL_TOPK) idxb += 4.0 * k;
             else if (comp == FEDNL_TOPLEK) idxb += 4.0 + 4.0 * cnt[c];

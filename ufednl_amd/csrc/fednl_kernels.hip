@@ -120,6 +120,13 @@ __device__ inline bool pos_is_diag(uint64_t p)
     return r * r == v;
 }
 
+// dopt's Natural compressor on one value (MatrixNMD.h:2009-2029): the mantissa
+// bits cleared, sign and exponent kept
+__device__ inline double natural_round(double v)
+{
+    return __longlong_as_double(__double_as_longlong(v) & (long long)0xFFF0000000000000ull);
+}
+
 // value of a double held by lane `src` (uniform), without an LDS round trip
 __device__ inline double bcast_lane(double v, int src)
 {
@@ -1690,7 +1697,7 @@ __global__ __launch_bounds__(256) void k_emit(FEDNL_KARGS)
         else if (t == ins) p = s_pop;
         else p = sc[t - 1];
         const double dv = dc[p];
-        const double item = scaled ? dv * mult : dv;
+        const double item = scaled ? dv * mult : (comp == FEDNL_NATURAL ? natural_round(dv) : dv);
         vals_out[(size_t)c * k + t] = item * prescale;
         const double upd = alpha * item;
         lc[p] = lc[p] + upd;
@@ -1726,6 +1733,7 @@ __global__ __launch_bounds__(256) void k_master_h(FEDNL_KARGS)
     const size_t S = PD.S;
     const int n = PD.n, bm_words = PD.bm_words, dp = PD.dp;
     const int scaled = (PD.comp == FEDNL_RANDK || PD.comp == FEDNL_RANDSEQK) ? 1 : 0;
+    const bool natural = (PD.comp == FEDNL_NATURAL);
     const double* __restrict__ delta = PD.delta;
     const uint32_t* __restrict__ sel_bm = PD.sel_bm;
     const double mult = PD.mult, prescale = PD.prescale;
@@ -1759,7 +1767,7 @@ __global__ __launch_bounds__(256) void k_master_h(FEDNL_KARGS)
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             if (on[u]) {
-                const double item = scaled ? v[u] * mult : v[u];
+                const double item = scaled ? v[u] * mult : (natural ? natural_round(v[u]) : v[u]);
                 h = h + item * prescale;
             }
         }
@@ -2704,8 +2712,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     e->lda = round_up(d, 4);
     e->S = (size_t)d * (d + 1) / 2;
     const int comp = opts->compressor;
-    if (comp == FEDNL_IDENTICAL)
-        e->k = (int)e->S;  // TrainLoop.h:354-357
+    if (comp == FEDNL_IDENTICAL || comp == FEDNL_NATURAL)
+        e->k = (int)e->S;  // TrainLoop.h:354-377
     else
         e->k = (int)(opts->k_mult * d);  // TrainLoop.h:362
     if (e->k <= 0 || (size_t)e->k > e->S) {
@@ -2714,7 +2722,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         return FEDNL_E_WRONG_ARGUMENT;
     }
     if (comp != FEDNL_TOPK && comp != FEDNL_TOPLEK && comp != FEDNL_RANDK && comp != FEDNL_RANDSEQK &&
-        comp != FEDNL_IDENTICAL) {
+        comp != FEDNL_IDENTICAL && comp != FEDNL_NATURAL) {
         set_err("fednl_create: compressor not supported by this engine");
         delete e;
         return FEDNL_E_UNSUPPORTED;
@@ -3122,7 +3130,7 @@ static size_t engine_footprint(const fednl_opts* o, int n, int m, int d)
 {
     const size_t m_pad = (size_t)round_up(m, kKB), d_pad = (size_t)round_up(d, kTile), dp = (size_t)round_up(d, kNB);
     const size_t S = (size_t)d * (d + 1) / 2;
-    size_t k = (o->compressor == FEDNL_IDENTICAL) ? S : (size_t)(o->k_mult * d);
+    size_t k = (o->compressor == FEDNL_IDENTICAL || o->compressor == FEDNL_NATURAL) ? S : (size_t)(o->k_mult * d);
     size_t b = 8 * (size_t)n * m_pad * d_pad;                 // A
     if (o->hessian_mode == 1) b += 4 * (size_t)n * m_pad * d_pad;
     if (o->hessian_mode == 2) b += 4 * (size_t)n * m_pad * d_pad;
