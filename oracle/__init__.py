@@ -79,6 +79,12 @@ def lib():
         L.orc_fednl_round.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_dp, _c_dp, _c_ip, _c_up, _c_dp, _c_dp, _c_dp]
         L.orc_fednl_get_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
         L.orc_fednl_set_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
+        L.orc_fednl_set_line_search.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int]
+        L.orc_fednl_line_search_count.restype = ctypes.c_int
+        L.orc_fednl_line_search_count.argtypes = [ctypes.c_void_p]
+        L.orc_natural.restype = ctypes.c_double
+        L.orc_natural.argtypes = [ctypes.c_double]
         _lib = L
     return _lib
 
@@ -198,7 +204,8 @@ def cholesky_solve(H_up: np.ndarray, d: int, shift: float, b: np.ndarray):
 class FedNL:
     """The oracle's FedNL run (forcesequpdate semantics)."""
 
-    def __init__(self, prob, compressor="topk", k_mult=8.0, alpha=None, option_b=True, workers=1):
+    def __init__(self, prob, compressor="topk", k_mult=8.0, alpha=None, option_b=True, workers=1,
+                 line_search=None):
         self.prob = prob
         n, m, d = prob.features.shape
         self.n, self.m, self.d = n, m, d
@@ -214,6 +221,9 @@ class FedNL:
         self._lab = np.ascontiguousarray(prob.labels, dtype=np.float64)
         self.h = lib().orc_fednl_create(dp(self._feat), dp(self._lab), n, m, d, prob.lam, self.comp,
                                         self.k, alpha, 1 if option_b else 0, workers)
+        if line_search:
+            c, gamma, steps = line_search
+            lib().orc_fednl_set_line_search(self.h, 1, c, gamma, int(steps))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -228,7 +238,8 @@ class FedNL:
         ok = lib().orc_fednl_round(self.h, 1 if tracking else 0, dp(g), dp(lk), ip(cnt), up(pos), dp(vals),
                                    dp(dx), dp(st))
         return dict(ok=bool(ok), grad=g, lk=lk, cnt=cnt, pos=pos.reshape(n, kk), vals=vals.reshape(n, kk),
-                    dx=dx, grad_norm=st[0], f=st[1], x_norm=st[2])
+                    dx=dx, grad_norm=st[0], f=st[1], x_norm=st[2],
+                    ls_count=lib().orc_fednl_line_search_count(self.h))
 
     def get_state(self):
         x = np.zeros(self.d); L = np.zeros(self.n * self.S); H = np.zeros(self.S)
@@ -260,7 +271,8 @@ def theoretical_alpha(compressor: str, k: int, d: int, option: int = 2) -> float
 
 
 def ref_fednl_run(prob, compressor="topk", k_mult=8.0, alpha=None, option_b=True, rounds=10, workers=1,
-                  tracking=True, dump_rounds=0):
+                  tracking=True, dump_rounds=0, line_search=None):
+    """line_search: None, or (c, gamma, max_steps) for FedNL-LS."""
     """Run the reference building blocks (oracle/_ref) for `rounds` rounds."""
     n, m, d = prob.features.shape
     k = d * (d + 1) // 2 if compressor in ("identical", "natural") else int(k_mult * d)
@@ -274,9 +286,12 @@ def ref_fednl_run(prob, compressor="topk", k_mult=8.0, alpha=None, option_b=True
     kk = max(k, S)
     dg = np.zeros(max(dr, 1) * d); dl = np.zeros(max(dr, 1) * n); dc = np.zeros(max(dr, 1) * n, dtype=np.int32)
     di = np.zeros(max(dr, 1) * n * k, dtype=np.uint32); ddx = np.zeros(max(dr, 1) * d)
+    ls = line_search if line_search else (0.5, 0.5, 0)
+    lsc = np.zeros(rounds, dtype=np.int32)
     st = ref().ref_fednl_run(dp(feat), dp(lab), n, m, d, prob.lam, COMPRESSORS[compressor], k, alpha,
                              1 if option_b else 0, rounds, workers, 1 if tracking else 0, dp(gn), dp(fv), dp(xn),
-                             dp(x), dr, dp(dg), dp(dl), ip(dc), up(di), dp(ddx), dp(sec))
-    return dict(status=st, grad_norm=gn, f=fv, x_norm=xn, x=x, seconds=sec,
+                             dp(x), dr, dp(dg), dp(dl), ip(dc), up(di), dp(ddx), dp(sec),
+                             ctypes.c_int(ls[2]), ctypes.c_double(ls[0]), ctypes.c_double(ls[1]), ip(lsc))
+    return dict(status=st, grad_norm=gn, f=fv, x_norm=xn, x=x, seconds=sec, ls_counts=lsc,
                 dump=dict(grad=dg.reshape(-1, d)[:dr], lk=dl.reshape(-1, n)[:dr], cnt=dc.reshape(-1, n)[:dr],
                           idx=di.reshape(-1, n, k)[:dr], dx=ddx.reshape(-1, d)[:dr]))

@@ -598,7 +598,22 @@ typedef struct
     double *grads, *lks, *fis, *hx, *delta, *vals;
     uint32_t *pos;
     int *cnt;
+    /* FedNL-LS (TrainLoop.h:344-355, 1546-1640): Armijo backtracking on the
+     * Newton direction from t = 1 */
+    int ls, ls_max, ls_count;
+    double ls_c, ls_gamma;
 } orc_fednl;
+
+ORC_API void orc_fednl_set_line_search(void *h, int on, double c, double gamma, int max_steps)
+{
+    orc_fednl *f = (orc_fednl *)h;
+    f->ls = on;
+    f->ls_c = c;
+    f->ls_gamma = gamma;
+    f->ls_max = max_steps;
+}
+
+ORC_API int orc_fednl_line_search_count(void *h) { return ((orc_fednl *)h)->ls_count; }
 
 ORC_API void *orc_fednl_create(const double *feat, const double *labels, int n, int m, int d,
                                double lambda, int compressor, int k, double alpha, int option_b,
@@ -707,7 +722,7 @@ ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int
         double *L = f->learned + (size_t)c * S;
         double fi = 0.0;
         orc_client_eval(feat, lab, d, m, f->lambda, f->x, NULL, NULL, f->grads + (size_t)c * d, f->hx,
-                        tracking ? &fi : NULL, NULL);
+                        (tracking || f->ls) ? &fi : NULL, NULL);
         f->fis[c] = fi;
         for (size_t p = 0; p < S; ++p)
             f->delta[p] = f->hx[p] - L[p];
@@ -789,8 +804,45 @@ ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int
     int ok = orc_cholesky_packed(f->H, d, shift, U);
     if (ok)
         orc_tri_solve_packed(U, d, g, step);
-    for (int i = 0; i < d; ++i)
-        f->x[i] = f->x[i] - step[i];
+    if (!f->ls)
+    {
+        for (int i = 0; i < d; ++i)
+            f->x[i] = f->x[i] - step[i];
+    }
+    else
+    {
+        /* TrainLoop.h:1550-1637: t = 1, f(x_k) from the round, d^k = -dx;
+         * accept the first t with f(x_k - t dx) <= f(x_k) + c t (-(dx, g)),
+         * else t *= gamma; after max_steps the last trial point is kept */
+        double dot = 0.0;
+        for (int i = 0; i < d; ++i)
+            dot = dot + step[i] * g[i];
+        const double dd = -dot;
+        double t = 1.0;
+        double *xprev = (double *)malloc(sizeof(double) * d);
+        memcpy(xprev, f->x, sizeof(double) * d);
+        int kls = 0;
+        while (kls < f->ls_max)
+        {
+            for (int i = 0; i < d; ++i)
+                f->x[i] = xprev[i] - t * step[i];
+            double freq = 0.0;
+            for (int c = 0; c < n; ++c)
+            {
+                double fi = 0.0;
+                orc_client_eval(f->feat + (size_t)c * m * d, f->labels + (size_t)c * m, d, m, f->lambda, f->x,
+                                NULL, NULL, NULL, NULL, &fi, NULL);
+                freq = freq + fi;
+            }
+            freq = freq * f->inv_n;
+            ++kls;
+            if (freq <= fagg + (f->ls_c * t) * dd)
+                break;
+            t = t * f->ls_gamma;
+        }
+        f->ls_count = kls;
+        free(xprev);
+    }
 
     double gg = 0.0;
     for (int i = 0; i < d; ++i)
@@ -807,7 +859,7 @@ ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int
     if (stats)
     {
         stats[0] = sqrt(gg);
-        stats[1] = tracking ? fagg : 0.0;
+        stats[1] = (tracking || f->ls) ? fagg : 0.0;
         stats[2] = xnorm;
     }
     free(g);

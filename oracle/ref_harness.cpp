@@ -328,8 +328,10 @@ extern "C"
                       int rounds, int workers, int tracking,
                       double* grad_norm, double* fval, double* x_norm, double* x_out,
                       int dump_rounds, double* dump_grad, double* dump_lk, int* dump_cnt,
-                      uint32_t* dump_idx, double* dump_dx, double* seconds)
+                      uint32_t* dump_idx, double* dump_dx, double* seconds,
+                      int ls_max, double ls_c, double ls_gamma, int* ls_counts)
     {
+        // ls_max > 0: FedNL-LS (TrainLoop.h:344-355, 1546-1640)
         if (workers < 1)
             workers = 1;
         if (workers > n)
@@ -436,7 +438,7 @@ extern "C"
                         }
                         idxs[c].swap(kc);
                         vals[c].swap(vc);
-                        if (tracking)
+                        if (tracking || ls_max > 0)
                             fis[c] = p->evaluateFunction(x, t, s);
                     }
                 });
@@ -489,14 +491,43 @@ extern "C"
                 }
             }
 
-            x -= dx;
-
             double fagg = 0.0;
-            if (tracking)
+            if (tracking || ls_max > 0)
             {
                 for (int c = 0; c < n; ++c)
                     fagg += fis[c];
                 fagg *= inv_n;
+            }
+            if (ls_max <= 0)
+            {
+                x -= dx;
+            }
+            else
+            {
+                // TrainLoop.h:1550-1637, with the reference's own vector ops
+                double t = 1.0;
+                const double dd = -(dx & g);
+                Vec xprev = x;
+                int kls = 0;
+                while (kls < ls_max)
+                {
+                    x = xprev - (t * dx);
+                    double freq = 0.0;
+                    for (int c = 0; c < n; ++c)
+                    {
+                        Problem* p = probs[c];
+                        Vec tm = p->evaluateClassificationMargin(x);
+                        Vec sm = p->evaluateClassificationMarginSigmoid(tm);
+                        freq += p->evaluateFunction(x, tm, sm);
+                    }
+                    freq *= inv_n;
+                    ++kls;
+                    if (freq <= fagg + (ls_c * t) * dd)
+                        break;
+                    t *= ls_gamma;
+                }
+                if (ls_counts)
+                    ls_counts[r] = kls;
             }
             if (grad_norm)
                 grad_norm[r] = g.vectorL2Norm();
