@@ -83,6 +83,7 @@ def lib():
                                                 ctypes.c_int]
         L.orc_fednl_line_search_count.restype = ctypes.c_int
         L.orc_fednl_line_search_count.argtypes = [ctypes.c_void_p]
+        L.orc_gauss_solve.argtypes = [_c_dp, _c_dp, ctypes.c_int, _c_dp]
         L.orc_natural.restype = ctypes.c_double
         L.orc_natural.argtypes = [ctypes.c_double]
         _lib = L
@@ -111,7 +112,8 @@ def ref():
         R.ref_fednl_run.argtypes = [_c_dp, _c_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, _c_dp, _c_dp, _c_dp, _c_dp, ctypes.c_int,
-                                    _c_dp, _c_dp, _c_ip, _c_up, _c_dp, _c_dp]
+                                    _c_dp, _c_dp, _c_ip, _c_up, _c_dp, _c_dp, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, _c_ip, _c_ip]
         _ref = R
     return _ref
 
@@ -272,8 +274,8 @@ def theoretical_alpha(compressor: str, k: int, d: int, option: int = 2) -> float
 
 def ref_fednl_run(prob, compressor="topk", k_mult=8.0, alpha=None, option_b=True, rounds=10, workers=1,
                   tracking=True, dump_rounds=0, line_search=None):
-    """line_search: None, or (c, gamma, max_steps) for FedNL-LS."""
-    """Run the reference building blocks (oracle/_ref) for `rounds` rounds."""
+    """Run the reference building blocks (oracle/_ref) for `rounds` rounds.
+    line_search: None, or (c, gamma, max_steps) for FedNL-LS."""
     n, m, d = prob.features.shape
     k = d * (d + 1) // 2 if compressor in ("identical", "natural") else int(k_mult * d)
     if alpha is None:
@@ -288,10 +290,11 @@ def ref_fednl_run(prob, compressor="topk", k_mult=8.0, alpha=None, option_b=True
     di = np.zeros(max(dr, 1) * n * k, dtype=np.uint32); ddx = np.zeros(max(dr, 1) * d)
     ls = line_search if line_search else (0.5, 0.5, 0)
     lsc = np.zeros(rounds, dtype=np.int32)
+    cok = np.zeros(rounds, dtype=np.int32)
     st = ref().ref_fednl_run(dp(feat), dp(lab), n, m, d, prob.lam, COMPRESSORS[compressor], k, alpha,
                              1 if option_b else 0, rounds, workers, 1 if tracking else 0, dp(gn), dp(fv), dp(xn),
                              dp(x), dr, dp(dg), dp(dl), ip(dc), up(di), dp(ddx), dp(sec),
-                             ctypes.c_int(ls[2]), ctypes.c_double(ls[0]), ctypes.c_double(ls[1]), ip(lsc))
-    return dict(status=st, grad_norm=gn, f=fv, x_norm=xn, x=x, seconds=sec, ls_counts=lsc,
+                             ctypes.c_int(ls[2]), ctypes.c_double(ls[0]), ctypes.c_double(ls[1]), ip(lsc), ip(cok))
+    return dict(status=st, grad_norm=gn, f=fv, x_norm=xn, x=x, seconds=sec, ls_counts=lsc, chol_ok=cok,
                 dump=dict(grad=dg.reshape(-1, d)[:dr], lk=dl.reshape(-1, n)[:dr], cnt=dc.reshape(-1, n)[:dr],
                           idx=di.reshape(-1, n, k)[:dr], dx=ddx.reshape(-1, d)[:dr]))

@@ -583,6 +583,75 @@ ORC_API double orc_natural(double v)
     return v;
 }
 
+/* Gauss-Jordan elimination with partial pivoting to reduced row-echelon
+ * form (GaussEliminationSolvers.h:27-120, zero tolerance): for each column,
+ * the row with the first largest |a(i, col)| at or below the current row is
+ * swapped up, the row and b are divided by the pivot (b by division, the row
+ * by the reciprocal), and the column is eliminated from every other row.
+ * M: d x d row-major (not modified), x[pivot column of row i] = b[i]. */
+ORC_API void orc_gauss_solve(const double *M, const double *b, int d, double *x)
+{
+    double *a = (double *)malloc(sizeof(double) * (size_t)d * d);
+    double *bb = (double *)malloc(sizeof(double) * d);
+    int *piv = (int *)malloc(sizeof(int) * d);
+    memcpy(a, M, sizeof(double) * (size_t)d * d);
+    memcpy(bb, b, sizeof(double) * d);
+    int row = 0, col = 0, np = 0;
+    while (row < d && col < d)
+    {
+        double best = fabs(a[(size_t)row * d + col]);
+        int br = row;
+        for (int i = row + 1; i < d; ++i)
+        {
+            const double v = fabs(a[(size_t)i * d + col]);
+            if (v > best)
+            {
+                br = i;
+                best = v;
+            }
+        }
+        if (br != row)
+        {
+            for (int j = 0; j < d; ++j)
+            {
+                const double t = a[(size_t)row * d + j];
+                a[(size_t)row * d + j] = a[(size_t)br * d + j];
+                a[(size_t)br * d + j] = t;
+            }
+            const double t = bb[row];
+            bb[row] = bb[br];
+            bb[br] = t;
+        }
+        if (best < 0.0)
+        {
+            col += 1;
+            continue;
+        }
+        bb[row] = bb[row] / a[(size_t)row * d + col];
+        const double inv = 1.0 / a[(size_t)row * d + col];
+        for (int j = 0; j < d; ++j)
+            a[(size_t)row * d + j] = a[(size_t)row * d + j] * inv;
+        for (int i = 0; i < d; ++i)
+        {
+            if (i == row)
+                continue;
+            const double kq = a[(size_t)i * d + col];
+            bb[i] = bb[i] - bb[row] * kq;
+            for (int j = 0; j < d; ++j)
+                a[(size_t)i * d + j] = a[(size_t)i * d + j] + a[(size_t)row * d + j] * (-kq);
+            a[(size_t)i * d + col] = 0.0;
+        }
+        piv[np++] = col;
+        ++row;
+        ++col;
+    }
+    for (int i = 0; i < d; ++i)
+        x[i] = 0.0;
+    for (int i = 0; i < np; ++i)
+        x[piv[i]] = bb[i];
+    free(a); free(bb); free(piv);
+}
+
 typedef struct
 {
     int n, m, d, lda, k, compressor, option_b, workers;
@@ -592,6 +661,8 @@ typedef struct
     double *x;                   /* d */
     double *learned;             /* n*S */
     double *H;                   /* S */
+    double *H0;                  /* S: H after creation; the master's full H keeps it as its
+                                    lower triangle (only upper entries are ever updated) */
     uint32_t *rng;               /* n client RandK/SeqK generators, seed 123+c */
     uint32_t *lek;               /* workers TopLEK streams, seed 1234 */
     /* scratch */
@@ -659,6 +730,8 @@ ORC_API void *orc_fednl_create(const double *feat, const double *labels, int n, 
     }
     for (size_t p = 0; p < S; ++p)
         f->H[p] = f->H[p] * f->inv_n;
+    f->H0 = (double *)malloc(sizeof(double) * S);
+    memcpy(f->H0, f->H, sizeof(double) * S);
     return f;
 }
 
@@ -667,7 +740,7 @@ ORC_API void orc_fednl_destroy(void *h)
     orc_fednl *f = (orc_fednl *)h;
     if (!f)
         return;
-    free(f->x); free(f->learned); free(f->H); free(f->rng); free(f->lek);
+    free(f->x); free(f->learned); free(f->H); free(f->H0); free(f->rng); free(f->lek);
     free(f->grads); free(f->lks); free(f->fis); free(f->hx); free(f->delta);
     free(f->vals); free(f->pos); free(f->cnt);
     free(f);
@@ -804,6 +877,23 @@ ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int
     int ok = orc_cholesky_packed(f->H, d, shift, U);
     if (ok)
         orc_tri_solve_packed(U, d, g, step);
+    else
+    {
+        /* TrainLoop.h:1410-1420 / 1476-1486: "SST FACTORIZATION FAILED", then
+         * dx = gausEleminationSolver(H + shift I, g) on the master's full H:
+         * the current upper triangle and, below it, the creation-time values */
+        double *M = (double *)malloc(sizeof(double) * (size_t)d * d);
+        for (int j = 0; j < d; ++j)
+            for (int i = 0; i <= j; ++i)
+            {
+                const size_t p = (size_t)j * (j + 1) / 2 + i;
+                M[(size_t)i * d + j] = f->H[p] + (i == j ? shift : 0.0);
+                if (i != j)
+                    M[(size_t)j * d + i] = f->H0[p];
+            }
+        orc_gauss_solve(M, g, d, step);
+        free(M);
+    }
     if (!f->ls)
     {
         for (int i = 0; i < d; ++i)

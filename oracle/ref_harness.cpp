@@ -24,6 +24,7 @@
 #include "dopt/random/include/Shuffle.h"
 #include "dopt/linalg_matrices/include/factorization/CholeskyFactorization.h"
 #include "dopt/linalg_linsolvers/include/ElementarySolvers.h"
+#include "dopt/linalg_linsolvers/include/GaussEliminationSolvers.h"
 
 #include <stdint.h>
 #include <string.h>
@@ -329,7 +330,7 @@ extern "C"
                       double* grad_norm, double* fval, double* x_norm, double* x_out,
                       int dump_rounds, double* dump_grad, double* dump_lk, int* dump_cnt,
                       uint32_t* dump_idx, double* dump_dx, double* seconds,
-                      int ls_max, double ls_c, double ls_gamma, int* ls_counts)
+                      int ls_max, double ls_c, double ls_gamma, int* ls_counts, int* chol_ok)
     {
         // ls_max > 0: FedNL-LS (TrainLoop.h:344-355, 1546-1640)
         if (workers < 1)
@@ -472,8 +473,15 @@ extern "C"
             }
             else
             {
+                // TrainLoop.h:1410-1420, 1476-1486: Gauss elimination on the
+                // master's full H (its lower triangle never updated) + shift I
+                Mat slau = H;
+                slau.addToAllDiagonalEntries(shift);
+                dx = dopt::gausEleminationSolver(slau, g);
                 status = 1;
             }
+            if (chol_ok)
+                chol_ok[r] = ok ? 1 : 0;
 
             if (r < dump_rounds)
             {
@@ -537,8 +545,6 @@ extern "C"
                 x_norm[r] = xnorm;
             if (seconds)
                 seconds[r] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            if (!ok)
-                break;
         }
 
         for (int i = 0; i < d; ++i)

@@ -68,7 +68,7 @@ def _read_tracking(path):
     return strs, ints, dbl, rec
 
 
-def _oracle_grad_norms(oracle, X, y, order, rounds, k_mult):
+def _oracle_grad_norms(oracle, X, y, order, rounds, k_mult, line_search=None, trials=False, option_b=True):
     from ufednl_amd.datagen import Problem
     m = N_SAMPLES // N_CLIENTS
     Xi = np.hstack([X, np.ones((N_SAMPLES, 1))])  # --add_intercept: last column
@@ -76,8 +76,10 @@ def _oracle_grad_norms(oracle, X, y, order, rounds, k_mult):
     feats = Xi[idx].reshape(N_CLIENTS, m, D_FEAT + 1)
     labs = y[idx].reshape(N_CLIENTS, m)
     prob = Problem("cli", np.ascontiguousarray(feats), np.ascontiguousarray(labs), 1e-3, np.zeros(D_FEAT + 1))
-    f = oracle.FedNL(prob, "randk", k_mult=k_mult)
-    return np.array([f.round(tracking=False)["grad_norm"] for _ in range(rounds)])
+    f = oracle.FedNL(prob, "randk", k_mult=k_mult, option_b=option_b, line_search=line_search)
+    outs = [f.round(tracking=False) for _ in range(rounds)]
+    g = np.array([o["grad_norm"] for o in outs])
+    return (g, np.array([o["ls_count"] for o in outs])) if trials else g
 
 
 def _args(data, out, rounds, extra=()):
@@ -114,6 +116,44 @@ def test_cli_tracking_file_matches_oracle(oracle, tmp_path, reshuffle_seed):
     # f(x) is tracked and decreases; ||x|| is finite
     fv = np.array([r[2] for r in rec])
     assert np.all(np.isfinite(fv)) and fv[-1] < fv[0]
+
+
+def test_cli_line_search(oracle, tmp_path):
+    """--line_search: tracking header says FedNL-LS, the trajectory and the
+    per-round trial counts follow the oracle, every trial adds one 8-byte f_i
+    per client to the received bytes (TrainLoop.h:760-761, 1593, 1671)."""
+    X, y = _dataset()
+    data = str(tmp_path / "train.libsvm")
+    _write_libsvm(data, X, y)
+    out = str(tmp_path / "ls.bin")
+    rounds = 15
+    ls = ("--line_search", "--c_line_search", "0.49", "--gamma_line_search", "0.5", "--max_line_search_steps", "4")
+    args = _args(data, out, rounds, ls)
+    i = args.index("--fednl-option-b")
+    args[i] = "--fednl-option-a"
+    i = args.index("--silent")
+    del args[i]
+    p = subprocess.run([BIN, *args], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    strs, ints, dbl, rec = _read_tracking(out)
+    assert strs[1] == "FedNL-LS (a)"
+    g_ref, cnt_ref = _oracle_grad_norms(oracle, X, y, np.arange(N_SAMPLES), rounds, 2.0, (0.49, 0.5, 4),
+                                        trials=True, option_b=False)
+    g = np.array([r[1] for r in rec])
+    live = int(np.argmax(g_ref <= 1e-7)) if np.any(g_ref <= 1e-7) else rounds
+    assert_grad_norms_close(g[:live + 1], g_ref[:live + 1])
+    printed = [int(line.rsplit(":", 1)[1]) for line in p.stdout.splitlines() if "line-search num:" in line]
+    assert len(printed) == rounds
+    np.testing.assert_array_equal(printed[:live], cnt_ref[:live])
+    assert "Line Search: [YES]" in p.stdout
+    # option a: gradient (8d) + k values (8k) per client, no Lk message, and
+    # the trials' f_i messages
+    d = D_FEAT + 1
+    k = 2 * d
+    acc = np.array([r[4] for r in rec])
+    per_round = np.diff(acc)
+    expect = [N_CLIENTS * (8.0 * d + 8.0 * k + 8.0 * t) for t in printed[:-1]]
+    np.testing.assert_array_equal(per_round, expect)
 
 
 def test_dopt_execute_local_callback(oracle, tmp_path):

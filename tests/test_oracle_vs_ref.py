@@ -120,3 +120,46 @@ def test_config1_topk_first_rounds(oracle, ref):
     for rr in range(2):
         for c in range(prob.n):
             np.testing.assert_array_equal(oracle.pos_to_flat(outs[rr]["pos"][c], prob.d), r["dump"]["idx"][rr, c])
+
+
+# A failed Cholesky factorization (a negative pivot of H + shift I, here with
+# option a and a small k) falls back to Gauss-Jordan elimination on the
+# master's full H, whose lower triangle keeps its creation-time values
+# (TrainLoop.h:1410-1420, 1476-1486; GaussEliminationSolvers.h:27-120).
+FALLBACK_CASES = [("topk", 4, 1.0, None), ("topk", 10, 2.0, None), ("toplek", 6, 4.0, (0.49, 0.5, 5)),
+                  ("topk", 8, 2.0, (0.49, 0.5, 2))]
+
+
+@pytest.mark.parametrize("comp,seed,scale,ls", FALLBACK_CASES)
+def test_fednl_factorization_fallback(oracle, comp, seed, scale, ls):
+    prob = datagen.small_problem(n=5, m=40, d_feat=11, seed=seed)
+    prob.features[...] *= scale
+    R = 25
+    r = oracle.ref_fednl_run(prob, comp, 0.5, option_b=False, rounds=R, tracking=False, line_search=ls)
+    f = oracle.FedNL(prob, comp, k_mult=0.5, option_b=False, line_search=ls)
+    outs = [f.round(tracking=False) for _ in range(R)]
+    ok = np.array([o["ok"] for o in outs], dtype=np.int32)
+    assert (ok == 0).any()
+    np.testing.assert_array_equal(ok, r["chol_ok"])
+    g = np.array([o["grad_norm"] for o in outs])
+    live = int(np.argmin(r["grad_norm"] > 1e-7)) if np.any(r["grad_norm"] <= 1e-7) else R
+    assert_grad_norms_close(g[:live + 1], r["grad_norm"][:live + 1])
+
+
+def test_gauss_solve_known_answers(oracle):
+    """Gauss-Jordan restatement: exact on an integer system that needs row
+    swaps (zero leading entry) and equal to numpy's solve on random
+    non-symmetric matrices."""
+    M = np.array([[0.0, 2.0, 1.0], [1.0, -2.0, -3.0], [-1.0, 1.0, 2.0]])
+    xs = np.array([1.0, -2.0, 3.0])
+    b = M @ xs
+    x = np.zeros(3)
+    oracle.lib().orc_gauss_solve(oracle.dp(M.copy()), oracle.dp(b.copy()), 3, oracle.dp(x))
+    np.testing.assert_allclose(x, xs, rtol=0, atol=1e-14)
+    rng = np.random.default_rng(3)
+    for d in (1, 7, 40):
+        A = rng.standard_normal((d, d))
+        bb = rng.standard_normal(d)
+        x = np.zeros(d)
+        oracle.lib().orc_gauss_solve(oracle.dp(A.copy()), oracle.dp(bb.copy()), d, oracle.dp(x))
+        np.testing.assert_allclose(x, np.linalg.solve(A, bb), rtol=1e-10, atol=1e-12)
