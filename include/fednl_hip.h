@@ -63,7 +63,12 @@ typedef struct fednl_opts {
     int hessian_mode;     /* A^T D A contraction: 0 FP64 MFMA (default); 1 FP32 MFMA;
                              2 BF16 hi/lo split with FP32 accumulation. Gradient, Delta,
                              Lk, aggregation and the Newton solve stay FP64. */
-    int reserved[7];
+    int line_search;      /* 1: FedNL-LS, Armijo backtracking on the Newton step from t = 1
+                             (--line_search, TrainLoop.h:344-355, 1546-1640) */
+    int max_line_search_steps;  /* trial points per round (--max_line_search_steps, default 5) */
+    double c_line_search;       /* sufficient-decrease constant c (--c_line_search, default 0.5) */
+    double gamma_line_search;   /* step shrink factor in (0, 1) (--gamma_line_search, default 0.5) */
+    int reserved[4];
 } fednl_opts;
 
 void fednl_default_opts(fednl_opts *o);
@@ -91,9 +96,10 @@ double fednl_alpha(void *handle);
 
 /* Enqueues `rounds` FedNL rounds on the engine's stream and waits for them.
  * One round = dopt's client loop (LocalWorkerForFedNL_*.h) for every client
- * plus the master update (TrainLoop.h:1054-1508). Returns FEDNL_OK, or
- * FEDNL_E_INTERNAL if a Cholesky factorization failed (the round records it
- * in the trace and the iterate is left unchanged). */
+ * plus the master update (TrainLoop.h:1054-1640). A round whose Cholesky
+ * factorization fails solves the system by Gauss-Jordan elimination instead,
+ * as dopt's master does (TrainLoop.h:1410-1420, 1476-1486), and records the
+ * failure in the trace (chol_ok = 0). */
 int fednl_run(void *handle, int rounds);
 
 /* Enqueues rounds without waiting (graph replay when enabled). */
@@ -101,10 +107,15 @@ int fednl_enqueue(void *handle, int rounds);
 int fednl_sync(void *handle);
 
 /* Per-round trace since creation: ||grad f(x_r)||, f(x_r) (0 without
- * tracking), ||x_r||, seconds since the first round was enqueued (from
- * device events), factorization flag (1 ok). Arrays sized >= *count. */
+ * tracking or line search), ||x_r||, seconds since the first round was
+ * enqueued (from device events), factorization flag (1 ok, 0: the round used
+ * the Gauss-Jordan fallback). Arrays sized >= *count. */
 int fednl_trace(void *handle, double *grad_norm, double *fval, double *x_norm, double *seconds,
                 int *chol_ok, int *count);
+
+/* FedNL-LS: trial points evaluated in each round since creation (1..max;
+ * 0 when line search is off). Arrays sized >= *count. */
+int fednl_trace_line_search(void *handle, int *trials, int *count);
 
 int fednl_get_x(void *handle, double *x);
 

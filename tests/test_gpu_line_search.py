@@ -11,7 +11,7 @@ values that differ by less than their own rounding, on either side.
 import numpy as np
 import pytest
 
-from tests.test_oracle_vs_ref import LS_CASES
+from tests.test_oracle_vs_ref import FALLBACK_CASES, LS_CASES
 from tests.tolerances import assert_grad_norms_close
 
 pytestmark = pytest.mark.gpu
@@ -44,7 +44,7 @@ def test_line_search_trajectory_matches_oracle(oracle, comp, seed, scale, option
     tr = eng.trace()
     live = _live(g_ref)
     assert live >= 5
-    assert_grad_norms_close(tr["grad_norm"][:live + 1], g_ref[:live + 1])
+    assert_grad_norms_close(tr["grad_norm"][:live + 1], g_ref[:live + 1], feature_scale=scale)
     np.testing.assert_array_equal(tr["ls_trials"][:live], cnt_ref[:live])
     assert np.all((tr["ls_trials"] >= 1) & (tr["ls_trials"] <= ls[2]))
     # f(x_k) of each round is reported with line search on, as with tracking
@@ -125,3 +125,58 @@ def test_line_search_in_batch_matches_solo(use_graph):
         np.testing.assert_array_equal(tr["ls_trials"], cnt, err_msg=f"member {i}")
         np.testing.assert_array_equal(e.x(), x, err_msg=f"member {i}")
         e.close()
+
+
+@pytest.mark.parametrize("comp,seed,scale,ls", FALLBACK_CASES)
+def test_factorization_fallback_matches_oracle(oracle, comp, seed, scale, ls):
+    """Rounds whose Cholesky factorization fails are solved by Gauss-Jordan
+    elimination on the master's full H (upper part current, lower part as at
+    creation), with or without line search; the failing rounds and the
+    trajectory follow the oracle."""
+    from ufednl_amd.engine import Engine
+    prob = _problem(seed, scale)
+    R = 25
+    f = oracle.FedNL(prob, comp, k_mult=0.5, option_b=False, line_search=ls)
+    outs = [f.round(tracking=False) for _ in range(R)]
+    g_ref = np.array([o["grad_norm"] for o in outs])
+    ok_ref = np.array([o["ok"] for o in outs], dtype=np.int32)
+    assert (ok_ref == 0).any()
+    eng = Engine.from_problem(prob, compressor=comp, k_mult=0.5, option_b=False, tracking=False, line_search=ls)
+    eng.run(R)
+    tr = eng.trace()
+    live = _live(g_ref)
+    np.testing.assert_array_equal(tr["chol_ok"][:live + 1], ok_ref[:live + 1])
+    assert_grad_norms_close(tr["grad_norm"][:live + 1], g_ref[:live + 1], feature_scale=scale)
+    eng.close()
+
+
+def test_fallback_step_equals_oracle_solve(oracle):
+    """The fallback's dx for a failing round, from the engine's own H and g,
+    equals the oracle's Gauss-Jordan solve of the same system bit for bit
+    (both round every product and sum separately)."""
+    from ufednl_amd.engine import Engine
+    comp, seed, scale, ls = FALLBACK_CASES[0]
+    prob = _problem(seed, scale)
+    eng = Engine.from_problem(prob, compressor=comp, k_mult=0.5, option_b=False, tracking=False)
+    n, m, d = prob.features.shape
+    H0 = eng.get_state()["H"].copy()
+    failed = False
+    for r in range(10):
+        eng.run(1)
+        if eng.trace()["chol_ok"][r] == 0:
+            failed = True
+            break
+    assert failed
+    H = eng.get_state()["H"]
+    last = eng.last_round()
+    M = np.zeros((d, d))
+    for j in range(d):
+        for i in range(j + 1):
+            p = j * (j + 1) // 2 + i
+            M[i, j] = H[p] + (prob.lam if i == j else 0.0)
+            if i != j:
+                M[j, i] = H0[p]
+    x = np.zeros(d)
+    oracle.lib().orc_gauss_solve(oracle.dp(M), oracle.dp(last["grad"].copy()), d, oracle.dp(x))
+    np.testing.assert_array_equal(last["dx"], x)
+    eng.close()

@@ -104,7 +104,7 @@ def test_fednl_line_search_trajectories(oracle, comp, seed, scale, option_b, ls)
     # excluded) follow acceptance decisions that are not rounding-bound
     live = int(np.argmin(r["grad_norm"] > 1e-7)) if np.any(r["grad_norm"] <= 1e-7) else R
     assert live >= 5 and all(o["ok"] for o in outs)
-    assert_grad_norms_close(g[:live + 1], r["grad_norm"][:live + 1])
+    assert_grad_norms_close(g[:live + 1], r["grad_norm"][:live + 1], feature_scale=scale)
     np.testing.assert_array_equal(cnt[:live], np.asarray(r["ls_counts"])[:live])
     assert np.all((cnt >= 1) & (cnt <= ls[2]))
     return cnt[:live]
@@ -143,7 +143,7 @@ def test_fednl_factorization_fallback(oracle, comp, seed, scale, ls):
     np.testing.assert_array_equal(ok, r["chol_ok"])
     g = np.array([o["grad_norm"] for o in outs])
     live = int(np.argmin(r["grad_norm"] > 1e-7)) if np.any(r["grad_norm"] <= 1e-7) else R
-    assert_grad_norms_close(g[:live + 1], r["grad_norm"][:live + 1])
+    assert_grad_norms_close(g[:live + 1], r["grad_norm"][:live + 1], feature_scale=scale)
 
 
 def test_gauss_solve_known_answers(oracle):

@@ -461,10 +461,6 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
         if (print) std::cerr << "You have specified unknown type of compressor: " << a.compressor << '\n';
         return kWrongArgument;
     }
-    if (a.line_search) {
-        std::cerr << "--line_search is not provided by this engine\n";
-        return kWrongArgument;
-    }
     const int d = ds.d;
     const size_t S = (size_t)d * (d + 1) / 2;
     const bool whole = (comp == FEDNL_IDENTICAL || comp == FEDNL_NATURAL);  // TrainLoop.h:354-377
@@ -509,6 +505,10 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
     o.option_b = a.option_b ? 1 : 0;
     o.workers = a.workers;
     o.tracking = a.tracking ? 1 : 0;
+    o.line_search = a.line_search ? 1 : 0;
+    o.c_line_search = a.c_ls;
+    o.gamma_line_search = a.gamma_ls;
+    o.max_line_search_steps = (int)a.max_ls;
     void* eng = nullptr;
     rc = fednl_create(&o, n, mpc, d, feats.data(), labs.data(), &eng);
     if (rc != FEDNL_OK) {
@@ -537,9 +537,15 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
     std::vector<double> x(d), glast(d);
     fednl_get_x(eng, x.data());
     fednl_last_round(eng, glast.data(), nullptr, cnt.data(), nullptr, nullptr, nullptr, nullptr);
+    std::vector<int> ls_trials(R, 0);
+    fednl_trace_line_search(eng, ls_trials.data(), &R);
 
-    // bytes received from the clients per round (TrainLoop.h:1104,1131-1132,1287,1533)
-    auto round_bytes = [&](double& scal, double& idxb) {
+    // bytes received from the clients per round (TrainLoop.h:1104,1131-1132,1287,1533,1593):
+    // the f_i message of the round is counted with the size of the Lk message
+    // (8 bytes with option b, none with option a); each line-search trial adds
+    // one 8-byte f_i per client
+    const bool send_fi = a.tracking || a.line_search;  // TrainLoop.h:620
+    auto round_bytes = [&](int trials, double& scal, double& idxb) {
         scal = 0.0;
         idxb = 0.0;
         for (int c = 0; c < n; ++c) {
@@ -548,7 +554,8 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
             // (LocalWorkerForFedNL_Natural.h:80-120)
             const double item_bytes = (comp == FEDNL_NATURAL) ? 3.0 * (double)(S / 2 + 1) : 8.0 * items;
             scal += 8.0 * d + item_bytes + (a.option_b ? 8.0 : 0.0);
-            if (a.tracking) scal += a.option_b ? 8.0 : 0.0;
+            if (send_fi) scal += a.option_b ? 8.0 : 0.0;
+            scal += 8.0 * trials;
             if (comp == FEDNL_TOPK) idxb += 4.0 * k;
             else if (comp == FEDNL_TOPLEK) idxb += 4.0 + 4.0 * cnt[c];
             else if ((comp == FEDNL_RANDK || comp == FEDNL_RANDSEQK) && a.transfer_idx) idxb += 4.0 * k;
@@ -576,16 +583,17 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
     }
     double acc_s = 0.0, acc_i = 0.0;
     for (int r = 0; r < R; ++r) {
+        // the master reports the fallback solve before the round line
+        if (!ok[r] && print)
+            std::cout << "SST FACTORIZATION FAILED\n";
         if (!a.silent && print) {
             const double t = sec[r] > 0 ? sec[r] : 1e-9;
             std::cout << "  Round #" << r << ": norm of full gradient: " << gn[r];
-            if (a.tracking) std::cout << ": f(x): " << fv[r];
+            if (send_fi) std::cout << ": f(x): " << fv[r];
             else std::cout << ": f(x): [N/A]";
             std::cout << ": avg. time per round: " << 1e3 * t / (r + 1) << " msec"
-                      << ": rounds/second: " << (r + 1) / t << ": line-search num: 0\n";
+                      << ": rounds/second: " << (r + 1) / t << ": line-search num: " << ls_trials[r] << '\n';
         }
-        if (!ok[r] && print)
-            std::cout << "SST FACTORIZATION FAILED\n";
         if (a.tracking) {
             tr.i32(r);
             tr.f64(gn[r]);
@@ -596,7 +604,7 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
             tr.f64(sec[r]);
         }
         double bs, bi;
-        round_bytes(bs, bi);
+        round_bytes(ls_trials[r], bs, bi);
         acc_s += bs;
         acc_i += bi;
     }
@@ -624,6 +632,9 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
         std::cout << "  Clients compressor: " << a.compressor << '\n';
         std::cout << "  Clients compressor K: " << k << " / Hessian shape is [d,d] where d: " << d
                   << " / Maximum K ( {d(d+1)}/2 ): " << S << '\n';
+        std::cout << '\n';
+        std::cout << "  Line Search: " << (a.line_search ? "[YES]" : "[NO]") << '\n';
+        std::cout << "  Line Search parameters: " << "c: " << a.c_ls << ", gamma: " << a.gamma_ls << '\n';
         std::cout << "=========================================================================\n";
         std::cout << "Solution Properties\n\n";
         std::cout << "  Last iterate CRC-32: " << crc32_bytes(x.data(), sizeof(double) * d) << '\n';

@@ -89,6 +89,12 @@ constexpr int kKC = 16;            // samples staged per LDS chunk
 constexpr int kPadLds = 4;         // LDS row padding (doubles)
 constexpr int kMaxHeapK = 8192;    // LDS-resident heap limit
 constexpr int kMaxClients = 1024;  // master scatter client table
+// per-round trace record: ||grad f||, f, ||x||, factorization flag, line-search trials
+constexpr int kTraceStride = 5;
+constexpr int kMaxLineSearchSteps = 64;  // trial launches enqueued per round
+// master state (mst): line-search scalars, the factorization-failure flag,
+// then x_k (d) from kLsHead on
+enum MstSlot { LS_FXK, LS_T, LS_DD, LS_DONE, LS_TRIALS, MST_CHOL_FAIL, kLsHead = 8 };
 constexpr int kNB = 32;            // Cholesky panel width
 constexpr int kMaxNewtonD = 512;   // single-workgroup Newton solve limit (LDS panel)
 
@@ -196,15 +202,17 @@ __device__ inline uint64_t round_nearest(double v)
 // ---------------------------------------------------------------------------
 struct ProblemDev {
     int n, m, d, k, lda, m_pad, d_pad, tiles_up, dp, workers, comp, hmode, option_b, tracking, seg_len, bm_words,
-        trace_cap, update_x;
+        trace_cap, update_x, ls, ls_max;
     size_t S;
-    double lambda, alpha, prescale, inv_n, mult, lek_delta;
+    double lambda, alpha, prescale, inv_n, mult, lek_delta, ls_c, ls_gamma;
     const double* A;
     const float* A32;
     const __bf16* AThi;
     const __bf16* ATlo;
     double *x, *Dw, *gw, *nlog, *learned, *delta, *grad_c, *part, *lk, *vals, *H, *g_avg, *shift, *W, *dx, *trace,
         *lek_p;
+    double* mst;        // master state (MstSlot header, then x_k)
+    const double* H0;   // H after creation: the lower triangle of the master's full H
     const double* hup;  // Newton: packed H to fill W from (standalone solve), or null
     const double* nb;   // Newton: right-hand side
     uint32_t *sel, *pos, *idx, *rng, *lek_state, *lek_popped, *seq_start, *sel_bm;
@@ -221,11 +229,15 @@ struct ProblemDev {
 #define FEDNL_KARGS const ProblemDev* __restrict__ probs, const int2* __restrict__ bmap
 
 // ---------------------------------------------------------------------------
-// k_margin: one wave per sample row
+// k_margin: one wave per sample row. kTrial: a line-search trial point (only
+// the -log sigma partials for f; nothing once the line search has stopped)
 // ---------------------------------------------------------------------------
+template <bool kTrial>
 __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
 {
     FEDNL_BLOCK();
+    if (kTrial && (!PD.ls || PD.mst[LS_DONE] != 0.0))
+        return;
     const double* __restrict__ A = PD.A;
     const double* __restrict__ x = PD.x;
     const int m = PD.m, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad;
@@ -259,14 +271,18 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
             if (s < m) {
                 // logistic_regression.h:98-101,148-161,398-404
                 const double sg = 1.0 / (1.0 + exp(-acc));
-                const double z1 = 1.0 - sg;
-                const double z2 = z1 * sg;
-                Dw[o] = inv_m * z2;
-                gw[o] = z1 * (-inv_m);
+                if (!kTrial) {
+                    const double z1 = 1.0 - sg;
+                    const double z2 = z1 * sg;
+                    Dw[o] = inv_m * z2;
+                    gw[o] = z1 * (-inv_m);
+                }
                 nl[wave * 4 + q] = -log(sg);
             } else {
-                Dw[o] = 0.0;
-                gw[o] = 0.0;
+                if (!kTrial) {
+                    Dw[o] = 0.0;
+                    gw[o] = 0.0;
+                }
                 nl[wave * 4 + q] = 0.0;
             }
         }
@@ -1841,6 +1857,16 @@ __device__ inline double block_sum(double v, double* scratch)
     return s;
 }
 
+// f_c = mean_s(-log sigma_s) + lambda/2 ||x||^2 (logistic_regression.h:119-123)
+// from client c's per-block partials of sum_s -log sigma_s
+__device__ inline double client_f(const double* __restrict__ nlog_part, int c, int nlog_blocks, int m,
+                                  double lambda, double xx)
+{
+    double sacc = 0.0;
+    for (int q = 0; q < nlog_blocks; ++q) sacc = sacc + nlog_part[(size_t)c * nlog_blocks + q];
+    return sacc / (double)m + (lambda / 2.0) * xx;
+}
+
 // ---------------------------------------------------------------------------
 // k_master_vec: Lk and f averages (TrainLoop.h:1278-1333, 1512-1544), the
 // Newton shift, norms for the record (TrainLoop.h:1679-1691), TopLEK stream
@@ -1873,30 +1899,29 @@ __global__ __launch_bounds__(256) void k_master_vec(FEDNL_KARGS)
     }
     gg = block_sum(gg, scratch);
     xx = block_sum(xx, scratch);
+    const bool with_f = tracking || PD.ls;  // f_i is sent (TrainLoop.h:620, 1512-1544)
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         lk[c] = sqrt(fro2_from_parts(part, c, tiles_up));
-        if (tracking) {
-            // f_c = mean_s(-log sigma_s) + lambda/2 ||x||^2 (logistic_regression.h:119-123)
-            double sacc = 0.0;
-            for (int q = 0; q < nlog_blocks; ++q) sacc = sacc + nlog_part[(size_t)c * nlog_blocks + q];
-            fc_s[c] = sacc / (double)m + (lambda / 2.0) * xx;
-        }
+        if (with_f)
+            fc_s[c] = client_f(nlog_part, c, nlog_blocks, m, lambda, xx);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double l = 0.0, fsum = 0.0;
         for (int c = 0; c < n; ++c) l = l + lk[c];
         l = l * inv_n;
-        if (tracking) {
+        if (with_f) {
             for (int c = 0; c < n; ++c) fsum = fsum + fc_s[c];
             fsum = fsum * inv_n;
         }
+        if (PD.ls)
+            PD.mst[LS_FXK] = fsum;
         *shift_out = option_b ? l : lambda;  // option a: mu(f) = lambda (logistic_regression.h:702-705)
         const int r = *round_ctr;
         if (r < trace_cap) {
-            trace[(size_t)r * 4 + 0] = sqrt(gg);
-            trace[(size_t)r * 4 + 1] = fsum;
-            trace[(size_t)r * 4 + 2] = sqrt(xx);
+            trace[(size_t)r * kTraceStride + 0] = sqrt(gg);
+            trace[(size_t)r * kTraceStride + 1] = fsum;
+            trace[(size_t)r * kTraceStride + 2] = sqrt(xx);
         }
     }
     if (comp == FEDNL_TOPLEK) {
@@ -1906,6 +1931,96 @@ __global__ __launch_bounds__(256) void k_master_vec(FEDNL_KARGS)
                 if (lek_hit[c]) st = lcg_real_step(st);
             lek_state[w] = st;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FedNL-LS (TrainLoop.h:1546-1640), after k_newton has written dx and left x
+// at x_k. k_ls_begin keeps x_k, computes d = -(dx, g) and sets x to the first
+// trial x_k - 1 * dx. Each trial is then k_margin<true> (the clients' f_i at
+// the trial point) and k_ls_check: f_req = mean_c f_c is accepted when
+// f_req <= f(x_k) + (c t) d, else t *= gamma and x = x_k - t dx; after
+// max_steps trials x stays at the last one. Once accepted, the remaining
+// trial launches return at once. One workgroup per problem.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ls_begin(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    (void)blk;
+    if (!PD.ls)
+        return;
+    const int d = PD.d;
+    double* __restrict__ x = PD.x;
+    const double* __restrict__ dx = PD.dx;
+    const double* __restrict__ g = PD.g_avg;
+    double* __restrict__ st = PD.mst;
+    double* __restrict__ xk = st + kLsHead;
+    const bool move = PD.ls_max > 0;
+    __shared__ double scratch[16];
+    double dot = 0.0;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) {
+        const double xi = x[i];
+        dot = fma(dx[i], g[i], dot);
+        xk[i] = xi;
+        if (move)
+            x[i] = xi - 1.0 * dx[i];
+    }
+    dot = block_sum(dot, scratch);
+    if (threadIdx.x == 0) {
+        st[LS_T] = 1.0;
+        st[LS_DD] = -dot;
+        st[LS_DONE] = move ? 0.0 : 1.0;
+        st[LS_TRIALS] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ls_check(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    (void)blk;
+    if (!PD.ls)
+        return;
+    double* __restrict__ st = PD.mst;
+    if (st[LS_DONE] != 0.0)
+        return;
+    const int n = PD.n, d = PD.d, m = PD.m, nlog_blocks = PD.m_pad / 16;
+    double* __restrict__ x = PD.x;
+    const double* __restrict__ dx = PD.dx;
+    const double* __restrict__ xk = st + kLsHead;
+    __shared__ double scratch[16];
+    __shared__ double fc_s[kMaxClients];
+    __shared__ double t_next;
+    __shared__ int move;
+    // f at the trial point, as k_master_vec computes f(x_k)
+    double xx = 0.0;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) xx = fma(x[i], x[i], xx);
+    xx = block_sum(xx, scratch);
+    for (int c = threadIdx.x; c < n; c += blockDim.x)
+        fc_s[c] = client_f(PD.nlog, c, nlog_blocks, m, PD.lambda, xx);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double freq = 0.0;
+        for (int c = 0; c < n; ++c) freq = freq + fc_s[c];
+        freq = freq * PD.inv_n;
+        double t = st[LS_T];
+        const int trials = (int)st[LS_TRIALS] + 1;
+        const bool accept = freq <= st[LS_FXK] + (PD.ls_c * t) * st[LS_DD];
+        if (!accept)
+            t = t * PD.ls_gamma;
+        const bool done = accept || trials >= PD.ls_max;
+        st[LS_T] = t;
+        st[LS_TRIALS] = (double)trials;
+        st[LS_DONE] = done ? 1.0 : 0.0;
+        const int r = *PD.round_ctr - 1;  // k_newton has counted the round
+        if (r >= 0 && r < PD.trace_cap)
+            PD.trace[(size_t)r * kTraceStride + 4] = (double)trials;
+        t_next = t;
+        move = done ? 0 : 1;
+    }
+    __syncthreads();
+    if (move) {
+        const double t = t_next;
+        for (int i = threadIdx.x; i < d; i += blockDim.x) x[i] = xk[i] - t * dx[i];
     }
 }
 
@@ -2272,10 +2387,13 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
 
     const int rnd = *round_ctr;
     if (s_fail) {
+        // k_newton_fallback solves this round's system
         if (tid == 0) {
             if (rnd < trace_cap)
-                trace[(size_t)rnd * 4 + 3] = 0.0;
+                trace[(size_t)rnd * kTraceStride + 3] = 0.0;
             *round_ctr = rnd + 1;
+            if (PD.mst)
+                PD.mst[MST_CHOL_FAIL] = 1.0;
         }
         for (int i = tid; i < d; i += nthr)
             dx_out[i] = 0.0;
@@ -2333,9 +2451,142 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
     }
     if (tid == 0) {
         if (rnd < trace_cap)
-            trace[(size_t)rnd * 4 + 3] = 1.0;
+            trace[(size_t)rnd * kTraceStride + 3] = 1.0;
         *round_ctr = rnd + 1;
     }
+}
+
+// ---------------------------------------------------------------------------
+// k_newton_fallback: after a failed factorization (a pivot < 0), dopt's master
+// prints "SST FACTORIZATION FAILED" and solves (H + shift I) dx = g by
+// Gauss-Jordan elimination with partial pivoting (TrainLoop.h:1410-1420,
+// 1476-1486; GaussEliminationSolvers.h:27-120) on its full H, whose lower
+// triangle is never updated after creation: M(i, j) = H(i, j) (+ shift on the
+// diagonal) for i <= j, H0(j, i) below. M is built row-major in W (ld = dp).
+// Per column: the first largest |M(i, col)| at or below the diagonal is
+// swapped up, the pivot row and b are scaled (b by division, the row by the
+// reciprocal), the column is eliminated from every other row. The products
+// and sums are rounded separately, as in the reference's loops. A zero
+// tolerance never skips a column, so dx = b at the end. One workgroup per
+// problem; returns at once when the round's factorization succeeded.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNewtonThreads) void k_newton_fallback(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    (void)blk;
+    double* __restrict__ mst = PD.mst;
+    if (mst[MST_CHOL_FAIL] == 0.0)
+        return;
+    const int d = PD.d, ld = PD.dp, tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    double* __restrict__ a = PD.W;
+    const double* __restrict__ H = PD.H;
+    const double* __restrict__ H0 = PD.H0;
+    const double shift = *PD.shift;
+    __shared__ double bv[kMaxNewtonD], kcol[kMaxNewtonD], prow[kMaxNewtonD];
+    __shared__ double red_v[kNewtonThreads / 64];
+    __shared__ int red_i[kNewtonThreads / 64];
+    __shared__ int s_br;
+    for (int e = tid; e < d * d; e += nthr) {
+        const int i = e / d, j = e - i * d;
+        double v;
+        if (i <= j) {
+            v = H[(size_t)j * (j + 1) / 2 + i];
+            if (i == j)
+                v = v + shift;
+        } else {
+            v = H0[(size_t)i * (i + 1) / 2 + j];
+        }
+        a[(size_t)i * ld + j] = v;
+    }
+    for (int i = tid; i < d; i += nthr) bv[i] = PD.nb[i];
+    __syncthreads();
+    for (int col = 0; col < d; ++col) {
+        // pivot: first row of the largest |M(i, col)|, i > col (NaN never wins)
+        double bvv = -1.0;
+        int bi = d;
+        for (int i = col + 1 + tid; i < d; i += nthr) {
+            const double v = fabs(a[(size_t)i * ld + col]);
+            if (v > bvv) {  // ascending i: the first of equal values stays
+                bvv = v;
+                bi = i;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(bvv, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ov > bvv || (ov == bvv && oi < bi)) {
+                bvv = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            red_v[wave] = bvv;
+            red_i[wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double v = red_v[0];
+            int ii = red_i[0];
+            for (int w = 1; w < nthr / 64; ++w)
+                if (red_v[w] > v || (red_v[w] == v && red_i[w] < ii)) {
+                    v = red_v[w];
+                    ii = red_i[w];
+                }
+            const double v0 = fabs(a[(size_t)col * ld + col]);
+            s_br = (ii < d && v > v0) ? ii : col;  // v0 NaN: nothing is larger
+        }
+        __syncthreads();
+        const int br = s_br;
+        if (br != col) {
+            for (int j = col + tid; j < d; j += nthr) {
+                const double t = a[(size_t)col * ld + j];
+                a[(size_t)col * ld + j] = a[(size_t)br * ld + j];
+                a[(size_t)br * ld + j] = t;
+            }
+            if (tid == 0) {
+                const double t = bv[col];
+                bv[col] = bv[br];
+                bv[br] = t;
+            }
+            __syncthreads();
+        }
+        const double piv = a[(size_t)col * ld + col];
+        const double inv = 1.0 / piv;
+        for (int j = col + tid; j < d; j += nthr) {
+            const double v = __dmul_rn(a[(size_t)col * ld + j], inv);
+            prow[j] = v;
+            a[(size_t)col * ld + j] = v;
+        }
+        for (int i = tid; i < d; i += nthr) kcol[i] = a[(size_t)i * ld + col];
+        if (tid == 0)
+            bv[col] = bv[col] / piv;
+        __syncthreads();
+        const double bc = bv[col];
+        for (int i = tid; i < d; i += nthr)
+            if (i != col)
+                bv[i] = __dadd_rn(bv[i], -__dmul_rn(bc, kcol[i]));
+        // rows other than col, columns col..d-1 (entries left of col are zero
+        // in the pivot row)
+        const int w = d - col;
+        for (int e = tid; e < (d - 1) * w; e += nthr) {
+            int i = e / w;
+            const int j = col + (e - i * w);
+            i += (i >= col) ? 1 : 0;
+            double* pa = a + (size_t)i * ld + j;
+            *pa = (j == col) ? 0.0 : __dadd_rn(*pa, __dmul_rn(prow[j], -kcol[i]));
+        }
+        __syncthreads();
+    }
+    double* __restrict__ x = PD.x;
+    double* __restrict__ dx_out = PD.dx;
+    for (int i = tid; i < d; i += nthr) {
+        dx_out[i] = bv[i];
+        if (PD.update_x)
+            x[i] = x[i] - bv[i];
+    }
+    if (tid == 0)
+        mst[MST_CHOL_FAIL] = 0.0;
 }
 
 // Lane layout check of v_mfma_f64_16x16x4f64: with A supplied as (lane+1) and
@@ -2382,9 +2633,11 @@ __global__ __launch_bounds__(64) void k_mfma_probe_mixed(const float* __restrict
 // ---------------------------------------------------------------------------
 // host engine
 // ---------------------------------------------------------------------------
-enum KernelId { KID_MARGIN, KID_HESS, KID_SELECT, KID_EMIT, KID_MASTER_H, KID_MASTER_VEC, KID_NEWTON, KID_COUNT };
+enum KernelId {
+    KID_MARGIN, KID_HESS, KID_SELECT, KID_EMIT, KID_MASTER_H, KID_MASTER_VEC, KID_NEWTON, KID_LS, KID_COUNT
+};
 static const char* kKernelNames[KID_COUNT] = {"k_margin", "k_hessian_f64", "k_select", "k_emit",
-                                              "k_master_h", "k_master_vec", "k_newton"};
+                                              "k_master_h", "k_master_vec", "k_newton", "k_line_search"};
 
 struct Engine {
     fednl_opts o;
@@ -2401,6 +2654,8 @@ struct Engine {
     int *sel_cnt = nullptr, *cnt = nullptr, *lek_hit = nullptr, *round_ctr = nullptr;
     double *lek_p = nullptr, *vals = nullptr, *H = nullptr, *g_avg = nullptr, *shift = nullptr, *W = nullptr;
     double *dx = nullptr, *trace = nullptr;
+    double* mst = nullptr;  // master state: MstSlot header + x_k
+    double* H0 = nullptr;   // H at creation (S, packed upper)
     int trace_cap = 0;
     int rounds_enqueued = 0;
 
@@ -2478,10 +2733,10 @@ static int ensure_trace(Engine* e, int need)
         return FEDNL_OK;
     int cap = std::max(need, std::max(1024, e->trace_cap * 2));
     double* nt = nullptr;
-    HIP_OK(hipMalloc(&nt, sizeof(double) * 4 * (size_t)cap));
-    HIP_OK(hipMemsetAsync(nt, 0, sizeof(double) * 4 * (size_t)cap, e->st));
+    HIP_OK(hipMalloc(&nt, sizeof(double) * kTraceStride * (size_t)cap));
+    HIP_OK(hipMemsetAsync(nt, 0, sizeof(double) * kTraceStride * (size_t)cap, e->st));
     if (e->trace) {
-        HIP_OK(hipMemcpyAsync(nt, e->trace, sizeof(double) * 4 * (size_t)e->trace_cap, hipMemcpyDeviceToDevice, e->st));
+        HIP_OK(hipMemcpyAsync(nt, e->trace, sizeof(double) * kTraceStride * (size_t)e->trace_cap, hipMemcpyDeviceToDevice, e->st));
         HIP_OK(hipStreamSynchronize(e->st));
         HIP_OK(hipFree(e->trace));
     }
@@ -2497,7 +2752,9 @@ static ProblemDev make_desc(const Engine* e)
     P.n = e->n; P.m = e->m; P.d = e->d; P.k = e->k; P.lda = e->lda; P.m_pad = e->m_pad; P.d_pad = e->d_pad;
     P.tiles_up = e->tiles_up; P.dp = e->dp; P.workers = e->workers; P.comp = e->o.compressor; P.hmode = e->hmode;
     P.option_b = e->o.option_b; P.tracking = e->o.tracking; P.seg_len = e->seg_len; P.bm_words = e->bm_words;
-    P.trace_cap = e->trace_cap; P.update_x = 1; P.S = e->S;
+    P.trace_cap = e->trace_cap; P.update_x = e->o.line_search ? 0 : 1; P.S = e->S;
+    P.ls = e->o.line_search ? 1 : 0; P.ls_max = e->o.max_line_search_steps;
+    P.ls_c = e->o.c_line_search; P.ls_gamma = e->o.gamma_line_search; P.mst = e->mst; P.H0 = e->H0;
     P.lambda = e->o.lambda; P.alpha = e->alpha; P.prescale = e->prescale; P.inv_n = e->inv_n; P.mult = e->mult;
     P.lek_delta = e->lek_delta;
     P.A = e->A; P.A32 = e->A32; P.AThi = e->AThi; P.ATlo = e->ATlo;
@@ -2575,7 +2832,8 @@ static int launch_round(Engine* e)
     const int n = e->n;
     {
         KTime kt(e, KID_MARGIN);
-        hipLaunchKernelGGL(k_margin, dim3(blocks_margin(P)), dim3(256), sizeof(double) * e->d_pad, e->st, pd, nomap);
+        hipLaunchKernelGGL(k_margin<false>, dim3(blocks_margin(P)), dim3(256), sizeof(double) * e->d_pad, e->st, pd,
+                           nomap);
     }
     {
         KTime kt(e, KID_HESS);
@@ -2610,6 +2868,17 @@ static int launch_round(Engine* e)
     {
         KTime kt(e, KID_NEWTON);
         hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), e->lds_newton, e->st, pd, nomap);
+        hipLaunchKernelGGL(k_newton_fallback, dim3(1), dim3(kNewtonThreads), 0, e->st, pd, nomap);
+    }
+    if (e->o.line_search) {
+        // every trial is enqueued; the ones after acceptance return at once
+        KTime kt(e, KID_LS);
+        hipLaunchKernelGGL(k_ls_begin, dim3(1), dim3(256), 0, e->st, pd, nomap);
+        for (int t = 0; t < e->o.max_line_search_steps; ++t) {
+            hipLaunchKernelGGL(k_margin<true>, dim3(blocks_margin(P)), dim3(256), sizeof(double) * e->d_pad, e->st,
+                               pd, nomap);
+            hipLaunchKernelGGL(k_ls_check, dim3(1), dim3(256), 0, e->st, pd, nomap);
+        }
     }
     return FEDNL_OK;
 }
@@ -2642,7 +2911,7 @@ static void free_engine(Engine* e)
     void* ptrs[] = {e->A, e->x, e->Dw, e->gw, e->nlog, e->learned, e->delta, e->grad_c, e->part, e->lk, e->sel_bm,
                     e->sel, e->pos, e->idx, e->rng, e->lek_state, e->lek_popped, e->seq_start, e->sel_cnt,
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
-                    e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev};
+                    e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev, e->mst, e->H0};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -2701,6 +2970,12 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     }
     if (opts->hessian_mode < 0 || opts->hessian_mode > 2) {
         set_err("fednl_create: hessian_mode must be 0 (FP64), 1 (FP32) or 2 (BF16 split)");
+        return FEDNL_E_WRONG_ARGUMENT;
+    }
+    if (opts->line_search &&
+        (!(opts->c_line_search > 0.0) || !(opts->gamma_line_search > 0.0 && opts->gamma_line_search < 1.0) ||
+         opts->max_line_search_steps < 0 || opts->max_line_search_steps > kMaxLineSearchSteps)) {
+        set_err("fednl_create: line search needs c > 0, 0 < gamma < 1 and 0 <= max steps <= 64");
         return FEDNL_E_WRONG_ARGUMENT;
     }
     Engine* e = new Engine();
@@ -2805,6 +3080,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         }
     }
     ALLOC(e->dx, e->d_pad);
+    ALLOC(e->mst, kLsHead + (size_t)e->d_pad);
+    ALLOC(e->H0, e->S);
 #undef ALLOC
     if ((rc = ensure_trace(e, 1024)) != FEDNL_OK) {
         free_engine(e);
@@ -2867,6 +3144,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     hipMemcpyAsync(e->lek_state, lek.data(), sizeof(uint32_t) * e->workers, hipMemcpyHostToDevice, e->st);
     hipMemsetAsync(e->x, 0, sizeof(double) * e->d_pad, e->st);
     hipMemsetAsync(e->round_ctr, 0, sizeof(int), e->st);
+    hipMemsetAsync(e->mst, 0, sizeof(double) * kLsHead, e->st);
     hipMemsetAsync(e->lek_hit, 0, sizeof(int) * n, e->st);
     hipMemsetAsync(e->grad_c, 0, sizeof(double) * (size_t)n * e->d_pad, e->st);
 
@@ -2877,11 +3155,12 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         return FEDNL_E_DEVICE;
     }
     {
-        hipLaunchKernelGGL(k_margin, dim3(blocks_margin(e->hdesc)), dim3(256), sizeof(double) * e->d_pad, e->st,
+        hipLaunchKernelGGL(k_margin<false>, dim3(blocks_margin(e->hdesc)), dim3(256), sizeof(double) * e->d_pad, e->st,
                            (const ProblemDev*)e->pdev, (const int2*)nullptr);
         launch_hessian(e, true, nullptr);
         hipLaunchKernelGGL(k_init_H, dim3((unsigned)((e->S + 255) / 256)), dim3(256), 0, e->st, e->H, e->learned,
                            e->S, n, e->inv_n);
+        hipMemcpyAsync(e->H0, e->H, sizeof(double) * e->S, hipMemcpyDeviceToDevice, e->st);
     }
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
@@ -2932,7 +3211,7 @@ static int enqueue_rounds(Engine* e, int rounds)
 // ---------------------------------------------------------------------------
 enum BatchKernel {
     BK_MARGIN, BK_HESS64, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
-    BK_EMIT, BK_MGRAD, BK_MVEC, BK_MH, BK_NEWTON, BK_COUNT
+    BK_EMIT, BK_MGRAD, BK_MVEC, BK_MH, BK_NEWTON, BK_LS_MARGIN, BK_LS, BK_COUNT
 };
 
 struct Batch {
@@ -2947,6 +3226,7 @@ struct Batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int rounds = 0;
+    int ls_steps = 0;  // most line-search trials of any member
 };
 
 static void free_batch(Batch* b)
@@ -2989,6 +3269,11 @@ static int batch_build(Batch* b)
         add(BK_MVEC, p, 1, false);
         add(BK_MH, p, blocks_master_h(P), false);
         add(BK_NEWTON, p, 1, false);
+        if (e->o.line_search) {
+            add(BK_LS_MARGIN, p, blocks_margin(P), false);
+            add(BK_LS, p, 1, false);
+            b->ls_steps = std::max(b->ls_steps, e->o.max_line_search_steps);
+        }
         b->lds_margin = std::max(b->lds_margin, sizeof(double) * (size_t)e->d_pad);
         b->lds_select = std::max(b->lds_select, e->lds_select);
         b->lds_emit = std::max(b->lds_emit, sizeof(uint32_t) * (size_t)e->bm_words);
@@ -3035,7 +3320,8 @@ static void batch_launch_round(Batch* b)
     hipStream_t st = b->st;
     auto map = [&](int k) { return mp + b->off[k]; };
     if (b->cnt[BK_MARGIN])
-        hipLaunchKernelGGL(k_margin, dim3(b->cnt[BK_MARGIN]), dim3(256), b->lds_margin, st, pd, map(BK_MARGIN));
+        hipLaunchKernelGGL(k_margin<false>, dim3(b->cnt[BK_MARGIN]), dim3(256), b->lds_margin, st, pd,
+                           map(BK_MARGIN));
     if (b->cnt[BK_HESS64])
         hipLaunchKernelGGL(k_hessian_f64<false>, dim3(b->cnt[BK_HESS64]), dim3(256), 0, st, pd, map(BK_HESS64),
                            (double*)nullptr);
@@ -3064,6 +3350,16 @@ static void batch_launch_round(Batch* b)
     hipLaunchKernelGGL(k_master_h, dim3(b->cnt[BK_MH]), dim3(256), 0, st, pd, map(BK_MH));
     hipLaunchKernelGGL(k_newton, dim3(b->cnt[BK_NEWTON]), dim3(kNewtonThreads), b->lds_newton, st, pd,
                        map(BK_NEWTON));
+    hipLaunchKernelGGL(k_newton_fallback, dim3(b->cnt[BK_NEWTON]), dim3(kNewtonThreads), 0, st, pd,
+                       map(BK_NEWTON));
+    if (b->cnt[BK_LS]) {
+        hipLaunchKernelGGL(k_ls_begin, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));
+        for (int t = 0; t < b->ls_steps; ++t) {
+            hipLaunchKernelGGL(k_margin<true>, dim3(b->cnt[BK_LS_MARGIN]), dim3(256), b->lds_margin, st, pd,
+                               map(BK_LS_MARGIN));
+            hipLaunchKernelGGL(k_ls_check, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));
+        }
+    }
 }
 
 static int batch_enqueue(Batch* b, int rounds)
@@ -3162,6 +3458,10 @@ void fednl_default_opts(fednl_opts* o)
     o->workers = 1;
     o->tracking = 1;
     o->hessian_mode = 0;
+    o->line_search = 0;
+    o->max_line_search_steps = 5;  // Args.h:300-306
+    o->c_line_search = 0.5;
+    o->gamma_line_search = 0.5;
 }
 
 const char* fednl_last_error(void) { return g_err.c_str(); }
@@ -3216,21 +3516,10 @@ int fednl_run(void* handle, int rounds)
     Engine* e = (Engine*)handle;
     if (!e)
         return FEDNL_E_MISSING_ARGUMENT;
-    const int first = e->rounds_enqueued;
     int rc = enqueue_rounds(e, rounds);
     if (rc != FEDNL_OK)
         return rc;
-    rc = fednl_sync(handle);
-    if (rc != FEDNL_OK)
-        return rc;
-    std::vector<double> tr((size_t)4 * rounds);
-    HIP_OK(hipMemcpy(tr.data(), e->trace + (size_t)4 * first, sizeof(double) * tr.size(), hipMemcpyDeviceToHost));
-    for (int r = 0; r < rounds; ++r)
-        if (tr[(size_t)4 * r + 3] != 1.0) {
-            set_err("Cholesky factorization failed (non-positive pivot)");
-            return FEDNL_E_INTERNAL;
-        }
-    return FEDNL_OK;
+    return fednl_sync(handle);
 }
 
 int fednl_batch_create(void* const* engines, int count, int use_graph, void** batch)
@@ -3405,11 +3694,11 @@ int fednl_serve(const fednl_request* reqs, int count, double hbm_budget_bytes, i
             const fednl_request& R = reqs[sl.req];
             const int first = sl.rounds;
             sl.rounds += rounds_per_step;
-            std::vector<double> tr((size_t)4 * rounds_per_step);
-            hipMemcpy(tr.data(), sl.e->trace + (size_t)4 * first, sizeof(double) * tr.size(), hipMemcpyDeviceToHost);
+            std::vector<double> tr((size_t)kTraceStride * rounds_per_step);
+            hipMemcpy(tr.data(), sl.e->trace + (size_t)kTraceStride * first, sizeof(double) * tr.size(), hipMemcpyDeviceToHost);
             int hit = -1;
             for (int r = 0; r < rounds_per_step; ++r) {
-                const double g = tr[(size_t)4 * r];
+                const double g = tr[(size_t)kTraceStride * r];
                 if (g * g <= R.tol) {
                     hit = first + r;
                     break;
@@ -3425,7 +3714,7 @@ int fednl_serve(const fednl_request* reqs, int count, double hbm_budget_bytes, i
             o.rounds = sl.rounds;
             o.rounds_to_tol = hit;
             o.status = (hit >= 0) ? 0 : 2;
-            o.grad_norm = tr[(size_t)4 * (rounds_per_step - 1)];
+            o.grad_norm = tr[(size_t)kTraceStride * (rounds_per_step - 1)];
             if (R.x_out)
                 hipMemcpy(R.x_out, sl.e->x, sizeof(double) * R.d, hipMemcpyDeviceToHost);
             used -= sl.bytes;
@@ -3458,13 +3747,13 @@ int fednl_trace(void* handle, double* grad_norm, double* fval, double* x_norm, d
     *count = R;
     if (R == 0)
         return FEDNL_OK;
-    std::vector<double> tr((size_t)4 * R);
+    std::vector<double> tr((size_t)kTraceStride * R);
     HIP_OK(hipMemcpy(tr.data(), e->trace, sizeof(double) * tr.size(), hipMemcpyDeviceToHost));
     for (int r = 0; r < R; ++r) {
-        if (grad_norm) grad_norm[r] = tr[(size_t)4 * r + 0];
-        if (fval) fval[r] = tr[(size_t)4 * r + 1];
-        if (x_norm) x_norm[r] = tr[(size_t)4 * r + 2];
-        if (chol_ok) chol_ok[r] = tr[(size_t)4 * r + 3] == 1.0 ? 1 : 0;
+        if (grad_norm) grad_norm[r] = tr[(size_t)kTraceStride * r + 0];
+        if (fval) fval[r] = tr[(size_t)kTraceStride * r + 1];
+        if (x_norm) x_norm[r] = tr[(size_t)kTraceStride * r + 2];
+        if (chol_ok) chol_ok[r] = tr[(size_t)kTraceStride * r + 3] == 1.0 ? 1 : 0;
         if (seconds) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, e->ev_start, e->round_ev[r]) != hipSuccess)
@@ -3472,6 +3761,22 @@ int fednl_trace(void* handle, double* grad_norm, double* fval, double* x_norm, d
             seconds[r] = ms * 1e-3;
         }
     }
+    return FEDNL_OK;
+}
+
+int fednl_trace_line_search(void* handle, int* trials, int* count)
+{
+    Engine* e = (Engine*)handle;
+    if (!e || !count)
+        return FEDNL_E_MISSING_ARGUMENT;
+    HIP_OK(hipStreamSynchronize(e->st));
+    const int R = e->rounds_enqueued;
+    *count = R;
+    if (R == 0 || !trials)
+        return FEDNL_OK;
+    std::vector<double> tr((size_t)kTraceStride * R);
+    HIP_OK(hipMemcpy(tr.data(), e->trace, sizeof(double) * tr.size(), hipMemcpyDeviceToHost));
+    for (int r = 0; r < R; ++r) trials[r] = (int)tr[(size_t)kTraceStride * r + 4];
     return FEDNL_OK;
 }
 
@@ -3852,7 +4157,7 @@ int fednl_hessians_at(void* handle, const double* x, double* out)
         err = hipMemcpy(dpd, &hp, sizeof(ProblemDev), hipMemcpyHostToDevice);
     if (err == hipSuccess) {
         // the margin weights of x, then the Hessians in the engine's mode
-        hipLaunchKernelGGL(k_margin, dim3(blocks_margin(hp)), dim3(256), sizeof(double) * e->d_pad, e->st,
+        hipLaunchKernelGGL(k_margin<false>, dim3(blocks_margin(hp)), dim3(256), sizeof(double) * e->d_pad, e->st,
                            (const ProblemDev*)dpd, (const int2*)nullptr);
         launch_hessian(e, true, dout, dpd);
         err = hipGetLastError();

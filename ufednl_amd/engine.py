@@ -29,7 +29,11 @@ class FednlOpts(ctypes.Structure):
         ("workers", ctypes.c_int),
         ("tracking", ctypes.c_int),
         ("hessian_mode", ctypes.c_int),
-        ("reserved", ctypes.c_int * 7),
+        ("line_search", ctypes.c_int),
+        ("max_line_search_steps", ctypes.c_int),
+        ("c_line_search", ctypes.c_double),
+        ("gamma_line_search", ctypes.c_double),
+        ("reserved", ctypes.c_int * 4),
     ]
 
 
@@ -84,6 +88,7 @@ def lib():
         L.fednl_sync.argtypes = [ctypes.c_void_p]
         L.fednl_trace.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_dp, _c_ip, _c_ip]
         L.fednl_get_x.argtypes = [ctypes.c_void_p, _c_dp]
+        L.fednl_trace_line_search.argtypes = [ctypes.c_void_p, _c_ip, _c_ip]
         L.fednl_last_round.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_ip, _c_up, _c_dp, _c_up, _c_dp]
         L.fednl_get_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
         L.fednl_set_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
@@ -137,22 +142,12 @@ class Engine:
     """One FedNL problem resident on the GPU."""
 
     def __init__(self, features, labels, lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0,
-                 option_b=True, workers=1, tracking=True, hessian="fp64"):
+                 option_b=True, workers=1, tracking=True, hessian="fp64", line_search=None):
         L = lib()
         feats = np.ascontiguousarray(features, dtype=np.float64)
         labs = np.ascontiguousarray(labels, dtype=np.float64)
         n, m, d = feats.shape
-        o = FednlOpts()
-        L.fednl_default_opts(ctypes.byref(o))
-        o.lambda_ = lam
-        o.compressor = COMPRESSORS[compressor]
-        o.k_mult = k_mult
-        o.alpha_mode = alpha_mode
-        o.alpha = alpha
-        o.option_b = 1 if option_b else 0
-        o.workers = workers
-        o.tracking = 1 if tracking else 0
-        o.hessian_mode = HESSIAN_MODES[hessian]
+        o = make_opts(lam, compressor, k_mult, alpha_mode, alpha, option_b, workers, tracking, hessian, line_search)
         h = ctypes.c_void_p()
         _err(L.fednl_create(ctypes.byref(o), n, m, d, dp(feats), dp(labs), ctypes.byref(h)), "fednl_create")
         self.h = h
@@ -204,7 +199,9 @@ class Engine:
         R = cnt.value
         g = np.zeros(R); f = np.zeros(R); xn = np.zeros(R); sec = np.zeros(R); ok = np.zeros(R, dtype=np.int32)
         _err(lib().fednl_trace(self.h, dp(g), dp(f), dp(xn), dp(sec), ip(ok), ctypes.byref(cnt)), "fednl_trace")
-        return dict(grad_norm=g, f=f, x_norm=xn, seconds=sec, chol_ok=ok)
+        ls = np.zeros(R, dtype=np.int32)
+        _err(lib().fednl_trace_line_search(self.h, ip(ls), ctypes.byref(cnt)), "fednl_trace_line_search")
+        return dict(grad_norm=g, f=f, x_norm=xn, seconds=sec, chol_ok=ok, ls_trials=ls)
 
     def x(self):
         out = np.zeros(self.d)
@@ -300,7 +297,8 @@ class Batch:
 
 
 def make_opts(lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0, option_b=True, workers=1,
-              tracking=False, hessian="fp64"):
+              tracking=False, hessian="fp64", line_search=None):
+    """fednl_opts; line_search: None (FedNL) or (c, gamma, max_steps) for FedNL-LS."""
     o = FednlOpts()
     lib().fednl_default_opts(ctypes.byref(o))
     o.lambda_ = lam
@@ -312,6 +310,12 @@ def make_opts(lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0, 
     o.workers = workers
     o.tracking = 1 if tracking else 0
     o.hessian_mode = HESSIAN_MODES[hessian]
+    if line_search is not None:
+        c, gamma, steps = line_search
+        o.line_search = 1
+        o.c_line_search = c
+        o.gamma_line_search = gamma
+        o.max_line_search_steps = int(steps)
     return o
 
 
