@@ -92,9 +92,8 @@ constexpr int kMaxClients = 1024;  // master scatter client table
 // per-round trace record: ||grad f||, f, ||x||, factorization flag, line-search trials
 constexpr int kTraceStride = 5;
 constexpr int kMaxLineSearchSteps = 64;  // trial launches enqueued per round
-// master state (mst): line-search scalars, the factorization-failure flag,
-// then x_k (d) from kLsHead on
-enum MstSlot { LS_FXK, LS_T, LS_DD, LS_DONE, LS_TRIALS, MST_CHOL_FAIL, kLsHead = 8 };
+// master state (mst): line-search scalars, then x_k (d) from kLsHead on
+enum MstSlot { LS_FXK, LS_T, LS_DD, LS_DONE, LS_TRIALS, kLsHead = 8 };
 constexpr int kNB = 32;            // Cholesky panel width
 constexpr int kMaxNewtonD = 512;   // single-workgroup Newton solve limit (LDS panel)
 
@@ -2025,6 +2024,135 @@ __global__ __launch_bounds__(256) void k_ls_check(FEDNL_KARGS)
 }
 
 // ---------------------------------------------------------------------------
+// newton_fallback_solve: after a failed factorization (a pivot < 0), dopt's master
+// prints "SST FACTORIZATION FAILED" and solves (H + shift I) dx = g by
+// Gauss-Jordan elimination with partial pivoting (TrainLoop.h:1410-1420,
+// 1476-1486; GaussEliminationSolvers.h:27-120) on its full H, whose lower
+// triangle is never updated after creation: M(i, j) = H(i, j) (+ shift on the
+// diagonal) for i <= j, H0(j, i) below. M is built row-major in W (ld = dp).
+// Per column: the first largest |M(i, col)| at or below the diagonal is
+// swapped up, the pivot row and b are scaled (b by division, the row by the
+// reciprocal), the column is eliminated from every other row. The products
+// and sums are rounded separately, as in the reference's loops. A zero
+// tolerance never skips a column, so dx = b at the end. Called by every
+// thread of k_newton's workgroup; kept out of line, off the solve's hot path.
+// ---------------------------------------------------------------------------
+__device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double* __restrict__ lds)
+{
+    const int d = PD.d, ld = PD.dp, tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    double* __restrict__ a = PD.W;
+    const double* __restrict__ H = PD.H;
+    const double* __restrict__ H0 = PD.H0;
+    const double shift = *PD.shift;
+    // k_newton's LDS: b, the eliminated column, the pivot row, reduction slots
+    double* bv = lds;
+    double* kcol = lds + ld;
+    double* prow = lds + 2 * ld;
+    double* red_v = lds + 3 * ld;
+    int* red_i = (int*)(lds + 3 * ld + 8);
+    int* s_br = red_i + 8;
+    for (int e = tid; e < d * d; e += nthr) {
+        const int i = e / d, j = e - i * d;
+        double v;
+        if (i <= j) {
+            v = H[(size_t)j * (j + 1) / 2 + i];
+            if (i == j)
+                v = v + shift;
+        } else {
+            v = H0[(size_t)i * (i + 1) / 2 + j];
+        }
+        a[(size_t)i * ld + j] = v;
+    }
+    for (int i = tid; i < d; i += nthr) bv[i] = PD.nb[i];
+    __syncthreads();
+    for (int col = 0; col < d; ++col) {
+        // pivot: first row of the largest |M(i, col)|, i > col (NaN never wins)
+        double bvv = -1.0;
+        int bi = d;
+        for (int i = col + 1 + tid; i < d; i += nthr) {
+            const double v = fabs(a[(size_t)i * ld + col]);
+            if (v > bvv) {  // ascending i: the first of equal values stays
+                bvv = v;
+                bi = i;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(bvv, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ov > bvv || (ov == bvv && oi < bi)) {
+                bvv = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            red_v[wave] = bvv;
+            red_i[wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double v = red_v[0];
+            int ii = red_i[0];
+            for (int w = 1; w < nthr / 64; ++w)
+                if (red_v[w] > v || (red_v[w] == v && red_i[w] < ii)) {
+                    v = red_v[w];
+                    ii = red_i[w];
+                }
+            const double v0 = fabs(a[(size_t)col * ld + col]);
+            *s_br = (ii < d && v > v0) ? ii : col;  // v0 NaN: nothing is larger
+        }
+        __syncthreads();
+        const int br = *s_br;
+        if (br != col) {
+            for (int j = col + tid; j < d; j += nthr) {
+                const double t = a[(size_t)col * ld + j];
+                a[(size_t)col * ld + j] = a[(size_t)br * ld + j];
+                a[(size_t)br * ld + j] = t;
+            }
+            if (tid == 0) {
+                const double t = bv[col];
+                bv[col] = bv[br];
+                bv[br] = t;
+            }
+            __syncthreads();
+        }
+        const double piv = a[(size_t)col * ld + col];
+        const double inv = 1.0 / piv;
+        for (int j = col + tid; j < d; j += nthr) {
+            const double v = __dmul_rn(a[(size_t)col * ld + j], inv);
+            prow[j] = v;
+            a[(size_t)col * ld + j] = v;
+        }
+        for (int i = tid; i < d; i += nthr) kcol[i] = a[(size_t)i * ld + col];
+        if (tid == 0)
+            bv[col] = bv[col] / piv;
+        __syncthreads();
+        const double bc = bv[col];
+        for (int i = tid; i < d; i += nthr)
+            if (i != col)
+                bv[i] = __dadd_rn(bv[i], -__dmul_rn(bc, kcol[i]));
+        // rows other than col, columns col..d-1 (entries left of col are zero
+        // in the pivot row)
+        const int w = d - col;
+        for (int e = tid; e < (d - 1) * w; e += nthr) {
+            int i = e / w;
+            const int j = col + (e - i * w);
+            i += (i >= col) ? 1 : 0;
+            double* pa = a + (size_t)i * ld + j;
+            *pa = (j == col) ? 0.0 : __dadd_rn(*pa, __dmul_rn(prow[j], -kcol[i]));
+        }
+        __syncthreads();
+    }
+    double* __restrict__ x = PD.x;
+    double* __restrict__ dx_out = PD.dx;
+    for (int i = tid; i < d; i += nthr) {
+        dx_out[i] = bv[i];
+        if (PD.update_x)
+            x[i] = x[i] - bv[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_newton: Cholesky of (H + shift I) and the two triangular solves of the
 // master (TrainLoop.h:1449-1466; CholeskyFactorization.h:153-272;
 // ElementarySolvers.h:72-226) in one workgroup of 8 waves. W holds the lower
@@ -2048,8 +2176,11 @@ static_assert(kMaxNewtonD <= kNewtonThreads, "k_newton back solve assumes one ro
 
 __host__ __device__ constexpr size_t newton_lds_bytes(int dp)
 {
-    // rows below the diagonal block, row stride kNB + 1 (at least one row)
-    return sizeof(double) * (size_t)(kNB + 1) * (size_t)(dp > kNB ? dp - kNB : 1);
+    // rows below the diagonal block, row stride kNB + 1 (at least one row);
+    // after a failed factorization the fallback solve reuses it (3 dp + 16)
+    const size_t panel = (size_t)(kNB + 1) * (size_t)(dp > kNB ? dp - kNB : 1);
+    const size_t fallback = 3 * (size_t)dp + 16;
+    return sizeof(double) * (panel > fallback ? panel : fallback);
 }
 
 // 1/sqrt(a) and sqrt(a) for a > 0: v_rsq_f64 and two Newton steps, then one
@@ -2387,16 +2518,20 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
 
     const int rnd = *round_ctr;
     if (s_fail) {
-        // k_newton_fallback solves this round's system
+        // the engine's rounds solve by Gauss-Jordan elimination instead; the
+        // standalone solve (no master H) reports dx = 0
+        if (PD.H0) {
+            __syncthreads();  // every wave is past its last use of the panel LDS
+            newton_fallback_solve(PD, lds);
+        } else {
+            for (int i = tid; i < d; i += nthr)
+                dx_out[i] = 0.0;
+        }
         if (tid == 0) {
             if (rnd < trace_cap)
                 trace[(size_t)rnd * kTraceStride + 3] = 0.0;
             *round_ctr = rnd + 1;
-            if (PD.mst)
-                PD.mst[MST_CHOL_FAIL] = 1.0;
         }
-        for (int i = tid; i < d; i += nthr)
-            dx_out[i] = 0.0;
         return;
     }
 
@@ -2454,139 +2589,6 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
             trace[(size_t)rnd * kTraceStride + 3] = 1.0;
         *round_ctr = rnd + 1;
     }
-}
-
-// ---------------------------------------------------------------------------
-// k_newton_fallback: after a failed factorization (a pivot < 0), dopt's master
-// prints "SST FACTORIZATION FAILED" and solves (H + shift I) dx = g by
-// Gauss-Jordan elimination with partial pivoting (TrainLoop.h:1410-1420,
-// 1476-1486; GaussEliminationSolvers.h:27-120) on its full H, whose lower
-// triangle is never updated after creation: M(i, j) = H(i, j) (+ shift on the
-// diagonal) for i <= j, H0(j, i) below. M is built row-major in W (ld = dp).
-// Per column: the first largest |M(i, col)| at or below the diagonal is
-// swapped up, the pivot row and b are scaled (b by division, the row by the
-// reciprocal), the column is eliminated from every other row. The products
-// and sums are rounded separately, as in the reference's loops. A zero
-// tolerance never skips a column, so dx = b at the end. One workgroup per
-// problem; returns at once when the round's factorization succeeded.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNewtonThreads) void k_newton_fallback(FEDNL_KARGS)
-{
-    FEDNL_BLOCK();
-    (void)blk;
-    double* __restrict__ mst = PD.mst;
-    if (mst[MST_CHOL_FAIL] == 0.0)
-        return;
-    const int d = PD.d, ld = PD.dp, tid = threadIdx.x, nthr = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    double* __restrict__ a = PD.W;
-    const double* __restrict__ H = PD.H;
-    const double* __restrict__ H0 = PD.H0;
-    const double shift = *PD.shift;
-    __shared__ double bv[kMaxNewtonD], kcol[kMaxNewtonD], prow[kMaxNewtonD];
-    __shared__ double red_v[kNewtonThreads / 64];
-    __shared__ int red_i[kNewtonThreads / 64];
-    __shared__ int s_br;
-    for (int e = tid; e < d * d; e += nthr) {
-        const int i = e / d, j = e - i * d;
-        double v;
-        if (i <= j) {
-            v = H[(size_t)j * (j + 1) / 2 + i];
-            if (i == j)
-                v = v + shift;
-        } else {
-            v = H0[(size_t)i * (i + 1) / 2 + j];
-        }
-        a[(size_t)i * ld + j] = v;
-    }
-    for (int i = tid; i < d; i += nthr) bv[i] = PD.nb[i];
-    __syncthreads();
-    for (int col = 0; col < d; ++col) {
-        // pivot: first row of the largest |M(i, col)|, i > col (NaN never wins)
-        double bvv = -1.0;
-        int bi = d;
-        for (int i = col + 1 + tid; i < d; i += nthr) {
-            const double v = fabs(a[(size_t)i * ld + col]);
-            if (v > bvv) {  // ascending i: the first of equal values stays
-                bvv = v;
-                bi = i;
-            }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(bvv, off);
-            const int oi = __shfl_xor(bi, off);
-            if (ov > bvv || (ov == bvv && oi < bi)) {
-                bvv = ov;
-                bi = oi;
-            }
-        }
-        if (lane == 0) {
-            red_v[wave] = bvv;
-            red_i[wave] = bi;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double v = red_v[0];
-            int ii = red_i[0];
-            for (int w = 1; w < nthr / 64; ++w)
-                if (red_v[w] > v || (red_v[w] == v && red_i[w] < ii)) {
-                    v = red_v[w];
-                    ii = red_i[w];
-                }
-            const double v0 = fabs(a[(size_t)col * ld + col]);
-            s_br = (ii < d && v > v0) ? ii : col;  // v0 NaN: nothing is larger
-        }
-        __syncthreads();
-        const int br = s_br;
-        if (br != col) {
-            for (int j = col + tid; j < d; j += nthr) {
-                const double t = a[(size_t)col * ld + j];
-                a[(size_t)col * ld + j] = a[(size_t)br * ld + j];
-                a[(size_t)br * ld + j] = t;
-            }
-            if (tid == 0) {
-                const double t = bv[col];
-                bv[col] = bv[br];
-                bv[br] = t;
-            }
-            __syncthreads();
-        }
-        const double piv = a[(size_t)col * ld + col];
-        const double inv = 1.0 / piv;
-        for (int j = col + tid; j < d; j += nthr) {
-            const double v = __dmul_rn(a[(size_t)col * ld + j], inv);
-            prow[j] = v;
-            a[(size_t)col * ld + j] = v;
-        }
-        for (int i = tid; i < d; i += nthr) kcol[i] = a[(size_t)i * ld + col];
-        if (tid == 0)
-            bv[col] = bv[col] / piv;
-        __syncthreads();
-        const double bc = bv[col];
-        for (int i = tid; i < d; i += nthr)
-            if (i != col)
-                bv[i] = __dadd_rn(bv[i], -__dmul_rn(bc, kcol[i]));
-        // rows other than col, columns col..d-1 (entries left of col are zero
-        // in the pivot row)
-        const int w = d - col;
-        for (int e = tid; e < (d - 1) * w; e += nthr) {
-            int i = e / w;
-            const int j = col + (e - i * w);
-            i += (i >= col) ? 1 : 0;
-            double* pa = a + (size_t)i * ld + j;
-            *pa = (j == col) ? 0.0 : __dadd_rn(*pa, __dmul_rn(prow[j], -kcol[i]));
-        }
-        __syncthreads();
-    }
-    double* __restrict__ x = PD.x;
-    double* __restrict__ dx_out = PD.dx;
-    for (int i = tid; i < d; i += nthr) {
-        dx_out[i] = bv[i];
-        if (PD.update_x)
-            x[i] = x[i] - bv[i];
-    }
-    if (tid == 0)
-        mst[MST_CHOL_FAIL] = 0.0;
 }
 
 // Lane layout check of v_mfma_f64_16x16x4f64: with A supplied as (lane+1) and
@@ -2868,7 +2870,6 @@ static int launch_round(Engine* e)
     {
         KTime kt(e, KID_NEWTON);
         hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), e->lds_newton, e->st, pd, nomap);
-        hipLaunchKernelGGL(k_newton_fallback, dim3(1), dim3(kNewtonThreads), 0, e->st, pd, nomap);
     }
     if (e->o.line_search) {
         // every trial is enqueued; the ones after acceptance return at once
@@ -3349,8 +3350,6 @@ static void batch_launch_round(Batch* b)
     hipLaunchKernelGGL(k_master_vec, dim3(b->cnt[BK_MVEC]), dim3(256), 0, st, pd, map(BK_MVEC));
     hipLaunchKernelGGL(k_master_h, dim3(b->cnt[BK_MH]), dim3(256), 0, st, pd, map(BK_MH));
     hipLaunchKernelGGL(k_newton, dim3(b->cnt[BK_NEWTON]), dim3(kNewtonThreads), b->lds_newton, st, pd,
-                       map(BK_NEWTON));
-    hipLaunchKernelGGL(k_newton_fallback, dim3(b->cnt[BK_NEWTON]), dim3(kNewtonThreads), 0, st, pd,
                        map(BK_NEWTON));
     if (b->cnt[BK_LS]) {
         hipLaunchKernelGGL(k_ls_begin, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));

@@ -237,7 +237,8 @@ class Engine:
         ms = np.zeros(KERNEL_NAMES_MAX); lc = np.zeros(KERNEL_NAMES_MAX, dtype=np.int32)
         names = (ctypes.c_char_p * KERNEL_NAMES_MAX)()
         nk = _err(lib().fednl_kernel_times(self.h, dp(ms), ip(lc), KERNEL_NAMES_MAX, names), "fednl_kernel_times")
-        return {names[q].decode(): (float(ms[q]), int(lc[q])) for q in range(nk)}
+        # kernels that ran: (total ms, launches)
+        return {names[q].decode(): (float(ms[q]), int(lc[q])) for q in range(nk) if lc[q] > 0}
 
 
 def select(compressor, delta, d, k, rng=None, toplek_rng=None, workers=1):
