@@ -84,6 +84,9 @@ def lib():
         L.orc_fednl_line_search_count.restype = ctypes.c_int
         L.orc_fednl_line_search_count.argtypes = [ctypes.c_void_p]
         L.orc_gauss_solve.argtypes = [_c_dp, _c_dp, ctypes.c_int, _c_dp]
+        L.orc_fednl_set_pp.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32]
+        L.orc_fednl_pp_round.restype = ctypes.c_int
+        L.orc_fednl_pp_round.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_ip, _c_dp, _c_dp]
         L.orc_natural.restype = ctypes.c_double
         L.orc_natural.argtypes = [ctypes.c_double]
         _lib = L
@@ -114,6 +117,11 @@ def ref():
                                     ctypes.c_int, ctypes.c_int, _c_dp, _c_dp, _c_dp, _c_dp, ctypes.c_int,
                                     _c_dp, _c_dp, _c_ip, _c_up, _c_dp, _c_dp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, _c_ip, _c_ip]
+        R.ref_fednl_pp_run.restype = ctypes.c_int
+        R.ref_fednl_pp_run.argtypes = [_c_dp, _c_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _c_dp, _c_dp,
+                                       _c_dp, _c_dp, _c_ip, _c_ip]
         _ref = R
     return _ref
 
@@ -207,7 +215,7 @@ class FedNL:
     """The oracle's FedNL run (forcesequpdate semantics)."""
 
     def __init__(self, prob, compressor="topk", k_mult=8.0, alpha=None, option_b=True, workers=1,
-                 line_search=None):
+                 line_search=None, pp=None):
         self.prob = prob
         n, m, d = prob.features.shape
         self.n, self.m, self.d = n, m, d
@@ -226,6 +234,10 @@ class FedNL:
         if line_search:
             c, gamma, steps = line_search
             lib().orc_fednl_set_line_search(self.h, 1, c, gamma, int(steps))
+        if pp:
+            # FedNL-PP: (clients per round, client sampler seed)
+            tau, seed = pp
+            lib().orc_fednl_set_pp(self.h, int(tau), int(seed))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -242,6 +254,12 @@ class FedNL:
         return dict(ok=bool(ok), grad=g, lk=lk, cnt=cnt, pos=pos.reshape(n, kk), vals=vals.reshape(n, kk),
                     dx=dx, grad_norm=st[0], f=st[1], x_norm=st[2],
                     ls_count=lib().orc_fednl_line_search_count(self.h))
+
+    def pp_round(self, tracking=True):
+        """One FedNL-PP round: selected clients, full gradient at x_r, stats."""
+        sel = np.zeros(self.n, dtype=np.int32); g = np.zeros(self.d); st = np.zeros(3)
+        ok = lib().orc_fednl_pp_round(self.h, 1 if tracking else 0, ip(sel), dp(g), dp(st))
+        return dict(ok=bool(ok), sel=sel, grad=g, grad_norm=st[0], f=st[1], x_norm=st[2])
 
     def get_state(self):
         x = np.zeros(self.d); L = np.zeros(self.n * self.S); H = np.zeros(self.S)
@@ -298,3 +316,21 @@ def ref_fednl_run(prob, compressor="topk", k_mult=8.0, alpha=None, option_b=True
     return dict(status=st, grad_norm=gn, f=fv, x_norm=xn, x=x, seconds=sec, ls_counts=lsc, chol_ok=cok,
                 dump=dict(grad=dg.reshape(-1, d)[:dr], lk=dl.reshape(-1, n)[:dr], cnt=dc.reshape(-1, n)[:dr],
                           idx=di.reshape(-1, n, k)[:dr], dx=ddx.reshape(-1, d)[:dr]))
+
+
+def ref_fednl_pp_run(prob, compressor="topk", k_mult=8.0, tau=1, seed=1234, alpha=None, option_b=True,
+                     rounds=10, workers=1, tracking=True):
+    """FedNL-PP with the reference's own building blocks (oracle/_ref): tau
+    clients per round, client sampler seeded `seed`."""
+    n, m, d = prob.features.shape
+    k = d * (d + 1) // 2 if compressor in ("identical", "natural") else int(k_mult * d)
+    if alpha is None:
+        alpha = theoretical_alpha(compressor, k, d, option=2)
+    feat = np.ascontiguousarray(prob.features, dtype=np.float64)
+    lab = np.ascontiguousarray(prob.labels, dtype=np.float64)
+    gn = np.zeros(rounds); fv = np.zeros(rounds); xn = np.zeros(rounds); x = np.zeros(d)
+    cok = np.zeros(rounds, dtype=np.int32); sel = np.zeros(rounds * n, dtype=np.int32)
+    st = ref().ref_fednl_pp_run(dp(feat), dp(lab), n, m, d, prob.lam, COMPRESSORS[compressor], k, alpha,
+                                1 if option_b else 0, rounds, workers, 1 if tracking else 0, int(tau), int(seed),
+                                dp(gn), dp(fv), dp(xn), dp(x), ip(cok), ip(sel))
+    return dict(status=st, grad_norm=gn, f=fv, x_norm=xn, x=x, chol_ok=cok, sel=sel.reshape(rounds, n))

@@ -673,6 +673,14 @@ typedef struct
      * Newton direction from t = 1 */
     int ls, ls_max, ls_count;
     double ls_c, ls_gamma;
+    /* FedNL-PP (bin_fednl_local_pp/include/TrainLoop.h): tau clients per
+     * round, the client sampler, each client's last direction and Lk, the
+     * master's running direction average and Lk average */
+    int pp_tau;
+    uint32_t pp_rng;
+    double *pp_dir, *pp_lk, *pp_g, *pp_gmsg, *pp_lkmsg;
+    double pp_li;
+    int *pp_sel;
 } orc_fednl;
 
 ORC_API void orc_fednl_set_line_search(void *h, int on, double c, double gamma, int max_steps)
@@ -743,6 +751,7 @@ ORC_API void orc_fednl_destroy(void *h)
     free(f->x); free(f->learned); free(f->H); free(f->H0); free(f->rng); free(f->lek);
     free(f->grads); free(f->lks); free(f->fis); free(f->hx); free(f->delta);
     free(f->vals); free(f->pos); free(f->cnt);
+    free(f->pp_dir); free(f->pp_lk); free(f->pp_g); free(f->pp_gmsg); free(f->pp_lkmsg); free(f->pp_sel);
     free(f);
 }
 
@@ -769,33 +778,20 @@ ORC_API void orc_fednl_set_state(void *h, const double *x, const double *learned
     if (lek) memcpy(f->lek, lek, sizeof(uint32_t) * f->workers);
 }
 
-/* One round. Outputs (any may be NULL):
- *   grad[d]   averaged gradient at x_r,
- *   lk[n]     per-client Lk,
- *   cnt[n]    selected counts, pos[n*k] selected list positions (sorted,
- *             RandSeqK in window order), vals[n*k] values sent to the master,
- *   dx[d]     Newton step, stats[3] = (||grad||, f(x_r) or 0, ||x_r||).
- * Returns 1 on success, 0 if the Cholesky factorization failed. */
-ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int *cnt, uint32_t *pos,
-                            double *vals, double *dx, double *stats)
+/* The client step of a round (LocalWorkerForFedNL_*.h): H_c(x), Delta,
+ * Lk, selection, message values and the learned update; f->hx keeps H_c(x). */
+static void client_step(orc_fednl *f, int c, int want_f)
 {
-    orc_fednl *f = (orc_fednl *)h;
-    int n = f->n, m = f->m, d = f->d;
-    size_t S = f->S, k = (size_t)f->k;
-    size_t cap = S > k ? S : k;
-    double xx = 0.0;
-    for (int i = 0; i < d; ++i)
-        xx = xx + f->x[i] * f->x[i];
-    double xnorm = sqrt(xx);
-
-    for (int c = 0; c < n; ++c)
+    const int m = f->m, d = f->d;
+    const size_t S = f->S, k = (size_t)f->k;
+    const size_t cap = S > k ? S : k;
     {
         const double *feat = f->feat + (size_t)c * m * d;
         const double *lab = f->labels + (size_t)c * m;
         double *L = f->learned + (size_t)c * S;
         double fi = 0.0;
         orc_client_eval(feat, lab, d, m, f->lambda, f->x, NULL, NULL, f->grads + (size_t)c * d, f->hx,
-                        (tracking || f->ls) ? &fi : NULL, NULL);
+                        want_f ? &fi : NULL, NULL);
         f->fis[c] = fi;
         for (size_t p = 0; p < S; ++p)
             f->delta[p] = f->hx[p] - L[p];
@@ -850,6 +846,29 @@ ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int
             L[pc[q]] = L[pc[q]] + upd;
         }
     }
+}
+
+/* One round. Outputs (any may be NULL):
+ *   grad[d]   averaged gradient at x_r,
+ *   lk[n]     per-client Lk,
+ *   cnt[n]    selected counts, pos[n*k] selected list positions (sorted,
+ *             RandSeqK in window order), vals[n*k] values sent to the master,
+ *   dx[d]     Newton step, stats[3] = (||grad||, f(x_r) or 0, ||x_r||).
+ * Returns 1 on success, 0 if the Cholesky factorization failed. */
+ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int *cnt, uint32_t *pos,
+                            double *vals, double *dx, double *stats)
+{
+    orc_fednl *f = (orc_fednl *)h;
+    int n = f->n, m = f->m, d = f->d;
+    size_t S = f->S, k = (size_t)f->k;
+    size_t cap = S > k ? S : k;
+    double xx = 0.0;
+    for (int i = 0; i < d; ++i)
+        xx = xx + f->x[i] * f->x[i];
+    double xnorm = sqrt(xx);
+
+    for (int c = 0; c < n; ++c)
+        client_step(f, c, tracking || f->ls);
 
     /* master: clients in order 0..n-1 */
     double *g = (double *)calloc(d, sizeof(double));
@@ -955,5 +974,200 @@ ORC_API int orc_fednl_round(void *h, int tracking, double *grad, double *lk, int
     free(g);
     free(U);
     free(step);
+    return ok;
+}
+
+/* y = L x for the full symmetric matrix held as packed upper L, columns j
+ * ascending (MatrixNMD.h:1299-1333 on the full matrix). */
+static void sym_matvec_packed(const double *L, int d, const double *x, double *y)
+{
+    for (int i = 0; i < d; ++i)
+        y[i] = 0.0;
+    for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i)
+        {
+            const size_t p = i <= j ? (size_t)j * (j + 1) / 2 + i : (size_t)i * (i + 1) / 2 + j;
+            y[i] = y[i] + L[p] * x[j];
+        }
+}
+
+/* Turns the run into FedNL-PP with tau clients per round and the client
+ * sampler seeded `seed` (bin_fednl_local_pp: Args.h:242-264,
+ * TrainLoop.h:279-287). Every client starts from l_i = 0 and the direction
+ * H_i x - grad f_i(x) at the current x; the master's averages start from
+ * their means (LocalWorkerForFedNL_TopK.h:20-30; TrainLoop.h:1060-1080). */
+ORC_API void orc_fednl_set_pp(void *h, int tau, uint32_t seed)
+{
+    orc_fednl *f = (orc_fednl *)h;
+    const int n = f->n, m = f->m, d = f->d;
+    const size_t S = f->S;
+    f->pp_tau = tau;
+    f->pp_rng = seed;
+    f->pp_dir = (double *)calloc((size_t)n * d, sizeof(double));
+    f->pp_lk = (double *)calloc(n, sizeof(double));
+    f->pp_g = (double *)calloc(d, sizeof(double));
+    f->pp_gmsg = (double *)calloc((size_t)n * d, sizeof(double));
+    f->pp_lkmsg = (double *)calloc(n, sizeof(double));
+    f->pp_sel = (int *)calloc(n, sizeof(int));
+    double *gr = (double *)malloc(sizeof(double) * d);
+    double *mv = (double *)malloc(sizeof(double) * d);
+    for (int c = 0; c < n; ++c)
+    {
+        orc_client_eval(f->feat + (size_t)c * m * d, f->labels + (size_t)c * m, d, m, f->lambda, f->x, NULL,
+                        NULL, gr, NULL, NULL, NULL);
+        sym_matvec_packed(f->learned + (size_t)c * S, d, f->x, mv);
+        for (int i = 0; i < d; ++i)
+            f->pp_dir[(size_t)c * d + i] = mv[i] - gr[i];
+    }
+    double li = 0.0;
+    for (int c = 0; c < n; ++c)
+    {
+        for (int i = 0; i < d; ++i)
+            f->pp_g[i] = f->pp_g[i] + f->pp_dir[(size_t)c * d + i];
+        li = li + f->pp_lk[c];
+    }
+    for (int i = 0; i < d; ++i)
+        f->pp_g[i] = f->pp_g[i] * f->inv_n;
+    f->pp_li = li * f->inv_n;
+    free(gr);
+    free(mv);
+}
+
+/* One FedNL-PP round (bin_fednl_local_pp/include/TrainLoop.h:1148-1720,
+ * LocalWorkerForFedNL_*.h). Selected clients: the FedNL client step, then
+ * (option b) Lk_new = ||learned_new - H_i(x)||_F and the message
+ * Lk_new - Lk_old, and the direction l_i x + learned_new x - grad f_i(x),
+ * sent as its change. Master, clients in order: g = (sum of changes) / n +
+ * g_prev, l = (sum of Lk changes) / n + l_prev, H[idx] += val, then
+ * x = (H + shift I)^-1 g (Cholesky, Gauss-Jordan if it fails; shift = l or
+ * lambda). Outputs: sel[n] (0/1), grad[d] = the full gradient at x_r (all
+ * clients, sum / n), stats[3] = (||grad||, f or 0, ||x_r||), where f sums
+ * the selected clients' f_i / n (TrainLoop.h:1648-1676). Returns 1 if the
+ * factorization succeeded. */
+ORC_API int orc_fednl_pp_round(void *h, int tracking, int *sel, double *grad, double *stats)
+{
+    orc_fednl *f = (orc_fednl *)h;
+    const int n = f->n, m = f->m, d = f->d;
+    const size_t S = f->S, k = (size_t)f->k;
+    const size_t cap = S > k ? S : k;
+    uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    for (int c = 0; c < n; ++c)
+        order[c] = (uint32_t)c;
+    orc_shuffle_partial(order, (size_t)n, (size_t)f->pp_tau, &f->pp_rng);
+    for (int c = 0; c < n; ++c)
+        f->pp_sel[c] = 0;
+    for (int q = 0; q < f->pp_tau; ++q)
+        f->pp_sel[order[q]] = 1;
+    free(order);
+
+    double xx = 0.0;
+    for (int i = 0; i < d; ++i)
+        xx = xx + f->x[i] * f->x[i];
+    const double xnorm = sqrt(xx);
+    double *tg = (double *)calloc(d, sizeof(double));
+    double *gr = (double *)malloc(sizeof(double) * d);
+    for (int c = 0; c < n; ++c)
+    {
+        orc_client_eval(f->feat + (size_t)c * m * d, f->labels + (size_t)c * m, d, m, f->lambda, f->x, NULL,
+                        NULL, gr, NULL, NULL, NULL);
+        for (int i = 0; i < d; ++i)
+            tg[i] = tg[i] + gr[i];
+    }
+    for (int i = 0; i < d; ++i)
+        tg[i] = tg[i] / (double)n;
+
+    double *res = (double *)malloc(sizeof(double) * S);
+    double *mv = (double *)malloc(sizeof(double) * d);
+    for (int c = 0; c < n; ++c)
+    {
+        if (!f->pp_sel[c])
+            continue;
+        client_step(f, c, tracking);
+        const double *L = f->learned + (size_t)c * S;
+        if (f->option_b)
+        {
+            for (size_t p = 0; p < S; ++p)
+                res[p] = L[p] - f->hx[p];
+            const double lk_new = orc_frob_upper_packed(res, d);
+            f->pp_lkmsg[c] = lk_new - f->pp_lk[c];
+            f->pp_lk[c] = lk_new;
+        }
+        sym_matvec_packed(L, d, f->x, mv);
+        const double *gc = f->grads + (size_t)c * d;
+        for (int i = 0; i < d; ++i)
+        {
+            const double dir = (f->x[i] * f->pp_lk[c] + mv[i]) - gc[i];
+            f->pp_gmsg[(size_t)c * d + i] = dir - f->pp_dir[(size_t)c * d + i];
+            f->pp_dir[(size_t)c * d + i] = dir;
+        }
+    }
+    free(res);
+    free(mv);
+
+    double *g = (double *)calloc(d, sizeof(double));
+    double li = 0.0, fagg = 0.0;
+    for (int c = 0; c < n; ++c)
+    {
+        if (!f->pp_sel[c])
+            continue;
+        for (int i = 0; i < d; ++i)
+            g[i] = g[i] + f->pp_gmsg[(size_t)c * d + i];
+        const uint32_t *pc = f->pos + (size_t)c * cap;
+        const double *vc = f->vals + (size_t)c * cap;
+        for (int q = 0; q < f->cnt[c]; ++q)
+            f->H[pc[q]] = f->H[pc[q]] + vc[q];
+        li = li + f->pp_lkmsg[c];
+        fagg = fagg + f->fis[c];
+    }
+    for (int i = 0; i < d; ++i)
+    {
+        g[i] = g[i] * f->inv_n;
+        g[i] = g[i] + f->pp_g[i];
+        f->pp_g[i] = g[i];
+    }
+    if (f->option_b)
+    {
+        li = li * f->inv_n;
+        li = li + f->pp_li;
+        f->pp_li = li;
+    }
+    fagg = fagg * f->inv_n;
+
+    const double shift = f->option_b ? li : f->lambda;
+    double *U = (double *)malloc(sizeof(double) * S);
+    int ok = orc_cholesky_packed(f->H, d, shift, U);
+    if (ok)
+        orc_tri_solve_packed(U, d, g, f->x);
+    else
+    {
+        double *M = (double *)malloc(sizeof(double) * (size_t)d * d);
+        for (int j = 0; j < d; ++j)
+            for (int i = 0; i <= j; ++i)
+            {
+                const size_t p = (size_t)j * (j + 1) / 2 + i;
+                M[(size_t)i * d + j] = f->H[p] + (i == j ? shift : 0.0);
+                if (i != j)
+                    M[(size_t)j * d + i] = f->H0[p];
+            }
+        orc_gauss_solve(M, g, d, f->x);
+        free(M);
+    }
+    free(U);
+    double gg = 0.0;
+    for (int i = 0; i < d; ++i)
+        gg = gg + tg[i] * tg[i];
+    if (sel)
+        memcpy(sel, f->pp_sel, sizeof(int) * n);
+    if (grad)
+        memcpy(grad, tg, sizeof(double) * d);
+    if (stats)
+    {
+        stats[0] = sqrt(gg);
+        stats[1] = tracking ? fagg : 0.0;
+        stats[2] = xnorm;
+    }
+    free(g);
+    free(tg);
+    free(gr);
     return ok;
 }

@@ -564,4 +564,247 @@ extern "C"
             delete probs[c];
         return status;
     }
+
+    // FedNL-PP (dopt/bin_fednl_local_pp): each round `tau` clients drawn by a
+    // partial shuffle of 0..n-1 with one RandomGenIntegerLinear seeded
+    // `client_seed` (TrainLoop.h:279-287,1155-1157) do the FedNL client step,
+    // then send Lk_new - Lk_old (option b) and the change of their direction
+    // l_i x + H_i x - grad f_i(x) (LocalWorkerForFedNL_TopK.h:20-30,129-145).
+    // The master keeps running averages of both and solves
+    // (H + shift I) x_{k+1} = g (TrainLoop.h:1060-1080,1175-1640). Clients of a
+    // worker run in ascending order and the master consumes 0..n-1 in order
+    // (forcesequpdate). Per round: ||grad f(x_r)|| of all clients, f (sum
+    // over the selected clients / n, with tracking), ||x_r||, factorization
+    // flag, and the selected clients (sel_out[r*n + c] = 0/1).
+    int ref_fednl_pp_run(const double* feat, const double* labels, int n, int m, int d,
+                         double lambda, int compressor, int k, double alpha, int option_b,
+                         int rounds, int workers, int tracking, int tau, uint32_t client_seed,
+                         double* grad_norm, double* fval, double* x_norm, double* x_out, int* chol_ok,
+                         int* sel_out)
+    {
+        if (workers < 1)
+            workers = 1;
+        if (workers > n)
+            workers = n;
+        if (tau < 1 || tau > n)
+            return -2;
+
+        std::vector<Problem*> probs(n);
+        std::vector<Mat> learned(n);
+        std::vector<Vec> local_dir(n);
+        std::vector<double> local_lk(n, 0.0);
+        Mat H = Mat::getZeroSquareMatrix(d);
+        Vec x0(d);
+        Vec prev_g(d);
+        double prev_li = 0.0;
+        for (int c = 0; c < n; ++c)
+        {
+            Mat atr = designTr(feat + (size_t)c * m * d, d, m);
+            Vec b = vecFrom(labels + (size_t)c * m, m);
+            probs[c] = new Problem(atr, b, lambda);
+            learned[c] = probs[c]->evaluateHessian(x0);
+        }
+        // vectors a worker reassigns are allocated (and at the end released)
+        // on that worker: the reference's vector memory pools are per thread
+        std::vector<Worker*> pool(workers);
+        for (int w = 0; w < workers; ++w)
+            pool[w] = new Worker();
+        for (int w = 0; w < workers; ++w)
+            pool[w]->submit([&, w] {
+                for (int c = w; c < n; c += workers)
+                    local_dir[c] = Mat::matrixVectorMultiply(learned[c], x0) - probs[c]->evaluateGradient(x0);
+            });
+        for (int w = 0; w < workers; ++w)
+            pool[w]->wait();
+        for (int c = 0; c < n; ++c)
+        {
+            H += learned[c];
+            prev_g += local_dir[c];
+            prev_li += local_lk[c];
+        }
+        double inv_n = 1.0 / double(n);
+        H *= inv_n;
+        prev_g *= inv_n;
+        prev_li *= inv_n;
+        double prescale = inv_n * alpha;
+        size_t S = ((size_t)(d + 1) * d) / 2;
+        double mult = double(S) / double(k);
+        double delta_toplek = double(k) * (1.0 / double(S));
+        std::vector<uint32_t> up = dopt::indiciesForUpperTriangularPart(H);
+        std::vector<dopt::RandomGenIntegerLinear> gens(n);
+        for (int c = 0; c < n; ++c)
+            gens[c].setSeed((uint32_t)(123 + c));
+        dopt::RandomGenIntegerLinear sampler;
+        sampler.setSeed(client_seed);
+
+        Vec x = x0;
+        std::vector<Vec> gmsg(n);
+        std::vector<double> lkmsg(n), fis(n);
+        std::vector<std::vector<uint32_t>> idxs(n);
+        std::vector<std::vector<double>> vals(n);
+        std::vector<char> selected(n, 0);
+        int status = 0;
+
+        for (int r = 0; r < rounds; ++r)
+        {
+            std::vector<uint32_t> order(n);
+            for (int c = 0; c < n; ++c)
+                order[c] = (uint32_t)c;
+            dopt::shuffle(order, (size_t)tau, sampler);
+            std::fill(selected.begin(), selected.end(), 0);
+            for (int q = 0; q < tau; ++q)
+                selected[order[q]] = 1;
+            if (sel_out)
+                for (int c = 0; c < n; ++c)
+                    sel_out[(size_t)r * n + c] = selected[c];
+
+            double xnorm = x.vectorL2Norm();
+            Vec tg(d);
+            for (int c = 0; c < n; ++c)
+                tg += probs[c]->evaluateGradient(x);
+            tg = tg / double(n);
+
+            for (int w = 0; w < workers; ++w)
+            {
+                pool[w]->submit([&, w] {
+                    for (int c = w; c < n; c += workers)
+                    {
+                        if (!selected[c])
+                            continue;
+                        Problem* p = probs[c];
+                        Vec t = p->evaluateClassificationMargin(x);
+                        Vec s = p->evaluateClassificationMarginSigmoid(t);
+                        Mat hx = p->evaluateHessian(x, t, s);
+                        Mat diff = hx - learned[c];
+                        std::vector<uint32_t> kc;
+                        double scale = 1.0;
+                        if (compressor == C_TOPK)
+                            kc = dopt::getTopKFromUpperDiagonalPart<true>(diff, (size_t)k, up);
+                        else if (compressor == C_TOPLEK)
+                            kc = dopt::getTopLEKFromUpperDiagonalPart<true>(diff, (size_t)k, up, delta_toplek);
+                        else if (compressor == C_RANDK)
+                        {
+                            kc = dopt::generateRandKItemsInUpperTriangularPart(gens[c], (size_t)k, learned[c], up);
+                            scale = mult;
+                        }
+                        else if (compressor == C_RANDSEQK)
+                        {
+                            kc = dopt::generateRandSeqKItemsInUpperTriangularPart(gens[c], (size_t)k, learned[c], up);
+                            scale = mult;
+                        }
+                        else
+                            kc = up;
+                        if (compressor == C_NATURAL)
+                            diff.applyNaturalCompressor();
+                        std::vector<double> vc(kc.size());
+                        for (size_t q = 0; q < kc.size(); ++q)
+                        {
+                            double item = diff.matrixByCols.get(kc[q]);
+                            if (scale != 1.0)
+                                item = item * scale;
+                            vc[q] = item * prescale;
+                        }
+                        for (size_t q = 0; q < kc.size(); ++q)
+                        {
+                            size_t first = kc[q];
+                            size_t iRow = 0, iCol = 0;
+                            diff.getPositionFromFlatternIndex(iRow, iCol, first);
+                            double item = diff.matrixByCols.get(first);
+                            if (scale != 1.0)
+                                item = item * scale;
+                            learned[c].matrixByCols[first] += alpha * item;
+                            if (iRow != iCol)
+                            {
+                                size_t second = diff.getFlattenIndexFromPosition(iCol, iRow);
+                                learned[c].matrixByCols[second] += alpha * item;
+                            }
+                        }
+                        if (option_b)
+                        {
+                            double newLk = (learned[c] - hx).frobeniusNormForSymmetricMatrixFromUpPart();
+                            lkmsg[c] = newLk - local_lk[c];
+                            local_lk[c] = newLk;
+                        }
+                        Vec dir = (x * local_lk[c]) + Mat::matrixVectorMultiply(learned[c], x) -
+                                  p->evaluateGradient(x, t, s);
+                        gmsg[c] = dir - local_dir[c];
+                        local_dir[c] = dir;
+                        idxs[c].swap(kc);
+                        vals[c].swap(vc);
+                        if (tracking)
+                            fis[c] = p->evaluateFunction(x, t, s);
+                    }
+                });
+            }
+            for (int w = 0; w < workers; ++w)
+                pool[w]->wait();
+
+            Vec g(d);
+            double li = 0.0, fagg = 0.0;
+            for (int c = 0; c < n; ++c)
+            {
+                if (!selected[c])
+                    continue;
+                g += gmsg[c];
+                for (size_t q = 0; q < idxs[c].size(); ++q)
+                    H.matrixByCols[idxs[c][q]] += vals[c][q];
+                if (option_b)
+                    li += lkmsg[c];
+                if (tracking)
+                    fagg += fis[c];
+            }
+            g *= inv_n;
+            g += prev_g;
+            prev_g = g;
+            if (option_b)
+            {
+                li *= inv_n;
+                li += prev_li;
+                prev_li = li;
+            }
+            fagg *= inv_n;
+
+            double shift = option_b ? li : probs[0]->computeMuStrongConvexity();
+            Mat ltr;
+            bool ok = dopt::cholFactorization::choleskyFactorization<Mat, true, true>(ltr, H, shift);
+            if (ok)
+            {
+                Vec y = dopt::forwardSubstitutionWithATranspose(ltr, g);
+                x = dopt::backwardSubstitutionWithATranspose(ltr, y);
+            }
+            else
+            {
+                Mat slau = H;
+                slau.addToAllDiagonalEntries(shift);
+                x = dopt::gausEleminationSolver(slau, g);
+                status = 1;
+            }
+            if (chol_ok)
+                chol_ok[r] = ok ? 1 : 0;
+            if (grad_norm)
+                grad_norm[r] = tg.vectorL2Norm();
+            if (fval)
+                fval[r] = tracking ? fagg : 0.0;
+            if (x_norm)
+                x_norm[r] = xnorm;
+        }
+
+        for (int i = 0; i < d; ++i)
+            x_out[i] = x[i];
+        for (int w = 0; w < workers; ++w)
+            pool[w]->submit([&, w] {
+                for (int c = w; c < n; c += workers)
+                {
+                    gmsg[c] = Vec();
+                    local_dir[c] = Vec();
+                }
+            });
+        for (int w = 0; w < workers; ++w)
+            pool[w]->wait();
+        for (int w = 0; w < workers; ++w)
+            delete pool[w];
+        for (int c = 0; c < n; ++c)
+            delete probs[c];
+        return status;
+    }
 }

@@ -163,3 +163,40 @@ def test_gauss_solve_known_answers(oracle):
         x = np.zeros(d)
         oracle.lib().orc_gauss_solve(oracle.dp(A.copy()), oracle.dp(bb.copy()), d, oracle.dp(x))
         np.testing.assert_allclose(x, np.linalg.solve(A, bb), rtol=1e-10, atol=1e-12)
+
+
+# FedNL-PP (bin_fednl_local_pp): tau clients per round from the client
+# sampler (seed 1234, Args.h:242-244), running averages of the clients'
+# direction and Lk changes, x_{k+1} = (H + shift I)^-1 g.
+PP_CASES = [("topk", 3, 2, True), ("randk", 4, 3, True), ("seqk", 5, 1, False), ("toplek", 6, 4, True),
+            ("natural", 7, 2, True), ("identical", 8, 3, False), ("topk", 9, 5, True)]
+
+
+@pytest.mark.parametrize("comp,seed,tau,option_b", PP_CASES)
+def test_fednl_pp_trajectories(oracle, comp, seed, tau, option_b):
+    prob = datagen.small_problem(n=5, m=40, d_feat=11, seed=seed)
+    R = 30
+    r = oracle.ref_fednl_pp_run(prob, comp, 1.0, tau=tau, seed=1234, option_b=option_b, rounds=R, workers=2)
+    f = oracle.FedNL(prob, comp, k_mult=1.0, option_b=option_b, workers=2, pp=(tau, 1234))
+    outs = [f.pp_round(tracking=True) for _ in range(R)]
+    np.testing.assert_array_equal(np.array([o["sel"] for o in outs]), r["sel"])
+    assert np.all(r["sel"].sum(axis=1) == tau)
+    np.testing.assert_array_equal([int(o["ok"]) for o in outs], r["chol_ok"])
+    assert_grad_norms_close(np.array([o["grad_norm"] for o in outs]), r["grad_norm"])
+    np.testing.assert_allclose([o["f"] for o in outs], r["f"], rtol=1e-13)
+    np.testing.assert_allclose([o["x_norm"] for o in outs], r["x_norm"], rtol=1e-10, atol=1e-14)
+
+
+def test_pp_client_sampler_is_the_reference_shuffle(oracle):
+    """The clients of a round are the first tau entries of dopt::shuffle of
+    0..n-1 (Shuffle.h:31-93), one generator across rounds."""
+    n, tau, R = 9, 4, 6
+    prob = datagen.small_problem(n=n, m=20, d_feat=5, seed=2)
+    r = oracle.ref_fednl_pp_run(prob, "topk", 1.0, tau=tau, seed=1234, rounds=R)
+    st = np.array([1234], dtype=np.uint32)
+    for rr in range(R):
+        v = np.arange(n, dtype=np.uint32)
+        oracle.lib().orc_shuffle_partial(oracle.up(v), n, tau, oracle.up(st))
+        want = np.zeros(n, dtype=np.int32)
+        want[v[:tau]] = 1
+        np.testing.assert_array_equal(r["sel"][rr], want)
