@@ -68,7 +68,10 @@ typedef struct fednl_opts {
     int max_line_search_steps;  /* trial points per round (--max_line_search_steps, default 5) */
     double c_line_search;       /* sufficient-decrease constant c (--c_line_search, default 0.5) */
     double gamma_line_search;   /* step shrink factor in (0, 1) (--gamma_line_search, default 0.5) */
-    int reserved[4];
+    int clients_per_round;      /* 0: FedNL; tau in [1, n]: FedNL-PP with tau clients drawn per round
+                                   (bin_fednl_local_pp, --clients-per-round, TrainLoop.h:279-287) */
+    unsigned client_selection_seed;  /* FedNL-PP client sampler seed (default 1234, Args.h:244) */
+    int reserved[2];
 } fednl_opts;
 
 void fednl_default_opts(fednl_opts *o);
@@ -117,9 +120,19 @@ int fednl_trace(void *handle, double *grad_norm, double *fval, double *x_norm, d
  * 0 when line search is off). Arrays sized >= *count. */
 int fednl_trace_line_search(void *handle, int *trials, int *count);
 
+/* Per round: clients that took part (n, or tau for FedNL-PP) and the Hessian
+ * items they sent (k each; TopLEK: its per-client counts). The inputs of
+ * dopt's received-bytes counters (TrainLoop.h:1104-1287). Arrays sized >= *count. */
+int fednl_trace_clients(void *handle, int *participants, double *items, int *count);
+
+/* FedNL-PP: the clients selected in the last round (n flags 0/1), the first
+ * tau entries of dopt::shuffle of 0..n-1 (TrainLoop.h:1155-1157). */
+int fednl_pp_selection(void *handle, int *selected);
+
 int fednl_get_x(void *handle, double *x);
 
-/* Outputs of the last round, for parity checks: averaged gradient (d),
+/* Outputs of the last round, for parity checks: averaged gradient (d; for
+ * FedNL-PP the full gradient at x_r),
  * per-client Lk (n), selected counts (n), selected list positions (n*k,
  * ascending), values sent to the master (n*k), flat indices (n*k), Newton
  * step (d). Any pointer may be NULL. */
@@ -233,6 +246,15 @@ int doptExecuteLocal(const char *command, ResultCallbackLocal resultCallback);
 
 /* The CLI entry (bin_fednl_local main, dopt/bin_fednl_local/src/main.cpp:25-59). */
 int fednl_cli_main(int argc, char **argv);
+
+/* FedNL-PP drop-ins: bin_fednl_local_pp's main (dopt/bin_fednl_local_pp/src/main.cpp:22-56)
+ * and its shared-library entry (dopt/bin_fednl_local_pp/exported/FedNLPPLocalShared.h:8-17,
+ * src/mainShared.cpp:15-39): the bin_fednl_local flags without the line
+ * search, plus --clients-per-round. The callback receives d once, on success
+ * (TrainLoop.h:1870-1872). */
+typedef void (*ResultCallbackLocalPP)(int xDimension);
+int doptExecuteLocalPP(const char *command, ResultCallbackLocalPP resultCallback);
+int fednl_cli_pp_main(int argc, char **argv);
 
 #ifdef __cplusplus
 }

@@ -86,7 +86,7 @@ def lib():
         L.orc_gauss_solve.argtypes = [_c_dp, _c_dp, ctypes.c_int, _c_dp]
         L.orc_fednl_set_pp.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32]
         L.orc_fednl_pp_round.restype = ctypes.c_int
-        L.orc_fednl_pp_round.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_ip, _c_dp, _c_dp]
+        L.orc_fednl_pp_round.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_ip, _c_ip, _c_dp, _c_dp]
         L.orc_natural.restype = ctypes.c_double
         L.orc_natural.argtypes = [ctypes.c_double]
         _lib = L
@@ -257,9 +257,10 @@ class FedNL:
 
     def pp_round(self, tracking=True):
         """One FedNL-PP round: selected clients, full gradient at x_r, stats."""
-        sel = np.zeros(self.n, dtype=np.int32); g = np.zeros(self.d); st = np.zeros(3)
-        ok = lib().orc_fednl_pp_round(self.h, 1 if tracking else 0, ip(sel), dp(g), dp(st))
-        return dict(ok=bool(ok), sel=sel, grad=g, grad_norm=st[0], f=st[1], x_norm=st[2])
+        sel = np.zeros(self.n, dtype=np.int32); cnt = np.zeros(self.n, dtype=np.int32)
+        g = np.zeros(self.d); st = np.zeros(3)
+        ok = lib().orc_fednl_pp_round(self.h, 1 if tracking else 0, ip(sel), ip(cnt), dp(g), dp(st))
+        return dict(ok=bool(ok), sel=sel, cnt=cnt, grad=g, grad_norm=st[0], f=st[1], x_norm=st[2])
 
     def get_state(self):
         x = np.zeros(self.d); L = np.zeros(self.n * self.S); H = np.zeros(self.S)

@@ -1040,11 +1040,12 @@ ORC_API void orc_fednl_set_pp(void *h, int tau, uint32_t seed)
  * sent as its change. Master, clients in order: g = (sum of changes) / n +
  * g_prev, l = (sum of Lk changes) / n + l_prev, H[idx] += val, then
  * x = (H + shift I)^-1 g (Cholesky, Gauss-Jordan if it fails; shift = l or
- * lambda). Outputs: sel[n] (0/1), grad[d] = the full gradient at x_r (all
+ * lambda). Outputs: sel[n] (0/1), cnt[n] (items each selected client sent,
+ * 0 for the others), grad[d] = the full gradient at x_r (all
  * clients, sum / n), stats[3] = (||grad||, f or 0, ||x_r||), where f sums
  * the selected clients' f_i / n (TrainLoop.h:1648-1676). Returns 1 if the
  * factorization succeeded. */
-ORC_API int orc_fednl_pp_round(void *h, int tracking, int *sel, double *grad, double *stats)
+ORC_API int orc_fednl_pp_round(void *h, int tracking, int *sel, int *cnt, double *grad, double *stats)
 {
     orc_fednl *f = (orc_fednl *)h;
     const int n = f->n, m = f->m, d = f->d;
@@ -1158,6 +1159,9 @@ ORC_API int orc_fednl_pp_round(void *h, int tracking, int *sel, double *grad, do
         gg = gg + tg[i] * tg[i];
     if (sel)
         memcpy(sel, f->pp_sel, sizeof(int) * n);
+    if (cnt)
+        for (int c = 0; c < n; ++c)
+            cnt[c] = f->pp_sel[c] ? f->cnt[c] : 0;
     if (grad)
         memcpy(grad, tg, sizeof(double) * d);
     if (stats)

@@ -2,6 +2,7 @@
 
   ufednl_amd/libfednl_hip.so   HIP kernels + host engine + C ABI (hipcc, gfx950)
   ufednl_amd/bin_fednl_local   drop-in CLI linked against it
+  ufednl_amd/bin_fednl_local_pp  the FedNL-PP CLI, linked the same way
 
 Run: python -m ufednl_amd.build  (or __graft_entry__.build()).
 """
@@ -18,6 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 INC = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libfednl_hip.so")
 CLI = os.path.join(HERE, "bin_fednl_local")
+CLI_PP = os.path.join(HERE, "bin_fednl_local_pp")
 
 ARCH = os.environ.get("FEDNL_OFFLOAD_ARCH", "gfx950")
 
@@ -54,11 +56,10 @@ def build_lib():
 
 
 def build_cli():
-    main = os.path.join(CSRC, "bin_fednl_local_main.cpp")
-    if not os.path.exists(main):
-        return
-    run(["g++", "-O2", "-std=c++17", f"-I{INC}", "-o", CLI, main, f"-L{HERE}", "-lfednl_hip",
-         f"-Wl,-rpath,$ORIGIN"])
+    for src, out in (("bin_fednl_local_main.cpp", CLI), ("bin_fednl_local_pp_main.cpp", CLI_PP)):
+        main = os.path.join(CSRC, src)
+        run(["g++", "-O2", "-std=c++17", f"-I{INC}", "-o", out, main, f"-L{HERE}", "-lfednl_hip",
+             f"-Wl,-rpath,$ORIGIN"])
 
 
 def build_oracle():

@@ -116,6 +116,9 @@ struct Args {
     bool silent_print = false, meminfo = false, forceseq = false, crc_data = false, print_train = false;
     bool silent = false;
     bool transfer_idx = false;
+    // FedNL-PP (bin_fednl_local_pp/include/Args.h:242-264)
+    bool pp = false;
+    int clients_per_round = -1;
 };
 
 static int parse_debug(const CmdLine& cl, Args& a)
@@ -157,6 +160,7 @@ static bool file_exists(const std::string& p)
 
 static int parse_args(const CmdLine& cl, Args& a)
 {
+    // a.pp is set by the caller (which executable this is)
     int rc = parse_debug(cl, a);
     if (rc != kOk)
         return rc;
@@ -203,7 +207,7 @@ static int parse_args(const CmdLine& cl, Args& a)
             std::cerr << "Please specify argument \"alpha_step_size\" with value of step size for FedNL\n";
             return kMissingArgument;
         }
-        a.line_search = cl.flag("line_search");
+        a.line_search = !a.pp && cl.flag("line_search");  // bin_fednl_local_pp has no line search
         if (a.line_search) {
             if (!cl.num("c_line_search", a.c_ls)) {
                 std::cerr << "Please specify argument \"c_line_search\" for line search. Line search parameter c in (0, 0.5). \n";
@@ -224,6 +228,8 @@ static int parse_args(const CmdLine& cl, Args& a)
         return kMissingArgument;
     }
     cl.num("x0_seed", a.x0_seed);
+    if (a.pp)
+        cl.num("clients-per-round", a.clients_per_round);
     a.option_a = cl.flag("fednl-option-a");
     a.option_b = cl.flag("fednl-option-b");
     a.silent = cl.flag("silent");
@@ -393,9 +399,10 @@ struct Bytes {
     void f64(double v) { const char* p = (const char*)&v; b.insert(b.end(), p, p + 8); }
 };
 
-static int train(const CmdLine& cl, ResultCallbackLocal cb)
+static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalPP cb_pp, bool pp)
 {
     Args a;
+    a.pp = pp;
     int rc = parse_args(cl, a);
     if (rc != kOk)
         return rc;
@@ -426,6 +433,11 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
     const int n = a.clients;
     if (n <= 0 || n > ds.N) {
         std::cerr << "Number of clients must be in [1, number of samples]\n";
+        return kWrongArgument;
+    }
+    if (a.pp && (a.clients_per_round < 1 || a.clients_per_round > n)) {
+        // bin_fednl_local_pp asserts 0 < clients-per-round <= clients (TrainLoop.h:282)
+        std::cerr << "Please specify --clients-per-round in [1, " << n << "]\n";
         return kWrongArgument;
     }
     const int mpc = ds.N / n;  // TrainLoop.h:212-250: residual samples are dropped
@@ -509,6 +521,7 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
     o.c_line_search = a.c_ls;
     o.gamma_line_search = a.gamma_ls;
     o.max_line_search_steps = (int)a.max_ls;
+    o.clients_per_round = a.pp ? a.clients_per_round : 0;
     void* eng = nullptr;
     rc = fednl_create(&o, n, mpc, d, feats.data(), labs.data(), &eng);
     if (rc != FEDNL_OK) {
@@ -537,34 +550,34 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
     std::vector<double> x(d), glast(d);
     fednl_get_x(eng, x.data());
     fednl_last_round(eng, glast.data(), nullptr, cnt.data(), nullptr, nullptr, nullptr, nullptr);
-    std::vector<int> ls_trials(R, 0);
+    std::vector<int> ls_trials(R, 0), parts(R, n);
+    std::vector<double> items(R, 0.0);
     fednl_trace_line_search(eng, ls_trials.data(), &R);
+    fednl_trace_clients(eng, parts.data(), items.data(), &R);
 
     // bytes received from the clients per round (TrainLoop.h:1104,1131-1132,1287,1533,1593):
-    // the f_i message of the round is counted with the size of the Lk message
-    // (8 bytes with option b, none with option a); each line-search trial adds
-    // one 8-byte f_i per client
+    // every participating client (all n, or tau for FedNL-PP) sends its
+    // gradient (8d), the Hessian items and, with option b, Lk (8); the f_i
+    // message of the round is counted with the size of the Lk message (8 with
+    // option b, none with option a); each line-search trial adds one 8-byte
+    // f_i per client. Items: k per client, TopLEK its counts, Natural three
+    // bytes per pair of values with one trailing pair
+    // (LocalWorkerForFedNL_Natural.h:80-120)
     const bool send_fi = a.tracking || a.line_search;  // TrainLoop.h:620
-    auto round_bytes = [&](int trials, double& scal, double& idxb) {
-        scal = 0.0;
+    auto round_bytes = [&](int r, double& scal, double& idxb) {
+        const double np = (double)parts[r];
+        const double item_bytes = (comp == FEDNL_NATURAL) ? np * 3.0 * (double)(S / 2 + 1) : 8.0 * items[r];
+        scal = np * (8.0 * d + (a.option_b ? 8.0 : 0.0) + ((send_fi && a.option_b) ? 8.0 : 0.0) +
+                     8.0 * ls_trials[r]) + item_bytes;
         idxb = 0.0;
-        for (int c = 0; c < n; ++c) {
-            const double items = (comp == FEDNL_TOPLEK) ? (double)cnt[c] : (double)k;
-            // Natural: sign + exponent of two values per 3 bytes, one trailing pair
-            // (LocalWorkerForFedNL_Natural.h:80-120)
-            const double item_bytes = (comp == FEDNL_NATURAL) ? 3.0 * (double)(S / 2 + 1) : 8.0 * items;
-            scal += 8.0 * d + item_bytes + (a.option_b ? 8.0 : 0.0);
-            if (send_fi) scal += a.option_b ? 8.0 : 0.0;
-            scal += 8.0 * trials;
-            if (comp == FEDNL_TOPK) idxb += 4.0 * k;
-            else if (comp == FEDNL_TOPLEK) idxb += 4.0 + 4.0 * cnt[c];
-            else if ((comp == FEDNL_RANDK || comp == FEDNL_RANDSEQK) && a.transfer_idx) idxb += 4.0 * k;
-        }
+        if (comp == FEDNL_TOPK) idxb = np * 4.0 * k;
+        else if (comp == FEDNL_TOPLEK) idxb = 4.0 * np + 4.0 * items[r];
+        else if ((comp == FEDNL_RANDK || comp == FEDNL_RANDSEQK) && a.transfer_idx) idxb = np * 4.0 * k;
     };
     Bytes tr;
     if (a.tracking) {
         tr.str(base_name(a.path), true);
-        std::string alg = "FedNL";
+        std::string alg = a.pp ? "FedNL-PP" : "FedNL";
         if (a.line_search) alg += "-LS";
         if (a.option_a) alg += " (a)";
         if (a.option_b) alg += " (b)";
@@ -589,10 +602,16 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
         if (!a.silent && print) {
             const double t = sec[r] > 0 ? sec[r] : 1e-9;
             std::cout << "  Round #" << r << ": norm of full gradient: " << gn[r];
-            if (send_fi) std::cout << ": f(x): " << fv[r];
-            else std::cout << ": f(x): [N/A]";
-            std::cout << ": avg. time per round: " << 1e3 * t / (r + 1) << " msec"
-                      << ": rounds/second: " << (r + 1) / t << ": line-search num: " << ls_trials[r] << '\n';
+            if (a.pp) {
+                // bin_fednl_local_pp/include/TrainLoop.h:1688-1696
+                std::cout << ": f(x): " << fv[r] << ": avg. time per round: " << 1e3 * t / (r + 1)
+                          << " milliseconds" << '\n';
+            } else {
+                if (send_fi) std::cout << ": f(x): " << fv[r];
+                else std::cout << ": f(x): [N/A]";
+                std::cout << ": avg. time per round: " << 1e3 * t / (r + 1) << " msec"
+                          << ": rounds/second: " << (r + 1) / t << ": line-search num: " << ls_trials[r] << '\n';
+            }
         }
         if (a.tracking) {
             tr.i32(r);
@@ -604,7 +623,7 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
             tr.f64(sec[r]);
         }
         double bs, bi;
-        round_bytes(ls_trials[r], bs, bi);
+        round_bytes(r, bs, bi);
         acc_s += bs;
         acc_i += bi;
     }
@@ -623,6 +642,8 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
         std::cout << "  Average time per round: " << (R ? elapsed_ms / R : 0.0) << " milliseconds\n";
         std::cout << "  Total number of rounds: " << a.rounds << '\n';
         std::cout << "  Total number of clients: " << n << '\n';
+        if (a.pp)
+            std::cout << "  Number of clients per round selected u.a.r: " << a.clients_per_round << '\n';
         std::cout << "  Algorithm: " << a.algorithm << '\n';
         std::cout << "  Data set: " << a.path << '\n';
         std::cout << "  d: " << d << '\n';
@@ -632,15 +653,20 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
         std::cout << "  Clients compressor: " << a.compressor << '\n';
         std::cout << "  Clients compressor K: " << k << " / Hessian shape is [d,d] where d: " << d
                   << " / Maximum K ( {d(d+1)}/2 ): " << S << '\n';
-        std::cout << '\n';
-        std::cout << "  Line Search: " << (a.line_search ? "[YES]" : "[NO]") << '\n';
-        std::cout << "  Line Search parameters: " << "c: " << a.c_ls << ", gamma: " << a.gamma_ls << '\n';
+        if (!a.pp) {
+            std::cout << '\n';
+            std::cout << "  Line Search: " << (a.line_search ? "[YES]" : "[NO]") << '\n';
+            std::cout << "  Line Search parameters: " << "c: " << a.c_ls << ", gamma: " << a.gamma_ls << '\n';
+        }
         std::cout << "=========================================================================\n";
         std::cout << "Solution Properties\n\n";
         std::cout << "  Last iterate CRC-32: " << crc32_bytes(x.data(), sizeof(double) * d) << '\n';
         std::cout << "  Gradient in last iterate CRC-32: " << crc32_bytes(glast.data(), sizeof(double) * d) << '\n';
         std::cout << "  Norm of last iterate: " << sqrt(xx) << '\n';
-        std::cout << "  Norm of full gradient in last iterate: " << sqrt(gg) << '\n';
+        if (a.pp)
+            std::cout << "  True of norm of full gradient in last iterate: " << sqrt(gg) << '\n';
+        else
+            std::cout << "  Norm of full gradient in last iterate: " << sqrt(gg) << '\n';
         std::cout << "  Norm square of full gradient in last iterate: " << gg << '\n';
         std::cout << "  Suboptimality gap of Function, f(x) - f*: " << f_gap << '\n';
         std::cout << "  Suboptimality distance to Optimal Solution |x - x*|: " << x_gap << '\n';
@@ -673,33 +699,16 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb)
     }
     if (cb)
         cb(d, sqrt(gg), f_gap, x_gap);
+    if (cb_pp)
+        cb_pp(d);
     return kOk;
 }
 
-}  // namespace fednl_cli
-
-extern "C" {
-
-int fednl_cli_main(int argc, char** argv)
+// "train " + command, split on single spaces (src/mainShared.cpp:15-39 of
+// both shared libraries); argv keeps pointers into `parts`
+static std::vector<char*> split_command(const char* command, const char* prog, std::vector<std::string>& parts)
 {
-    fednl_cli::CmdLine cl(argc, argv);
-    if (cl.flag("version")) {
-        std::cout << "fednl-mi355x engine (FedNL round engine for gfx950)\n";
-        if (!cl.flag("train"))
-            return 0;
-    }
-    if (!cl.flag("train")) {
-        std::cout << "Please specify one of the action [train]\n";
-        return -1;
-    }
-    return fednl_cli::train(cl, nullptr);
-}
-
-int doptExecuteLocal(const char* command, ResultCallbackLocal resultCallback)
-{
-    // "train " + command, split on single spaces, heap copies (src/mainShared.cpp:15-39)
     std::string full = std::string("train ") + (command ? command : "");
-    std::vector<std::string> parts;
     std::string cur;
     for (char ch : full) {
         if (ch == ' ') {
@@ -711,13 +720,51 @@ int doptExecuteLocal(const char* command, ResultCallbackLocal resultCallback)
     }
     parts.push_back(cur);
     std::vector<char*> argv;
-    argv.push_back(const_cast<char*>("bin_fednl_local"));
+    argv.push_back(const_cast<char*>(prog));
     for (auto& p : parts) {
         if (p == "train") p = "--train";
         argv.push_back(p.data());
     }
+    return argv;
+}
+
+static int cli_main(int argc, char** argv, bool pp)
+{
+    CmdLine cl(argc, argv);
+    if (cl.flag("version")) {
+        std::cout << "fednl-mi355x engine (FedNL round engine for gfx950)\n";
+        if (!cl.flag("train"))
+            return 0;
+    }
+    if (!cl.flag("train")) {
+        std::cout << "Please specify one of the action [train]\n";
+        return -1;
+    }
+    return train(cl, nullptr, nullptr, pp);
+}
+
+}  // namespace fednl_cli
+
+extern "C" {
+
+int fednl_cli_main(int argc, char** argv) { return fednl_cli::cli_main(argc, argv, false); }
+
+int fednl_cli_pp_main(int argc, char** argv) { return fednl_cli::cli_main(argc, argv, true); }
+
+int doptExecuteLocal(const char* command, ResultCallbackLocal resultCallback)
+{
+    std::vector<std::string> parts;
+    std::vector<char*> argv = fednl_cli::split_command(command, "bin_fednl_local", parts);
     fednl_cli::CmdLine cl((int)argv.size(), argv.data());
-    return fednl_cli::train(cl, resultCallback);
+    return fednl_cli::train(cl, resultCallback, nullptr, false);
+}
+
+int doptExecuteLocalPP(const char* command, ResultCallbackLocalPP resultCallback)
+{
+    std::vector<std::string> parts;
+    std::vector<char*> argv = fednl_cli::split_command(command, "bin_fednl_local_pp", parts);
+    fednl_cli::CmdLine cl((int)argv.size(), argv.data());
+    return fednl_cli::train(cl, nullptr, resultCallback, true);
 }
 
 }  // extern "C"

@@ -89,11 +89,13 @@ constexpr int kKC = 16;            // samples staged per LDS chunk
 constexpr int kPadLds = 4;         // LDS row padding (doubles)
 constexpr int kMaxHeapK = 8192;    // LDS-resident heap limit
 constexpr int kMaxClients = 1024;  // master scatter client table
-// per-round trace record: ||grad f||, f, ||x||, factorization flag, line-search trials
-constexpr int kTraceStride = 5;
+// per-round trace record: ||grad f||, f, ||x||, factorization flag,
+// line-search trials, participating clients, Hessian items they sent
+constexpr int kTraceStride = 7;
 constexpr int kMaxLineSearchSteps = 64;  // trial launches enqueued per round
-// master state (mst): line-search scalars, then x_k (d) from kLsHead on
-enum MstSlot { LS_FXK, LS_T, LS_DD, LS_DONE, LS_TRIALS, kLsHead = 8 };
+// master state (mst): line-search scalars, FedNL-PP's running Lk average,
+// then x_k (d) from kLsHead on
+enum MstSlot { LS_FXK, LS_T, LS_DD, LS_DONE, LS_TRIALS, PP_LI, kLsHead = 8 };
 constexpr int kNB = 32;            // Cholesky panel width
 constexpr int kMaxNewtonD = 512;   // single-workgroup Newton solve limit (LDS panel)
 
@@ -201,7 +203,7 @@ __device__ inline uint64_t round_nearest(double v)
 // ---------------------------------------------------------------------------
 struct ProblemDev {
     int n, m, d, k, lda, m_pad, d_pad, tiles_up, dp, workers, comp, hmode, option_b, tracking, seg_len, bm_words,
-        trace_cap, update_x, ls, ls_max;
+        trace_cap, update_x, ls, ls_max, pp, pp_tau;
     size_t S;
     double lambda, alpha, prescale, inv_n, mult, lek_delta, ls_c, ls_gamma;
     const double* A;
@@ -211,6 +213,11 @@ struct ProblemDev {
     double *x, *Dw, *gw, *nlog, *learned, *delta, *grad_c, *part, *lk, *vals, *H, *g_avg, *shift, *W, *dx, *trace,
         *lek_p;
     double* mst;        // master state (MstSlot header, then x_k)
+    // FedNL-PP: selected flags, client sampler state, per-client direction
+    // (n x d_pad) and Lk, this round's messages, H_c(x) (option b), full gradient
+    int* pp_sel;
+    uint32_t* pp_rng;
+    double *pp_dir, *pp_lk, *pp_gmsg, *pp_lkmsg, *pp_h, *pp_tg;
     const double* H0;   // H after creation: the lower triangle of the master's full H
     const double* hup;  // Newton: packed H to fill W from (standalone solve), or null
     const double* nb;   // Newton: right-hand side
@@ -226,6 +233,9 @@ struct ProblemDev {
     const int blk = bmv_.y
 
 #define FEDNL_KARGS const ProblemDev* __restrict__ probs, const int2* __restrict__ bmap
+
+// FedNL-PP: client c sits this round out
+__device__ inline bool pp_idle(const ProblemDev& P, int c) { return P.pp && P.pp_sel[c] == 0; }
 
 // ---------------------------------------------------------------------------
 // k_margin: one wave per sample row. kTrial: a line-search trial point (only
@@ -309,7 +319,8 @@ template <bool kInit>
 __device__ __forceinline__ void hess_epilogue(const double (&hval)[16], const int (&iv)[16], const int (&jv)[16],
                                               int c, int d, int d_pad, double lambda, size_t S, int tiles_up, int t,
                                               const double* __restrict__ x, double* __restrict__ learned,
-                                              double* __restrict__ delta, double* __restrict__ part,
+                                              double* __restrict__ delta, double* __restrict__ hout,
+                                              double* __restrict__ part,
                                               double* __restrict__ grad_c, bool with_grad, double gacc, int i0,
                                               double (*red)[4], double (*gred)[kTile])
 {
@@ -340,6 +351,8 @@ __device__ __forceinline__ void hess_epilogue(const double (&hval)[16], const in
             } else {
                 const double dv = h - lv[q];
                 delta[o] = dv;
+                if (hout)
+                    hout[o] = h;
                 if (i == j)
                     sdiag = fma(dv, dv, sdiag);
                 else
@@ -418,6 +431,8 @@ __global__ __launch_bounds__(256) void k_hessian_f32(FEDNL_KARGS, double* __rest
     int c, t, i0, j0;
     if (!hess_tile_of_block(blk, n, tiles_up, c, t, i0, j0))
         return;
+    if (!kInit && pp_idle(PD, c))
+        return;
     __shared__ __align__(16) float sI[2][kKC][kTile + kPadLds];
     __shared__ __align__(16) float sJ[2][kKC][kTile + kPadLds];
     __shared__ double red[2][4];
@@ -491,8 +506,8 @@ __global__ __launch_bounds__(256) void k_hessian_f32(FEDNL_KARGS, double* __rest
                 iv[q] = i0 + wi * 32 + it * 16 + l16;
                 jv[q] = j0 + wj * 32 + jt * 16 + 4 * kq + r;
             }
-    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, part, nullptr,
-                         false, 0.0, i0, red, gred);
+    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, PD.pp_h, part,
+                         nullptr, false, 0.0, i0, red, gred);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,6 +542,8 @@ __global__ __launch_bounds__(256) void k_hessian_bf16(FEDNL_KARGS, double* __res
     double* __restrict__ part = PD.part;
     int c, t, i0, j0;
     if (!hess_tile_of_block(blk, n, tiles_up, c, t, i0, j0))
+        return;
+    if (!kInit && pp_idle(PD, c))
         return;
     __shared__ __align__(16) __bf16 tIh[2][kTile][kKBPad];
     __shared__ __align__(16) __bf16 tIl[2][kTile][kKBPad];
@@ -618,8 +635,8 @@ __global__ __launch_bounds__(256) void k_hessian_bf16(FEDNL_KARGS, double* __res
                 iv[q] = i0 + wi * 32 + it * 16 + l16;
                 jv[q] = j0 + wj * 32 + jt * 16 + 4 * kq + r;
             }
-    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, part, nullptr,
-                         false, 0.0, i0, red, gred);
+    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, PD.pp_h, part,
+                         nullptr, false, 0.0, i0, red, gred);
 }
 
 // ---------------------------------------------------------------------------
@@ -709,6 +726,8 @@ __global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __rest
     double* __restrict__ part = PD.part;
     int c, t, i0, j0;
     if (!hess_tile_of_block(blk, n, tiles_up, c, t, i0, j0))
+        return;
+    if (!kInit && pp_idle(PD, c))
         return;
     const bool dg = (i0 == j0);
 
@@ -819,8 +838,9 @@ __global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __rest
                 iv[q] = i0 + wi * 32 + it * 16 + l16;
                 jv[q] = j0 + wj * 32 + jt * 16 + kq + 4 * r;
             }
-    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, part, grad_c,
-                         !kInit && dg, gacc, i0, red, gred);
+    // FedNL-PP takes every client's gradient from k_grad_f64
+    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, PD.pp_h, part,
+                         grad_c, !kInit && dg && !PD.pp, gacc, i0, red, gred);
 }
 
 // ---------------------------------------------------------------------------
@@ -1264,6 +1284,11 @@ __global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = blk;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (pp_idle(PD, c)) {
+        if (kLek && tid == 0)
+            lek_hit[c] = 0;  // no draw from the worker's stream
+        return;
+    }
     const double* dc = delta + (size_t)c * S;
     __shared__ double s_fro2;
     PROBE_T(t_sel0);
@@ -1518,6 +1543,8 @@ __global__ __launch_bounds__(256) void k_select_randk(FEDNL_KARGS)
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = blk;
     const int tid = threadIdx.x;
+    if (pp_idle(PD, c))
+        return;
     const int P = pow2_at_least(k);
     uint64_t* keys = (uint64_t*)smem;            // P
     int* qprev = (int*)(keys + P);                // k
@@ -1611,6 +1638,8 @@ __global__ __launch_bounds__(256) void k_select_window(FEDNL_KARGS)
     int* __restrict__ sel_cnt = PD.sel_cnt;
     uint32_t* __restrict__ seq_start = PD.seq_start;
     const int c = blk;
+    if (pp_idle(PD, c))
+        return;
     __shared__ uint64_t s_start;
     if (threadIdx.x == 0) {
         uint64_t st = 0;
@@ -1667,6 +1696,8 @@ __global__ __launch_bounds__(256) void k_emit(FEDNL_KARGS)
     uint32_t* __restrict__ sel_bm = PD.sel_bm;
     extern __shared__ __align__(16) uint32_t bml[];  // bm_words
     const int c = blk;
+    if (pp_idle(PD, c))
+        return;
     for (int w = threadIdx.x; w < bm_words; w += blockDim.x)
         bml[w] = 0u;
     __shared__ int s_cnt, s_ins;
@@ -1760,7 +1791,7 @@ __global__ __launch_bounds__(256) void k_master_h(FEDNL_KARGS)
     const int nw = min(kMasterPos / 32, bm_words - w0);
     for (int e = threadIdx.x; e < n * (kMasterPos / 32); e += blockDim.x) {
         const int cc = e / (kMasterPos / 32), q = e % (kMasterPos / 32);
-        bw[cc][q] = (q < nw) ? sel_bm[(size_t)cc * bm_words + w0 + q] : 0u;
+        bw[cc][q] = (q < nw && !pp_idle(PD, cc)) ? sel_bm[(size_t)cc * bm_words + w0 + q] : 0u;
     }
     __syncthreads();
     const size_t p = p0 + threadIdx.x;
@@ -1825,8 +1856,35 @@ __global__ __launch_bounds__(256) void k_master_grad(FEDNL_KARGS)
             for (int cc = 0; cc < cn; ++cc) s = s + tile[cc][col];
         __syncthreads();
     }
-    if (threadIdx.x < kGradCols && i0 + col < d)
-        g_avg[i0 + col] = s * inv_n;
+    if (!PD.pp) {
+        if (threadIdx.x < kGradCols && i0 + col < d)
+            g_avg[i0 + col] = s * inv_n;
+        return;
+    }
+    // FedNL-PP (bin_fednl_local_pp/include/TrainLoop.h:1166,1400-1426): s is
+    // the full gradient sum (reported, divided by n); the direction estimate
+    // is (sum of the selected clients' changes) / n + the previous estimate
+    const double* __restrict__ gmsg = PD.pp_gmsg;
+    double sg = 0.0;
+    for (int c0 = 0; c0 < n; c0 += kGradClients) {
+        const int cn = min(kGradClients, n - c0);
+        for (int e = threadIdx.x; e < cn * kGradCols; e += blockDim.x) {
+            const int cc = e / kGradCols, ii = e % kGradCols;
+            const int i = i0 + ii;
+            tile[cc][ii] = (i < d) ? gmsg[(size_t)(c0 + cc) * d_pad + i] : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x < kGradCols)
+            for (int cc = 0; cc < cn; ++cc)
+                if (!pp_idle(PD, c0 + cc))
+                    sg = sg + tile[cc][col];
+        __syncthreads();
+    }
+    if (threadIdx.x < kGradCols && i0 + col < d) {
+        const int i = i0 + col;
+        PD.pp_tg[i] = s / (double)n;
+        g_avg[i] = sg * inv_n + g_avg[i];
+    }
 }
 
 // H = (sum_c learned_c) / n, clients in order (TrainLoop.h:639-643, 732-735)
@@ -1891,8 +1949,9 @@ __global__ __launch_bounds__(256) void k_master_vec(FEDNL_KARGS)
     __shared__ double scratch[16];
     __shared__ double fc_s[kMaxClients];
     double gg = 0.0, xx = 0.0;
+    const double* __restrict__ g_rep = PD.pp ? PD.pp_tg : g_avg;  // FedNL-PP reports the full gradient
     for (int i = threadIdx.x; i < d; i += blockDim.x) {
-        const double g = g_avg[i];
+        const double g = g_rep[i];
         gg = fma(g, g, gg);
         xx = fma(x[i], x[i], xx);
     }
@@ -1910,17 +1969,38 @@ __global__ __launch_bounds__(256) void k_master_vec(FEDNL_KARGS)
         for (int c = 0; c < n; ++c) l = l + lk[c];
         l = l * inv_n;
         if (with_f) {
-            for (int c = 0; c < n; ++c) fsum = fsum + fc_s[c];
+            // FedNL-PP: the selected clients' f_i over n (TrainLoop.h:1648-1676)
+            for (int c = 0; c < n; ++c)
+                if (!pp_idle(PD, c))
+                    fsum = fsum + fc_s[c];
             fsum = fsum * inv_n;
         }
         if (PD.ls)
             PD.mst[LS_FXK] = fsum;
+        if (PD.pp && option_b) {
+            // running Lk average (TrainLoop.h:1436-1445)
+            double sl = 0.0;
+            for (int c = 0; c < n; ++c)
+                if (!pp_idle(PD, c))
+                    sl = sl + PD.pp_lkmsg[c];
+            l = sl * inv_n + PD.mst[PP_LI];
+            PD.mst[PP_LI] = l;
+        }
         *shift_out = option_b ? l : lambda;  // option a: mu(f) = lambda (logistic_regression.h:702-705)
         const int r = *round_ctr;
         if (r < trace_cap) {
+            int parts = 0;
+            double items = 0.0;
+            for (int c = 0; c < n; ++c)
+                if (!pp_idle(PD, c)) {
+                    parts += 1;
+                    items += (comp == FEDNL_TOPLEK) ? (double)PD.cnt[c] : (double)PD.k;
+                }
             trace[(size_t)r * kTraceStride + 0] = sqrt(gg);
             trace[(size_t)r * kTraceStride + 1] = fsum;
             trace[(size_t)r * kTraceStride + 2] = sqrt(xx);
+            trace[(size_t)r * kTraceStride + 5] = (double)parts;
+            trace[(size_t)r * kTraceStride + 6] = items;
         }
     }
     if (comp == FEDNL_TOPLEK) {
@@ -2020,6 +2100,120 @@ __global__ __launch_bounds__(256) void k_ls_check(FEDNL_KARGS)
     if (move) {
         const double t = t_next;
         for (int i = threadIdx.x; i < d; i += blockDim.x) x[i] = xk[i] - t * dx[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FedNL-PP (dopt/bin_fednl_local_pp). k_pp_sample draws the round's clients:
+// the first tau entries of a partial shuffle of 0..n-1 (Shuffle.h:31-93)
+// with the master's RandomGenIntegerLinear (TrainLoop.h:279-287,1155-1157).
+// One workgroup per problem; the draws are sequential in thread 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_pp_sample(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    (void)blk;
+    if (!PD.pp)
+        return;
+    const int n = PD.n, tau = PD.pp_tau;
+    __shared__ uint32_t ord[kMaxClients];
+    __shared__ int flag[kMaxClients];
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        ord[c] = (uint32_t)c;
+        flag[c] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t st = *PD.pp_rng;
+        for (int i = 0; i < tau; ++i) {
+            st = kLcgA * st + kLcgC;
+            const double u = (double)st * (1.0 / 4294967295.0);
+            const uint64_t off = round_nearest(u * (double)(n - 1 - i));
+            if (off) {
+                const uint32_t t = ord[i];
+                ord[i] = ord[i + off];
+                ord[i + off] = t;
+            }
+        }
+        *PD.pp_rng = st;
+        for (int q = 0; q < tau; ++q) flag[ord[q]] = 1;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < n; c += blockDim.x) PD.pp_sel[c] = flag[c];
+}
+
+// FedNL-PP client step after the learned update (LocalWorkerForFedNL_*.h,
+// bin_fednl_local_pp): option b sends Lk_new - Lk_old with
+// Lk_new = ||learned_new - H_c(x)||_F over the upper part; every selected
+// client sends the change of its direction l_c x + learned_new x - grad f_c(x)
+// and keeps the new one. kInit: the starting direction (l_c = 0) of every
+// client (LocalWorkerForFedNL_TopK.h:20-30). One workgroup per client.
+template <bool kInit>
+__global__ __launch_bounds__(256) void k_pp_client(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    const int c = blk;
+    if (!PD.pp || (!kInit && pp_idle(PD, c)))
+        return;
+    const int d = PD.d, d_pad = PD.d_pad;
+    const size_t S = PD.S;
+    const double* __restrict__ L = PD.learned + (size_t)c * S;
+    __shared__ double xs[kMaxNewtonD];
+    __shared__ double scratch[16];
+    __shared__ double s_lk;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) xs[i] = PD.x[i];
+    double lk_c = 0.0;
+    if (!kInit) {
+        lk_c = PD.pp_lk[c];
+        if (PD.option_b) {
+            const double* __restrict__ Hc = PD.pp_h + (size_t)c * S;
+            double so = 0.0, sd = 0.0;
+            for (size_t p = threadIdx.x; p < S; p += blockDim.x) {
+                const double r = L[p] - Hc[p];
+                if (pos_is_diag(p))
+                    sd = fma(r, r, sd);
+                else
+                    so = fma(r, r, so);
+            }
+            so = block_sum(so, scratch);
+            sd = block_sum(sd, scratch);
+            if (threadIdx.x == 0) {
+                const double lk_new = sqrt((so + so) + sd);
+                PD.pp_lkmsg[c] = lk_new - PD.pp_lk[c];
+                PD.pp_lk[c] = lk_new;
+                s_lk = lk_new;
+            }
+            __syncthreads();
+            lk_c = s_lk;
+        }
+    }
+    __syncthreads();
+    const double* __restrict__ gc = PD.grad_c + (size_t)c * d_pad;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) {
+        // row i of the full symmetric learned Hessian times x, columns ascending
+        double mv = 0.0;
+        const size_t ri = (size_t)i * (i + 1) / 2;
+        for (int j = 0; j < i; ++j) mv = mv + L[ri + j] * xs[j];
+        for (int j = i; j < d; ++j) mv = mv + L[(size_t)j * (j + 1) / 2 + i] * xs[j];
+        const double dir = (xs[i] * lk_c + mv) - gc[i];
+        const size_t o = (size_t)c * d_pad + i;
+        if (!kInit)
+            PD.pp_gmsg[o] = dir - PD.pp_dir[o];
+        PD.pp_dir[o] = dir;
+    }
+}
+
+// FedNL-PP starting direction estimate: the clients' directions averaged in
+// client order (TrainLoop.h:1066-1080)
+__global__ __launch_bounds__(256) void k_pp_init_avg(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    (void)blk;
+    const int n = PD.n, d = PD.d, d_pad = PD.d_pad;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) {
+        double s = 0.0;
+        for (int c = 0; c < n; ++c) s = s + PD.pp_dir[(size_t)c * d_pad + i];
+        PD.g_avg[i] = s * PD.inv_n;
     }
 }
 
@@ -2147,8 +2341,10 @@ __device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double*
     double* __restrict__ dx_out = PD.dx;
     for (int i = tid; i < d; i += nthr) {
         dx_out[i] = bv[i];
-        if (PD.update_x)
+        if (PD.update_x == 1)
             x[i] = x[i] - bv[i];
+        else if (PD.update_x == 2)
+            x[i] = bv[i];
     }
 }
 
@@ -2581,8 +2777,10 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
     PROBE_ADD(5, t_b0);
     for (int i = tid; i < d; i += nthr) {
         dx_out[i] = yv[i];
-        if (update_x)
+        if (update_x == 1)
             x[i] = x[i] - yv[i];
+        else if (update_x == 2)
+            x[i] = yv[i];  // FedNL-PP: the solution is the next iterate
     }
     if (tid == 0) {
         if (rnd < trace_cap)
@@ -2658,6 +2856,11 @@ struct Engine {
     double *dx = nullptr, *trace = nullptr;
     double* mst = nullptr;  // master state: MstSlot header + x_k
     double* H0 = nullptr;   // H at creation (S, packed upper)
+    // FedNL-PP state (clients_per_round > 0)
+    int* pp_sel = nullptr;
+    uint32_t* pp_rng = nullptr;
+    double *pp_dir = nullptr, *pp_lk = nullptr, *pp_gmsg = nullptr, *pp_lkmsg = nullptr, *pp_h = nullptr;
+    double* pp_tg = nullptr;
     int trace_cap = 0;
     int rounds_enqueued = 0;
 
@@ -2754,9 +2957,14 @@ static ProblemDev make_desc(const Engine* e)
     P.n = e->n; P.m = e->m; P.d = e->d; P.k = e->k; P.lda = e->lda; P.m_pad = e->m_pad; P.d_pad = e->d_pad;
     P.tiles_up = e->tiles_up; P.dp = e->dp; P.workers = e->workers; P.comp = e->o.compressor; P.hmode = e->hmode;
     P.option_b = e->o.option_b; P.tracking = e->o.tracking; P.seg_len = e->seg_len; P.bm_words = e->bm_words;
-    P.trace_cap = e->trace_cap; P.update_x = e->o.line_search ? 0 : 1; P.S = e->S;
+    P.trace_cap = e->trace_cap; P.S = e->S;
+    P.pp = e->o.clients_per_round > 0 ? 1 : 0; P.pp_tau = e->o.clients_per_round;
+    // x -= dx (FedNL), x = solution (FedNL-PP), x left to the line search (FedNL-LS)
+    P.update_x = P.pp ? 2 : (e->o.line_search ? 0 : 1);
     P.ls = e->o.line_search ? 1 : 0; P.ls_max = e->o.max_line_search_steps;
     P.ls_c = e->o.c_line_search; P.ls_gamma = e->o.gamma_line_search; P.mst = e->mst; P.H0 = e->H0;
+    P.pp_sel = e->pp_sel; P.pp_rng = e->pp_rng; P.pp_dir = e->pp_dir; P.pp_lk = e->pp_lk; P.pp_gmsg = e->pp_gmsg;
+    P.pp_lkmsg = e->pp_lkmsg; P.pp_h = e->pp_h; P.pp_tg = e->pp_tg;
     P.lambda = e->o.lambda; P.alpha = e->alpha; P.prescale = e->prescale; P.inv_n = e->inv_n; P.mult = e->mult;
     P.lek_delta = e->lek_delta;
     P.A = e->A; P.A32 = e->A32; P.AThi = e->AThi; P.ATlo = e->ATlo;
@@ -2832,6 +3040,9 @@ static int launch_round(Engine* e)
     const ProblemDev* pd = e->pdev;
     const int2* nomap = nullptr;
     const int n = e->n;
+    const bool pp = e->pp_sel != nullptr;
+    if (pp)
+        hipLaunchKernelGGL(k_pp_sample, dim3(1), dim3(64), 0, e->st, pd, nomap);
     {
         KTime kt(e, KID_MARGIN);
         hipLaunchKernelGGL(k_margin<false>, dim3(blocks_margin(P)), dim3(256), sizeof(double) * e->d_pad, e->st, pd,
@@ -2839,6 +3050,10 @@ static int launch_round(Engine* e)
     }
     {
         KTime kt(e, KID_HESS);
+        // FedNL-PP reports the full gradient: every client's, from k_grad_f64
+        // (the reduced-precision modes launch it themselves)
+        if (pp && e->hmode == 0)
+            hipLaunchKernelGGL(k_grad_f64, dim3(blocks_grad(P)), dim3(256), 0, e->st, pd, nomap);
         launch_hessian(e, false, nullptr);
     }
     const int comp = e->o.compressor;
@@ -2856,6 +3071,8 @@ static int launch_round(Engine* e)
     {
         KTime kt(e, KID_EMIT);
         hipLaunchKernelGGL(k_emit, dim3(n), dim3(256), sizeof(uint32_t) * e->bm_words, e->st, pd, nomap);
+        if (pp)
+            hipLaunchKernelGGL(k_pp_client<false>, dim3(n), dim3(256), 0, e->st, pd, nomap);
     }
     {
         KTime kt(e, KID_MASTER_VEC);
@@ -2912,7 +3129,8 @@ static void free_engine(Engine* e)
     void* ptrs[] = {e->A, e->x, e->Dw, e->gw, e->nlog, e->learned, e->delta, e->grad_c, e->part, e->lk, e->sel_bm,
                     e->sel, e->pos, e->idx, e->rng, e->lek_state, e->lek_popped, e->seq_start, e->sel_cnt,
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
-                    e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev, e->mst, e->H0};
+                    e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev, e->mst, e->H0, e->pp_sel, e->pp_rng,
+                    e->pp_dir, e->pp_lk, e->pp_gmsg, e->pp_lkmsg, e->pp_h, e->pp_tg};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -2978,6 +3196,15 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
          opts->max_line_search_steps < 0 || opts->max_line_search_steps > kMaxLineSearchSteps)) {
         set_err("fednl_create: line search needs c > 0, 0 < gamma < 1 and 0 <= max steps <= 64");
         return FEDNL_E_WRONG_ARGUMENT;
+    }
+    if (opts->clients_per_round < 0 || opts->clients_per_round > n) {
+        set_err("fednl_create: clients_per_round must be in [1, n] (FedNL-PP) or 0 (FedNL)");
+        return FEDNL_E_WRONG_ARGUMENT;
+    }
+    if (opts->clients_per_round > 0 && opts->line_search) {
+        // bin_fednl_local_pp has no line search (its Args.h drops the flags)
+        set_err("fednl_create: FedNL-PP runs without line search");
+        return FEDNL_E_UNSUPPORTED;
     }
     Engine* e = new Engine();
     e->hmode = opts->hessian_mode;
@@ -3083,6 +3310,17 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     ALLOC(e->dx, e->d_pad);
     ALLOC(e->mst, kLsHead + (size_t)e->d_pad);
     ALLOC(e->H0, e->S);
+    if (opts->clients_per_round > 0) {
+        ALLOC(e->pp_sel, n);
+        ALLOC(e->pp_rng, 1);
+        ALLOC(e->pp_dir, (size_t)n * e->d_pad);
+        ALLOC(e->pp_lk, n);
+        ALLOC(e->pp_gmsg, (size_t)n * e->d_pad);
+        ALLOC(e->pp_lkmsg, n);
+        ALLOC(e->pp_tg, e->d_pad);
+        if (opts->option_b)
+            ALLOC(e->pp_h, (size_t)n * e->S);
+    }
 #undef ALLOC
     if ((rc = ensure_trace(e, 1024)) != FEDNL_OK) {
         free_engine(e);
@@ -3162,6 +3400,23 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         hipLaunchKernelGGL(k_init_H, dim3((unsigned)((e->S + 255) / 256)), dim3(256), 0, e->st, e->H, e->learned,
                            e->S, n, e->inv_n);
         hipMemcpyAsync(e->H0, e->H, sizeof(double) * e->S, hipMemcpyDeviceToDevice, e->st);
+        if (e->pp_sel) {
+            // FedNL-PP: every client's gradient and direction at x0, their
+            // average, l_c = 0, the client sampler's seed (Args.h:242-244)
+            const ProblemDev* pd = e->pdev;
+            hipMemsetAsync(e->pp_sel, 0, sizeof(int) * n, e->st);
+            hipMemsetAsync(e->pp_lk, 0, sizeof(double) * n, e->st);
+            hipMemsetAsync(e->pp_lkmsg, 0, sizeof(double) * n, e->st);
+            hipMemsetAsync(e->pp_gmsg, 0, sizeof(double) * (size_t)n * e->d_pad, e->st);
+            hipMemsetAsync(e->pp_dir, 0, sizeof(double) * (size_t)n * e->d_pad, e->st);
+            hipMemsetAsync(e->pp_tg, 0, sizeof(double) * e->d_pad, e->st);
+            const uint32_t seed = opts->client_selection_seed;
+            hipMemcpyAsync(e->pp_rng, &seed, sizeof(uint32_t), hipMemcpyHostToDevice, e->st);
+            hipLaunchKernelGGL(k_grad_f64, dim3(blocks_grad(e->hdesc)), dim3(256), 0, e->st, pd, (const int2*)nullptr);
+            hipLaunchKernelGGL(k_pp_client<true>, dim3(n), dim3(256), 0, e->st, pd, (const int2*)nullptr);
+            hipLaunchKernelGGL(k_pp_init_avg, dim3(1), dim3(256), 0, e->st, pd, (const int2*)nullptr);
+            hipStreamSynchronize(e->st);  // the seed's source is on this stack frame
+        }
     }
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
@@ -3432,7 +3687,9 @@ static size_t engine_footprint(const fednl_opts* o, int n, int m, int d)
     b += 8 * 2 * (size_t)n * S;                                 // learned, delta
     b += 8 * 3 * (size_t)n * m_pad + 8 * (size_t)n * d_pad;     // Dw, gw, nlog, grad_c
     b += (size_t)n * k * (4 + 4 + 4 + 8) + 4 * (size_t)n * ((S + 31) / 32);
-    b += 8 * (S + dp * dp) + (1 << 20);
+    b += 8 * (2 * S + dp * dp) + (1 << 20);                     // H, H0, W
+    if (o->clients_per_round > 0)                               // FedNL-PP state
+        b += 8 * (2 * (size_t)n * d_pad + 2 * (size_t)n + d_pad) + (o->option_b ? 8 * (size_t)n * S : 0);
     return b;
 }
 
@@ -3461,6 +3718,8 @@ void fednl_default_opts(fednl_opts* o)
     o->max_line_search_steps = 5;  // Args.h:300-306
     o->c_line_search = 0.5;
     o->gamma_line_search = 0.5;
+    o->clients_per_round = 0;            // FedNL; > 0: FedNL-PP
+    o->client_selection_seed = 1234u;    // bin_fednl_local_pp/include/Args.h:244
 }
 
 const char* fednl_last_error(void) { return g_err.c_str(); }
@@ -3527,6 +3786,11 @@ int fednl_batch_create(void* const* engines, int count, int use_graph, void** ba
         set_err("fednl_batch_create: empty batch");
         return FEDNL_E_MISSING_ARGUMENT;
     }
+    for (int i = 0; i < count; ++i)
+        if (engines[i] && ((Engine*)engines[i])->pp_sel) {
+            set_err("fednl_batch_create: FedNL-PP engines run on their own");
+            return FEDNL_E_UNSUPPORTED;
+        }
     Batch* b = new Batch();
     b->use_graph = use_graph != 0;
     for (int i = 0; i < count; ++i) {
@@ -3608,6 +3872,11 @@ int fednl_serve(const fednl_request* reqs, int count, double hbm_budget_bytes, i
         set_err("fednl_serve: empty request list");
         return FEDNL_E_MISSING_ARGUMENT;
     }
+    for (int i = 0; i < count; ++i)
+        if (reqs[i].opts.clients_per_round > 0) {
+            set_err("fednl_serve: FedNL-PP requests are not batched");
+            return FEDNL_E_UNSUPPORTED;
+        }
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
@@ -3763,6 +4032,39 @@ int fednl_trace(void* handle, double* grad_norm, double* fval, double* x_norm, d
     return FEDNL_OK;
 }
 
+int fednl_trace_clients(void* handle, int* participants, double* items, int* count)
+{
+    Engine* e = (Engine*)handle;
+    if (!e || !count)
+        return FEDNL_E_MISSING_ARGUMENT;
+    HIP_OK(hipStreamSynchronize(e->st));
+    const int R = e->rounds_enqueued;
+    *count = R;
+    if (R == 0)
+        return FEDNL_OK;
+    std::vector<double> tr((size_t)kTraceStride * R);
+    HIP_OK(hipMemcpy(tr.data(), e->trace, sizeof(double) * tr.size(), hipMemcpyDeviceToHost));
+    for (int r = 0; r < R; ++r) {
+        if (participants) participants[r] = (int)tr[(size_t)kTraceStride * r + 5];
+        if (items) items[r] = tr[(size_t)kTraceStride * r + 6];
+    }
+    return FEDNL_OK;
+}
+
+int fednl_pp_selection(void* handle, int* selected)
+{
+    Engine* e = (Engine*)handle;
+    if (!e || !selected)
+        return FEDNL_E_MISSING_ARGUMENT;
+    if (!e->pp_sel) {
+        set_err("fednl_pp_selection: not a FedNL-PP engine");
+        return FEDNL_E_WRONG_ARGUMENT;
+    }
+    HIP_OK(hipStreamSynchronize(e->st));
+    HIP_OK(hipMemcpy(selected, e->pp_sel, sizeof(int) * e->n, hipMemcpyDeviceToHost));
+    return FEDNL_OK;
+}
+
 int fednl_trace_line_search(void* handle, int* trials, int* count)
 {
     Engine* e = (Engine*)handle;
@@ -3796,7 +4098,8 @@ int fednl_last_round(void* handle, double* grad, double* lk, int* cnt, uint32_t*
     if (!e)
         return FEDNL_E_MISSING_ARGUMENT;
     HIP_OK(hipStreamSynchronize(e->st));
-    if (grad) HIP_OK(hipMemcpy(grad, e->g_avg, sizeof(double) * e->d, hipMemcpyDeviceToHost));
+    // FedNL-PP: the full gradient at x_r (the direction estimate stays internal)
+    if (grad) HIP_OK(hipMemcpy(grad, e->pp_tg ? e->pp_tg : e->g_avg, sizeof(double) * e->d, hipMemcpyDeviceToHost));
     if (lk) HIP_OK(hipMemcpy(lk, e->lk, sizeof(double) * e->n, hipMemcpyDeviceToHost));
     if (cnt) HIP_OK(hipMemcpy(cnt, e->cnt, sizeof(int) * e->n, hipMemcpyDeviceToHost));
     if (pos) HIP_OK(hipMemcpy(pos, e->pos, sizeof(uint32_t) * (size_t)e->n * e->k, hipMemcpyDeviceToHost));

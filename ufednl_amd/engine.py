@@ -33,7 +33,9 @@ class FednlOpts(ctypes.Structure):
         ("max_line_search_steps", ctypes.c_int),
         ("c_line_search", ctypes.c_double),
         ("gamma_line_search", ctypes.c_double),
-        ("reserved", ctypes.c_int * 4),
+        ("clients_per_round", ctypes.c_int),
+        ("client_selection_seed", ctypes.c_uint),
+        ("reserved", ctypes.c_int * 2),
     ]
 
 
@@ -89,6 +91,8 @@ def lib():
         L.fednl_trace.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_dp, _c_ip, _c_ip]
         L.fednl_get_x.argtypes = [ctypes.c_void_p, _c_dp]
         L.fednl_trace_line_search.argtypes = [ctypes.c_void_p, _c_ip, _c_ip]
+        L.fednl_trace_clients.argtypes = [ctypes.c_void_p, _c_ip, _c_dp, _c_ip]
+        L.fednl_pp_selection.argtypes = [ctypes.c_void_p, _c_ip]
         L.fednl_last_round.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_ip, _c_up, _c_dp, _c_up, _c_dp]
         L.fednl_get_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
         L.fednl_set_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
@@ -142,12 +146,13 @@ class Engine:
     """One FedNL problem resident on the GPU."""
 
     def __init__(self, features, labels, lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0,
-                 option_b=True, workers=1, tracking=True, hessian="fp64", line_search=None):
+                 option_b=True, workers=1, tracking=True, hessian="fp64", line_search=None, pp=None):
         L = lib()
         feats = np.ascontiguousarray(features, dtype=np.float64)
         labs = np.ascontiguousarray(labels, dtype=np.float64)
         n, m, d = feats.shape
-        o = make_opts(lam, compressor, k_mult, alpha_mode, alpha, option_b, workers, tracking, hessian, line_search)
+        o = make_opts(lam, compressor, k_mult, alpha_mode, alpha, option_b, workers, tracking, hessian, line_search,
+                      pp)
         h = ctypes.c_void_p()
         _err(L.fednl_create(ctypes.byref(o), n, m, d, dp(feats), dp(labs), ctypes.byref(h)), "fednl_create")
         self.h = h
@@ -201,7 +206,15 @@ class Engine:
         _err(lib().fednl_trace(self.h, dp(g), dp(f), dp(xn), dp(sec), ip(ok), ctypes.byref(cnt)), "fednl_trace")
         ls = np.zeros(R, dtype=np.int32)
         _err(lib().fednl_trace_line_search(self.h, ip(ls), ctypes.byref(cnt)), "fednl_trace_line_search")
-        return dict(grad_norm=g, f=f, x_norm=xn, seconds=sec, chol_ok=ok, ls_trials=ls)
+        parts = np.zeros(R, dtype=np.int32); items = np.zeros(R)
+        _err(lib().fednl_trace_clients(self.h, ip(parts), dp(items), ctypes.byref(cnt)), "fednl_trace_clients")
+        return dict(grad_norm=g, f=f, x_norm=xn, seconds=sec, chol_ok=ok, ls_trials=ls, participants=parts,
+                    items=items)
+
+    def pp_selection(self):
+        sel = np.zeros(self.n, dtype=np.int32)
+        _err(lib().fednl_pp_selection(self.h, ip(sel)), "fednl_pp_selection")
+        return sel
 
     def x(self):
         out = np.zeros(self.d)
@@ -298,8 +311,9 @@ class Batch:
 
 
 def make_opts(lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0, option_b=True, workers=1,
-              tracking=False, hessian="fp64", line_search=None):
-    """fednl_opts; line_search: None (FedNL) or (c, gamma, max_steps) for FedNL-LS."""
+              tracking=False, hessian="fp64", line_search=None, pp=None):
+    """fednl_opts; line_search: None (FedNL) or (c, gamma, max_steps) for
+    FedNL-LS; pp: None or (clients per round, sampler seed) for FedNL-PP."""
     o = FednlOpts()
     lib().fednl_default_opts(ctypes.byref(o))
     o.lambda_ = lam
@@ -317,6 +331,10 @@ def make_opts(lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0, 
         o.c_line_search = c
         o.gamma_line_search = gamma
         o.max_line_search_steps = int(steps)
+    if pp is not None:
+        tau, seed = pp
+        o.clients_per_round = int(tau)
+        o.client_selection_seed = int(seed)
     return o
 
 
