@@ -202,3 +202,73 @@ def test_cli_dataset_errors(tmp_path):
     p = subprocess.run([BIN, *_args(str(bad), str(tmp_path / "o.bin"), 3)], capture_output=True, text=True,
                        timeout=60)
     assert p.returncode == (-4 & 0xFF)
+
+
+BIN_PP = os.path.join(ROOT, "ufednl_amd", "bin_fednl_local_pp")
+
+
+def _pp_args(data, out, rounds, tau):
+    return ["--train", "--train_dataset", data, "--clients", str(N_CLIENTS), "--add_intercept", "--lambda", "0.001",
+            "--algorithm", "fednl1", "--compressor", "topk", "--k_compressor_as_d_mult", "2",
+            "--theoretical_alpha", "--use_theoretical_alpha_option_2", "--fednl-option-b", "--rounds", str(rounds),
+            "--tracking", "--out", out, "--clients-per-round", str(tau)]
+
+
+def test_cli_pp_tracking_file_matches_oracle(oracle, tmp_path):
+    """bin_fednl_local_pp: tracking header "FedNL-PP (b)", the full-gradient
+    trajectory of the oracle's FedNL-PP, and tau clients' messages in the
+    byte counters (TrainLoop.h:830-836,1688-1715 of bin_fednl_local_pp)."""
+    from ufednl_amd.datagen import Problem
+    X, y = _dataset()
+    data = str(tmp_path / "train.libsvm")
+    _write_libsvm(data, X, y)
+    out = str(tmp_path / "pp.bin")
+    rounds, tau = 20, 3
+    p = subprocess.run([BIN_PP, *_pp_args(data, out, rounds, tau)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    strs, ints, dbl, rec = _read_tracking(out)
+    assert strs[1] == "FedNL-PP (b)"
+    d = D_FEAT + 1
+    k = 2 * d
+    m = N_SAMPLES // N_CLIENTS
+    Xi = np.hstack([X, np.ones((N_SAMPLES, 1))])
+    prob = Problem("cli", np.ascontiguousarray(Xi[: m * N_CLIENTS].reshape(N_CLIENTS, m, d)),
+                   np.ascontiguousarray(y[: m * N_CLIENTS].reshape(N_CLIENTS, m)), 1e-3, np.zeros(d))
+    f = oracle.FedNL(prob, "topk", k_mult=2.0, pp=(tau, 1234))
+    outs = [f.pp_round(tracking=True) for _ in range(rounds)]
+    assert_grad_norms_close(np.array([r[1] for r in rec]), np.array([o["grad_norm"] for o in outs]))
+    np.testing.assert_allclose([r[2] for r in rec], [o["f"] for o in outs], rtol=1e-12)
+    # per selected client: gradient 8d, k values, Lk 8 and the f_i message
+    # counted with the Lk size 8 (option b); indices 4k (TopK)
+    scal = np.diff([r[4] for r in rec])
+    idx = np.diff([r[5] for r in rec])
+    np.testing.assert_array_equal(scal, tau * (8.0 * d + 8.0 * k + 16.0))
+    np.testing.assert_array_equal(idx, tau * 4.0 * k)
+    assert "Number of clients per round selected u.a.r: 3" in p.stdout
+    lines = [line for line in p.stdout.splitlines() if line.startswith("  Round #")]
+    assert len(lines) == rounds and lines[0].endswith("milliseconds")
+
+
+def test_cli_pp_clients_per_round_errors(tmp_path):
+    X, y = _dataset()
+    data = str(tmp_path / "train.libsvm")
+    _write_libsvm(data, X, y)
+    for tau in (0, N_CLIENTS + 1):
+        p = subprocess.run([BIN_PP, *_pp_args(data, str(tmp_path / "o.bin"), 3, tau)], capture_output=True,
+                           text=True, timeout=60)
+        assert p.returncode == (-2 & 0xFF)
+
+
+def test_dopt_execute_local_pp_callback(tmp_path):
+    X, y = _dataset()
+    data = str(tmp_path / "train.libsvm")
+    _write_libsvm(data, X, y)
+    lib = ctypes.CDLL(os.path.join(ROOT, "ufednl_amd", "libfednl_hip.so"))
+    CB = ctypes.CFUNCTYPE(None, ctypes.c_int)
+    lib.doptExecuteLocalPP.argtypes = [ctypes.c_char_p, CB]
+    lib.doptExecuteLocalPP.restype = ctypes.c_int
+    got = []
+    cb = CB(lambda d: got.append(d))
+    cmd = " ".join(_pp_args(data, str(tmp_path / "r.bin"), 5, 2)[1:])
+    assert lib.doptExecuteLocalPP(cmd.encode(), cb) == 0
+    assert got == [D_FEAT + 1]
