@@ -58,8 +58,9 @@ def algorithmic_work(name, n, m, d, k, S):
         # Cholesky d^3/3, two triangular solves 2 d^2
         return "mfma", d ** 3 / 3.0 + 2.0 * d * d
     if name == "k_select":
-        # |Delta_c| read once, k positions + count written
-        return "hbm", float(n) * (8.0 * S + 4.0 * k + 4.0)
+        # |Delta_c| read once, k positions + count written; then in the same
+        # workgroup the messages (as k_emit below)
+        return "hbm", float(n) * (8.0 * S + 4.0 * k + 4.0) + float(n) * (k * 40.0 + S / 8.0)
     if name == "k_margin":
         # features once, labels, x; sigmoid weights, gradient weights, partials
         return "hbm", float(n) * m * (8.0 * d + 8.0 + 16.0) + 8.0 * d
@@ -67,11 +68,10 @@ def algorithmic_work(name, n, m, d, k, S):
         # Delta and learned at the k positions, learned written back, message
         # (value, flat index, position) and the selection bitmap written
         return "hbm", float(n) * (k * (8.0 + 16.0 + 8.0 + 4.0 + 4.0) + S / 8.0)
-    if name == "k_master_h":
+    if name == "k_master":
         # selection bitmaps, the selected Delta values, H read and written
-        return "hbm", float(n) * (S / 8.0 + 8.0 * k) + 16.0 * S
-    if name == "k_master_grad":
-        return "hbm", float(n) * d * 8.0 + 8.0 * d
+        # (W written), client gradients read, the average written
+        return "hbm", float(n) * (S / 8.0 + 8.0 * k) + 24.0 * S + float(n) * d * 8.0 + 8.0 * d
     return "hbm", 8.0 * S
 
 
@@ -131,7 +131,14 @@ def hessian_rel_error(eng_mode, eng_ref, x):
     return float(np.max(fro(a - b) / fro(b)))
 
 
-def mixed_precision_leg(rounds_c2, rounds_c3):
+def _warm(eng, rounds=300):
+    """Sustained rounds on an existing engine right before a timed section,
+    so that it starts at the GPU's working clocks."""
+    if eng is not None:
+        eng.run(rounds, check=False)
+
+
+def mixed_precision_leg(rounds_c2, rounds_c3, warm=None):
     """FP64 / FP32 / BF16-split A^T D A on the 142 x 350, d=300 TopK workload
     and on the 80 x 407, d=123 RandSeqK / TopLEK workload."""
     from ufednl_amd import datagen
@@ -144,6 +151,7 @@ def mixed_precision_leg(rounds_c2, rounds_c3):
         ref = None
         for mode in ("fp64", "fp32", "bf16x2"):
             eng = Engine.from_problem(prob, compressor=comp, k_mult=8.0, hessian=mode, tracking=False)
+            _warm(warm)
             r, t, rate = time_to_tol(eng, rounds)
             eng.kernel_timing(True)
             eng.run(5, check=False)
@@ -172,7 +180,7 @@ def mixed_precision_leg(rounds_c2, rounds_c3):
     return out
 
 
-def batched_leg(count, rounds):
+def batched_leg(count, rounds, warm=None):
     """count independent dense-Gaussian problems (d~U[32,512], n~U[16,256],
     m~U[100,600], TopK k=8d) stepped as one batch (one launch per kernel,
     graph replay) against the same problems stepped one after another."""
@@ -187,12 +195,14 @@ def batched_leg(count, rounds):
         del prob
     setup = time.perf_counter() - t0
     # one after another (each engine's own stream, waited for in turn)
+    _warm(warm)
     t0 = time.perf_counter()
     for e in engs:
         e.run(2, check=False)
     t_seq = (time.perf_counter() - t0) / 2
     b = Batch(engs, use_graph=True)
-    b.run(2)  # warm-up (graph capture)
+    b.run(2)  # graph capture
+    _warm(warm)
     t0 = time.perf_counter()
     b.run(rounds)
     t_batch = (time.perf_counter() - t0) / rounds
@@ -213,7 +223,7 @@ def batched_leg(count, rounds):
                 solves_per_s=(len(conv) / (max(conv) * t_batch)) if conv else None)
 
 
-def serving_leg(count, rate, budget_gb, max_rounds, seed=5):
+def serving_leg(count, rate, budget_gb, max_rounds, seed=5, warm=None):
     """Poisson arrivals (rate per second, fixed seed) of FedNL solves with
     dense N(0,1) features, n in [32,512], m in [200,1000], d log-uniform in
     [64,2048] clipped to the engine's d <= 512, TopK k=8d, each run to
@@ -231,6 +241,7 @@ def serving_leg(count, rate, budget_gb, max_rounds, seed=5):
         reqs.append(dict(arrival_s=float(t), problem=prob, max_rounds=max_rounds, tol=TOL,
                          opts=dict(compressor="topk", k_mult=8.0)))
     gen_s = time.perf_counter() - t0
+    _warm(warm)
     res, _ = E.serve(reqs, hbm_budget_bytes=budget_gb * 1e9, rounds_per_step=4)
     lat = np.array([r["finished_s"] - rq["arrival_s"] for r, rq in zip(res, reqs)])
     ok = [r["status"] == 0 for r in res]
@@ -269,8 +280,11 @@ def cpu_baseline(prob, rounds, cores):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=500,
+                    help="untimed rounds first: the GPU clocks up under sustained load")
+    ap.add_argument("--calib-rounds", type=int, default=100,
+                    help="rounds with every kernel timed (per-kernel table), after the warmup")
     ap.add_argument("--tol-rounds", type=int, default=400, help="round budget of the time-to-tolerance solve")
     ap.add_argument("--cpu-rounds", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -305,22 +319,25 @@ def main():
     prob = datagen.config2()
     n, m, d = prob.features.shape
 
-    # time to tolerance from x0 = 0 (device timestamps of each round)
-    eng_tol = Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False)
-    eng_tol.run(args.tol_rounds, check=False)
-    tr = eng_tol.trace()
-    g2 = tr["grad_norm"] ** 2
-    hit = np.nonzero(g2 <= TOL)[0]
-    r_tol = int(hit[0]) if hit.size else None
-    t_tol = float(tr["seconds"][r_tol]) if r_tol is not None else None
-    final_g2 = float(g2[-1])
-    eng_tol.close()
-
-    # throughput
+    # warmup: sustained rounds on a second engine bring the GPU to its working
+    # clocks; the same engine then gives the per-kernel table (every kernel
+    # timed by HIP events on the engine stream) and stays the warm-up source
+    # of the other legs
+    eng_w = Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False)
     eng = Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False)
-    eng.run(args.warmup, check=False)
-    _sync_all(eng)
-    eng.kernel_timing(True)
+    eng_w.run(args.warmup, check=False)
+    eng_w.kernel_timing(True)
+    eng_w.enqueue(args.calib_rounds)
+    _sync_all(eng_w)
+    calib = eng_w.kernel_times()
+    eng_w.kernel_timing(False)
+    dominant = max(calib.items(), key=lambda kv: kv[1][0])[0]
+    _warm(eng_w)
+
+    # the timed region: K rounds from x0 = 0 on a fresh engine, HIP events
+    # around the dominant kernel only; the time to tolerance comes from the
+    # same run's per-round device timestamps
+    eng.kernel_timing(True, only=dominant)
     _sync_all(eng)
     t0 = time.perf_counter()
     eng.enqueue(args.steps)
@@ -330,10 +347,16 @@ def main():
     eng.kernel_timing(False)
     elapsed = t1 - t0
     rps = args.steps / elapsed
+    tr = eng.trace()
+    g2 = tr["grad_norm"] ** 2
+    hit = np.nonzero(g2 <= TOL)[0]
+    r_tol = int(hit[0]) if hit.size else None
+    t_tol = float(tr["seconds"][r_tol]) if r_tol is not None else None
+    final_g2 = float(g2[-1])
     traffic = load_traffic(os.path.join(ROOT, "profiles", TRAFFIC_FILE))
     kname, roof, kavg = kernel_roofline(eng, times, traffic)
     all_roofs = {}
-    for name_, (ms_, nl_) in times.items():
+    for name_, (ms_, nl_) in calib.items():
         b_, w_ = algorithmic_work(name_, eng.n, eng.m, eng.d, eng.k, eng.S)
         sec_ = ms_ * 1e-3 / max(nl_, 1)
         pk_ = PEAK_FP64_MFMA_TFLOPS * 1e12 if b_ == "mfma" else PEAK_HBM_GBS * 1e9
@@ -341,9 +364,9 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        extra["mixed_precision"] = mixed_precision_leg(args.tol_rounds, 500)
-        extra["batched"] = batched_leg(args.batch_problems, args.batch_rounds)
-        extra["serving"] = serving_leg(args.serve_requests, args.serve_rate, 64.0, 400)
+        extra["mixed_precision"] = mixed_precision_leg(args.tol_rounds, 500, warm=eng_w)
+        extra["batched"] = batched_leg(args.batch_problems, args.batch_rounds, warm=eng_w)
+        extra["serving"] = serving_leg(args.serve_requests, args.serve_rate, 64.0, 400, warm=eng_w)
 
     cpu = None
     if not args.no_cpu:
@@ -373,7 +396,10 @@ def main():
         "rounds_to_tol": r_tol,
         "tol": TOL,
         "final_grad_norm_sq": final_g2,
-        "kernel_ms_per_round": {k_: v[0] / max(v[1], 1) for k_, v in times.items()},
+        "kernel_ms_per_round": {k_: v[0] / max(v[1], 1) for k_, v in calib.items()},
+        "calib_rounds": args.calib_rounds,
+        "timed_region": "rounds 0..steps-1 from x0 = 0 after warmup rounds on a second engine; "
+                        "kernel_ms_per_round from calib_rounds with every kernel timed",
         "kernel_roofline_all": all_roofs,
         "roofline_kernel": kname,
         "roofline": roof,

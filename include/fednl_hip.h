@@ -150,6 +150,9 @@ int fednl_set_state(void *handle, const double *x, const double *learned, const 
 /* Per-kernel device time (milliseconds, summed over rounds since the last
  * reset) measured with HIP events on the engine stream when enabled. */
 int fednl_kernel_timing(void *handle, int enable);
+/* The same for a subset: bit q times the kernel fednl_kernel_times lists at
+ * index q (the events around each timed launch cost device time). */
+int fednl_kernel_timing_mask(void *handle, unsigned mask);
 int fednl_kernel_times(void *handle, double *ms, int *launches, int max_kernels, const char **names);
 
 /* Standalone compressor on identical inputs (parity of index selection).

@@ -1,16 +1,18 @@
 // fednl_kernels.hip -- MI355X (gfx950) kernels and host engine of the FedNL
 // round. One FedNL round of dopt's single-node simulation
 // (dopt/bin_fednl_local/include/TrainLoop.h:1054-1508 with the client loops of
-// LocalWorkerForFedNL_{TopK,TopLEK,RandK,RandSeqK}.h) runs as seven kernels on
-// one stream, every client simulated in place:
+// LocalWorkerForFedNL_{TopK,TopLEK,RandK,RandSeqK}.h) runs as five kernels on
+// one stream (six with TopLEK), every client simulated in place:
 //
 //   k_margin        t = A x, sigma, D = (1-sigma)sigma/m, g = (1-sigma)(-1/m), -log sigma
 //   k_hessian_f64   upper tiles of A^T diag(D) A on v_mfma_f64_16x16x4f64, fused with
 //                   +lambda, Delta = H - learned, Lk partial sums, and the client gradient
-//   k_select_*      index selection (heap scan of TopK/TopLEK, RandK shuffle, RandSeqK window)
-//   k_emit          message values, learned-Hessian update, TopLEK random rounding
-//   k_master_h      H[idx] += val in client order (deterministic)
-//   k_master_vec    gradient / Lk / f averages, norms, trace record
+//   k_select_*      index selection (heap scan of TopK/TopLEK, RandK shuffle, RandSeqK
+//                   window), followed in the same workgroup by the client's message
+//                   values and learned-Hessian update (k_emit on its own for TopLEK,
+//                   whose random rounding runs over the other clients)
+//   k_master        H[idx] += val in client order (deterministic), gradient / Lk / f
+//                   averages, the shift, the trace record
 //   k_newton        Cholesky of H + shift I, two triangular solves, x -= dx
 //
 // Device layout (per problem):
@@ -1267,9 +1269,8 @@ __device__ inline double fro2_from_parts(const double* part, int c, int tiles_up
 // Output: sorted selected list positions, count; TopLEK: hit flag, p, popped.
 // ---------------------------------------------------------------------------
 template <bool kLek>
-__global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
+__device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk)
 {
-    FEDNL_BLOCK();
     const double* __restrict__ delta = PD.delta;
     const size_t S = PD.S;
     const int k = PD.k, seg_len = PD.seg_len, tiles_up = PD.tiles_up;
@@ -1532,9 +1533,8 @@ __global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
 // target t_i = i + round(u_i (S-1-i)); the value left at slot i is found by
 // following the chain of earlier swaps that touched its source slot.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_select_randk(FEDNL_KARGS)
+__device__ __forceinline__ void select_randk_client(const ProblemDev& PD, int blk)
 {
-    FEDNL_BLOCK();
     const size_t S = PD.S;
     const int k = PD.k;
     uint32_t* __restrict__ rng = PD.rng;
@@ -1627,9 +1627,8 @@ __global__ __launch_bounds__(256) void k_select_randk(FEDNL_KARGS)
 
 // RandSeqK (SpecialMathRoutinesForMatrix.h:137-169): window of k list positions
 // from generateInteger() % S, cyclic; stored ascending. Identical: all positions.
-__global__ __launch_bounds__(256) void k_select_window(FEDNL_KARGS)
+__device__ __forceinline__ void select_window_client(const ProblemDev& PD, int blk)
 {
-    FEDNL_BLOCK();
     const int comp = PD.comp;
     const size_t S = PD.S;
     const int k = PD.k;
@@ -1674,9 +1673,11 @@ __global__ __launch_bounds__(256) void k_select_window(FEDNL_KARGS)
 // draw from their worker's RealLinear stream: the draw index is the number of
 // earlier hits among the clients of the same worker this round.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_emit(FEDNL_KARGS)
+// the client's message values, positions, flat indices, selection bitmap and
+// learned update (LocalWorkerForFedNL_*.h), after its selection; every
+// thread of the workgroup takes part; dynamic LDS holds the bitmap
+__device__ __forceinline__ void emit_client_body(const ProblemDev& PD, int blk)
 {
-    FEDNL_BLOCK();
     const int comp = PD.comp;
     const double* __restrict__ delta = PD.delta;
     const size_t S = PD.S;
@@ -1761,8 +1762,50 @@ __global__ __launch_bounds__(256) void k_emit(FEDNL_KARGS)
         sel_bm[(size_t)c * bm_words + w] = bml[w];
 }
 
+
+__global__ __launch_bounds__(256) void k_emit(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    emit_client_body(PD, blk);
+}
+
+// kEmit: the client's messages and learned update follow in the same
+// workgroup (every compressor but TopLEK, whose draw needs the other clients)
+template <bool kLek, bool kEmit>
+__global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    select_heap_client<kLek>(PD, blk);
+    if (kEmit) {
+        __syncthreads();
+        emit_client_body(PD, blk);
+    }
+}
+
+template <bool kEmit>
+__global__ __launch_bounds__(256) void k_select_randk(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    select_randk_client(PD, blk);
+    if (kEmit) {
+        __syncthreads();
+        emit_client_body(PD, blk);
+    }
+}
+
+template <bool kEmit>
+__global__ __launch_bounds__(256) void k_select_window(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    select_window_client(PD, blk);
+    if (kEmit) {
+        __syncthreads();
+        emit_client_body(PD, blk);
+    }
+}
+
 // ---------------------------------------------------------------------------
-// k_master_h: H[idx] += val for clients 0..n-1 in order (TrainLoop.h:1197-1274).
+// master_h_block: H[idx] += val for clients 0..n-1 in order (TrainLoop.h:1197-1274).
 // Each thread owns one list position and walks the clients in order; a
 // client contributes where its selection bitmap has the bit set, with the
 // value it sent recomputed from Delta by the same operations as k_emit
@@ -1772,9 +1815,8 @@ __global__ __launch_bounds__(256) void k_emit(FEDNL_KARGS)
 // ---------------------------------------------------------------------------
 constexpr int kMasterPos = 256;
 
-__global__ __launch_bounds__(256) void k_master_h(FEDNL_KARGS)
+__device__ __forceinline__ void master_h_block(const ProblemDev& PD, int blk)
 {
-    FEDNL_BLOCK();
     double* __restrict__ H = PD.H;
     const size_t S = PD.S;
     const int n = PD.n, bm_words = PD.bm_words, dp = PD.dp;
@@ -1784,7 +1826,6 @@ __global__ __launch_bounds__(256) void k_master_h(FEDNL_KARGS)
     const uint32_t* __restrict__ sel_bm = PD.sel_bm;
     const double mult = PD.mult, prescale = PD.prescale;
     double* __restrict__ W = PD.W;
-    const double* __restrict__ shift_p = PD.shift;
     __shared__ uint32_t bw[kMaxClients][kMasterPos / 32];
     const size_t p0 = (size_t)blk * kMasterPos;
     const int w0 = (int)(p0 / 32);
@@ -1820,11 +1861,11 @@ __global__ __launch_bounds__(256) void k_master_h(FEDNL_KARGS)
     }
     H[p] = h;
     if (W) {
-        // the Newton workspace: W(j, i) = H(i, j) (+shift on the diagonal),
-        // column-major lower, ld = dp (TrainLoop.h:1449-1454)
+        // the Newton workspace: W(j, i) = H(i, j), column-major lower, ld = dp;
+        // k_newton adds the shift on the diagonal (TrainLoop.h:1449-1454)
         const uint32_t j = pos_col(p);
         const uint32_t i = (uint32_t)(p - (size_t)j * (j + 1) / 2);
-        W[(size_t)i * dp + j] = (i == j) ? h + *shift_p : h;
+        W[(size_t)i * dp + j] = h;
     }
 }
 
@@ -1833,9 +1874,8 @@ __global__ __launch_bounds__(256) void k_master_h(FEDNL_KARGS)
 constexpr int kGradCols = 64;
 constexpr int kGradClients = 64;
 
-__global__ __launch_bounds__(256) void k_master_grad(FEDNL_KARGS)
+__device__ __forceinline__ void master_grad_block(const ProblemDev& PD, int blk)
 {
-    FEDNL_BLOCK();
     const int n = PD.n, d = PD.d, d_pad = PD.d_pad;
     const double* __restrict__ grad_c = PD.grad_c;
     const double inv_n = PD.inv_n;
@@ -1925,14 +1965,12 @@ __device__ inline double client_f(const double* __restrict__ nlog_part, int c, i
 }
 
 // ---------------------------------------------------------------------------
-// k_master_vec: Lk and f averages (TrainLoop.h:1278-1333, 1512-1544), the
+// master_vec_block: Lk and f averages (TrainLoop.h:1278-1333, 1512-1544), the
 // Newton shift, norms for the record (TrainLoop.h:1679-1691), TopLEK stream
 // advance. One workgroup.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_master_vec(FEDNL_KARGS)
+__device__ __forceinline__ void master_vec_block(const ProblemDev& PD)
 {
-    FEDNL_BLOCK();
-    (void)blk;
     const int comp = PD.comp, n = PD.n, d = PD.d, m = PD.m, nlog_blocks = PD.m_pad / 16, tiles_up = PD.tiles_up;
     const double* __restrict__ g_avg = PD.g_avg;
     const double* __restrict__ part = PD.part;
@@ -1948,14 +1986,9 @@ __global__ __launch_bounds__(256) void k_master_vec(FEDNL_KARGS)
     const int* __restrict__ round_ctr = PD.round_ctr;
     __shared__ double scratch[16];
     __shared__ double fc_s[kMaxClients];
-    double gg = 0.0, xx = 0.0;
-    const double* __restrict__ g_rep = PD.pp ? PD.pp_tg : g_avg;  // FedNL-PP reports the full gradient
-    for (int i = threadIdx.x; i < d; i += blockDim.x) {
-        const double g = g_rep[i];
-        gg = fma(g, g, gg);
-        xx = fma(x[i], x[i], xx);
-    }
-    gg = block_sum(gg, scratch);
+    (void)g_avg;
+    double xx = 0.0;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) xx = fma(x[i], x[i], xx);
     xx = block_sum(xx, scratch);
     const bool with_f = tracking || PD.ls;  // f_i is sent (TrainLoop.h:620, 1512-1544)
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
@@ -1996,7 +2029,6 @@ __global__ __launch_bounds__(256) void k_master_vec(FEDNL_KARGS)
                     parts += 1;
                     items += (comp == FEDNL_TOPLEK) ? (double)PD.cnt[c] : (double)PD.k;
                 }
-            trace[(size_t)r * kTraceStride + 0] = sqrt(gg);
             trace[(size_t)r * kTraceStride + 1] = fsum;
             trace[(size_t)r * kTraceStride + 2] = sqrt(xx);
             trace[(size_t)r * kTraceStride + 5] = (double)parts;
@@ -2011,6 +2043,27 @@ __global__ __launch_bounds__(256) void k_master_vec(FEDNL_KARGS)
             lek_state[w] = st;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// k_master: the master's aggregation in one launch. Blocks [0, nh) do
+// H[idx] += val and W = H, blocks [nh, nh + ng) the gradient average, the last
+// block the Lk / f averages, the shift and the record (see the three block
+// functions above for the reference lines).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int master_h_blocks(size_t S) { return (int)((S + kMasterPos - 1) / kMasterPos); }
+__host__ __device__ inline int master_grad_blocks(int d) { return (d + kGradCols - 1) / kGradCols; }
+
+__global__ __launch_bounds__(256) void k_master(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    const int nh = master_h_blocks(PD.S), ng = master_grad_blocks(PD.d);
+    if (blk < nh)
+        master_h_block(PD, blk);
+    else if (blk < nh + ng)
+        master_grad_block(PD, blk - nh);
+    else
+        master_vec_block(PD);
 }
 
 // ---------------------------------------------------------------------------
@@ -2070,7 +2123,7 @@ __global__ __launch_bounds__(256) void k_ls_check(FEDNL_KARGS)
     __shared__ double fc_s[kMaxClients];
     __shared__ double t_next;
     __shared__ int move;
-    // f at the trial point, as k_master_vec computes f(x_k)
+    // f at the trial point, as k_master computes f(x_k)
     double xx = 0.0;
     for (int i = threadIdx.x; i < d; i += blockDim.x) xx = fma(x[i], x[i], xx);
     xx = block_sum(xx, scratch);
@@ -2396,7 +2449,9 @@ __device__ inline void sqrt_and_inv(double a, double& root, double& inv)
 
 // Factors the 32 x 32 block held in Dg (row-major, lower part meaningful) in
 // wave 0 and solves its part of L y = b: on return Dg holds L11, invd /
-// invall the reciprocal pivots, yv[jb..jb+31] the solution.
+// invall the reciprocal pivots, yv[jb..jb+31] the solution. Column cc + 1 is
+// updated first, so the square root of the next pivot overlaps the rest of
+// column cc's rank-1 update.
 __device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double* invd, double* invall, double* yv,
                                                    int jb, int lane, int* s_fail)
 {
@@ -2411,14 +2466,14 @@ __device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double
     double yr = yv[jb + lr];
     double myinv = 1.0;
     bool fail = false;
+    double piv = bcast_lane(row[0], 0);
+    double l, inv;
+    sqrt_and_inv(piv, l, inv);
 #pragma unroll
     for (int cc = 0; cc < kNB; ++cc) {
-        const double piv = bcast_lane(row[cc], cc);
         fail = fail || (piv < 0.0);  // finish the (now meaningless) block, report after
         // piv < 0 is reported after the block; piv == 0 gives non-finite
         // entries as a division by a zero pivot would
-        double l, inv;
-        sqrt_and_inv(piv, l, inv);
         const double ycc = bcast_lane(yr, cc) * inv;
         const double scaled = inv * row[cc];
         const bool below = lane > cc;
@@ -2426,10 +2481,15 @@ __device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double
         row[cc] = me ? l : (below ? scaled : row[cc]);
         myinv = me ? inv : myinv;
         yr = me ? ycc : (below ? fma(-scaled, ycc, yr) : yr);
-        // L[c2][cc] lives in lane c2: broadcast by v_readlane
+        if (cc + 1 < kNB) {
+            // L[c2][cc] lives in lane c2: broadcast by v_readlane
+            row[cc + 1] = fma(-row[cc], bcast_lane(row[cc], cc + 1), row[cc + 1]);
+            piv = bcast_lane(row[cc + 1], cc + 1);
+            sqrt_and_inv(piv, l, inv);
 #pragma unroll
-        for (int c2 = cc + 1; c2 < kNB; ++c2)
-            row[c2] = fma(-row[cc], bcast_lane(row[cc], c2), row[c2]);
+            for (int c2 = cc + 2; c2 < kNB; ++c2)
+                row[c2] = fma(-row[cc], bcast_lane(row[cc], c2), row[c2]);
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
     if (fail) {
@@ -2508,8 +2568,25 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
         }
         __threadfence_block();
     }
+    if (!Hup) {
+        // W = H from k_master: the shift joins the diagonal here
+        for (int i = tid; i < d; i += nthr) W[(size_t)i * dp + i] = W[(size_t)i * dp + i] + shift;
+        __threadfence_block();
+    }
     for (int i = tid; i < dp; i += nthr)
         yv[i] = (i < d) ? b[i] : 0.0;
+    if (PD.mst) {
+        // an engine round: the record's ||grad f(x_r)|| (TrainLoop.h:1679-1691);
+        // FedNL-PP reports the full gradient
+        __shared__ double gsc[16];
+        const double* __restrict__ gr = PD.pp ? PD.pp_tg : b;
+        double gg = 0.0;
+        for (int i = tid; i < d; i += nthr) gg = fma(gr[i], gr[i], gg);
+        gg = block_sum(gg, gsc);
+        const int r0 = *round_ctr;
+        if (tid == 0 && r0 < trace_cap)
+            trace[(size_t)r0 * kTraceStride + 0] = sqrt(gg);
+    }
     __syncthreads();
     PROBE_ADD(0, t_n0);
 
@@ -2834,10 +2911,10 @@ __global__ __launch_bounds__(64) void k_mfma_probe_mixed(const float* __restrict
 // host engine
 // ---------------------------------------------------------------------------
 enum KernelId {
-    KID_MARGIN, KID_HESS, KID_SELECT, KID_EMIT, KID_MASTER_H, KID_MASTER_VEC, KID_NEWTON, KID_LS, KID_COUNT
+    KID_MARGIN, KID_HESS, KID_SELECT, KID_EMIT, KID_MASTER, KID_NEWTON, KID_LS, KID_COUNT
 };
 static const char* kKernelNames[KID_COUNT] = {"k_margin", "k_hessian_f64", "k_select", "k_emit",
-                                              "k_master_h", "k_master_vec", "k_newton", "k_line_search"};
+                                              "k_master", "k_newton", "k_line_search"};
 
 struct Engine {
     fednl_opts o;
@@ -2867,6 +2944,7 @@ struct Engine {
     hipEvent_t ev_start = nullptr;
     std::vector<hipEvent_t> round_ev;
     bool timing = false;
+    unsigned timing_mask = ~0u;    // KernelId bits timed when timing is on
     struct Pending { int kid; hipEvent_t a, b; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
@@ -2918,7 +2996,7 @@ struct KTime {
     hipEvent_t b = nullptr;
     KTime(Engine* e_, int kid_) : e(e_), kid(kid_)
     {
-        if (e->timing) {
+        if (e->timing && ((e->timing_mask >> kid) & 1u)) {
             hipEvent_t a = pool_event(e);
             b = pool_event(e);
             hipEventRecord(a, e->st);
@@ -2995,8 +3073,7 @@ static int sync_desc(Engine* e)
 static int blocks_margin(const ProblemDev& P) { return (P.m_pad / 16) * P.n; }
 static int blocks_hessian(const ProblemDev& P) { return 8 * ((P.n + 7) / 8) * P.tiles_up; }
 static int blocks_grad(const ProblemDev& P) { return (P.d_pad / kTile) * P.n; }
-static int blocks_master_grad(const ProblemDev& P) { return (P.d + kGradCols - 1) / kGradCols; }
-static int blocks_master_h(const ProblemDev& P) { return (int)((P.S + kMasterPos - 1) / kMasterPos); }
+static int blocks_master(const ProblemDev& P) { return master_h_blocks(P.S) + master_grad_blocks(P.d) + 1; }
 
 // client Hessians (and, for the reduced-precision modes, the FP64 client
 // gradients) at the current margin weights; init: write learned (or
@@ -3059,30 +3136,29 @@ static int launch_round(Engine* e)
     const int comp = e->o.compressor;
     {
         KTime kt(e, KID_SELECT);
+        // the messages follow the selection in the same workgroup, except
+        // for TopLEK (its draw runs over the other clients' outcomes)
+        const size_t lds_bm = sizeof(uint32_t) * e->bm_words;
+        const size_t lds_fused = std::max(e->lds_select, lds_bm);
         if (comp == FEDNL_TOPK)
-            hipLaunchKernelGGL(k_select_heap<false>, dim3(n), dim3(256), e->lds_select, e->st, pd, nomap);
+            hipLaunchKernelGGL((k_select_heap<false, true>), dim3(n), dim3(256), lds_fused, e->st, pd, nomap);
         else if (comp == FEDNL_TOPLEK)
-            hipLaunchKernelGGL(k_select_heap<true>, dim3(n), dim3(256), e->lds_select, e->st, pd, nomap);
+            hipLaunchKernelGGL((k_select_heap<true, false>), dim3(n), dim3(256), e->lds_select, e->st, pd, nomap);
         else if (comp == FEDNL_RANDK)
-            hipLaunchKernelGGL(k_select_randk, dim3(n), dim3(256), e->lds_select, e->st, pd, nomap);
+            hipLaunchKernelGGL(k_select_randk<true>, dim3(n), dim3(256), lds_fused, e->st, pd, nomap);
         else
-            hipLaunchKernelGGL(k_select_window, dim3(n), dim3(256), 0, e->st, pd, nomap);
+            hipLaunchKernelGGL(k_select_window<true>, dim3(n), dim3(256), lds_bm, e->st, pd, nomap);
     }
     {
         KTime kt(e, KID_EMIT);
-        hipLaunchKernelGGL(k_emit, dim3(n), dim3(256), sizeof(uint32_t) * e->bm_words, e->st, pd, nomap);
+        if (comp == FEDNL_TOPLEK)
+            hipLaunchKernelGGL(k_emit, dim3(n), dim3(256), sizeof(uint32_t) * e->bm_words, e->st, pd, nomap);
         if (pp)
             hipLaunchKernelGGL(k_pp_client<false>, dim3(n), dim3(256), 0, e->st, pd, nomap);
     }
     {
-        KTime kt(e, KID_MASTER_VEC);
-        hipLaunchKernelGGL(k_master_grad, dim3(blocks_master_grad(P)), dim3(256), 0, e->st, pd, nomap);
-        hipLaunchKernelGGL(k_master_vec, dim3(1), dim3(256), 0, e->st, pd, nomap);
-    }
-    {
-        // after the shift is known: H update and the Newton workspace in one pass
-        KTime kt(e, KID_MASTER_H);
-        hipLaunchKernelGGL(k_master_h, dim3(blocks_master_h(P)), dim3(256), 0, e->st, pd, nomap);
+        KTime kt(e, KID_MASTER);
+        hipLaunchKernelGGL(k_master, dim3(blocks_master(P)), dim3(256), 0, e->st, pd, nomap);
     }
     {
         KTime kt(e, KID_NEWTON);
@@ -3327,12 +3403,12 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         return rc;
     }
     if (e->lds_select > 64 * 1024) {
-        hipFuncSetAttribute((const void*)k_select_heap<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)e->lds_select);
-        hipFuncSetAttribute((const void*)k_select_heap<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)e->lds_select);
-        hipFuncSetAttribute((const void*)k_select_randk, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)e->lds_select);
+        const int lds = (int)std::max(e->lds_select, sizeof(uint32_t) * e->bm_words);
+        hipFuncSetAttribute((const void*)k_select_heap<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        hipFuncSetAttribute((const void*)k_select_heap<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        hipFuncSetAttribute((const void*)k_select_heap<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        hipFuncSetAttribute((const void*)k_select_randk<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        hipFuncSetAttribute((const void*)k_select_randk<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     }
     if (e->lds_newton > 64 * 1024)
         hipFuncSetAttribute((const void*)k_newton, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_newton);
@@ -3467,7 +3543,7 @@ static int enqueue_rounds(Engine* e, int rounds)
 // ---------------------------------------------------------------------------
 enum BatchKernel {
     BK_MARGIN, BK_HESS64, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
-    BK_EMIT, BK_MGRAD, BK_MVEC, BK_MH, BK_NEWTON, BK_LS_MARGIN, BK_LS, BK_COUNT
+    BK_EMIT, BK_MASTER, BK_NEWTON, BK_LS_MARGIN, BK_LS, BK_COUNT
 };
 
 struct Batch {
@@ -3520,10 +3596,9 @@ static int batch_build(Batch* b)
         add(comp == FEDNL_TOPK ? BK_SEL_TOPK
                                : comp == FEDNL_TOPLEK ? BK_SEL_TOPLEK : comp == FEDNL_RANDK ? BK_SEL_RANDK : BK_SEL_WINDOW,
             p, e->n, false);
-        add(BK_EMIT, p, e->n, false);
-        add(BK_MGRAD, p, blocks_master_grad(P), false);
-        add(BK_MVEC, p, 1, false);
-        add(BK_MH, p, blocks_master_h(P), false);
+        if (comp == FEDNL_TOPLEK)
+            add(BK_EMIT, p, e->n, false);
+        add(BK_MASTER, p, blocks_master(P), false);
         add(BK_NEWTON, p, 1, false);
         if (e->o.line_search) {
             add(BK_LS_MARGIN, p, blocks_margin(P), false);
@@ -3589,21 +3664,23 @@ static void batch_launch_round(Batch* b)
     if (b->cnt[BK_HESS16])
         hipLaunchKernelGGL(k_hessian_bf16<false>, dim3(b->cnt[BK_HESS16]), dim3(256), 0, st, pd, map(BK_HESS16),
                            (double*)nullptr);
+    // selection and messages in one workgroup per client (TopLEK: emit after)
+    const size_t lds_fused = std::max(b->lds_select, b->lds_emit);
     if (b->cnt[BK_SEL_TOPK])
-        hipLaunchKernelGGL(k_select_heap<false>, dim3(b->cnt[BK_SEL_TOPK]), dim3(256), b->lds_select, st, pd,
+        hipLaunchKernelGGL((k_select_heap<false, true>), dim3(b->cnt[BK_SEL_TOPK]), dim3(256), lds_fused, st, pd,
                            map(BK_SEL_TOPK));
     if (b->cnt[BK_SEL_TOPLEK])
-        hipLaunchKernelGGL(k_select_heap<true>, dim3(b->cnt[BK_SEL_TOPLEK]), dim3(256), b->lds_select, st, pd,
-                           map(BK_SEL_TOPLEK));
+        hipLaunchKernelGGL((k_select_heap<true, false>), dim3(b->cnt[BK_SEL_TOPLEK]), dim3(256), b->lds_select, st,
+                           pd, map(BK_SEL_TOPLEK));
     if (b->cnt[BK_SEL_RANDK])
-        hipLaunchKernelGGL(k_select_randk, dim3(b->cnt[BK_SEL_RANDK]), dim3(256), b->lds_select, st, pd,
+        hipLaunchKernelGGL(k_select_randk<true>, dim3(b->cnt[BK_SEL_RANDK]), dim3(256), lds_fused, st, pd,
                            map(BK_SEL_RANDK));
     if (b->cnt[BK_SEL_WINDOW])
-        hipLaunchKernelGGL(k_select_window, dim3(b->cnt[BK_SEL_WINDOW]), dim3(256), 0, st, pd, map(BK_SEL_WINDOW));
-    hipLaunchKernelGGL(k_emit, dim3(b->cnt[BK_EMIT]), dim3(256), b->lds_emit, st, pd, map(BK_EMIT));
-    hipLaunchKernelGGL(k_master_grad, dim3(b->cnt[BK_MGRAD]), dim3(256), 0, st, pd, map(BK_MGRAD));
-    hipLaunchKernelGGL(k_master_vec, dim3(b->cnt[BK_MVEC]), dim3(256), 0, st, pd, map(BK_MVEC));
-    hipLaunchKernelGGL(k_master_h, dim3(b->cnt[BK_MH]), dim3(256), 0, st, pd, map(BK_MH));
+        hipLaunchKernelGGL(k_select_window<true>, dim3(b->cnt[BK_SEL_WINDOW]), dim3(256), b->lds_emit, st, pd,
+                           map(BK_SEL_WINDOW));
+    if (b->cnt[BK_EMIT])
+        hipLaunchKernelGGL(k_emit, dim3(b->cnt[BK_EMIT]), dim3(256), b->lds_emit, st, pd, map(BK_EMIT));
+    hipLaunchKernelGGL(k_master, dim3(b->cnt[BK_MASTER]), dim3(256), 0, st, pd, map(BK_MASTER));
     hipLaunchKernelGGL(k_newton, dim3(b->cnt[BK_NEWTON]), dim3(kNewtonThreads), b->lds_newton, st, pd,
                        map(BK_NEWTON));
     if (b->cnt[BK_LS]) {
@@ -4140,11 +4217,17 @@ int fednl_set_state(void* handle, const double* x, const double* learned, const 
 
 int fednl_kernel_timing(void* handle, int enable)
 {
+    return fednl_kernel_timing_mask(handle, enable ? ~0u : 0u);
+}
+
+int fednl_kernel_timing_mask(void* handle, unsigned mask)
+{
     Engine* e = (Engine*)handle;
     if (!e)
         return FEDNL_E_MISSING_ARGUMENT;
     int rc = collect_timing(e);
-    e->timing = enable != 0;
+    e->timing = mask != 0;
+    e->timing_mask = mask;
     for (int q = 0; q < KID_COUNT; ++q) {
         e->kms[q] = 0.0;
         e->klaunch[q] = 0;
@@ -4222,9 +4305,11 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     const size_t lds = select_lds_bytes(std::min(k, kMaxHeapK));
     const int seg_len = select_seg_len(std::min(k, kMaxHeapK));
     if (lds > 64 * 1024) {
-        hipFuncSetAttribute((const void*)k_select_heap<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipFuncSetAttribute((const void*)k_select_heap<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipFuncSetAttribute((const void*)k_select_randk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipFuncSetAttribute((const void*)k_select_heap<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+        hipFuncSetAttribute((const void*)k_select_heap<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+        hipFuncSetAttribute((const void*)k_select_randk<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     }
     // TopLEK needs ||Delta||^2: computed on the host-side copy here as tile partials would be
     // in the round; the standalone path passes one partial per client.
@@ -4328,13 +4413,13 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     const ProblemDev* pd = dpd;
     const int2* nomap = nullptr;
     if (compressor == FEDNL_TOPK)
-        hipLaunchKernelGGL(k_select_heap<false>, dim3(nclients), dim3(256), lds, st, pd, nomap);
+        hipLaunchKernelGGL((k_select_heap<false, false>), dim3(nclients), dim3(256), lds, st, pd, nomap);
     else if (compressor == FEDNL_TOPLEK)
-        hipLaunchKernelGGL(k_select_heap<true>, dim3(nclients), dim3(256), lds, st, pd, nomap);
+        hipLaunchKernelGGL((k_select_heap<true, false>), dim3(nclients), dim3(256), lds, st, pd, nomap);
     else if (compressor == FEDNL_RANDK)
-        hipLaunchKernelGGL(k_select_randk, dim3(nclients), dim3(256), lds, st, pd, nomap);
+        hipLaunchKernelGGL(k_select_randk<false>, dim3(nclients), dim3(256), lds, st, pd, nomap);
     else if (compressor == FEDNL_RANDSEQK || compressor == FEDNL_IDENTICAL)
-        hipLaunchKernelGGL(k_select_window, dim3(nclients), dim3(256), 0, st, pd, nomap);
+        hipLaunchKernelGGL(k_select_window<false>, dim3(nclients), dim3(256), 0, st, pd, nomap);
     else {
         hipFree(dpd);
         hipFree(bm);

@@ -97,6 +97,7 @@ def lib():
         L.fednl_get_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
         L.fednl_set_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
         L.fednl_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.fednl_kernel_timing_mask.argtypes = [ctypes.c_void_p, ctypes.c_uint]
         L.fednl_kernel_times.argtypes = [ctypes.c_void_p, _c_dp, _c_ip, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_char_p)]
         L.fednl_select.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dp, _c_up, _c_up,
@@ -243,8 +244,19 @@ class Engine:
                                    dp(c(st.get("H"), np.float64)), up(c(st.get("rng"), np.uint32)),
                                    up(c(st.get("lek"), np.uint32))), "fednl_set_state")
 
-    def kernel_timing(self, enable=True):
-        _err(lib().fednl_kernel_timing(self.h, 1 if enable else 0), "fednl_kernel_timing")
+    def kernel_timing(self, enable=True, only=None):
+        """Per-kernel device timing; only: a kernel name to time alone."""
+        if only is None:
+            _err(lib().fednl_kernel_timing(self.h, 1 if enable else 0), "fednl_kernel_timing")
+            return
+        names = self.kernel_names()
+        mask = (1 << names.index(only)) if enable else 0
+        _err(lib().fednl_kernel_timing_mask(self.h, mask), "fednl_kernel_timing_mask")
+
+    def kernel_names(self):
+        names = (ctypes.c_char_p * KERNEL_NAMES_MAX)()
+        nk = _err(lib().fednl_kernel_times(self.h, None, None, KERNEL_NAMES_MAX, names), "fednl_kernel_times")
+        return [names[q].decode() for q in range(nk)]
 
     def kernel_times(self):
         ms = np.zeros(KERNEL_NAMES_MAX); lc = np.zeros(KERNEL_NAMES_MAX, dtype=np.int32)
