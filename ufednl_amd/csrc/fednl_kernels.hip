@@ -952,8 +952,11 @@ __device__ inline void heapify_cont(double* hv, uint32_t* hid, int i, int n, dou
 // minimum is below the running comparison value.
 // ---------------------------------------------------------------------------
 struct RegHeap {
-    double r0v, v1, v2, v3;
-    uint32_t r0i, i1, i2, i3;
+    // slot 0 in every lane; level 1 (slots 1..4) in lanes 0..3; level 2
+    // (5..20) in lanes 0..15; level 3 (21..84) one per lane; level 4
+    // (85..340) four per lane: lane o holds the children of level-3 slot 21 + o
+    double r0v, v1, v2, v3, v4[4];
+    uint32_t r0i, i1, i2, i3, i4[4];
     int n, lane;
 
     __device__ void load(const double* hv, const uint32_t* hid, int n_, int lane_)
@@ -969,6 +972,12 @@ struct RegHeap {
         i2 = (lane < 16 && 5 + lane < n) ? hid[5 + lane] : 0u;
         v3 = (21 + lane < n) ? hv[21 + lane] : inf;
         i3 = (21 + lane < n) ? hid[21 + lane] : 0u;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int sl = 85 + 4 * lane + c;
+            v4[c] = (sl < n) ? hv[sl] : inf;
+            i4[c] = (sl < n) ? hid[sl] : 0u;
+        }
     }
 
     __device__ void store(double* hv, uint32_t* hid) const
@@ -989,6 +998,29 @@ struct RegHeap {
             hv[21 + lane] = v3;
             hid[21 + lane] = i3;
         }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int sl = 85 + 4 * lane + c;
+            if (sl < n) {
+                hv[sl] = v4[c];
+                hid[sl] = i4[c];
+            }
+        }
+    }
+
+    // level-4 entry c (uniform) of this lane
+    __device__ double v4at(int c) const { return c == 0 ? v4[0] : (c == 1 ? v4[1] : (c == 2 ? v4[2] : v4[3])); }
+    __device__ uint32_t i4at(int c) const { return c == 0 ? i4[0] : (c == 1 ? i4[1] : (c == 2 ? i4[2] : i4[3])); }
+
+    // level-4 entry c (uniform) of lane o becomes (v, id)
+    __device__ void set4(int o, int c, double v, uint32_t id)
+    {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool hit = (lane == o) && (q == c);
+            v4[q] = hit ? v : v4[q];
+            i4[q] = hit ? id : i4[q];
+        }
     }
 
     // value and id of slot sidx (uniform), wherever it is held
@@ -1006,6 +1038,10 @@ struct RegHeap {
         } else if (sidx < 85) {
             v = bcast_lane(v3, sidx - 21);
             id = __builtin_amdgcn_readlane(i3, sidx - 21);
+        } else if (sidx < 341) {
+            const int q = sidx - 85;
+            v = bcast_lane(v4at(q & 3), q >> 2);
+            id = __builtin_amdgcn_readlane(i4at(q & 3), q >> 2);
         } else {
             v = hv[sidx];
             id = hid[sidx];
@@ -1023,6 +1059,8 @@ struct RegHeap {
             v2 = (lane == sidx - 5) ? inf : v2;
         else if (sidx >= 21 && sidx < 85)
             v3 = (lane == sidx - 21) ? inf : v3;
+        else if (sidx >= 85 && sidx < 341)
+            set4((sidx - 85) >> 2, (sidx - 85) & 3, inf, 0u);
     }
 
     // one level: children of offset o held in (cv_reg, ci_reg); on a move the
@@ -1081,8 +1119,34 @@ struct RegHeap {
         }
         o = q;
         lowv = mu;
-        // level 3 -> 4 and below, in LDS (all lanes run the same uniform steps)
-        const int i = 21 + o;
+        // level 3 -> 4: lane o holds the four children; their minimum (the
+        // first of equal values) in that lane
+        {
+            double mloc = v4[0];
+            int cloc = 0;
+#pragma unroll
+            for (int c = 1; c < 4; ++c)
+                if (v4[c] < mloc) {
+                    mloc = v4[c];
+                    cloc = c;
+                }
+            mu = bcast_lane(mloc, o);
+            if (!(lowv > mu)) {
+                v3 = (lane == o) ? key : v3;
+                i3 = (lane == o) ? id : i3;
+                return;
+            }
+            q = __builtin_amdgcn_readlane(cloc, o);
+        }
+        {
+            const uint32_t cid = __builtin_amdgcn_readlane(i4at(q), o);
+            v3 = (lane == o) ? mu : v3;
+            i3 = (lane == o) ? cid : i3;
+        }
+        const int o4 = o, c4 = q;
+        lowv = mu;
+        // level 4 -> 5 and below, in LDS (all lanes run the same uniform steps)
+        const int i = 85 + 4 * o4 + c4;
         const int first = 4 * i + 1;
         double cv[4];
         uint32_t ci[4];
@@ -1105,16 +1169,14 @@ struct RegHeap {
             }
         }
         if (lowest < 0) {
-            v3 = (lane == o) ? key : v3;
-            i3 = (lane == o) ? id : i3;
+            set4(o4, c4, key, id);
             return;
         }
         uint32_t cid = ci[0];
 #pragma unroll
         for (int c = 1; c < 4; ++c)
             cid = (lowest == c) ? ci[c] : cid;
-        v3 = (lane == o) ? lowv : v3;
-        i3 = (lane == o) ? cid : i3;
+        set4(o4, c4, lowv, cid);
         __builtin_amdgcn_wave_barrier();
         heapify_cont(hv, hid, first + lowest, n, lowv, key, id);
         __builtin_amdgcn_wave_barrier();
@@ -2447,14 +2509,43 @@ __device__ inline void sqrt_and_inv(double a, double& root, double& inv)
     inv = y;
 }
 
+// One column step of the diagonal-block factorisation (rows in lanes): the
+// pivot of column cc, L[r][cc] for the rows below, the forward-substitution
+// entry, and the rank-1 update of columns cc+1 .. kEnd-1.
+template <int kEnd>
+__device__ __forceinline__ void diag_column(double (&row)[kNB], double& yr, double& myinv, bool& fail, int cc,
+                                            int lane)
+{
+    const double piv = bcast_lane(row[cc], cc);
+    fail = fail || (piv < 0.0);  // finish the (now meaningless) block, report after
+    // piv < 0 is reported after the block; piv == 0 gives non-finite
+    // entries as a division by a zero pivot would
+    double l, inv;
+    sqrt_and_inv(piv, l, inv);
+    const double ycc = bcast_lane(yr, cc) * inv;
+    const double scaled = inv * row[cc];
+    const bool below = lane > cc;
+    const bool me = lane == cc;
+    row[cc] = me ? l : (below ? scaled : row[cc]);
+    myinv = me ? inv : myinv;
+    yr = me ? ycc : (below ? fma(-scaled, ycc, yr) : yr);
+    // L[c2][cc] lives in lane c2: broadcast by v_readlane
+#pragma unroll
+    for (int c2 = cc + 1; c2 < kEnd; ++c2)
+        row[c2] = fma(-row[cc], bcast_lane(row[cc], c2), row[c2]);
+}
+
 // Factors the 32 x 32 block held in Dg (row-major, lower part meaningful) in
 // wave 0 and solves its part of L y = b: on return Dg holds L11, invd /
-// invall the reciprocal pivots, yv[jb..jb+31] the solution. Column cc + 1 is
-// updated first, so the square root of the next pivot overlaps the rest of
-// column cc's rank-1 update.
-__device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double* invd, double* invall, double* yv,
-                                                   int jb, int lane, int* s_fail)
+// invall the reciprocal pivots, yv[jb..jb+31] the solution. Two 16-column
+// halves: the first is factored with rows in lanes, then the lower-right
+// quarter takes A22 -= L21 L21^T on FP64 MFMA (Dq holds the product), then
+// the second half is factored.
+__device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double (*Dq)[17], double* invd,
+                                                   double* invall, double* yv, int jb, int lane, int* s_fail)
 {
+    constexpr int kHalf = kNB / 2;
+    static_assert(kHalf == 16, "one 16 x 16 MFMA tile per quarter");
     double row[kNB];
     const bool act = lane < kNB;
     const int lr = act ? lane : 0;
@@ -2466,30 +2557,45 @@ __device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double
     double yr = yv[jb + lr];
     double myinv = 1.0;
     bool fail = false;
-    double piv = bcast_lane(row[0], 0);
-    double l, inv;
-    sqrt_and_inv(piv, l, inv);
 #pragma unroll
-    for (int cc = 0; cc < kNB; ++cc) {
-        fail = fail || (piv < 0.0);  // finish the (now meaningless) block, report after
-        // piv < 0 is reported after the block; piv == 0 gives non-finite
-        // entries as a division by a zero pivot would
-        const double ycc = bcast_lane(yr, cc) * inv;
-        const double scaled = inv * row[cc];
-        const bool below = lane > cc;
-        const bool me = lane == cc;
-        row[cc] = me ? l : (below ? scaled : row[cc]);
-        myinv = me ? inv : myinv;
-        yr = me ? ycc : (below ? fma(-scaled, ycc, yr) : yr);
-        if (cc + 1 < kNB) {
-            // L[c2][cc] lives in lane c2: broadcast by v_readlane
-            row[cc + 1] = fma(-row[cc], bcast_lane(row[cc], cc + 1), row[cc + 1]);
-            piv = bcast_lane(row[cc + 1], cc + 1);
-            sqrt_and_inv(piv, l, inv);
+    for (int cc = 0; cc < kHalf; ++cc) {
+        diag_column<kHalf>(row, yr, myinv, fail, cc, lane);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // A22 -= L21 L21^T: L21 (rows 16..31, columns 0..15) through LDS into the
+    // MFMA operands (A[m][k] = B[k][m] = L21[m][k]), the product back through Dq
+    if (lane >= kHalf && lane < kNB) {
 #pragma unroll
-            for (int c2 = cc + 2; c2 < kNB; ++c2)
-                row[c2] = fma(-row[cc], bcast_lane(row[cc], c2), row[c2]);
+        for (int c = 0; c < kHalf; ++c)
+            Dg[lane][c] = row[c];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    {
+        const int l16 = lane & 15, kq = lane >> 4;
+        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < kHalf; k0 += 4) {
+            const double av = Dg[kHalf + l16][k0 + kq];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, av, acc, 0, 0, 0);
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            Dq[4 * r + kq][l16] = acc[r];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    {
+        const int m = (lane >= kHalf && lane < kNB) ? lane - kHalf : 0;
+#pragma unroll
+        for (int c = 0; c < kHalf; ++c)
+            row[kHalf + c] = row[kHalf + c] - Dq[m][c];
+    }
+#pragma unroll
+    for (int cc = kHalf; cc < kNB; ++cc) {
+        diag_column<kNB>(row, yr, myinv, fail, cc, lane);
         __builtin_amdgcn_sched_barrier(0);
     }
     if (fail) {
@@ -2532,6 +2638,7 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
     constexpr int kPS = kNB + 1;                    // row stride of the panel in LDS
     double* P21 = lds;                              // rows below the diagonal block: P21[r * kPS + c]
     __shared__ double Dg[kNB][kNB + 1];             // L11 of the current panel, row-major (lower)
+    __shared__ double Dq[16][17];                   // the diagonal factor's MFMA quarter update
     __shared__ double invd[kNB];                    // 1 / L11[c][c]
     __shared__ double invall[kMaxNewtonD];          // 1 / L[i][i] for the back solve
     __shared__ double yv[kMaxNewtonD];              // right-hand side -> y -> dx
@@ -2656,6 +2763,7 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
         // matrix) goes to wave 0, which updates and factors it; waves 1..7
         // apply the rest of the trailing update
         if (wave == 0) {
+            PROBE_T(t_w0);
             if (jb >= 0) {
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
@@ -2686,8 +2794,9 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            PROBE_ADD(7, t_w0);
             PROBE_T(t_dg);
-            newton_diag_factor(Dg, invd, invall, yv, jb + kNB, lane, &s_fail);
+            newton_diag_factor(Dg, Dq, invd, invall, yv, jb + kNB, lane, &s_fail);
             PROBE_ADD(2, t_dg);
         } else if (jb >= 0) {
             PROBE_T(t_sy);
@@ -2757,6 +2866,10 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
                 }
             }
             PROBE_ADD_AT(6, t_sy, 64);
+#ifdef FEDNL_PROBE
+            if (lane == 0)
+                atomicAdd(&g_probe[8 + wave], wall_clock64() - t_sy);
+#endif
         }
         __syncthreads();
         PROBE_ADD(4, t_b);
