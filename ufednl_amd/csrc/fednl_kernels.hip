@@ -941,23 +941,69 @@ __device__ inline void heapify_cont(double* hv, uint32_t* hid, int i, int n, dou
 }
 
 // ---------------------------------------------------------------------------
-// The top four levels of the 4-ary heap (slots 0..84) held in the registers
+// The top five levels of the 4-ary heap (slots 0..340) held in the registers
 // of one wave: slot 0 in every lane (r0v, r0i), level 1 (slots 1..4) in lanes
 // 0..3 of (v1, i1), level 2 (slots 5..20) in lanes 0..15 of (v2, i2), level 3
-// (slots 21..84) in lanes 0..63 of (v3, i3). The children of the level-L slot
-// with offset o are the lanes 4o..4o+3 of level L+1, one DPP quad. Deeper
-// slots stay in LDS. admit() is dopt's increaseMinimumInMinHeap + minHeapify
-// (MinHeap.h:78-85, 255-290) with the same comparison sequence: the child that
-// moves up is the first of its group holding the group minimum, when that
-// minimum is below the running comparison value.
+// (slots 21..84) in lanes 0..63 of (v3, i3), level 4 (slots 85..340) four per
+// lane in (v4, i4): lane o holds the children of level-3 slot 21 + o. The
+// children of the level-L slot with offset o are the lanes 4o..4o+3 of level
+// L+1, one DPP quad. Deeper slots stay in LDS.
+//
+// Every child group's minimum and the first child holding it are kept beside
+// the values (m1/a1 for level 1, g2/b2 and g3/b3 per quad, g4/c4 per lane), so
+// one level of a sift is a lane read and a compare. admit() is dopt's
+// increaseMinimumInMinHeap + minHeapify (MinHeap.h:78-85, 255-290) with the
+// same comparison sequence: the child that moves up is the first of its group
+// holding the group minimum, when that minimum is below the running
+// comparison value. Most admissions stop at the root or one level below it.
 // ---------------------------------------------------------------------------
 struct RegHeap {
-    // slot 0 in every lane; level 1 (slots 1..4) in lanes 0..3; level 2
-    // (5..20) in lanes 0..15; level 3 (21..84) one per lane; level 4
-    // (85..340) four per lane: lane o holds the children of level-3 slot 21 + o
     double r0v, v1, v2, v3, v4[4];
     uint32_t r0i, i1, i2, i3, i4[4];
+    // group minima: m1 = min of level 1 (uniform), a1 its first lane; g2, g3 =
+    // minimum of the lane's quad, b2, b3 = ballot of lanes equal to it; g4 =
+    // minimum of the lane's four level-4 slots, c4 the first of them
+    double m1, g2, g3, g4;
+    uint64_t b2, b3;
+    int a1, c4;
     int n, lane;
+
+    __device__ void refresh1()
+    {
+        const double t = quad_min(v1);
+        m1 = bcast_lane(t, 0);
+        a1 = __builtin_ctzll(__ballot(v1 == m1) & 0xFull);
+    }
+    __device__ void refresh2()
+    {
+        g2 = quad_min(v2);
+        b2 = __ballot(v2 == g2);
+    }
+    __device__ void refresh3()
+    {
+        g3 = quad_min(v3);
+        b3 = __ballot(v3 == g3);
+    }
+    __device__ void refresh4()
+    {
+        double mloc = v4[0];
+        int cloc = 0;
+#pragma unroll
+        for (int c = 1; c < 4; ++c)
+            if (v4[c] < mloc) {
+                mloc = v4[c];
+                cloc = c;
+            }
+        g4 = mloc;
+        c4 = cloc;
+    }
+    __device__ void refresh_all()
+    {
+        refresh4();
+        refresh3();
+        refresh2();
+        refresh1();
+    }
 
     __device__ void load(const double* hv, const uint32_t* hid, int n_, int lane_)
     {
@@ -978,6 +1024,7 @@ struct RegHeap {
             v4[c] = (sl < n) ? hv[sl] : inf;
             i4[c] = (sl < n) ? hid[sl] : 0u;
         }
+        refresh_all();
     }
 
     __device__ void store(double* hv, uint32_t* hid) const
@@ -1053,100 +1100,92 @@ struct RegHeap {
     __device__ void kill(int sidx)
     {
         const double inf = __builtin_inf();
-        if (sidx >= 1 && sidx < 5)
+        if (sidx >= 1 && sidx < 5) {
             v1 = (lane == sidx - 1) ? inf : v1;
-        else if (sidx >= 5 && sidx < 21)
+            refresh1();
+        } else if (sidx >= 5 && sidx < 21) {
             v2 = (lane == sidx - 5) ? inf : v2;
-        else if (sidx >= 21 && sidx < 85)
+            refresh2();
+        } else if (sidx >= 21 && sidx < 85) {
             v3 = (lane == sidx - 21) ? inf : v3;
-        else if (sidx >= 85 && sidx < 341)
+            refresh3();
+        } else if (sidx >= 85 && sidx < 341) {
             set4((sidx - 85) >> 2, (sidx - 85) & 3, inf, 0u);
-    }
-
-    // one level: children of offset o held in (cv_reg, ci_reg); on a move the
-    // minimum and its lane come back in (mu, q)
-    __device__ bool step(double cv_reg, int o, double lowv, double& mu, int& q) const
-    {
-        const bool mine = (lane >> 2) == o;
-        const double cv = mine ? cv_reg : __builtin_inf();
-        const double mn = quad_min(cv);
-        mu = bcast_lane(mn, 4 * o);
-        if (!(lowv > mu))
-            return false;
-        const uint64_t eq = __ballot(mine && cv == mu);
-        q = __builtin_ctzll(eq);
-        return true;
+            refresh4();
+        }
     }
 
     // slot 0 receives (key, id) and sifts down
     __device__ void admit(double* hv, uint32_t* hid, double key, uint32_t id)
     {
-        double mu;
-        int q;
-        // level 0 -> 1
-        if (!step(v1, 0, key, mu, q)) {
+        if (key > m1)
+            admit_deep(hv, hid, key, id);
+        else {
             r0v = key;
             r0i = id;
-            return;
         }
-        r0v = mu;
-        r0i = __builtin_amdgcn_readlane(i1, q);
-        int o = q;
-        double lowv = mu;
+    }
+
+    // admit() for a key above the level-1 minimum: the root takes that
+    // minimum and the sift continues below it
+    __device__ void admit_deep(double* hv, uint32_t* hid, double key, uint32_t id)
+    {
+        // level 0 -> 1
+        const int o1 = a1;
+        r0v = m1;
+        r0i = __builtin_amdgcn_readlane(i1, o1);
+        double lowv = m1;
         // level 1 -> 2
-        if (!step(v2, o, lowv, mu, q)) {
-            v1 = (lane == o) ? key : v1;
-            i1 = (lane == o) ? id : i1;
+        double mu = bcast_lane(g2, 4 * o1);
+        if (!(lowv > mu)) {
+            v1 = (lane == o1) ? key : v1;
+            i1 = (lane == o1) ? id : i1;
+            refresh1();
             return;
         }
+        const int o2 = 4 * o1 + __builtin_ctzll((b2 >> (4 * o1)) & 0xFull);
         {
-            const uint32_t cid = __builtin_amdgcn_readlane(i2, q);
-            v1 = (lane == o) ? mu : v1;
-            i1 = (lane == o) ? cid : i1;
+            const uint32_t cid = __builtin_amdgcn_readlane(i2, o2);
+            v1 = (lane == o1) ? mu : v1;
+            i1 = (lane == o1) ? cid : i1;
         }
-        o = q;
         lowv = mu;
         // level 2 -> 3
-        if (!step(v3, o, lowv, mu, q)) {
-            v2 = (lane == o) ? key : v2;
-            i2 = (lane == o) ? id : i2;
+        mu = bcast_lane(g3, 4 * o2);
+        if (!(lowv > mu)) {
+            v2 = (lane == o2) ? key : v2;
+            i2 = (lane == o2) ? id : i2;
+            refresh2();
+            refresh1();
             return;
         }
+        const int o3 = 4 * o2 + __builtin_ctzll((b3 >> (4 * o2)) & 0xFull);
         {
-            const uint32_t cid = __builtin_amdgcn_readlane(i3, q);
-            v2 = (lane == o) ? mu : v2;
-            i2 = (lane == o) ? cid : i2;
+            const uint32_t cid = __builtin_amdgcn_readlane(i3, o3);
+            v2 = (lane == o2) ? mu : v2;
+            i2 = (lane == o2) ? cid : i2;
         }
-        o = q;
         lowv = mu;
-        // level 3 -> 4: lane o holds the four children; their minimum (the
-        // first of equal values) in that lane
-        {
-            double mloc = v4[0];
-            int cloc = 0;
-#pragma unroll
-            for (int c = 1; c < 4; ++c)
-                if (v4[c] < mloc) {
-                    mloc = v4[c];
-                    cloc = c;
-                }
-            mu = bcast_lane(mloc, o);
-            if (!(lowv > mu)) {
-                v3 = (lane == o) ? key : v3;
-                i3 = (lane == o) ? id : i3;
-                return;
-            }
-            q = __builtin_amdgcn_readlane(cloc, o);
+        // level 3 -> 4: the children of level-3 slot o3 are lane o3's four
+        // level-4 entries
+        mu = bcast_lane(g4, o3);
+        if (!(lowv > mu)) {
+            v3 = (lane == o3) ? key : v3;
+            i3 = (lane == o3) ? id : i3;
+            refresh3();
+            refresh2();
+            refresh1();
+            return;
         }
+        const int c4s = __builtin_amdgcn_readlane(c4, o3);
         {
-            const uint32_t cid = __builtin_amdgcn_readlane(i4at(q), o);
-            v3 = (lane == o) ? mu : v3;
-            i3 = (lane == o) ? cid : i3;
+            const uint32_t cid = __builtin_amdgcn_readlane(i4at(c4s), o3);
+            v3 = (lane == o3) ? mu : v3;
+            i3 = (lane == o3) ? cid : i3;
         }
-        const int o4 = o, c4 = q;
         lowv = mu;
         // level 4 -> 5 and below, in LDS (all lanes run the same uniform steps)
-        const int i = 85 + 4 * o4 + c4;
+        const int i = 85 + 4 * o3 + c4s;
         const int first = 4 * i + 1;
         double cv[4];
         uint32_t ci[4];
@@ -1169,17 +1208,21 @@ struct RegHeap {
             }
         }
         if (lowest < 0) {
-            set4(o4, c4, key, id);
-            return;
-        }
-        uint32_t cid = ci[0];
+            set4(o3, c4s, key, id);
+        } else {
+            uint32_t cid = ci[0];
 #pragma unroll
-        for (int c = 1; c < 4; ++c)
-            cid = (lowest == c) ? ci[c] : cid;
-        set4(o4, c4, lowv, cid);
-        __builtin_amdgcn_wave_barrier();
-        heapify_cont(hv, hid, first + lowest, n, lowv, key, id);
-        __builtin_amdgcn_wave_barrier();
+            for (int c = 1; c < 4; ++c)
+                cid = (lowest == c) ? ci[c] : cid;
+            set4(o3, c4s, lowv, cid);
+            __builtin_amdgcn_wave_barrier();
+            heapify_cont(hv, hid, first + lowest, n, lowv, key, id);
+            __builtin_amdgcn_wave_barrier();
+        }
+        refresh4();
+        refresh3();
+        refresh2();
+        refresh1();
     }
 };
 
@@ -1452,7 +1495,13 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
 #ifdef FEDNL_PROBE
                             const unsigned long long ta = clock64();
 #endif
-                            rh.admit(hv, hid, key, (uint32_t)(sbase + e0 + f));
+                            const uint32_t kid = (uint32_t)(sbase + e0 + f);
+                            if (key > rh.m1)
+                                rh.admit_deep(hv, hid, key, kid);
+                            else {
+                                rh.r0v = key;
+                                rh.r0i = kid;
+                            }
 #ifdef FEDNL_PROBE
                             adm_cnt += 1;
                             adm_cyc += clock64() - ta;
