@@ -90,6 +90,8 @@ constexpr int kTile = 64;          // Hessian output tile (rows and columns)
 constexpr int kKC = 16;            // samples staged per LDS chunk
 constexpr int kPadLds = 4;         // LDS row padding (doubles)
 constexpr int kMaxHeapK = 8192;    // LDS-resident heap limit
+constexpr int kScanGroup = 8;      // TopK scan: chunks of 64 positions per staged group
+static_assert(kScanGroup % 2 == 0, "the scan tests chunk pairs");
 constexpr int kMaxClients = 1024;  // master scatter client table
 // per-round trace record: ||grad f||, f, ||x||, factorization flag,
 // line-search trials, participating clients, Hessian items they sent
@@ -135,6 +137,15 @@ __device__ inline double natural_round(double v)
 {
     return __longlong_as_double(__double_as_longlong(v) & (long long)0xFFF0000000000000ull);
 }
+
+// the TopK tail's bitmap of S list positions fits in the heap's k + 4 values
+__host__ __device__ inline bool select_bitmap_fits(size_t S, int k)
+{
+    return ((S + 31) / 32) * 4 <= (size_t)8 * (k + 4);
+}
+
+// global-memory view of a double array (global_load rather than flat_load)
+typedef __attribute__((address_space(1))) const double gdouble;
 
 // value of a double held by lane `src` (uniform), without an LDS round trip
 __device__ inline double bcast_lane(double v, int src)
@@ -204,7 +215,7 @@ __device__ inline uint64_t round_nearest(double v)
 // 8 so that blockIdx % 8 still names the XCD).
 // ---------------------------------------------------------------------------
 struct ProblemDev {
-    int n, m, d, k, lda, m_pad, d_pad, tiles_up, dp, workers, comp, hmode, option_b, tracking, seg_len, bm_words,
+    int n, m, d, k, lda, m_pad, d_pad, tiles_up, dp, workers, comp, hmode, option_b, tracking, bm_words,
         trace_cap, update_x, ls, ls_max, pp, pp_tau;
     size_t S;
     double lambda, alpha, prescale, inv_n, mult, lek_delta, ls_c, ls_gamma;
@@ -1378,7 +1389,7 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
 {
     const double* __restrict__ delta = PD.delta;
     const size_t S = PD.S;
-    const int k = PD.k, seg_len = PD.seg_len, tiles_up = PD.tiles_up;
+    const int k = PD.k, tiles_up = PD.tiles_up;
     const double* __restrict__ part = PD.part;
     const double lek_delta = PD.lek_delta;
     double* __restrict__ lk_out = nullptr;
@@ -1415,9 +1426,13 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
         return;
     }
     const int P = pow2_at_least(k);
+    const size_t words = (S + 31) / 32;
+    // the tail orders the selection through a bitmap in the value area, or
+    // (when that is too small) by a bitonic sort of P ids
+    const bool bm_tail = select_bitmap_fits(S, k);
     double* hv = (double*)smem;                                // k + 4 (padding read by heapify)
-    uint32_t* hid = (uint32_t*)(hv + k + 4);                   // P + 4
-    double* seg = (double*)(hid + round_up(P + 4, 2));
+    uint32_t* hid = (uint32_t*)(hv + k + 4);                   // k + 4, or P + 4 for the sort
+    double* ring = (double*)(hid + round_up((bm_tail ? k : P) + 4, 2));  // 2 * kScanGroup * 64
 
     stage_abs(hv, dc, k, tid, blockDim.x);
     for (int q = tid; q < k; q += blockDim.x)
@@ -1441,104 +1456,102 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
 
     PROBE_ADD(1000 + 8 * c + 0, t_sel0);
     PROBE_T(t_sel1);
-    // scan of list positions k .. S-1 in segments of seg_len: wave 0 scans the
-    // current segment while waves 1-3 load the next one (all loads in flight)
-    const size_t rest = S - (size_t)k;
-    const int nseg = (int)((rest + seg_len - 1) / seg_len);
-    {
-        const int len0 = (int)min((size_t)seg_len, rest);
-        stage_abs(seg, dc + k, len0, tid, blockDim.x);
-    }
-    __syncthreads();
+    // scan of list positions k .. S-1 by wave 0 alone, in groups of
+    // kScanGroup chunks of 64 positions. The next group's global loads are in
+    // flight in registers while the current group, staged in LDS, is tested:
+    // one ballot per chunk against the current root (the root only grows, so
+    // a position not above it is never admitted), then the admissions in
+    // list order.
     RegHeap rh;
-    if (wave == 0)
+    if (wave == 0) {
         rh.load(hv, hid, k, lane);
 #ifdef FEDNL_PROBE
-    const unsigned long long tscan = clock64();
-    unsigned long long adm_cnt = 0, adm_cyc = 0, adm_wait = 0;
+        const unsigned long long tscan = clock64();
+        unsigned long long adm_cnt = 0, adm_cyc = 0;
 #endif
-    for (int sgi = 0; sgi < nseg; ++sgi) {
-        const double* cur = seg + (size_t)(sgi & 1) * seg_len;
-        const size_t sbase = (size_t)k + (size_t)sgi * seg_len;
-        const int slen = (int)min((size_t)seg_len, S - sbase);
-        if (wave == 0) {
-            // chunks of 256 positions, the next chunk's loads in flight while
-            // the current one is tested
-            double v4[4], nx[4];
+        const gdouble* __restrict__ src = (const gdouble*)(dc + k);
+        const int rest = (int)(S - (size_t)k);
+        const int ngroups = (rest + kScanGroup * 64 - 1) / (kScanGroup * 64);
+        double ra[kScanGroup];
+        // group 0 to ring slot 0, group 1's loads in flight
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int e = 64 * j + lane;
-                const double t = cur[e < slen ? e : slen - 1];
-                v4[j] = (e < slen) ? t : -1.0;
-            }
-            for (int e4 = 0; e4 < slen; e4 += 256) {
+        for (int j = 0; j < kScanGroup; ++j) {
+            const int e = 64 * j + lane;
+            ra[j] = src[e < rest ? e : rest - 1];  // clamped; masked when tested
+        }
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int e = e4 + 256 + 64 * j + lane;
-                    const double t = cur[e < slen ? e : slen - 1];
-                    nx[j] = (e < slen) ? t : -1.0;
-                }
-                // one ballot for 256 positions: the root only grows, so a position
-                // not above it now is never admitted in this pass
-                double rv = rh.r0v;
-                const bool any = (v4[0] > rv) || (v4[1] > rv) || (v4[2] > rv) || (v4[3] > rv);
-                if (__ballot(any) != 0ull) {
+        for (int j = 0; j < kScanGroup; ++j) ring[64 * j + lane] = ra[j];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const double v = v4[j];
-                        const int e0 = e4 + 64 * j;
-                        uint64_t mask = __ballot(v > rv);
-                        while (mask) {
-                            const int f = __builtin_ctzll(mask);
-                            const double key = bcast_lane(v, f);
-                            // increaseMinimumInMinHeap (MinHeap.h:78-85)
+        for (int j = 0; j < kScanGroup; ++j) {
+            const int e = 64 * (kScanGroup + j) + lane;
+            ra[j] = src[e < rest ? e : rest - 1];
+        }
+        // the admissions of one chunk (positions base .. base + 63 in v)
+        auto scan_chunk = [&](double v, int base) {
+            double rv = rh.r0v;
+            uint64_t mask = __ballot(v > rv);
+            while (mask) {
+                const int f = __builtin_ctzll(mask);
+                const double key = bcast_lane(v, f);
+                const uint32_t kid = (uint32_t)(k + base + f);
 #ifdef FEDNL_PROBE
-                            const unsigned long long ta = clock64();
+                const unsigned long long ta = clock64();
 #endif
-                            const uint32_t kid = (uint32_t)(sbase + e0 + f);
-                            if (key > rh.m1)
-                                rh.admit_deep(hv, hid, key, kid);
-                            else {
-                                rh.r0v = key;
-                                rh.r0i = kid;
-                            }
-#ifdef FEDNL_PROBE
-                            adm_cnt += 1;
-                            adm_cyc += clock64() - ta;
-#endif
-                            rv = rh.r0v;
-                            const uint64_t above = (f == 63) ? 0ull : (~0ull << (f + 1));
-                            mask = __ballot(v > rv) & above;
-                        }
-                    }
+                // increaseMinimumInMinHeap (MinHeap.h:78-85)
+                if (key > rh.m1)
+                    rh.admit_deep(hv, hid, key, kid);
+                else {
+                    rh.r0v = key;
+                    rh.r0i = kid;
                 }
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    v4[j] = nx[j];
+#ifdef FEDNL_PROBE
+                adm_cnt += 1;
+                adm_cyc += clock64() - ta;
+#endif
+                rv = rh.r0v;
+                const uint64_t above = (f == 63) ? 0ull : (~0ull << (f + 1));
+                mask = __ballot(v > rv) & above;
             }
-        } else if (sgi + 1 < nseg) {
-            double* nxt = seg + (size_t)((sgi + 1) & 1) * seg_len;
-            const size_t nbase = sbase + seg_len;
-            const int nlen = (int)min((size_t)seg_len, S - nbase);
-            stage_abs(nxt, dc + nbase, nlen, tid - 64, blockDim.x - 64);
+        };
+        for (int g = 0; g < ngroups; ++g) {
+            const double* cur = ring + (g & 1) * (kScanGroup * 64);
+            const int gbase = g * kScanGroup * 64;
+            // chunk pairs, the next pair's LDS reads in flight
+            double n0 = cur[lane], n1 = cur[64 + lane];
+            for (int j = 0; j < kScanGroup; j += 2) {
+                const int e0 = gbase + 64 * j + lane;
+                const double v0 = (e0 < rest) ? fabs(n0) : -1.0;
+                const double v1 = (e0 + 64 < rest) ? fabs(n1) : -1.0;
+                if (j + 2 < kScanGroup) {
+                    n0 = cur[64 * (j + 2) + lane];
+                    n1 = cur[64 * (j + 3) + lane];
+                }
+                const double rv = rh.r0v;
+                if (__ballot(v0 > rv || v1 > rv) == 0ull)
+                    continue;
+                scan_chunk(v0, gbase + 64 * j);
+                scan_chunk(v1, gbase + 64 * j + 64);
+            }
+            // group g+1 (arrived during the tests) to the other slot, group
+            // g+2's loads in flight
+            double* nxt = ring + ((g + 1) & 1) * (kScanGroup * 64);
+#pragma unroll
+            for (int j = 0; j < kScanGroup; ++j) nxt[64 * j + lane] = ra[j];
+            const int nbase = (g + 2) * kScanGroup * 64;
+#pragma unroll
+            for (int j = 0; j < kScanGroup; ++j) {
+                const int e = nbase + 64 * j + lane;
+                ra[j] = src[e < rest ? e : rest - 1];
+            }
         }
 #ifdef FEDNL_PROBE
-        const unsigned long long tw0 = clock64();
-#endif
-        __syncthreads();
-#ifdef FEDNL_PROBE
-        if (tid == 0)
-            adm_wait += clock64() - tw0;
+        if (lane == 0) {
+            g_probe[1000 + 8 * c + 5] += clock64() - tscan;
+            g_probe[1000 + 8 * c + 3] += adm_cnt;
+            g_probe[1000 + 8 * c + 4] += adm_cyc;
+        }
 #endif
     }
-#ifdef FEDNL_PROBE
-    if (tid == 0) {
-        g_probe[1000 + 8 * c + 5] += clock64() - tscan;
-        g_probe[1000 + 8 * c + 3] += adm_cnt;
-        g_probe[1000 + 8 * c + 4] += adm_cyc;
-        g_probe[1000 + 8 * c + 6] += adm_wait;
-    }
-#endif
     if (wave == 0)
         rh.store(hv, hid);
     __syncthreads();
@@ -1613,10 +1626,10 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
         cnt = s_cnt;
     }
 
-    const size_t words = (S + 31) / 32;
-    if (words * 4 <= (size_t)seg_len * 16) {
-        // ascending order through a bitmap of the selected list positions
-        uint32_t* bm = (uint32_t*)seg;
+    if (bm_tail) {
+        // ascending order through a bitmap of the selected list positions, in
+        // the heap value area (not read past this point)
+        uint32_t* bm = (uint32_t*)hv;
         for (size_t w = tid; w < words; w += blockDim.x)
             bm[w] = 0u;
         __syncthreads();
@@ -3113,7 +3126,6 @@ struct Engine {
     double kms[KID_COUNT] = {0};
     int klaunch[KID_COUNT] = {0};
     size_t lds_select = 0, lds_newton = 0;
-    int seg_len = 256;
     int hmode = 0;                 // 0 FP64, 1 FP32, 2 BF16 split (A^T D A only)
     ProblemDev hdesc{};            // descriptor as last uploaded
     ProblemDev* pdev = nullptr;    // device copy read by every round kernel
@@ -3196,7 +3208,7 @@ static ProblemDev make_desc(const Engine* e)
     ProblemDev P{};
     P.n = e->n; P.m = e->m; P.d = e->d; P.k = e->k; P.lda = e->lda; P.m_pad = e->m_pad; P.d_pad = e->d_pad;
     P.tiles_up = e->tiles_up; P.dp = e->dp; P.workers = e->workers; P.comp = e->o.compressor; P.hmode = e->hmode;
-    P.option_b = e->o.option_b; P.tracking = e->o.tracking; P.seg_len = e->seg_len; P.bm_words = e->bm_words;
+    P.option_b = e->o.option_b; P.tracking = e->o.tracking; P.bm_words = e->bm_words;
     P.trace_cap = e->trace_cap; P.S = e->S;
     P.pp = e->o.clients_per_round > 0 ? 1 : 0; P.pp_tau = e->o.clients_per_round;
     // x -= dx (FedNL), x = solution (FedNL-PP), x left to the line search (FedNL-LS)
@@ -3339,23 +3351,11 @@ static int launch_round(Engine* e)
     return FEDNL_OK;
 }
 
-constexpr size_t kLdsBudget = 160 * 1024 - 4096;  // dynamic LDS of the selection kernels
-
-// staging segment of the heap scan: what is left of the LDS budget after the
-// heap, two buffers, a multiple of 256 list positions, at most 8192
-static int select_seg_len(int k)
+static size_t select_lds_bytes(int k, size_t S)
 {
     const int P = pow2_at_least(k);
-    const size_t heap = (size_t)8 * (k + 4) + (size_t)4 * round_up(P + 4, 2);
-    const size_t left = kLdsBudget > heap ? kLdsBudget - heap : 0;
-    int seg = (int)std::min<size_t>(8192, left / 16 / 256 * 256);
-    return std::max(seg, 256);
-}
-
-static size_t select_lds_bytes(int k)
-{
-    const int P = pow2_at_least(k);
-    const size_t heap = (size_t)8 * (k + 4) + (size_t)4 * round_up(P + 4, 2) + (size_t)16 * select_seg_len(k);
+    const int ids = select_bitmap_fits(S, k) ? k : P;
+    const size_t heap = (size_t)8 * (k + 4) + (size_t)4 * round_up(ids + 4, 2) + (size_t)8 * 2 * kScanGroup * 64;
     const size_t rk = (size_t)8 * P + (size_t)8 * k;
     return std::max(heap, rk);
 }
@@ -3485,8 +3485,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     e->prescale = e->inv_n * e->alpha;
     e->mult = (double)e->S / (double)e->k;
     e->lek_delta = (double)e->k * (1.0 / (double)e->S);
-    e->lds_select = select_lds_bytes(std::min<int>(e->k, kMaxHeapK));
-    e->seg_len = select_seg_len(std::min<int>(e->k, kMaxHeapK));
+    e->lds_select = select_lds_bytes(std::min<int>(e->k, kMaxHeapK), e->S);
     e->lds_newton = newton_lds_bytes(e->dp);
 
     int rc = FEDNL_OK;
@@ -4464,8 +4463,7 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     hipMemsetAsync(hit, 0, sizeof(int) * nclients, st);
     if (rng) hipMemcpyAsync(drng, rng, sizeof(uint32_t) * nclients, hipMemcpyHostToDevice, st);
     if (toplek_rng) hipMemcpyAsync(dlek, toplek_rng, sizeof(uint32_t) * workers, hipMemcpyHostToDevice, st);
-    const size_t lds = select_lds_bytes(std::min(k, kMaxHeapK));
-    const int seg_len = select_seg_len(std::min(k, kMaxHeapK));
+    const size_t lds = select_lds_bytes(std::min(k, kMaxHeapK), S);
     if (lds > 64 * 1024) {
         hipFuncSetAttribute((const void*)k_select_heap<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);
@@ -4538,7 +4536,6 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     hp.k = k;
     hp.S = S;
     hp.lda = round_up(d, 4);
-    hp.seg_len = seg_len;
     hp.tiles_up = (compressor == FEDNL_TOPLEK) ? 1 : 0;
     hp.part = parts;
     hp.lek_delta = delta_param;
