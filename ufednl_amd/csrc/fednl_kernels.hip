@@ -144,8 +144,26 @@ __host__ __device__ inline bool select_bitmap_fits(size_t S, int k)
     return ((S + 31) / 32) * 4 <= (size_t)8 * (k + 4);
 }
 
-// global-memory view of a double array (global_load rather than flat_load)
-typedef __attribute__((address_space(1))) const double gdouble;
+// Global-memory views of the descriptor's arrays. The pointers in ProblemDev
+// are generic; accesses through them compile to flat instructions, which
+// count against the LDS wait counter too, so every LDS wait would also wait
+// for the flat loads in flight. Kernels that mix LDS work with global
+// prefetches take their arrays through G() and access them with global_*.
+// (The host pass parses the same device code; there the views are plain
+// pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+#else
+template <class T>
+using gmem = T;
+#endif
+template <class T>
+__device__ __forceinline__ gmem<T>* G(T* p)
+{
+    return (gmem<T>*)p;
+}
+typedef gmem<const double> gdouble;
 
 // value of a double held by lane `src` (uniform), without an LDS round trip
 __device__ inline double bcast_lane(double v, int src)
@@ -260,13 +278,13 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
     FEDNL_BLOCK();
     if (kTrial && (!PD.ls || PD.mst[LS_DONE] != 0.0))
         return;
-    const double* __restrict__ A = PD.A;
-    const double* __restrict__ x = PD.x;
+    const gmem<double>* __restrict__ A = G(PD.A);
+    const gmem<double>* __restrict__ x = G(PD.x);
     const int m = PD.m, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad;
     const double inv_m = 1.0 / (double)m;
-    double* __restrict__ Dw = PD.Dw;
-    double* __restrict__ gw = PD.gw;
-    double* __restrict__ nlog_part = PD.nlog;
+    gmem<double>* __restrict__ Dw = G(PD.Dw);
+    gmem<double>* __restrict__ gw = G(PD.gw);
+    gmem<double>* __restrict__ nlog_part = G(PD.nlog);
     const int gx = m_pad / 16, bx = blk % gx, by = blk / gx;
     extern __shared__ double xs[];
     __shared__ double nl[16];
@@ -277,36 +295,44 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
         xs[i] = (i < d) ? x[i] : 0.0;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // four rows per wave (m_pad is a multiple of 16, so all four exist), their
+    // loads interleaved; every lane's sum runs over the same columns in the
+    // same order for each row
     const int s0 = bx * 16 + wave * 4;
-    for (int q = 0; q < 4; ++q) {
-        const int s = s0 + q;
-        if (s >= m_pad)
-            break;
-        const double* row = A + ((size_t)c * m_pad + s) * d_pad;
-        double acc = 0.0;
-        for (int i = lane; i < d_pad; i += 64)
-            acc = fma(row[i], xs[i], acc);
+    const double* row0 = A + ((size_t)c * m_pad + s0) * d_pad;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int i = lane; i < d_pad; i += 64) {
+        const double xi = xs[i];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            acc[q] = fma(row0[(size_t)q * d_pad + i], xi, acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
         for (int off = 32; off > 0; off >>= 1)
-            acc += __shfl_xor(acc, off);
-        if (lane == 0) {
-            const size_t o = (size_t)c * m_pad + s;
-            if (s < m) {
-                // logistic_regression.h:98-101,148-161,398-404
-                const double sg = 1.0 / (1.0 + exp(-acc));
-                if (!kTrial) {
-                    const double z1 = 1.0 - sg;
-                    const double z2 = z1 * sg;
-                    Dw[o] = inv_m * z2;
-                    gw[o] = z1 * (-inv_m);
-                }
-                nl[wave * 4 + q] = -log(sg);
-            } else {
-                if (!kTrial) {
-                    Dw[o] = 0.0;
-                    gw[o] = 0.0;
-                }
-                nl[wave * 4 + q] = 0.0;
+            acc[q] += __shfl_xor(acc[q], off);
+    // every lane holds the four sums; lane q finishes row s0 + q
+    if (lane < 4) {
+        const double t = (lane == 0) ? acc[0] : (lane == 1) ? acc[1] : (lane == 2) ? acc[2] : acc[3];
+        const int s = s0 + lane;
+        const size_t o = (size_t)c * m_pad + s;
+        if (s < m) {
+            // logistic_regression.h:98-101,148-161,398-404
+            const double sg = 1.0 / (1.0 + exp(-t));
+            if (!kTrial) {
+                const double z1 = 1.0 - sg;
+                const double z2 = z1 * sg;
+                Dw[o] = inv_m * z2;
+                gw[o] = z1 * (-inv_m);
             }
+            nl[wave * 4 + lane] = -log(sg);
+        } else {
+            if (!kTrial) {
+                Dw[o] = 0.0;
+                gw[o] = 0.0;
+            }
+            nl[wave * 4 + lane] = 0.0;
         }
     }
     __syncthreads();
@@ -331,10 +357,10 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
 template <bool kInit>
 __device__ __forceinline__ void hess_epilogue(const double (&hval)[16], const int (&iv)[16], const int (&jv)[16],
                                               int c, int d, int d_pad, double lambda, size_t S, int tiles_up, int t,
-                                              const double* __restrict__ x, double* __restrict__ learned,
-                                              double* __restrict__ delta, double* __restrict__ hout,
-                                              double* __restrict__ part,
-                                              double* __restrict__ grad_c, bool with_grad, double gacc, int i0,
+                                              const gmem<double>* __restrict__ x, gmem<double>* __restrict__ learned,
+                                              gmem<double>* __restrict__ delta, gmem<double>* __restrict__ hout,
+                                              gmem<double>* __restrict__ part,
+                                              gmem<double>* __restrict__ grad_c, bool with_grad, double gacc, int i0,
                                               double (*red)[4], double (*gred)[kTile])
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -432,15 +458,15 @@ template <bool kInit>
 __global__ __launch_bounds__(256) void k_hessian_f32(FEDNL_KARGS, double* __restrict__ learned_out)
 {
     FEDNL_BLOCK();
-    const float* __restrict__ A32 = PD.A32;
-    const double* __restrict__ Dw = PD.Dw;
-    const double* __restrict__ x = PD.x;
+    const gmem<float>* __restrict__ A32 = G(PD.A32);
+    const gmem<double>* __restrict__ Dw = G(PD.Dw);
+    const gmem<double>* __restrict__ x = G(PD.x);
     const int n = PD.n, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad, tiles_up = PD.tiles_up;
     const double lambda = PD.lambda;
     const size_t S = PD.S;
-    double* __restrict__ learned = learned_out ? learned_out : PD.learned;
-    double* __restrict__ delta = PD.delta;
-    double* __restrict__ part = PD.part;
+    gmem<double>* __restrict__ learned = G(learned_out ? learned_out : PD.learned);
+    gmem<double>* __restrict__ delta = G(PD.delta);
+    gmem<double>* __restrict__ part = G(PD.part);
     int c, t, i0, j0;
     if (!hess_tile_of_block(blk, n, tiles_up, c, t, i0, j0))
         return;
@@ -460,16 +486,16 @@ __global__ __launch_bounds__(256) void k_hessian_f32(FEDNL_KARGS, double* __rest
 #pragma unroll
         for (int it = 0; it < 2; ++it) acc[jt][it] = flt4{0.f, 0.f, 0.f, 0.f};
 
-    const float* Ac = A32 + (size_t)c * m_pad * d_pad;
-    const double* Dc = Dw + (size_t)c * m_pad;
+    const gmem<float>* Ac = A32 + (size_t)c * m_pad * d_pad;
+    const gmem<double>* Dc = Dw + (size_t)c * m_pad;
     // rows lr of a chunk, columns lc .. lc + 3 of both column blocks
     const int lr = tid >> 4, lc = (tid & 15) * 4;
     float4 ri, rj;
     float rd;
     auto gload = [&](int s0) {
-        const float* row = Ac + (size_t)(s0 + lr) * d_pad;
-        ri = *reinterpret_cast<const float4*>(row + i0 + lc);
-        rj = *reinterpret_cast<const float4*>(row + j0 + lc);
+        const gmem<float>* row = Ac + (size_t)(s0 + lr) * d_pad;
+        ri = *reinterpret_cast<const gmem<float4>*>(row + i0 + lc);
+        rj = *reinterpret_cast<const gmem<float4>*>(row + j0 + lc);
         rd = (float)Dc[s0 + lr];
     };
     auto lstore = [&](int buf) {
@@ -519,7 +545,7 @@ __global__ __launch_bounds__(256) void k_hessian_f32(FEDNL_KARGS, double* __rest
                 iv[q] = i0 + wi * 32 + it * 16 + l16;
                 jv[q] = j0 + wj * 32 + jt * 16 + 4 * kq + r;
             }
-    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, PD.pp_h, part,
+    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, G(PD.pp_h), part,
                          nullptr, false, 0.0, i0, red, gred);
 }
 
@@ -531,6 +557,7 @@ __global__ __launch_bounds__(256) void k_hessian_f32(FEDNL_KARGS, double* __rest
 // accumulation; the lo * lo term is dropped. Output layout as k_hessian_f32.
 // ---------------------------------------------------------------------------
 constexpr int kKB = 32;        // samples per chunk (K of one bf16 MFMA)
+static_assert(kKB % 16 == 0, "k_margin takes m_pad (a multiple of kKB) in blocks of 16 rows");
 constexpr int kKBPad = 40;     // LDS row length in bf16 (80 B rows)
 
 __device__ inline void split_bf16(float v, __bf16& hi, __bf16& lo)
@@ -543,16 +570,16 @@ template <bool kInit>
 __global__ __launch_bounds__(256) void k_hessian_bf16(FEDNL_KARGS, double* __restrict__ learned_out)
 {
     FEDNL_BLOCK();
-    const __bf16* __restrict__ AThi = PD.AThi;
-    const __bf16* __restrict__ ATlo = PD.ATlo;
-    const double* __restrict__ Dw = PD.Dw;
-    const double* __restrict__ x = PD.x;
+    const gmem<__bf16>* __restrict__ AThi = G(PD.AThi);
+    const gmem<__bf16>* __restrict__ ATlo = G(PD.ATlo);
+    const gmem<double>* __restrict__ Dw = G(PD.Dw);
+    const gmem<double>* __restrict__ x = G(PD.x);
     const int n = PD.n, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad, tiles_up = PD.tiles_up;
     const double lambda = PD.lambda;
     const size_t S = PD.S;
-    double* __restrict__ learned = learned_out ? learned_out : PD.learned;
-    double* __restrict__ delta = PD.delta;
-    double* __restrict__ part = PD.part;
+    gmem<double>* __restrict__ learned = G(learned_out ? learned_out : PD.learned);
+    gmem<double>* __restrict__ delta = G(PD.delta);
+    gmem<double>* __restrict__ part = G(PD.part);
     int c, t, i0, j0;
     if (!hess_tile_of_block(blk, n, tiles_up, c, t, i0, j0))
         return;
@@ -575,16 +602,16 @@ __global__ __launch_bounds__(256) void k_hessian_bf16(FEDNL_KARGS, double* __res
         for (int it = 0; it < 2; ++it) acc[jt][it] = flt4{0.f, 0.f, 0.f, 0.f};
 
     const size_t cbase = (size_t)c * d_pad * m_pad;
-    const double* Dc = Dw + (size_t)c * m_pad;
+    const gmem<double>* Dc = Dw + (size_t)c * m_pad;
     // column col of each block, samples ks .. ks + 7 of the chunk
     const int col = tid >> 2, ks = (tid & 3) * 8;
     uint4 rih, ril, rjh, rjl;
     double rdv[8];
     auto gload = [&](int s0) {
-        rih = *reinterpret_cast<const uint4*>(AThi + cbase + (size_t)(i0 + col) * m_pad + s0 + ks);
-        ril = *reinterpret_cast<const uint4*>(ATlo + cbase + (size_t)(i0 + col) * m_pad + s0 + ks);
-        rjh = *reinterpret_cast<const uint4*>(AThi + cbase + (size_t)(j0 + col) * m_pad + s0 + ks);
-        rjl = *reinterpret_cast<const uint4*>(ATlo + cbase + (size_t)(j0 + col) * m_pad + s0 + ks);
+        rih = *reinterpret_cast<const gmem<uint4>*>(AThi + cbase + (size_t)(i0 + col) * m_pad + s0 + ks);
+        ril = *reinterpret_cast<const gmem<uint4>*>(ATlo + cbase + (size_t)(i0 + col) * m_pad + s0 + ks);
+        rjh = *reinterpret_cast<const gmem<uint4>*>(AThi + cbase + (size_t)(j0 + col) * m_pad + s0 + ks);
+        rjl = *reinterpret_cast<const gmem<uint4>*>(ATlo + cbase + (size_t)(j0 + col) * m_pad + s0 + ks);
 #pragma unroll
         for (int u = 0; u < 8; ++u)
             rdv[u] = Dc[s0 + ks + u];
@@ -648,7 +675,7 @@ __global__ __launch_bounds__(256) void k_hessian_bf16(FEDNL_KARGS, double* __res
                 iv[q] = i0 + wi * 32 + it * 16 + l16;
                 jv[q] = j0 + wj * 32 + jt * 16 + 4 * kq + r;
             }
-    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, PD.pp_h, part,
+    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, G(PD.pp_h), part,
                          nullptr, false, 0.0, i0, red, gred);
 }
 
@@ -661,18 +688,18 @@ __global__ __launch_bounds__(256) void k_hessian_bf16(FEDNL_KARGS, double* __res
 __global__ __launch_bounds__(256) void k_grad_f64(FEDNL_KARGS)
 {
     FEDNL_BLOCK();
-    const double* __restrict__ A = PD.A;
-    const double* __restrict__ gw = PD.gw;
-    const double* __restrict__ x = PD.x;
+    const gmem<double>* __restrict__ A = G(PD.A);
+    const gmem<double>* __restrict__ gw = G(PD.gw);
+    const gmem<double>* __restrict__ x = G(PD.x);
     const int m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad;
     const double lambda = PD.lambda;
-    double* __restrict__ grad_c = PD.grad_c;
+    gmem<double>* __restrict__ grad_c = G(PD.grad_c);
     __shared__ double gp[4][kTile];
     const int gx = d_pad / kTile;
     const int c = blk / gx, i0 = (blk % gx) * kTile;
     const int ii = threadIdx.x & 63, ph = threadIdx.x >> 6;
-    const double* Ac = A + (size_t)c * m_pad * d_pad;
-    const double* Gc = gw + (size_t)c * m_pad;
+    const gmem<double>* Ac = A + (size_t)c * m_pad * d_pad;
+    const gmem<double>* Gc = gw + (size_t)c * m_pad;
     double acc = 0.0;
     for (int s0 = ph * 8; s0 < m_pad; s0 += 32) {
         double av[8], gv[8];
@@ -726,17 +753,17 @@ template <bool kInit>
 __global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __restrict__ learned_out)
 {
     FEDNL_BLOCK();
-    const double* __restrict__ A = PD.A;
-    const double* __restrict__ Dw = PD.Dw;
-    const double* __restrict__ gw = PD.gw;
-    const double* __restrict__ x = PD.x;
+    const gmem<double>* __restrict__ A = G(PD.A);
+    const gmem<double>* __restrict__ Dw = G(PD.Dw);
+    const gmem<double>* __restrict__ gw = G(PD.gw);
+    const gmem<double>* __restrict__ x = G(PD.x);
     const int n = PD.n, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad, tiles_up = PD.tiles_up;
     const double lambda = PD.lambda;
     const size_t S = PD.S;
-    double* __restrict__ learned = learned_out ? learned_out : PD.learned;
-    double* __restrict__ delta = PD.delta;
-    double* __restrict__ grad_c = PD.grad_c;
-    double* __restrict__ part = PD.part;
+    gmem<double>* __restrict__ learned = G(learned_out ? learned_out : PD.learned);
+    gmem<double>* __restrict__ delta = G(PD.delta);
+    gmem<double>* __restrict__ grad_c = G(PD.grad_c);
+    gmem<double>* __restrict__ part = G(PD.part);
     int c, t, i0, j0;
     if (!hess_tile_of_block(blk, n, tiles_up, c, t, i0, j0))
         return;
@@ -764,9 +791,9 @@ __global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __rest
         for (int it = 0; it < 2; ++it) acc[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
     double gacc = 0.0;
 
-    const double* Ac = A + (size_t)c * m_pad * d_pad;
-    const double* Dc = Dw + (size_t)c * m_pad;
-    const double* Gc = gw + (size_t)c * m_pad;
+    const gmem<double>* Ac = A + (size_t)c * m_pad * d_pad;
+    const gmem<double>* Dc = Dw + (size_t)c * m_pad;
+    const gmem<double>* Gc = gw + (size_t)c * m_pad;
 
     // load assignment: rows lr and lr + 8 of a chunk, columns lc, lc + 1 of
     // both column blocks (diagonal tiles read the same block twice, from cache)
@@ -776,12 +803,12 @@ __global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __rest
     double2 ri0, ri1, rj0, rj1;
     double rd0, rd1, rg0, rg1;
     auto gload = [&](int s0) {
-        const double* row0 = Ac + (size_t)(s0 + lr) * d_pad;
-        const double* row1 = Ac + (size_t)(s0 + lr + 8) * d_pad;
-        ri0 = *reinterpret_cast<const double2*>(row0 + i0 + lc);
-        rj0 = *reinterpret_cast<const double2*>(row0 + j0 + lc);
-        ri1 = *reinterpret_cast<const double2*>(row1 + i0 + lc);
-        rj1 = *reinterpret_cast<const double2*>(row1 + j0 + lc);
+        const gmem<double>* row0 = Ac + (size_t)(s0 + lr) * d_pad;
+        const gmem<double>* row1 = Ac + (size_t)(s0 + lr + 8) * d_pad;
+        ri0 = *reinterpret_cast<const gmem<double2>*>(row0 + i0 + lc);
+        rj0 = *reinterpret_cast<const gmem<double2>*>(row0 + j0 + lc);
+        ri1 = *reinterpret_cast<const gmem<double2>*>(row1 + i0 + lc);
+        rj1 = *reinterpret_cast<const gmem<double2>*>(row1 + j0 + lc);
         rd0 = Dc[s0 + lr];
         rd1 = Dc[s0 + lr + 8];
         rg0 = Gc[s0 + lr];
@@ -852,7 +879,7 @@ __global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __rest
                 jv[q] = j0 + wj * 32 + jt * 16 + kq + 4 * r;
             }
     // FedNL-PP takes every client's gradient from k_grad_f64
-    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, PD.pp_h, part,
+    hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, G(PD.pp_h), part,
                          grad_c, !kInit && dg && !PD.pp, gacc, i0, red, gred);
 }
 
@@ -1387,17 +1414,17 @@ __device__ inline double fro2_from_parts(const double* part, int c, int tiles_up
 template <bool kLek>
 __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk)
 {
-    const double* __restrict__ delta = PD.delta;
+    const gmem<double>* __restrict__ delta = G(PD.delta);
     const size_t S = PD.S;
     const int k = PD.k, tiles_up = PD.tiles_up;
-    const double* __restrict__ part = PD.part;
+    const gmem<double>* __restrict__ part = G(PD.part);
     const double lek_delta = PD.lek_delta;
     double* __restrict__ lk_out = nullptr;
-    uint32_t* __restrict__ sel = PD.sel;
-    int* __restrict__ sel_cnt = PD.sel_cnt;
-    double* __restrict__ lek_p = PD.lek_p;
-    int* __restrict__ lek_hit = PD.lek_hit;
-    uint32_t* __restrict__ lek_popped = PD.lek_popped;
+    gmem<uint32_t>* __restrict__ sel = G(PD.sel);
+    gmem<int>* __restrict__ sel_cnt = G(PD.sel_cnt);
+    gmem<double>* __restrict__ lek_p = G(PD.lek_p);
+    gmem<int>* __restrict__ lek_hit = G(PD.lek_hit);
+    gmem<uint32_t>* __restrict__ lek_popped = G(PD.lek_popped);
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = blk;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -1661,9 +1688,9 @@ __device__ __forceinline__ void select_randk_client(const ProblemDev& PD, int bl
 {
     const size_t S = PD.S;
     const int k = PD.k;
-    uint32_t* __restrict__ rng = PD.rng;
-    uint32_t* __restrict__ sel = PD.sel;
-    int* __restrict__ sel_cnt = PD.sel_cnt;
+    gmem<uint32_t>* __restrict__ rng = G(PD.rng);
+    gmem<uint32_t>* __restrict__ sel = G(PD.sel);
+    gmem<int>* __restrict__ sel_cnt = G(PD.sel_cnt);
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = blk;
     const int tid = threadIdx.x;
@@ -1756,10 +1783,10 @@ __device__ __forceinline__ void select_window_client(const ProblemDev& PD, int b
     const int comp = PD.comp;
     const size_t S = PD.S;
     const int k = PD.k;
-    uint32_t* __restrict__ rng = PD.rng;
-    uint32_t* __restrict__ sel = PD.sel;
-    int* __restrict__ sel_cnt = PD.sel_cnt;
-    uint32_t* __restrict__ seq_start = PD.seq_start;
+    gmem<uint32_t>* __restrict__ rng = G(PD.rng);
+    gmem<uint32_t>* __restrict__ sel = G(PD.sel);
+    gmem<int>* __restrict__ sel_cnt = G(PD.sel_cnt);
+    gmem<uint32_t>* __restrict__ seq_start = G(PD.seq_start);
     const int c = blk;
     if (pp_idle(PD, c))
         return;
@@ -1803,22 +1830,22 @@ __device__ __forceinline__ void select_window_client(const ProblemDev& PD, int b
 __device__ __forceinline__ void emit_client_body(const ProblemDev& PD, int blk)
 {
     const int comp = PD.comp;
-    const double* __restrict__ delta = PD.delta;
+    const gmem<double>* __restrict__ delta = G(PD.delta);
     const size_t S = PD.S;
     const int k = PD.k, lda = PD.lda, workers = PD.workers, bm_words = PD.bm_words;
-    const uint32_t* __restrict__ sel = PD.sel;
-    const int* __restrict__ sel_cnt = PD.sel_cnt;
-    const double* __restrict__ lek_p = PD.lek_p;
-    const int* __restrict__ lek_hit = PD.lek_hit;
-    const uint32_t* __restrict__ lek_popped = PD.lek_popped;
-    const uint32_t* __restrict__ lek_state = PD.lek_state;
+    const gmem<uint32_t>* __restrict__ sel = G(PD.sel);
+    const gmem<int>* __restrict__ sel_cnt = G(PD.sel_cnt);
+    const gmem<double>* __restrict__ lek_p = G(PD.lek_p);
+    const gmem<int>* __restrict__ lek_hit = G(PD.lek_hit);
+    const gmem<uint32_t>* __restrict__ lek_popped = G(PD.lek_popped);
+    const gmem<uint32_t>* __restrict__ lek_state = G(PD.lek_state);
     const double alpha = PD.alpha, prescale = PD.prescale, mult = PD.mult;
-    double* __restrict__ learned = PD.learned;
-    uint32_t* __restrict__ pos_out = PD.pos;
-    uint32_t* __restrict__ idx_out = PD.idx;
-    double* __restrict__ vals_out = PD.vals;
-    int* __restrict__ cnt_out = PD.cnt;
-    uint32_t* __restrict__ sel_bm = PD.sel_bm;
+    gmem<double>* __restrict__ learned = G(PD.learned);
+    gmem<uint32_t>* __restrict__ pos_out = G(PD.pos);
+    gmem<uint32_t>* __restrict__ idx_out = G(PD.idx);
+    gmem<double>* __restrict__ vals_out = G(PD.vals);
+    gmem<int>* __restrict__ cnt_out = G(PD.cnt);
+    gmem<uint32_t>* __restrict__ sel_bm = G(PD.sel_bm);
     extern __shared__ __align__(16) uint32_t bml[];  // bm_words
     const int c = blk;
     if (pp_idle(PD, c))
@@ -1935,21 +1962,21 @@ __global__ __launch_bounds__(256) void k_select_window(FEDNL_KARGS)
 // value it sent recomputed from Delta by the same operations as k_emit
 // (fl(Delta * prescale), or fl(fl(Delta * S/k) * prescale) for RandK/RandSeqK).
 // The bitmap words of the workgroup's 256 positions are staged in LDS; the
-// Delta loads of 16 clients are issued together.
+// Delta loads of 48 clients are issued together.
 // ---------------------------------------------------------------------------
 constexpr int kMasterPos = 256;
 
 __device__ __forceinline__ void master_h_block(const ProblemDev& PD, int blk)
 {
-    double* __restrict__ H = PD.H;
+    gmem<double>* __restrict__ H = G(PD.H);
     const size_t S = PD.S;
     const int n = PD.n, bm_words = PD.bm_words, dp = PD.dp;
     const int scaled = (PD.comp == FEDNL_RANDK || PD.comp == FEDNL_RANDSEQK) ? 1 : 0;
     const bool natural = (PD.comp == FEDNL_NATURAL);
-    const double* __restrict__ delta = PD.delta;
-    const uint32_t* __restrict__ sel_bm = PD.sel_bm;
+    const gmem<double>* __restrict__ delta = G(PD.delta);
+    const gmem<uint32_t>* __restrict__ sel_bm = G(PD.sel_bm);
     const double mult = PD.mult, prescale = PD.prescale;
-    double* __restrict__ W = PD.W;
+    gmem<double>* __restrict__ W = G(PD.W);
     __shared__ uint32_t bw[kMaxClients][kMasterPos / 32];
     const size_t p0 = (size_t)blk * kMasterPos;
     const int w0 = (int)(p0 / 32);
@@ -1965,7 +1992,7 @@ __device__ __forceinline__ void master_h_block(const ProblemDev& PD, int blk)
     const int q = threadIdx.x >> 5;
     const uint32_t bit = 1u << (threadIdx.x & 31);
     double h = H[p];
-    constexpr int kBatch = 16;
+    constexpr int kBatch = 48;
     for (int c0 = 0; c0 < n; c0 += kBatch) {
         double v[kBatch];
         bool on[kBatch];
@@ -2001,9 +2028,9 @@ constexpr int kGradClients = 64;
 __device__ __forceinline__ void master_grad_block(const ProblemDev& PD, int blk)
 {
     const int n = PD.n, d = PD.d, d_pad = PD.d_pad;
-    const double* __restrict__ grad_c = PD.grad_c;
+    const gmem<double>* __restrict__ grad_c = G(PD.grad_c);
     const double inv_n = PD.inv_n;
-    double* __restrict__ g_avg = PD.g_avg;
+    gmem<double>* __restrict__ g_avg = G(PD.g_avg);
     __shared__ double tile[kGradClients][kGradCols + 1];
     const int i0 = blk * kGradCols;
     const int col = threadIdx.x & (kGradCols - 1);
@@ -2028,7 +2055,7 @@ __device__ __forceinline__ void master_grad_block(const ProblemDev& PD, int blk)
     // FedNL-PP (bin_fednl_local_pp/include/TrainLoop.h:1166,1400-1426): s is
     // the full gradient sum (reported, divided by n); the direction estimate
     // is (sum of the selected clients' changes) / n + the previous estimate
-    const double* __restrict__ gmsg = PD.pp_gmsg;
+    const gmem<double>* __restrict__ gmsg = G(PD.pp_gmsg);
     double sg = 0.0;
     for (int c0 = 0; c0 < n; c0 += kGradClients) {
         const int cn = min(kGradClients, n - c0);
@@ -2096,18 +2123,18 @@ __device__ inline double client_f(const double* __restrict__ nlog_part, int c, i
 __device__ __forceinline__ void master_vec_block(const ProblemDev& PD)
 {
     const int comp = PD.comp, n = PD.n, d = PD.d, m = PD.m, nlog_blocks = PD.m_pad / 16, tiles_up = PD.tiles_up;
-    const double* __restrict__ g_avg = PD.g_avg;
-    const double* __restrict__ part = PD.part;
-    double* __restrict__ lk = PD.lk;
-    const double* __restrict__ nlog_part = PD.nlog;
-    const double* __restrict__ x = PD.x;
+    const gmem<double>* __restrict__ g_avg = G(PD.g_avg);
+    const gmem<double>* __restrict__ part = G(PD.part);
+    gmem<double>* __restrict__ lk = G(PD.lk);
+    const gmem<double>* __restrict__ nlog_part = G(PD.nlog);
+    const gmem<double>* __restrict__ x = G(PD.x);
     const double lambda = PD.lambda, inv_n = PD.inv_n;
     const int option_b = PD.option_b, tracking = PD.tracking, workers = PD.workers, trace_cap = PD.trace_cap;
-    const int* __restrict__ lek_hit = PD.lek_hit;
-    uint32_t* __restrict__ lek_state = PD.lek_state;
-    double* __restrict__ shift_out = PD.shift;
-    double* __restrict__ trace = PD.trace;
-    const int* __restrict__ round_ctr = PD.round_ctr;
+    const gmem<int>* __restrict__ lek_hit = G(PD.lek_hit);
+    gmem<uint32_t>* __restrict__ lek_state = G(PD.lek_state);
+    gmem<double>* __restrict__ shift_out = G(PD.shift);
+    gmem<double>* __restrict__ trace = G(PD.trace);
+    const gmem<int>* __restrict__ round_ctr = G(PD.round_ctr);
     __shared__ double scratch[16];
     __shared__ double fc_s[kMaxClients];
     (void)g_avg;
@@ -2206,10 +2233,10 @@ __global__ __launch_bounds__(256) void k_ls_begin(FEDNL_KARGS)
     if (!PD.ls)
         return;
     const int d = PD.d;
-    double* __restrict__ x = PD.x;
-    const double* __restrict__ dx = PD.dx;
-    const double* __restrict__ g = PD.g_avg;
-    double* __restrict__ st = PD.mst;
+    gmem<double>* __restrict__ x = G(PD.x);
+    const gmem<double>* __restrict__ dx = G(PD.dx);
+    const gmem<double>* __restrict__ g = G(PD.g_avg);
+    gmem<double>* __restrict__ st = G(PD.mst);
     double* __restrict__ xk = st + kLsHead;
     const bool move = PD.ls_max > 0;
     __shared__ double scratch[16];
@@ -2236,12 +2263,12 @@ __global__ __launch_bounds__(256) void k_ls_check(FEDNL_KARGS)
     (void)blk;
     if (!PD.ls)
         return;
-    double* __restrict__ st = PD.mst;
+    gmem<double>* __restrict__ st = G(PD.mst);
     if (st[LS_DONE] != 0.0)
         return;
     const int n = PD.n, d = PD.d, m = PD.m, nlog_blocks = PD.m_pad / 16;
-    double* __restrict__ x = PD.x;
-    const double* __restrict__ dx = PD.dx;
+    gmem<double>* __restrict__ x = G(PD.x);
+    const gmem<double>* __restrict__ dx = G(PD.dx);
     const double* __restrict__ xk = st + kLsHead;
     __shared__ double scratch[16];
     __shared__ double fc_s[kMaxClients];
@@ -2412,9 +2439,9 @@ __device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double*
 {
     const int d = PD.d, ld = PD.dp, tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    double* __restrict__ a = PD.W;
-    const double* __restrict__ H = PD.H;
-    const double* __restrict__ H0 = PD.H0;
+    gmem<double>* __restrict__ a = G(PD.W);
+    const gmem<double>* __restrict__ H = G(PD.H);
+    const gmem<double>* __restrict__ H0 = G(PD.H0);
     const double shift = *PD.shift;
     // k_newton's LDS: b, the eliminated column, the pivot row, reduction slots
     double* bv = lds;
@@ -2514,8 +2541,8 @@ __device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double*
         }
         __syncthreads();
     }
-    double* __restrict__ x = PD.x;
-    double* __restrict__ dx_out = PD.dx;
+    gmem<double>* __restrict__ x = G(PD.x);
+    gmem<double>* __restrict__ dx_out = G(PD.dx);
     for (int i = tid; i < d; i += nthr) {
         dx_out[i] = bv[i];
         if (PD.update_x == 1)
@@ -2688,14 +2715,14 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
     FEDNL_BLOCK();
     (void)blk;
     const int d = PD.d, dp = PD.dp, trace_cap = PD.trace_cap, update_x = PD.update_x;
-    const double* __restrict__ Hup = PD.hup;
-    const double* __restrict__ shift_p = PD.shift;
-    const double* __restrict__ b = PD.nb;
-    double* __restrict__ W = PD.W;
-    double* __restrict__ x = PD.x;
-    double* __restrict__ dx_out = PD.dx;
-    double* __restrict__ trace = PD.trace;
-    int* __restrict__ round_ctr = PD.round_ctr;
+    const gmem<double>* __restrict__ Hup = G(PD.hup);
+    const gmem<double>* __restrict__ shift_p = G(PD.shift);
+    const gmem<double>* __restrict__ b = G(PD.nb);
+    gmem<double>* __restrict__ W = G(PD.W);
+    gmem<double>* __restrict__ x = G(PD.x);
+    gmem<double>* __restrict__ dx_out = G(PD.dx);
+    gmem<double>* __restrict__ trace = G(PD.trace);
+    gmem<int>* __restrict__ round_ctr = G(PD.round_ctr);
     extern __shared__ __align__(16) double lds[];
     constexpr int kPS = kNB + 1;                    // row stride of the panel in LDS
     double* P21 = lds;                              // rows below the diagonal block: P21[r * kPS + c]
@@ -3323,7 +3350,7 @@ static int launch_round(Engine* e)
         else
             hipLaunchKernelGGL(k_select_window<true>, dim3(n), dim3(256), lds_bm, e->st, pd, nomap);
     }
-    {
+    if (comp == FEDNL_TOPLEK || pp) {
         KTime kt(e, KID_EMIT);
         if (comp == FEDNL_TOPLEK)
             hipLaunchKernelGGL(k_emit, dim3(n), dim3(256), sizeof(uint32_t) * e->bm_words, e->st, pd, nomap);
