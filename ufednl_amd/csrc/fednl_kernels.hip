@@ -1393,10 +1393,21 @@ __device__ inline void compact_bitmap(const uint32_t* bm, size_t words, uint32_t
 // summed in tile order.
 __device__ inline double fro2_from_parts(const double* part, int c, int tiles_up)
 {
+    // (off-diagonal, diagonal) pairs, eight tiles' loads in flight; summed in
+    // tile order
+    const gmem<double2>* pc = G(reinterpret_cast<const double2*>(part) + (size_t)c * tiles_up);
     double o = 0.0, dd = 0.0;
-    for (int t = 0; t < tiles_up; ++t) {
-        o = o + part[((size_t)c * tiles_up + t) * 2 + 0];
-        dd = dd + part[((size_t)c * tiles_up + t) * 2 + 1];
+    for (int t0 = 0; t0 < tiles_up; t0 += 8) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            v[u] = pc[t0 + u < tiles_up ? t0 + u : 0];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (t0 + u < tiles_up) {
+                o = o + v[u].x;
+                dd = dd + v[u].y;
+            }
     }
     return (o + o) + dd;
 }
@@ -1419,7 +1430,6 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
     const int k = PD.k, tiles_up = PD.tiles_up;
     const gmem<double>* __restrict__ part = G(PD.part);
     const double lek_delta = PD.lek_delta;
-    double* __restrict__ lk_out = nullptr;
     gmem<uint32_t>* __restrict__ sel = G(PD.sel);
     gmem<int>* __restrict__ sel_cnt = G(PD.sel_cnt);
     gmem<double>* __restrict__ lek_p = G(PD.lek_p);
@@ -1436,12 +1446,10 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
     const double* dc = delta + (size_t)c * S;
     __shared__ double s_fro2;
     PROBE_T(t_sel0);
-    if (tid == 0) {
-        const double f2 = part ? fro2_from_parts(part, c, tiles_up) : 0.0;
-        s_fro2 = f2;
-        if (lk_out)
-            lk_out[c] = sqrt(f2);
-    }
+    // TopLEK's energy target needs ||Delta||_F^2 (from the Hessian kernel's
+    // tile partials)
+    if (kLek && tid == 0)
+        s_fro2 = part ? fro2_from_parts(part, c, tiles_up) : 0.0;
     if ((size_t)k >= S) {
         for (size_t p = tid; p < S; p += blockDim.x)
             sel[(size_t)c * k + p] = (uint32_t)p;
@@ -1981,9 +1989,23 @@ __device__ __forceinline__ void master_h_block(const ProblemDev& PD, int blk)
     const size_t p0 = (size_t)blk * kMasterPos;
     const int w0 = (int)(p0 / 32);
     const int nw = min(kMasterPos / 32, bm_words - w0);
-    for (int e = threadIdx.x; e < n * (kMasterPos / 32); e += blockDim.x) {
-        const int cc = e / (kMasterPos / 32), q = e % (kMasterPos / 32);
-        bw[cc][q] = (q < nw && !pp_idle(PD, cc)) ? sel_bm[(size_t)cc * bm_words + w0 + q] : 0u;
+    // eight loads per thread in flight before the LDS stores
+    for (int e0 = threadIdx.x; e0 < n * (kMasterPos / 32); e0 += blockDim.x * 8) {
+        uint32_t wv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x;
+            const int cc = e / (kMasterPos / 32), q = e % (kMasterPos / 32);
+            wv[u] = (e < n * (kMasterPos / 32) && q < nw && !pp_idle(PD, cc))
+                        ? sel_bm[(size_t)cc * bm_words + w0 + q]
+                        : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < n * (kMasterPos / 32))
+                bw[e / (kMasterPos / 32)][e % (kMasterPos / 32)] = wv[u];
+        }
     }
     __syncthreads();
     const size_t p = p0 + threadIdx.x;
@@ -2025,6 +2047,29 @@ __device__ __forceinline__ void master_h_block(const ProblemDev& PD, int blk)
 constexpr int kGradCols = 64;
 constexpr int kGradClients = 64;
 
+// tile[cc][ii] = src[(c0 + cc) * d_pad + i0 + ii] for cn clients (zero past
+// d), all of one thread's loads in flight before the LDS stores (a block of
+// 256 threads covers the kGradClients x kGradCols tile in 16 steps)
+__device__ __forceinline__ void load_grad_tile(double (*tile)[kGradCols + 1], const gmem<const double>* src, int c0,
+                                               int cn, int i0, int d, int d_pad)
+{
+    constexpr int kSteps = kGradClients * kGradCols / 256;
+    const int tot = cn * kGradCols;
+    double v[kSteps];
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) {
+        const int e = threadIdx.x + u * 256;
+        const int cc = e / kGradCols, i = i0 + e % kGradCols;
+        v[u] = (e < tot && i < d) ? src[(size_t)(c0 + cc) * d_pad + i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) {
+        const int e = threadIdx.x + u * 256;
+        if (e < tot)
+            tile[e / kGradCols][e % kGradCols] = v[u];
+    }
+}
+
 __device__ __forceinline__ void master_grad_block(const ProblemDev& PD, int blk)
 {
     const int n = PD.n, d = PD.d, d_pad = PD.d_pad;
@@ -2037,11 +2082,7 @@ __device__ __forceinline__ void master_grad_block(const ProblemDev& PD, int blk)
     double s = 0.0;
     for (int c0 = 0; c0 < n; c0 += kGradClients) {
         const int cn = min(kGradClients, n - c0);
-        for (int e = threadIdx.x; e < cn * kGradCols; e += blockDim.x) {
-            const int cc = e / kGradCols, ii = e % kGradCols;
-            const int i = i0 + ii;
-            tile[cc][ii] = (i < d) ? grad_c[(size_t)(c0 + cc) * d_pad + i] : 0.0;
-        }
+        load_grad_tile(tile, grad_c, c0, cn, i0, d, d_pad);
         __syncthreads();
         if (threadIdx.x < kGradCols)
             for (int cc = 0; cc < cn; ++cc) s = s + tile[cc][col];
@@ -2059,11 +2100,7 @@ __device__ __forceinline__ void master_grad_block(const ProblemDev& PD, int blk)
     double sg = 0.0;
     for (int c0 = 0; c0 < n; c0 += kGradClients) {
         const int cn = min(kGradClients, n - c0);
-        for (int e = threadIdx.x; e < cn * kGradCols; e += blockDim.x) {
-            const int cc = e / kGradCols, ii = e % kGradCols;
-            const int i = i0 + ii;
-            tile[cc][ii] = (i < d) ? gmsg[(size_t)(c0 + cc) * d_pad + i] : 0.0;
-        }
+        load_grad_tile(tile, gmsg, c0, cn, i0, d, d_pad);
         __syncthreads();
         if (threadIdx.x < kGradCols)
             for (int cc = 0; cc < cn; ++cc)
@@ -2110,8 +2147,19 @@ __device__ inline double block_sum(double v, double* scratch)
 __device__ inline double client_f(const double* __restrict__ nlog_part, int c, int nlog_blocks, int m,
                                   double lambda, double xx)
 {
+    // the client's partials in block order, eight loads in flight
+    const gmem<const double>* pc = G(nlog_part + (size_t)c * nlog_blocks);
     double sacc = 0.0;
-    for (int q = 0; q < nlog_blocks; ++q) sacc = sacc + nlog_part[(size_t)c * nlog_blocks + q];
+    for (int q0 = 0; q0 < nlog_blocks; q0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            v[u] = pc[q0 + u < nlog_blocks ? q0 + u : 0];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (q0 + u < nlog_blocks)
+                sacc = sacc + v[u];
+    }
     return sacc / (double)m + (lambda / 2.0) * xx;
 }
 
@@ -2137,20 +2185,23 @@ __device__ __forceinline__ void master_vec_block(const ProblemDev& PD)
     const gmem<int>* __restrict__ round_ctr = G(PD.round_ctr);
     __shared__ double scratch[16];
     __shared__ double fc_s[kMaxClients];
+    __shared__ double lk_s[kMaxClients];
     (void)g_avg;
     double xx = 0.0;
     for (int i = threadIdx.x; i < d; i += blockDim.x) xx = fma(x[i], x[i], xx);
     xx = block_sum(xx, scratch);
     const bool with_f = tracking || PD.ls;  // f_i is sent (TrainLoop.h:620, 1512-1544)
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        lk[c] = sqrt(fro2_from_parts(part, c, tiles_up));
+        const double lkc = sqrt(fro2_from_parts(part, c, tiles_up));
+        lk[c] = lkc;
+        lk_s[c] = lkc;
         if (with_f)
             fc_s[c] = client_f(nlog_part, c, nlog_blocks, m, lambda, xx);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double l = 0.0, fsum = 0.0;
-        for (int c = 0; c < n; ++c) l = l + lk[c];
+        for (int c = 0; c < n; ++c) l = l + lk_s[c];
         l = l * inv_n;
         if (with_f) {
             // FedNL-PP: the selected clients' f_i over n (TrainLoop.h:1648-1676)
