@@ -2622,6 +2622,7 @@ __device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double*
 // The factorisation fails, as dopt's, when a pivot is < 0.
 // ---------------------------------------------------------------------------
 constexpr int kNewtonThreads = 512;
+static_assert(kNewtonThreads == 512, "k_newton's panel staging splits 32 columns over two halves of 256 threads");
 static_assert(kMaxNewtonD - 32 < (kNewtonThreads / 64) * 64, "k_newton TRSM assumes one 64-row chunk per wave");
 static_assert(kMaxNewtonD <= kNewtonThreads, "k_newton back solve assumes one row per thread");
 
@@ -2843,6 +2844,20 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
         if (s_fail)
             break;
         const int T = dp - jb - kNB;
+        // wave 0's three tiles of the next diagonal block, as the last
+        // trailing update left them (phases A and C do not touch them), are
+        // loaded now so that their latency is hidden behind phase A
+        double wpre[3][4];
+        if (wave == 0 && jb >= 0 && T > 0) {
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const int ti = (t == 0) ? 0 : 1, tj = (t == 2) ? 1 : 0;
+                const double* wp = W + (size_t)(jb + kNB + tj * 16 + kq) * dp + jb + kNB + ti * 16 + l16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    wpre[t][r] = wp[(size_t)4 * r * dp];
+            }
+        }
         PROBE_T(t_a);
         if (jb >= 0) {
             // A. L21 = A21 L11^-T, one 64-row chunk per wave (T < nwaves * 64).
@@ -2908,11 +2923,7 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
                     const int ti = (t == 0) ? 0 : 1, tj = (t == 2) ? 1 : 0;
-                    const double* wp = W + (size_t)(jb + kNB + tj * 16 + kq) * dp + jb + kNB + ti * 16 + l16;
-                    double wold[4];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        wold[r] = wp[(size_t)4 * r * dp];
+                    const double* wold = wpre[t];
                     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                     for (int k0 = 0; k0 < kNB; k0 += 4) {
@@ -3014,27 +3025,21 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
         __syncthreads();
         PROBE_ADD(4, t_b);
         PROBE_T(t_c);
-        // C. the next panel's rows below its diagonal block
+        // C. the next panel's rows below its diagonal block: thread t holds
+        // row t % 256 and columns 16 (t / 256) .. + 15, all sixteen loads in
+        // flight before the LDS stores
         {
             const int jn = jb + kNB;
             const int rows = T - kNB;
-            const int etot = kNB * rows;
-            for (int e0 = tid; e0 < etot; e0 += nthr * 8) {
-                double pv[8];
+            const int c0 = 16 * (tid >> 8);
+            for (int r = tid & 255; r < rows; r += 256) {
+                double pv[16];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int e = min(e0 + q * nthr, etot - 1);
-                    const int c = e / rows, r = e % rows;
-                    pv[q] = W[(size_t)(jn + c) * dp + jn + kNB + r];
-                }
+                for (int q = 0; q < 16; ++q)
+                    pv[q] = W[(size_t)(jn + c0 + q) * dp + jn + kNB + r];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int e = e0 + q * nthr;
-                    if (e < etot) {
-                        const int c = e / rows, r = e % rows;
-                        P21[r * kPS + c] = pv[q];
-                    }
-                }
+                for (int q = 0; q < 16; ++q)
+                    P21[r * kPS + c0 + q] = pv[q];
             }
         }
         __syncthreads();
