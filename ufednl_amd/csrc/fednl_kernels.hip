@@ -2960,14 +2960,17 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
             int rb[kGroup], cb[kGroup];
             double wv[kGroup][4];
             // first group's coordinates and loads
+            int ti = 0, tj = 0;
+            tri_tile(tcur, ti, tj);
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
                 const int tt = tcur + g;
-                int ti = 0, tj = 0;
-                if (tt < ntiles)
-                    tri_tile(tt, ti, tj);
+                if (g > 0 && ++tj > ti) {  // the next tile in row-major lower order
+                    ++ti;
+                    tj = 0;
+                }
                 rb[g] = (tt < ntiles) ? ti * 16 : -1;
-                cb[g] = tj * 16;
+                cb[g] = (tt < ntiles) ? tj * 16 : 0;  // past the last tile: an in-bounds dummy address
                 const double* wp = W + (size_t)(jb + kNB + cb[g] + kq) * dp + jb + kNB + max(rb[g], 0) + l16;
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
@@ -2977,14 +2980,17 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
                 // next group's loads in flight during this group's MFMAs
                 int rbn[kGroup], cbn[kGroup];
                 double wn[kGroup][4];
+                int ti = 0, tj = 0;
+                tri_tile(tcur + stride, ti, tj);
 #pragma unroll
                 for (int g = 0; g < kGroup; ++g) {
                     const int tt = tcur + stride + g;
-                    int ti = 0, tj = 0;
-                    if (tt < ntiles)
-                        tri_tile(tt, ti, tj);
+                    if (g > 0 && ++tj > ti) {
+                        ++ti;
+                        tj = 0;
+                    }
                     rbn[g] = (tt < ntiles) ? ti * 16 : -1;
-                    cbn[g] = tj * 16;
+                    cbn[g] = (tt < ntiles) ? tj * 16 : 0;
                     const double* wp =
                         W + (size_t)(jb + kNB + cbn[g] + kq) * dp + jb + kNB + max(rbn[g], 0) + l16;
 #pragma unroll
