@@ -33,6 +33,7 @@
 #include <chrono>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 #include <thread>
 
@@ -2860,9 +2861,13 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
         }
         PROBE_T(t_a);
         if (jb >= 0) {
-            // A. L21 = A21 L11^-T, one 64-row chunk per wave (T < nwaves * 64).
-            // Every wave holds L11 in lanes 0..31 (row r in lane r) and
-            // broadcasts its entries by v_readlane; lane l solves row rb + l.
+            // A. L21 = A21 L11^-T, one 64-row chunk per wave (T < nwaves * 64),
+            // lane l solving row rb + l, in two 16-column halves: the first
+            // half by substitution with L11's entries broadcast by
+            // v_readlane (every wave holds L11 in lanes 0..31, row r in lane
+            // r); then the second half's columns take -X1 L_ba^T on FP64 MFMA
+            // (X1 through the panel rows in LDS); then the second half by
+            // substitution.
             if (wave * 64 < T) {
                 const int rr = wave * 64 + lane;
                 const bool live = rr < T;
@@ -2880,19 +2885,56 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
                     a[cc] = pr[cc];
                 const int i = jb + kNB + (live ? rr : 0);
                 double bi = yv[i];
+                auto solve_half = [&](auto half) {
+                    constexpr int c0 = decltype(half)::value * 16;
 #pragma unroll
-                for (int cc = 0; cc < kNB; ++cc) {
-                    a[cc] = a[cc] * bcast_lane(invl, cc);
-                    bi = fma(-a[cc], bcast_lane(ypan, cc), bi);
-                    // ties this column's broadcasts to a[cc] so that they are
-                    // not all hoisted ahead of the solve (SGPR pressure)
-                    double lc = lrow[cc];
-                    asm volatile("" : "+v"(lc) : "v"(a[cc]));
+                    for (int cc = c0; cc < c0 + 16; ++cc) {
+                        a[cc] = a[cc] * bcast_lane(invl, cc);
+                        bi = fma(-a[cc], bcast_lane(ypan, cc), bi);
+                        // ties this column's broadcasts to a[cc] so that they
+                        // are not all hoisted ahead of the solve (SGPR pressure)
+                        double lc = lrow[cc];
+                        asm volatile("" : "+v"(lc) : "v"(a[cc]));
 #pragma unroll
-                    for (int c2 = cc + 1; c2 < kNB; ++c2)
-                        a[c2] = fma(-a[cc], bcast_lane(lc, c2), a[c2]);
-                    __builtin_amdgcn_sched_barrier(0);
+                        for (int c2 = cc + 1; c2 < c0 + 16; ++c2)
+                            a[c2] = fma(-a[cc], bcast_lane(lc, c2), a[c2]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                };
+                solve_half(std::integral_constant<int, 0>{});
+                // X1 to this wave's rows of the panel (columns 0..15)
+                if (live) {
+#pragma unroll
+                    for (int cc = 0; cc < 16; ++cc)
+                        pr[cc] = a[cc];
                 }
+                __builtin_amdgcn_wave_barrier();
+                // U = X1 L_ba^T per live 16-row tile: lane (l16, kq), item r
+                // holds U[row l16][column 4r + kq]; written over the tile's
+                // columns 16..31 (already in the owners' registers)
+#pragma unroll
+                for (int rt = 0; rt < 4; ++rt) {
+                    const int rb = wave * 64 + rt * 16;
+                    if (rb >= T)
+                        break;
+                    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int k0 = 0; k0 < 16; k0 += 4) {
+                        const double av = Dg[16 + l16][k0 + kq];            // A[m = column][k] = L_ba
+                        const double bv = P21[(rb + l16) * kPS + k0 + kq];  // B[k][n = row] = X1
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        P21[(rb + l16) * kPS + 16 + 4 * r + kq] = acc[r];
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (live) {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        a[16 + j] = a[16 + j] - pr[16 + j];
+                }
+                solve_half(std::integral_constant<int, 1>{});
                 if (live) {
                     yv[i] = bi;
 #pragma unroll
