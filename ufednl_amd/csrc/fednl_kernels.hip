@@ -2627,11 +2627,17 @@ static_assert(kNewtonThreads == 512, "k_newton's panel staging splits 32 columns
 static_assert(kMaxNewtonD - 32 < (kNewtonThreads / 64) * 64, "k_newton TRSM assumes one 64-row chunk per wave");
 static_assert(kMaxNewtonD <= kNewtonThreads, "k_newton back solve assumes one row per thread");
 
+// Row stride (doubles) of k_newton's panel rows in LDS. The MFMA operand
+// reads take rows l16 = 0..15 and columns kq = 0..3 in one instruction; with
+// a stride of 2 mod 32 the 32 lanes of each half-wave hit 32 distinct bank
+// pairs.
+constexpr int kNewtonPS = kNB + 2;
+
 __host__ __device__ constexpr size_t newton_lds_bytes(int dp)
 {
-    // rows below the diagonal block, row stride kNB + 1 (at least one row);
+    // rows below the diagonal block, row stride kNewtonPS (at least one row);
     // after a failed factorization the fallback solve reuses it (3 dp + 16)
-    const size_t panel = (size_t)(kNB + 1) * (size_t)(dp > kNB ? dp - kNB : 1);
+    const size_t panel = (size_t)kNewtonPS * (size_t)(dp > kNB ? dp - kNB : 1);
     const size_t fallback = 3 * (size_t)dp + 16;
     return sizeof(double) * (panel > fallback ? panel : fallback);
 }
@@ -2777,7 +2783,7 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
     gmem<double>* __restrict__ trace = G(PD.trace);
     gmem<int>* __restrict__ round_ctr = G(PD.round_ctr);
     extern __shared__ __align__(16) double lds[];
-    constexpr int kPS = kNB + 1;                    // row stride of the panel in LDS
+    constexpr int kPS = kNewtonPS;                  // row stride of the panel in LDS
     double* P21 = lds;                              // rows below the diagonal block: P21[r * kPS + c]
     __shared__ double Dg[kNB][kNB + 1];             // L11 of the current panel, row-major (lower)
     __shared__ double Dq[16][17];                   // the diagonal factor's MFMA quarter update
