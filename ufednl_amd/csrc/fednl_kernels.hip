@@ -1495,9 +1495,11 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
     // scan of list positions k .. S-1 by wave 0 alone, in groups of
     // kScanGroup chunks of 64 positions. The next group's global loads are in
     // flight in registers while the current group, staged in LDS, is tested:
-    // one ballot per chunk against the current root (the root only grows, so
-    // a position not above it is never admitted), then the admissions in
-    // list order.
+    // one ballot per chunk against the current root, then the admissions in
+    // list order, each later position of the chunk tested again against the
+    // root left by the admission before it. (The quirky heap's root can also
+    // drop after an admission, so no test is ever made against an older
+    // root.)
     RegHeap rh;
     if (wave == 0) {
         rh.load(hv, hid, k, lane);
