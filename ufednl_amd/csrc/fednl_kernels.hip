@@ -3122,17 +3122,31 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
         return;
     }
 
-    // L^T dx = y, 32-row blocks from the bottom. Thread rr < jb keeps row rr
-    // of the block's L^T columns (loaded before the block is solved); in the
-    // block, lane r < 32 keeps column r of L11.
+    // L^T dx = y, 32-row blocks from the bottom. Row rr < jb of the block's
+    // L^T columns is column rr of W at rows jb..jb+31 (256 contiguous bytes):
+    // waves 1-7 stage them into the panel LDS, 32 lanes per column so that
+    // the loads coalesce, while wave 0 solves the block (lane r < 32 keeps
+    // column r of L11); then thread rr folds its row into y.
     for (int jb = dp - kNB; jb >= 0; jb -= kNB) {
         const int rr = tid;
         const bool has = rr < jb;
-        double wl[kNB];
-        if (has) {
+        if (wave > 0) {
+            const int tot = jb * kNB;
+            const int t1 = tid - 64, n1 = nthr - 64;
+            for (int e0 = t1; e0 < tot; e0 += n1 * 8) {
+                double v[8];
 #pragma unroll
-            for (int cc = 0; cc < kNB; ++cc)
-                wl[cc] = W[(size_t)rr * dp + jb + cc];
+                for (int u = 0; u < 8; ++u) {
+                    const int e = min(e0 + u * n1, tot - 1);
+                    v[u] = W[(size_t)(e >> 5) * dp + jb + (e & (kNB - 1))];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int e = e0 + u * n1;
+                    if (e < tot)
+                        P21[(e >> 5) * kPS + (e & (kNB - 1))] = v[u];
+                }
+            }
         }
         if (wave == 0) {
             const bool act = lane < kNB;
@@ -3157,6 +3171,7 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
         }
         __syncthreads();
         if (has) {
+            const double* wl = P21 + rr * kPS;
             double s = yv[rr];
 #pragma unroll
             for (int cc = 0; cc < kNB; ++cc)
