@@ -3127,6 +3127,13 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
     // waves 1-7 stage them into the panel LDS, 32 lanes per column so that
     // the loads coalesce, while wave 0 solves the block (lane r < 32 keeps
     // column r of L11); then thread rr folds its row into y.
+    double colv_next[kNB];
+    if (wave == 0) {
+        const double* cp = W + (size_t)(dp - kNB + (lane & (kNB - 1))) * dp + dp - kNB;
+#pragma unroll
+        for (int c = 0; c < kNB; ++c)
+            colv_next[c] = cp[c];
+    }
     for (int jb = dp - kNB; jb >= 0; jb -= kNB) {
         const int rr = tid;
         const bool has = rr < jb;
@@ -3151,12 +3158,18 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
         if (wave == 0) {
             const bool act = lane < kNB;
             const int lr = act ? lane : 0;
+            // column lr of L11 (rows jb..jb+31; entries above the diagonal
+            // unused), loaded one block ahead
             double colv[kNB];
-            // column lr of L11 from rows jb..jb+31 (entries above the diagonal unused)
-            const double* cp = W + (size_t)(jb + lr) * dp + jb;
 #pragma unroll
             for (int c = 0; c < kNB; ++c)
-                colv[c] = cp[c];
+                colv[c] = colv_next[c];
+            if (jb >= kNB) {
+                const double* cp = W + (size_t)(jb - kNB + lr) * dp + jb - kNB;
+#pragma unroll
+                for (int c = 0; c < kNB; ++c)
+                    colv_next[c] = cp[c];
+            }
             double yr = yv[jb + lr];
 #pragma unroll
             for (int cc = kNB - 1; cc >= 0; --cc) {
