@@ -2171,6 +2171,28 @@ __device__ inline double client_f(const double* __restrict__ nlog_part, int c, i
 // Newton shift, norms for the record (TrainLoop.h:1679-1691), TopLEK stream
 // advance. One workgroup.
 // ---------------------------------------------------------------------------
+// v[0] + v[1] + ... in client order over the clients with on[c] (all when
+// on is null), eight reads in flight ahead of the additions
+__device__ __forceinline__ double ordered_sum(const double* v, const int* on, int n)
+{
+    double acc = 0.0;
+    for (int c0 = 0; c0 < n; c0 += 8) {
+        double t[8];
+        int f[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int c = c0 + u < n ? c0 + u : n - 1;
+            t[u] = v[c];
+            f[u] = (c0 + u < n) && (on == nullptr || on[c] != 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (f[u])
+                acc = acc + t[u];
+    }
+    return acc;
+}
+
 __device__ __forceinline__ void master_vec_block(const ProblemDev& PD)
 {
     const int comp = PD.comp, n = PD.n, d = PD.d, m = PD.m, nlog_blocks = PD.m_pad / 16, tiles_up = PD.tiles_up;
@@ -2188,57 +2210,65 @@ __device__ __forceinline__ void master_vec_block(const ProblemDev& PD)
     const gmem<int>* __restrict__ round_ctr = G(PD.round_ctr);
     __shared__ double scratch[16];
     __shared__ double fc_s[kMaxClients];
-    __shared__ double lk_s[kMaxClients];
     (void)g_avg;
+    PROBE_T(t_v0);
     double xx = 0.0;
     for (int i = threadIdx.x; i < d; i += blockDim.x) xx = fma(x[i], x[i], xx);
     xx = block_sum(xx, scratch);
+#ifdef FEDNL_PROBE
+    if (threadIdx.x == 0) atomicMax(&g_probe[3010], wall_clock64() - t_v0);
+#endif
     const bool with_f = tracking || PD.ls;  // f_i is sent (TrainLoop.h:620, 1512-1544)
+    const bool pp = PD.pp != 0;
+    __shared__ int s_parts, s_items;
+    if (threadIdx.x == 0) {
+        s_parts = 0;
+        s_items = 0;
+    }
+    __syncthreads();
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         const double lkc = sqrt(fro2_from_parts(part, c, tiles_up));
         lk[c] = lkc;
-        lk_s[c] = lkc;
+        // this round's participants and message items (integer counts)
+        if (!pp || PD.pp_sel[c] != 0) {
+            atomicAdd(&s_parts, 1);
+            atomicAdd(&s_items, comp == FEDNL_TOPLEK ? PD.cnt[c] : PD.k);
+        }
         if (with_f)
             fc_s[c] = client_f(nlog_part, c, nlog_blocks, m, lambda, xx);
     }
     __syncthreads();
+#ifdef FEDNL_PROBE
+    if (threadIdx.x == 0) atomicMax(&g_probe[3011], wall_clock64() - t_v0);
+#endif
     if (threadIdx.x == 0) {
         double l = 0.0, fsum = 0.0;
-        for (int c = 0; c < n; ++c) l = l + lk_s[c];
+        l = ordered_sum((const double*)lk, nullptr, n);
         l = l * inv_n;
         if (with_f) {
             // FedNL-PP: the selected clients' f_i over n (TrainLoop.h:1648-1676)
-            for (int c = 0; c < n; ++c)
-                if (!pp_idle(PD, c))
-                    fsum = fsum + fc_s[c];
+            fsum = ordered_sum(fc_s, pp ? PD.pp_sel : nullptr, n);
             fsum = fsum * inv_n;
         }
         if (PD.ls)
             PD.mst[LS_FXK] = fsum;
         if (PD.pp && option_b) {
             // running Lk average (TrainLoop.h:1436-1445)
-            double sl = 0.0;
-            for (int c = 0; c < n; ++c)
-                if (!pp_idle(PD, c))
-                    sl = sl + PD.pp_lkmsg[c];
+            const double sl = ordered_sum(PD.pp_lkmsg, PD.pp_sel, n);
             l = sl * inv_n + PD.mst[PP_LI];
             PD.mst[PP_LI] = l;
         }
         *shift_out = option_b ? l : lambda;  // option a: mu(f) = lambda (logistic_regression.h:702-705)
         const int r = *round_ctr;
         if (r < trace_cap) {
-            int parts = 0;
-            double items = 0.0;
-            for (int c = 0; c < n; ++c)
-                if (!pp_idle(PD, c)) {
-                    parts += 1;
-                    items += (comp == FEDNL_TOPLEK) ? (double)PD.cnt[c] : (double)PD.k;
-                }
             trace[(size_t)r * kTraceStride + 1] = fsum;
             trace[(size_t)r * kTraceStride + 2] = sqrt(xx);
-            trace[(size_t)r * kTraceStride + 5] = (double)parts;
-            trace[(size_t)r * kTraceStride + 6] = items;
+            trace[(size_t)r * kTraceStride + 5] = (double)s_parts;
+            trace[(size_t)r * kTraceStride + 6] = (double)s_items;
         }
+#ifdef FEDNL_PROBE
+        atomicMax(&g_probe[3012], wall_clock64() - t_v0);
+#endif
     }
     if (comp == FEDNL_TOPLEK) {
         for (int w = threadIdx.x; w < workers; w += blockDim.x) {
@@ -2263,12 +2293,19 @@ __global__ __launch_bounds__(256) void k_master(FEDNL_KARGS)
 {
     FEDNL_BLOCK();
     const int nh = master_h_blocks(PD.S), ng = master_grad_blocks(PD.d);
+#ifdef FEDNL_PROBE
+    const unsigned long long t_m0 = wall_clock64();
+#endif
     if (blk < nh)
         master_h_block(PD, blk);
     else if (blk < nh + ng)
         master_grad_block(PD, blk - nh);
     else
         master_vec_block(PD);
+#ifdef FEDNL_PROBE
+    if (threadIdx.x == 0)
+        atomicMax(&g_probe[3000 + (blk < nh ? 0 : (blk < nh + ng ? 1 : 2))], wall_clock64() - t_m0);
+#endif
 }
 
 // ---------------------------------------------------------------------------
