@@ -1988,26 +1988,27 @@ __device__ __forceinline__ void master_h_block(const ProblemDev& PD, int blk)
     const gmem<uint32_t>* __restrict__ sel_bm = G(PD.sel_bm);
     const double mult = PD.mult, prescale = PD.prescale;
     gmem<double>* __restrict__ W = G(PD.W);
-    __shared__ uint32_t bw[kMaxClients][kMasterPos / 32];
+    // the selection words of the block's positions, client-minor (bwT[q][c]:
+    // word q of client c) so that one 16-byte LDS read gives four clients
+    __shared__ __align__(16) uint32_t bwT[kMasterPos / 32][kMaxClients + 4];  // +4: store banks
     const size_t p0 = (size_t)blk * kMasterPos;
     const int w0 = (int)(p0 / 32);
     const int nw = min(kMasterPos / 32, bm_words - w0);
+    const int n4 = (n + 3) & ~3;  // padded with empty words
     // eight loads per thread in flight before the LDS stores
-    for (int e0 = threadIdx.x; e0 < n * (kMasterPos / 32); e0 += blockDim.x * 8) {
+    for (int e0 = threadIdx.x; e0 < n4 * (kMasterPos / 32); e0 += blockDim.x * 8) {
         uint32_t wv[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int e = e0 + u * blockDim.x;
             const int cc = e / (kMasterPos / 32), q = e % (kMasterPos / 32);
-            wv[u] = (e < n * (kMasterPos / 32) && q < nw && !pp_idle(PD, cc))
-                        ? sel_bm[(size_t)cc * bm_words + w0 + q]
-                        : 0u;
+            wv[u] = (cc < n && q < nw && !pp_idle(PD, cc)) ? sel_bm[(size_t)cc * bm_words + w0 + q] : 0u;
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int e = e0 + u * blockDim.x;
-            if (e < n * (kMasterPos / 32))
-                bw[e / (kMasterPos / 32)][e % (kMasterPos / 32)] = wv[u];
+            if (e < n4 * (kMasterPos / 32))
+                bwT[e % (kMasterPos / 32)][e / (kMasterPos / 32)] = wv[u];
         }
     }
     __syncthreads();
@@ -2015,16 +2016,29 @@ __device__ __forceinline__ void master_h_block(const ProblemDev& PD, int blk)
     if (p >= S)
         return;
     const int q = threadIdx.x >> 5;
-    const uint32_t bit = 1u << (threadIdx.x & 31);
+    const int sh = threadIdx.x & 31;
     double h = H[p];
+    // per batch: the clients' words (four per LDS read), then all the Delta
+    // loads of the clients whose bit is set, then the sums in client order
     constexpr int kBatch = 48;
     for (int c0 = 0; c0 < n; c0 += kBatch) {
+        uint32_t wq[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u += 4) {
+            const int cc = min(c0 + u, n4 - 4);
+            const uint4 w4 = *reinterpret_cast<const uint4*>(&bwT[q][cc]);
+            const bool in = c0 + u < n4;
+            wq[u] = in ? w4.x : 0u;
+            wq[u + 1] = in ? w4.y : 0u;
+            wq[u + 2] = in ? w4.z : 0u;
+            wq[u + 3] = in ? w4.w : 0u;
+        }
         double v[kBatch];
         bool on[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             const int cc = c0 + u;
-            on[u] = (cc < n) && (bw[cc < n ? cc : 0][q] & bit);
+            on[u] = (cc < n) && ((wq[u] >> sh) & 1u);
             v[u] = on[u] ? delta[(size_t)cc * S + p] : 0.0;
         }
 #pragma unroll
