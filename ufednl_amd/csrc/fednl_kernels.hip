@@ -751,7 +751,7 @@ __global__ void k_convert_a(const double* __restrict__ A, int n, int m_pad, int 
 // (blockIdx % 8 selects the XCD), keeping the client's rows of A in that L2.
 // ---------------------------------------------------------------------------
 template <bool kInit>
-__global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __restrict__ learned_out)
+__global__ __launch_bounds__(256, 4) void k_hessian_f64(FEDNL_KARGS, double* __restrict__ learned_out)
 {
     FEDNL_BLOCK();
     const gmem<double>* __restrict__ A = G(PD.A);
@@ -784,6 +784,7 @@ __global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __rest
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wi = wave & 1, wj = wave >> 1;
     const int l16 = lane & 15, kq = lane >> 4;
+    const bool below = dg && wi == 1 && wj == 0;
 
     dbl4 acc[2][2];
 #pragma unroll
@@ -840,6 +841,9 @@ __global__ __launch_bounds__(256) void k_hessian_f64(FEDNL_KARGS, double* __rest
         const bool more = ch + 1 < nch;
         if (more)
             gload((ch + 1) * kKC);
+        // on a diagonal tile the (wi = 1, wj = 0) quadrant lies below the
+        // diagonal: that wave stages and reduces but multiplies nothing
+        if (!below)
 #pragma unroll
         for (int ks = 0; ks < kKC; ks += 4) {
             const int kk = ks + kq;
