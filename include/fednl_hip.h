@@ -147,6 +147,12 @@ int fednl_get_state(void *handle, double *x, double *learned, double *H, uint32_
 int fednl_set_state(void *handle, const double *x, const double *learned, const double *H,
                     const uint32_t *rng, const uint32_t *toplek_rng);
 
+/* The tile shapes the engine picked when it was created, by timing each
+ * candidate on its own data: rows per wave of the margin pass (4 or 8; the
+ * environment variable FEDNL_MARGIN_ROWS=4|8 sets it instead). Both shapes
+ * compute the same bits. No counterpart in the reference. */
+int fednl_tuning(void *handle, int *margin_rows);
+
 /* Per-kernel device time (milliseconds, summed over rounds since the last
  * reset) measured with HIP events on the engine stream when enabled. */
 int fednl_kernel_timing(void *handle, int enable);

@@ -273,9 +273,14 @@ __device__ inline bool pp_idle(const ProblemDev& P, int c) { return P.pp && P.pp
 // k_margin: one wave per sample row. kTrial: a line-search trial point (only
 // the -log sigma partials for f; nothing once the line search has stopped)
 // ---------------------------------------------------------------------------
-template <bool kTrial>
+template <bool kTrial, int kRows>
 __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
 {
+    // kRows rows per wave, 4 kRows per workgroup (m_pad is a multiple of 32);
+    // the -log sigma partials stay per 16 rows, whatever kRows is, so that f
+    // is summed in the same order
+    static_assert(kRows == 4 || kRows == 8, "16 or 32 rows per workgroup");
+    constexpr int kRB = 4 * kRows;
     FEDNL_BLOCK();
     if (kTrial && (!PD.ls || PD.mst[LS_DONE] != 0.0))
         return;
@@ -286,36 +291,39 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
     gmem<double>* __restrict__ Dw = G(PD.Dw);
     gmem<double>* __restrict__ gw = G(PD.gw);
     gmem<double>* __restrict__ nlog_part = G(PD.nlog);
-    const int gx = m_pad / 16, bx = blk % gx, by = blk / gx;
+    const int gxb = m_pad / kRB, bx = blk % gxb, c = blk / gxb;
     extern __shared__ double xs[];
-    __shared__ double nl[16];
-    const int c = by;
-    if (threadIdx.x < 16)
+    __shared__ double nl[kRB];
+    if (threadIdx.x < kRB)
         nl[threadIdx.x] = 0.0;
     for (int i = threadIdx.x; i < d_pad; i += blockDim.x)
         xs[i] = (i < d) ? x[i] : 0.0;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // four rows per wave (m_pad is a multiple of 16, so all four exist), their
-    // loads interleaved; every lane's sum runs over the same columns in the
-    // same order for each row
-    const int s0 = bx * 16 + wave * 4;
-    const double* row0 = A + ((size_t)c * m_pad + s0) * d_pad;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // the wave's rows, their loads interleaved; every lane's sum runs over the
+    // same columns in the same order for each row
+    const int s0 = bx * kRB + wave * kRows;
+    const gmem<double>* row0 = A + ((size_t)c * m_pad + s0) * d_pad;
+    double acc[kRows];
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) acc[q] = 0.0;
 #pragma unroll 2
     for (int i = lane; i < d_pad; i += 64) {
         const double xi = xs[i];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < kRows; ++q)
             acc[q] = fma(row0[(size_t)q * d_pad + i], xi, acc[q]);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < kRows; ++q)
         for (int off = 32; off > 0; off >>= 1)
             acc[q] += __shfl_xor(acc[q], off);
-    // every lane holds the four sums; lane q finishes row s0 + q
-    if (lane < 4) {
-        const double t = (lane == 0) ? acc[0] : (lane == 1) ? acc[1] : (lane == 2) ? acc[2] : acc[3];
+    // every lane holds the sums; lane q finishes row s0 + q
+    if (lane < kRows) {
+        double t = acc[0];
+#pragma unroll
+        for (int q = 1; q < kRows; ++q)
+            t = (lane == q) ? acc[q] : t;
         const int s = s0 + lane;
         const size_t o = (size_t)c * m_pad + s;
         if (s < m) {
@@ -327,21 +335,22 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
                 Dw[o] = inv_m * z2;
                 gw[o] = z1 * (-inv_m);
             }
-            nl[wave * 4 + lane] = -log(sg);
+            nl[wave * kRows + lane] = -log(sg);
         } else {
             if (!kTrial) {
                 Dw[o] = 0.0;
                 gw[o] = 0.0;
             }
-            nl[wave * 4 + lane] = 0.0;
+            nl[wave * kRows + lane] = 0.0;
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // per-block partial of sum_s -log sigma_s (samples in order)
+    if (threadIdx.x < kRB / 16) {
+        // per-16-row partial of sum_s -log sigma_s (samples in order)
+        const int h = threadIdx.x;
         double sacc = 0.0;
-        for (int q = 0; q < 16; ++q) sacc = sacc + nl[q];
-        nlog_part[(size_t)c * gx + bx] = sacc;
+        for (int q = 0; q < 16; ++q) sacc = sacc + nl[16 * h + q];
+        nlog_part[(size_t)c * (m_pad / 16) + bx * (kRB / 16) + h] = sacc;
     }
 }
 
@@ -3341,7 +3350,8 @@ struct Engine {
     hipEvent_t ev_start = nullptr;
     std::vector<hipEvent_t> round_ev;
     bool timing = false;
-    unsigned timing_mask = ~0u;    // KernelId bits timed when timing is on
+    unsigned timing_mask = ~0u;
+    int margin_rows = 4;           // k_margin rows per wave, picked at creation    // KernelId bits timed when timing is on
     struct Pending { int kid; hipEvent_t a, b; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
@@ -3467,6 +3477,67 @@ static int sync_desc(Engine* e)
 
 // number of blocks each round kernel uses for one problem
 static int blocks_margin(const ProblemDev& P) { return (P.m_pad / 16) * P.n; }
+
+// the engine's margin pass with its tuned rows per wave
+static void launch_margin(Engine* e, bool trial)
+{
+    const ProblemDev& P = e->hdesc;
+    const unsigned grid = (unsigned)((P.m_pad / (4 * e->margin_rows)) * P.n);
+    const size_t lds = sizeof(double) * e->d_pad;
+    const ProblemDev* pd = e->pdev;
+    const int2* nomap = nullptr;
+    if (e->margin_rows == 8) {
+        if (trial)
+            hipLaunchKernelGGL((k_margin<true, 8>), dim3(grid), dim3(256), lds, e->st, pd, nomap);
+        else
+            hipLaunchKernelGGL((k_margin<false, 8>), dim3(grid), dim3(256), lds, e->st, pd, nomap);
+    } else {
+        if (trial)
+            hipLaunchKernelGGL((k_margin<true, 4>), dim3(grid), dim3(256), lds, e->st, pd, nomap);
+        else
+            hipLaunchKernelGGL((k_margin<false, 4>), dim3(grid), dim3(256), lds, e->st, pd, nomap);
+    }
+}
+
+// Tile autotuning at engine creation: each candidate shape of the margin
+// pass (4 or 8 rows per wave) runs on the engine's own data, timed with HIP
+// events on its stream after a warm-up, and the fastest is kept. Both shapes
+// give the same bits (same per-row sums, -log sigma partials per 16 rows).
+// FEDNL_MARGIN_ROWS=4 or 8 sets the shape instead.
+static void tune_margin(Engine* e)
+{
+    const char* env = getenv("FEDNL_MARGIN_ROWS");
+    if (env && (env[0] == '4' || env[0] == '8') && env[1] == 0) {
+        e->margin_rows = env[0] - '0';
+        return;
+    }
+    static const int cand[2] = {4, 8};
+    float best = 0.f;
+    int pick = 4;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        if (a) hipEventDestroy(a);
+        e->margin_rows = 4;
+        return;
+    }
+    for (int q = 0; q < 2; ++q) {
+        e->margin_rows = cand[q];
+        for (int w = 0; w < 3; ++w) launch_margin(e, false);
+        hipEventRecord(a, e->st);
+        for (int r = 0; r < 10; ++r) launch_margin(e, false);
+        hipEventRecord(b, e->st);
+        float ms = 0.f;
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
+            ms = 1e30f;
+        if (q == 0 || ms < best) {
+            best = ms;
+            pick = cand[q];
+        }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    e->margin_rows = pick;
+}
 static int blocks_hessian(const ProblemDev& P) { return 8 * ((P.n + 7) / 8) * P.tiles_up; }
 static int blocks_grad(const ProblemDev& P) { return (P.d_pad / kTile) * P.n; }
 static int blocks_master(const ProblemDev& P) { return master_h_blocks(P.S) + master_grad_blocks(P.d) + 1; }
@@ -3518,8 +3589,7 @@ static int launch_round(Engine* e)
         hipLaunchKernelGGL(k_pp_sample, dim3(1), dim3(64), 0, e->st, pd, nomap);
     {
         KTime kt(e, KID_MARGIN);
-        hipLaunchKernelGGL(k_margin<false>, dim3(blocks_margin(P)), dim3(256), sizeof(double) * e->d_pad, e->st, pd,
-                           nomap);
+        launch_margin(e, false);
     }
     {
         KTime kt(e, KID_HESS);
@@ -3565,8 +3635,7 @@ static int launch_round(Engine* e)
         KTime kt(e, KID_LS);
         hipLaunchKernelGGL(k_ls_begin, dim3(1), dim3(256), 0, e->st, pd, nomap);
         for (int t = 0; t < e->o.max_line_search_steps; ++t) {
-            hipLaunchKernelGGL(k_margin<true>, dim3(blocks_margin(P)), dim3(256), sizeof(double) * e->d_pad, e->st,
-                               pd, nomap);
+            launch_margin(e, true);
             hipLaunchKernelGGL(k_ls_check, dim3(1), dim3(256), 0, e->st, pd, nomap);
         }
     }
@@ -3853,8 +3922,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         return FEDNL_E_DEVICE;
     }
     {
-        hipLaunchKernelGGL(k_margin<false>, dim3(blocks_margin(e->hdesc)), dim3(256), sizeof(double) * e->d_pad, e->st,
-                           (const ProblemDev*)e->pdev, (const int2*)nullptr);
+        tune_margin(e);
+        launch_margin(e, false);
         launch_hessian(e, true, nullptr);
         hipLaunchKernelGGL(k_init_H, dim3((unsigned)((e->S + 255) / 256)), dim3(256), 0, e->st, e->H, e->learned,
                            e->S, n, e->inv_n);
@@ -4034,7 +4103,7 @@ static void batch_launch_round(Batch* b)
     hipStream_t st = b->st;
     auto map = [&](int k) { return mp + b->off[k]; };
     if (b->cnt[BK_MARGIN])
-        hipLaunchKernelGGL(k_margin<false>, dim3(b->cnt[BK_MARGIN]), dim3(256), b->lds_margin, st, pd,
+        hipLaunchKernelGGL((k_margin<false, 4>), dim3(b->cnt[BK_MARGIN]), dim3(256), b->lds_margin, st, pd,
                            map(BK_MARGIN));
     if (b->cnt[BK_HESS64])
         hipLaunchKernelGGL(k_hessian_f64<false>, dim3(b->cnt[BK_HESS64]), dim3(256), 0, st, pd, map(BK_HESS64),
@@ -4069,7 +4138,7 @@ static void batch_launch_round(Batch* b)
     if (b->cnt[BK_LS]) {
         hipLaunchKernelGGL(k_ls_begin, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));
         for (int t = 0; t < b->ls_steps; ++t) {
-            hipLaunchKernelGGL(k_margin<true>, dim3(b->cnt[BK_LS_MARGIN]), dim3(256), b->lds_margin, st, pd,
+            hipLaunchKernelGGL((k_margin<true, 4>), dim3(b->cnt[BK_LS_MARGIN]), dim3(256), b->lds_margin, st, pd,
                                map(BK_LS_MARGIN));
             hipLaunchKernelGGL(k_ls_check, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));
         }
@@ -4603,6 +4672,15 @@ int fednl_kernel_timing(void* handle, int enable)
     return fednl_kernel_timing_mask(handle, enable ? ~0u : 0u);
 }
 
+int fednl_tuning(void* handle, int* margin_rows)
+{
+    Engine* e = (Engine*)handle;
+    if (!e || !margin_rows)
+        return FEDNL_E_MISSING_ARGUMENT;
+    *margin_rows = e->margin_rows;
+    return FEDNL_OK;
+}
+
 int fednl_kernel_timing_mask(void* handle, unsigned mask)
 {
     Engine* e = (Engine*)handle;
@@ -4925,7 +5003,7 @@ int fednl_hessians_at(void* handle, const double* x, double* out)
         err = hipMemcpy(dpd, &hp, sizeof(ProblemDev), hipMemcpyHostToDevice);
     if (err == hipSuccess) {
         // the margin weights of x, then the Hessians in the engine's mode
-        hipLaunchKernelGGL(k_margin<false>, dim3(blocks_margin(hp)), dim3(256), sizeof(double) * e->d_pad, e->st,
+        hipLaunchKernelGGL((k_margin<false, 4>), dim3(blocks_margin(hp)), dim3(256), sizeof(double) * e->d_pad, e->st,
                            (const ProblemDev*)dpd, (const int2*)nullptr);
         launch_hessian(e, true, dout, dpd);
         err = hipGetLastError();

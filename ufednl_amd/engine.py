@@ -98,6 +98,7 @@ def lib():
         L.fednl_set_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
         L.fednl_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.fednl_kernel_timing_mask.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        L.fednl_tuning.argtypes = [ctypes.c_void_p, _c_ip]
         L.fednl_kernel_times.argtypes = [ctypes.c_void_p, _c_dp, _c_ip, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_char_p)]
         L.fednl_select.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dp, _c_up, _c_up,
@@ -243,6 +244,12 @@ class Engine:
         _err(lib().fednl_set_state(self.h, dp(c(st.get("x"), np.float64)), dp(c(st.get("learned"), np.float64)),
                                    dp(c(st.get("H"), np.float64)), up(c(st.get("rng"), np.uint32)),
                                    up(c(st.get("lek"), np.uint32))), "fednl_set_state")
+
+    def tuning(self):
+        """Tile shapes picked at creation: {"margin_rows": 4 or 8}."""
+        r = np.zeros(1, dtype=np.int32)
+        _err(lib().fednl_tuning(self.h, ip(r)), "fednl_tuning")
+        return {"margin_rows": int(r[0])}
 
     def kernel_timing(self, enable=True, only=None):
         """Per-kernel device timing; only: a kernel name to time alone."""
