@@ -1430,10 +1430,11 @@ __device__ inline double fro2_from_parts(const double* part, int c, int tiles_up
 // k_select_heap: dopt's getTopKFromUpperDiagonalPart<true> heap scan
 // (SpecialMathRoutinesForMatrix.h:276-327) and, for TopLEK, the energy walk of
 // getTopLEKFromUpperDiagonalPart<true> (:377-511) up to the random draw.
-// One workgroup per client. Wave 0 runs the sequential scan: each chunk of 64
-// list positions is tested against the current root with one ballot; only
-// admitted positions enter the (sequential) heap update. Waves 1-3 stage the
-// next segment of |Delta| into LDS meanwhile.
+// One workgroup per client: all four waves build the heap and order the
+// selection; wave 0 alone runs the sequential scan, reading |Delta| from
+// global memory through a small LDS ring: each chunk of 64 list positions is
+// tested against the current root with one ballot, and only admitted
+// positions enter the (sequential) heap update.
 // Output: sorted selected list positions, count; TopLEK: hit flag, p, popped.
 // ---------------------------------------------------------------------------
 template <bool kLek>
@@ -2678,14 +2679,18 @@ __device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double*
 // padding set to the identity). Right-looking with 32-column panels and a
 // look-ahead of one panel; the right-hand side is carried along, so the
 // forward substitution happens inside the factorisation. Per panel j:
-//   A. every wave solves its 64 rows of L21 = A21 L11^-T (L11 broadcast by
-//      v_readlane), folds L21 y into the right-hand side and writes L21 to W;
-//   B. wave 0 updates the next diagonal block with FP64 MFMA and factors it
-//      (rows in lanes, columns broadcast by v_readlane) while waves 1..7
-//      apply the rest of W22 -= L21 L21^T on 16 x 16 tiles with FP64 MFMA,
-//      the next tile group's loads in flight during the current one;
+//   A. every wave solves its 64 rows of L21 = A21 L11^-T in two 16-column
+//      halves (substitution with L11 broadcast by v_readlane, the cross term
+//      on FP64 MFMA), folds L21 y into the right-hand side and writes L21 to W;
+//   B. wave 0 updates the next diagonal block with FP64 MFMA (its tiles
+//      loaded before A) and factors it (rows in lanes, columns broadcast by
+//      v_readlane) while waves 1..7 apply the rest of W22 -= L21 L21^T on
+//      16 x 16 tiles with FP64 MFMA, the next tile group's loads in flight
+//      during the current one;
 //   C. the next panel's rows below its diagonal block are staged in LDS.
-// Then L^T dx = y by 32-row blocks from the bottom, and x -= dx.
+// Then L^T dx = y by 32-row blocks from the bottom (each block's L^T rows
+// staged through LDS by waves 1..7 while wave 0 solves the block), and
+// x -= dx.
 // The factorisation fails, as dopt's, when a pivot is < 0.
 // ---------------------------------------------------------------------------
 constexpr int kNewtonThreads = 512;
