@@ -1914,19 +1914,41 @@ __device__ __forceinline__ void emit_client_body(const ProblemDev& PD, int blk)
     const uint32_t* sc = sel + (size_t)c * k;
     const double* dc = delta + (size_t)c * S;
     double* lc = learned + (size_t)c * S;
-    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-        uint32_t p;
-        if (ins < 0 || t < ins) p = sc[t];
-        else if (t == ins) p = s_pop;
-        else p = sc[t - 1];
-        const double dv = dc[p];
-        const double item = scaled ? dv * mult : (comp == FEDNL_NATURAL ? natural_round(dv) : dv);
-        vals_out[(size_t)c * k + t] = item * prescale;
-        const double upd = alpha * item;
-        lc[p] = lc[p] + upd;
-        pos_out[(size_t)c * k + t] = p;
-        idx_out[(size_t)c * k + t] = pos_flat(p, lda);
-        atomicOr(&bml[p >> 5], 1u << (p & 31));
+    // eight items per thread at a time: their positions, then their Delta and
+    // learned values, all loads of a step in flight together
+    constexpr int kEmitBatch = 8;
+    for (int t0 = threadIdx.x; t0 < cnt; t0 += blockDim.x * kEmitBatch) {
+        uint32_t pv[kEmitBatch];
+#pragma unroll
+        for (int u = 0; u < kEmitBatch; ++u) {
+            const int t = t0 + u * blockDim.x;
+            const int tt = t < cnt ? t : cnt - 1;
+            uint32_t p;
+            if (ins < 0 || tt < ins) p = sc[tt];
+            else if (tt == ins) p = s_pop;
+            else p = sc[tt - 1];
+            pv[u] = p;
+        }
+        double dv[kEmitBatch], lv[kEmitBatch];
+#pragma unroll
+        for (int u = 0; u < kEmitBatch; ++u) {
+            dv[u] = dc[pv[u]];
+            lv[u] = lc[pv[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kEmitBatch; ++u) {
+            const int t = t0 + u * blockDim.x;
+            if (t >= cnt)
+                break;
+            const uint32_t p = pv[u];
+            const double item = scaled ? dv[u] * mult : (comp == FEDNL_NATURAL ? natural_round(dv[u]) : dv[u]);
+            vals_out[(size_t)c * k + t] = item * prescale;
+            const double upd = alpha * item;
+            lc[p] = lv[u] + upd;
+            pos_out[(size_t)c * k + t] = p;
+            idx_out[(size_t)c * k + t] = pos_flat(p, lda);
+            atomicOr(&bml[p >> 5], 1u << (p & 31));
+        }
     }
     for (int t = cnt + threadIdx.x; t < k; t += blockDim.x) {
         pos_out[(size_t)c * k + t] = 0xFFFFFFFFu;
