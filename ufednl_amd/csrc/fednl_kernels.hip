@@ -2801,7 +2801,6 @@ __device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double
 #pragma unroll
     for (int cc = 0; cc < kHalf; ++cc) {
         diag_column<kHalf>(row, yr, myinv, fail, cc, lane);
-        __builtin_amdgcn_sched_barrier(0);
     }
     // A22 -= L21 L21^T: L21 (rows 16..31, columns 0..15) through LDS into the
     // MFMA operands (A[m][k] = B[k][m] = L21[m][k]), the product back through Dq
@@ -2837,7 +2836,6 @@ __device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double
 #pragma unroll
     for (int cc = kHalf; cc < kNB; ++cc) {
         diag_column<kNB>(row, yr, myinv, fail, cc, lane);
-        __builtin_amdgcn_sched_barrier(0);
     }
     if (fail) {
         if (lane == 0)
