@@ -3266,7 +3266,6 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
                 const double xc = bcast_lane(q, cc);
                 const double upd = fma(-colv[cc], xc, yr);
                 yr = (lane == cc) ? xc : ((lane < cc) ? upd : yr);
-                __builtin_amdgcn_sched_barrier(0);
             }
             if (act)
                 yv[jb + lane] = yr;
