@@ -139,6 +139,12 @@ int fednl_get_x(void *handle, double *x);
 int fednl_last_round(void *handle, double *grad, double *lk, int *cnt, uint32_t *pos, double *vals,
                      uint32_t *flat_idx, double *dx);
 
+/* The last round's Hessian differences Delta_c = H_c(x_r) - learned_c as the
+ * client pass wrote them (n*S packed upper), the input of every compressor
+ * (LocalWorkerForFedNL_TopK.h:70-71). For parity checks: the selection,
+ * message values and learned update of a round can be recomputed from it. */
+int fednl_last_delta(void *handle, double *delta);
+
 /* Whole state for lockstep checks: x (d), learned Hessians (n*S packed),
  * master Hessian estimate H (S packed), RandK/RandSeqK client generator
  * states (n), TopLEK stream states (workers). Any pointer may be NULL. */

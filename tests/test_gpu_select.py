@@ -100,7 +100,7 @@ def test_randseqk_bit_exact(oracle, d, k):
 def test_newton_solve_matches_oracle(oracle):
     from ufednl_amd import engine as E
     rng = np.random.default_rng(3)
-    for d in (1, 7, 33, 64, 100, 300):
+    for d in (1, 7, 33, 64, 100, 300, 321, 480, 511, 512):
         A = rng.standard_normal((d, d))
         Hs = A @ A.T / d + 0.1 * np.eye(d)
         Hup = np.array([Hs[i, j] for j in range(d) for i in range(j + 1)])

@@ -4662,6 +4662,16 @@ int fednl_last_round(void* handle, double* grad, double* lk, int* cnt, uint32_t*
     return FEDNL_OK;
 }
 
+int fednl_last_delta(void* handle, double* delta)
+{
+    Engine* e = (Engine*)handle;
+    if (!e || !delta)
+        return FEDNL_E_MISSING_ARGUMENT;
+    HIP_OK(hipStreamSynchronize(e->st));
+    HIP_OK(hipMemcpy(delta, e->delta, sizeof(double) * (size_t)e->n * e->S, hipMemcpyDeviceToHost));
+    return FEDNL_OK;
+}
+
 int fednl_get_state(void* handle, double* x, double* learned, double* H, uint32_t* rng, uint32_t* toplek_rng)
 {
     Engine* e = (Engine*)handle;

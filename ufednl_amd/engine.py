@@ -95,6 +95,7 @@ def lib():
         L.fednl_pp_selection.argtypes = [ctypes.c_void_p, _c_ip]
         L.fednl_last_round.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_ip, _c_up, _c_dp, _c_up, _c_dp]
         L.fednl_get_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
+        L.fednl_last_delta.argtypes = [ctypes.c_void_p, _c_dp]
         L.fednl_set_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
         L.fednl_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.fednl_kernel_timing_mask.argtypes = [ctypes.c_void_p, ctypes.c_uint]
@@ -231,6 +232,12 @@ class Engine:
              "fednl_last_round")
         return dict(grad=g, lk=lk, cnt=cnt, pos=pos.reshape(n, k), vals=vals.reshape(n, k), idx=idx.reshape(n, k),
                     dx=dx)
+
+    def last_delta(self):
+        """Delta_c = H_c(x_r) - learned_c of the last round (n x S packed upper)."""
+        out = np.zeros(self.n * self.S)
+        _err(lib().fednl_last_delta(self.h, dp(out)), "fednl_last_delta")
+        return out.reshape(self.n, self.S)
 
     def get_state(self):
         x = np.zeros(self.d); L = np.zeros(self.n * self.S); H = np.zeros(self.S)
