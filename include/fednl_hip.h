@@ -185,6 +185,21 @@ int fednl_select(int compressor, int nclients, int d, int k, const double *delta
  * factorization succeeded, 0 otherwise, negative on error. */
 int fednl_newton_solve(int d, const double *H_up, double shift, const double *b, double *dx);
 
+/* No counterpart in the reference: fednl_newton_solve for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). H_up (S packed upper), b and dx (d
+ * doubles) and ok (one int) are DEVICE pointers; the solve is enqueued on
+ * `stream` (a hipStream_t; NULL for the null stream) and the call returns
+ * without waiting. The same kernels as fednl_newton_solve give the same dx;
+ * *ok becomes 1 when the factorization succeeded, else 0 (dx = 0). The call
+ * allocates nothing: its workspace (descriptor, shift, status and the
+ * 512 x 512 Cholesky workspace, 2 MiB + 2 KiB) is a device array of the
+ * library that each GPU holds until the process ends. d <= 512, larger
+ * returns FEDNL_E_UNSUPPORTED. Calls on one GPU share that array: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+int fednl_newton_solve_noalloc(int d, const double *H_up, double shift, const double *b, double *dx, int *ok,
+                               void *stream);
+
 /* Diagnostic: raw outputs of three v_mfma_f64_16x16x4f64 probes (768 doubles)
  * used by the tests to pin the lane layout the Hessian kernel relies on.
  * No counterpart in the reference. */

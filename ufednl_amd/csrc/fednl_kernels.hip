@@ -3706,6 +3706,35 @@ static hipError_t dalloc(T** p, size_t count)
     return hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1));
 }
 
+// Raises a kernel's dynamic-LDS limit to at least `bytes` (never lowers it:
+// engines of other sizes may still launch the kernel with their own larger
+// size). The limit is per device.
+static hipError_t raise_lds_limit(const void* func, size_t bytes)
+{
+    if (bytes <= 64 * 1024)
+        return hipSuccess;
+    static std::mutex mu;
+    static std::vector<std::pair<std::pair<int, const void*>, size_t>> set;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& s : set)
+        if (s.first.first == dev && s.first.second == func) {
+            if (s.second >= bytes)
+                return hipSuccess;
+            e = hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            if (e == hipSuccess)
+                s.second = bytes;
+            return e;
+        }
+    e = hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess)
+        set.push_back({{dev, func}, bytes});
+    return e;
+}
+
 static double theory_alpha(int comp, int alpha_mode, double alpha, int k, int d, size_t S)
 {
     // TrainLoop.h:409-518; SpecialMathRoutinesForMatrix.h:195-218
@@ -3878,16 +3907,23 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         free_engine(e);
         return rc;
     }
-    if (e->lds_select > 64 * 1024) {
-        const int lds = (int)std::max(e->lds_select, sizeof(uint32_t) * e->bm_words);
-        hipFuncSetAttribute((const void*)k_select_heap<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        hipFuncSetAttribute((const void*)k_select_heap<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        hipFuncSetAttribute((const void*)k_select_heap<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        hipFuncSetAttribute((const void*)k_select_randk<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        hipFuncSetAttribute((const void*)k_select_randk<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    {
+        const size_t lds = std::max(e->lds_select, sizeof(uint32_t) * e->bm_words);
+        hipError_t le = hipSuccess;
+        const void* sel_fns[] = {(const void*)k_select_heap<false, true>, (const void*)k_select_heap<false, false>,
+                                 (const void*)k_select_heap<true, false>, (const void*)k_select_randk<true>,
+                                 (const void*)k_select_randk<false>};
+        for (const void* f : sel_fns)
+            if (le == hipSuccess)
+                le = raise_lds_limit(f, lds);
+        if (le == hipSuccess)
+            le = raise_lds_limit((const void*)k_newton, e->lds_newton);
+        if (le != hipSuccess) {
+            set_err("hipFuncSetAttribute (LDS limit)", le);
+            free_engine(e);
+            return FEDNL_E_DEVICE;
+        }
     }
-    if (e->lds_newton > 64 * 1024)
-        hipFuncSetAttribute((const void*)k_newton, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_newton);
 
     // host staging of the sign-folded, padded design
     {
@@ -4244,6 +4280,90 @@ static size_t engine_footprint(const fednl_opts* o, int n, int m, int d)
     if (o->clients_per_round > 0)                               // FedNL-PP state
         b += 8 * (2 * (size_t)n * d_pad + 2 * (size_t)n + d_pad) + (o->option_b ? 8 * (size_t)n * S : 0);
     return b;
+}
+
+// ---------------------------------------------------------------------------
+// Newton solve without allocation. Every buffer of the standalone solve that
+// is neither an input nor an output (the descriptor, the shift, the status
+// record, the round counter and the Cholesky workspace W) lives in one
+// file-scope device array. Each GPU holds its own copy until the process
+// ends; its device address comes from hipGetSymbolAddress once per device
+// (the array's name in host code is only a stand-in). Calls on one GPU share
+// the array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int kSolveScratchMaxD = kMaxNewtonD;                 // d <= 512
+constexpr size_t kSolveHdrDoubles = 256;                       // 2 KiB header
+constexpr size_t kSolveSlotShift = 128, kSolveSlotTrace = 136, kSolveSlotCtr = 152;
+constexpr size_t kSolveScratchDoubles =
+    kSolveHdrDoubles + (size_t)kSolveScratchMaxD * (size_t)kSolveScratchMaxD;  // + W: 2 MiB
+static_assert(sizeof(ProblemDev) <= kSolveSlotShift * sizeof(double), "descriptor fits the header");
+static_assert(kSolveSlotTrace + kTraceStride <= kSolveSlotCtr, "status record fits before the counter");
+
+static __device__ __align__(256) double scratch[kSolveScratchDoubles];
+
+// One launch before the solve: the descriptor (by value) and the shift into
+// the header, the status record and round counter cleared, W cleared with
+// the identity on its padding (what the allocating solve uploads).
+__global__ __launch_bounds__(256) void k_solve_scratch_init(ProblemDev hp, double shift, double* base, int d, int dp)
+{
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nthr = (size_t)gridDim.x * blockDim.x;
+    if (tid == 0) {
+        *reinterpret_cast<ProblemDev*>(base) = hp;
+        base[kSolveSlotShift] = shift;
+        *reinterpret_cast<int*>(base + kSolveSlotCtr) = 0;
+    }
+    if (tid < (size_t)kTraceStride)
+        base[kSolveSlotTrace + tid] = 0.0;
+    double* W = base + kSolveHdrDoubles;
+    const size_t total = (size_t)dp * dp;
+    for (size_t e = tid; e < total; e += nthr) {
+        const size_t i = e / dp, j = e - i * dp;
+        W[e] = (i == j && (int)i >= d) ? 1.0 : 0.0;
+    }
+}
+
+// The factorization flag of the status record to the caller's int.
+__global__ void k_solve_status(const double* base, int* ok)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *ok = (base[kSolveSlotTrace + 3] == 1.0) ? 1 : 0;
+}
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64];
+
+// The calling thread's current device's copy of `scratch` (resolved once per
+// device, with that device current).
+static int solve_scratch(double** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_err("hipGetDevice", e);
+        return FEDNL_E_DEVICE;
+    }
+    if (dev < 0 || dev >= 64) {
+        set_err("fednl_newton_solve_noalloc: device index out of range");
+        return FEDNL_E_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_err("hipGetSymbolAddress (solve scratch)", e);
+            return FEDNL_E_DEVICE;
+        }
+        e = raise_lds_limit((const void*)k_newton, newton_lds_bytes(kSolveScratchMaxD));
+        if (e != hipSuccess) {
+            set_err("hipFuncSetAttribute (k_newton)", e);
+            return FEDNL_E_DEVICE;
+        }
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FEDNL_OK;
 }
 
 }  // namespace fednl
@@ -4798,13 +4918,9 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     if (rng) hipMemcpyAsync(drng, rng, sizeof(uint32_t) * nclients, hipMemcpyHostToDevice, st);
     if (toplek_rng) hipMemcpyAsync(dlek, toplek_rng, sizeof(uint32_t) * workers, hipMemcpyHostToDevice, st);
     const size_t lds = select_lds_bytes(std::min(k, kMaxHeapK), S);
-    if (lds > 64 * 1024) {
-        hipFuncSetAttribute((const void*)k_select_heap<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
-        hipFuncSetAttribute((const void*)k_select_heap<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
-        hipFuncSetAttribute((const void*)k_select_randk<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    }
+    raise_lds_limit((const void*)k_select_heap<false, false>, lds);
+    raise_lds_limit((const void*)k_select_heap<true, false>, lds);
+    raise_lds_limit((const void*)k_select_randk<false>, lds);
     // TopLEK needs ||Delta||^2: computed on the host-side copy here as tile partials would be
     // in the round; the standalone path passes one partial per client.
     double* parts = nullptr;
@@ -5087,8 +5203,7 @@ int fednl_newton_solve(int d, const double* H_up, double shift, const double* b,
         HIP_OK(hipMemcpy(dW, w0.data(), sizeof(double) * w0.size(), hipMemcpyHostToDevice));
     }
     const size_t lds = newton_lds_bytes(dp);
-    if (lds > 64 * 1024)
-        hipFuncSetAttribute((const void*)k_newton, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    HIP_OK(raise_lds_limit((const void*)k_newton, lds));
     ProblemDev hp{};
     hp.d = d;
     hp.dp = dp;
@@ -5113,6 +5228,49 @@ int fednl_newton_solve(int d, const double* H_up, double shift, const double* b,
     HIP_OK(hipMemcpy(dx, dx_d, sizeof(double) * d, hipMemcpyDeviceToHost));
     hipFree(dH); hipFree(dsh); hipFree(db); hipFree(dW); hipFree(dx_d); hipFree(dtr); hipFree(dctr); hipFree(dpd);
     return tr[3] == 1.0 ? 1 : 0;
+}
+
+int fednl_newton_solve_noalloc(int d, const double* H_up, double shift, const double* b, double* dx, int* ok,
+                               void* stream)
+{
+    if (d <= 0 || !H_up || !b || !dx || !ok) {
+        set_err("fednl_newton_solve_noalloc: null argument or d <= 0");
+        return FEDNL_E_MISSING_ARGUMENT;
+    }
+    if (d > kSolveScratchMaxD) {
+        set_err("fednl_newton_solve_noalloc: d > 512 (the solve scratch holds a 512 x 512 workspace)");
+        return FEDNL_E_UNSUPPORTED;
+    }
+    double* base = nullptr;
+    int rc = solve_scratch(&base);
+    if (rc != FEDNL_OK)
+        return rc;
+    const int dp = round_up(d, kNB);
+    hipStream_t st = (hipStream_t)stream;
+    ProblemDev hp{};
+    hp.d = d;
+    hp.dp = dp;
+    hp.trace_cap = 1;
+    hp.update_x = 0;
+    hp.hup = H_up;
+    hp.shift = base + kSolveSlotShift;
+    hp.nb = b;
+    hp.W = base + kSolveHdrDoubles;
+    hp.x = nullptr;
+    hp.dx = dx;
+    hp.trace = base + kSolveSlotTrace;
+    hp.round_ctr = reinterpret_cast<int*>(base + kSolveSlotCtr);
+    const int init_blocks = std::max(1, std::min(1024, (dp * dp + 255) / 256));
+    hipLaunchKernelGGL(k_solve_scratch_init, dim3(init_blocks), dim3(256), 0, st, hp, shift, base, d, dp);
+    hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), newton_lds_bytes(dp), st, (const ProblemDev*)base,
+                       (const int2*)nullptr);
+    hipLaunchKernelGGL(k_solve_status, dim3(1), dim3(64), 0, st, (const double*)base, ok);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_err("fednl_newton_solve_noalloc launch", e);
+        return FEDNL_E_DEVICE;
+    }
+    return FEDNL_OK;
 }
 
 }  // extern "C"

@@ -105,6 +105,8 @@ def lib():
         L.fednl_select.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dp, _c_up, _c_up,
                                    ctypes.c_int, ctypes.c_double, _c_up, _c_ip]
         L.fednl_newton_solve.argtypes = [ctypes.c_int, _c_dp, ctypes.c_double, _c_dp, _c_dp]
+        vp = ctypes.c_void_p
+        L.fednl_newton_solve_noalloc.argtypes = [ctypes.c_int, vp, ctypes.c_double, vp, vp, vp, vp]
         L.fednl_hessians_at.argtypes = [ctypes.c_void_p, _c_dp, _c_dp]
         L.fednl_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_void_p)]
@@ -301,6 +303,13 @@ def newton_solve(H_up, d, shift, b):
     dx = np.zeros(d)
     ok = _err(lib().fednl_newton_solve(d, dp(H_up), shift, dp(b), dp(dx)), "fednl_newton_solve")
     return bool(ok), dx
+
+
+def newton_solve_noalloc(d, H_up_ptr, shift, b_ptr, dx_ptr, ok_ptr, stream=None):
+    """fednl_newton_solve_noalloc on device pointers (ints), enqueued on
+    `stream` (a hipStream_t handle, None for the null stream)."""
+    _err(lib().fednl_newton_solve_noalloc(d, H_up_ptr, shift, b_ptr, dx_ptr, ok_ptr, stream),
+         "fednl_newton_solve_noalloc")
 
 
 class Batch:
