@@ -187,3 +187,18 @@ def test_config4_member_randk_trajectory_in_batch(oracle):
     eng.close()
     other.close()
     assert_grad_norms_close(g, g_ref, feature_scale=float(np.abs(big.features).max()))
+
+
+# ------------------------------------------------------------------ config5 shape
+
+def test_config5_d700_lockstep(oracle):
+    """A config5-shaped problem past the single-workgroup range (d = 700:
+    block columns of 22 panels, S = 245350, k = 5600): lockstep rounds."""
+    prob = datagen.gaussian_problem("c5_member_n32_m200_d700", 32, 200, 700, seed=51)
+    orc = oracle.FedNL(prob, "topk", k_mult=8.0)
+    orc.round()
+    eng = _engine(prob, "topk")
+    for r in range(2):
+        diag = lockstep_round(eng, orc, "topk", max_flip_frac=1e-3)
+        print(f"config5 d=700 lockstep round {r + 1}: {diag}", flush=True)
+    eng.close()

@@ -112,6 +112,37 @@ def test_newton_solve_matches_oracle(oracle):
         np.testing.assert_allclose(dx, dx_o, rtol=1e-10, atol=1e-12 * np.abs(dx_o).max())
 
 
+@pytest.mark.parametrize("d", [700, 1500, 2048])
+def test_newton_solve_large_d_matches_oracle(oracle, d):
+    """Block columns split into several 512-row segments (d > 512)."""
+    from ufednl_amd import engine as E
+    rng = np.random.default_rng(d)
+    A = rng.standard_normal((d, d))
+    Hs = A @ A.T / d + 0.1 * np.eye(d)
+    Hup = np.concatenate([Hs[: j + 1, j] for j in range(d)])
+    b = rng.standard_normal(d)
+    print(f"newton d={d}", flush=True)
+    ok, dx = E.newton_solve(Hup, d, 1e-3, b)
+    ok_o, dx_o, _ = oracle.cholesky_solve(Hup, d, 1e-3, b)
+    assert ok and ok_o
+    np.testing.assert_allclose(dx, dx_o, rtol=1e-9, atol=1e-11 * np.abs(dx_o).max())
+    # and the solve itself: (H + shift I) dx = b
+    r = (Hs + 1e-3 * np.eye(d)) @ dx - b
+    assert np.abs(r).max() <= 1e-10 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("d", [6, 300, 700])
+def test_newton_solve_reports_failure_any_size(d):
+    """A negative pivot in a late block column fails the factorization."""
+    from ufednl_amd import engine as E
+    Hs = np.eye(d)
+    Hs[d - 1, d - 1] = -1.0
+    Hup = np.concatenate([Hs[: j + 1, j] for j in range(d)])
+    ok, dx = E.newton_solve(Hup, d, 0.0, np.ones(d))
+    assert not ok
+    assert np.all(dx == 0.0)
+
+
 def test_newton_solve_reports_failure():
     from ufednl_amd import engine as E
     d = 6

@@ -13,7 +13,8 @@
 //                   whose random rounding runs over the other clients)
 //   k_master        H[idx] += val in client order (deterministic), gradient / Lk / f
 //                   averages, the shift, the trace record
-//   k_newton        Cholesky of H + shift I, two triangular solves, x -= dx
+//   k_newton_mc     Cholesky of H + shift I, two triangular solves, x -= dx, over
+//                   one workgroup per 32-column block (left-looking, flag handoffs)
 //
 // Device layout (per problem):
 //   A       n x m_pad x d_pad  sample rows b_s * a_s (sign folded as dopt's
@@ -54,8 +55,11 @@ __device__ unsigned long long g_probe[16384];
     do { if (threadIdx.x == 0) g_probe[(slot)] = (unsigned long long)(v); } while (0)
 #define PROBE_ADD_AT(slot, t0, thread) \
     do { if (threadIdx.x == (thread)) atomicAdd(&g_probe[(slot)], wall_clock64() - (t0)); } while (0)
+#define PROBE_STAMP_AT(slot, thread) \
+    do { if (threadIdx.x == (thread)) g_probe[(slot)] = wall_clock64(); } while (0)
 #else
 #define PROBE_ADD_AT(slot, t0, thread) do {} while (0)
+#define PROBE_STAMP_AT(slot, thread) do {} while (0)
 #define PROBE_T(var) do {} while (0)
 #define PROBE_ADD(slot, t0) do {} while (0)
 #define PROBE_SET(slot, v) do {} while (0)
@@ -102,7 +106,8 @@ constexpr int kMaxLineSearchSteps = 64;  // trial launches enqueued per round
 // then x_k (d) from kLsHead on
 enum MstSlot { LS_FXK, LS_T, LS_DD, LS_DONE, LS_TRIALS, PP_LI, kLsHead = 8 };
 constexpr int kNB = 32;            // Cholesky panel width
-constexpr int kMaxNewtonD = 512;   // single-workgroup Newton solve limit (LDS panel)
+constexpr int kMaxNewtonD = 512;   // largest d of the no-allocation Newton solve
+constexpr int kMaxD = 2048;        // features per problem
 
 __host__ __device__ inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
@@ -255,6 +260,12 @@ struct ProblemDev {
     const double* nb;   // Newton: right-hand side
     uint32_t *sel, *pos, *idx, *rng, *lek_state, *lek_popped, *seq_start, *sel_bm;
     int *sel_cnt, *cnt, *lek_hit, *round_ctr;
+    // Newton over many workgroups: sync words (epoch, ticket, failure epoch,
+    // error, per block column: segments done, diagonal block done), the
+    // forward solution y, the reciprocal pivots and each diagonal block's
+    // L^-T (32 x 32 per block column)
+    int* nsync;
+    double *ny, *ninv, *nxt;
 };
 
 #define FEDNL_BLOCK()                                                                     \
@@ -2088,7 +2099,7 @@ __device__ __forceinline__ void master_h_block(const ProblemDev& PD, int blk)
     H[p] = h;
     if (W) {
         // the Newton workspace: W(j, i) = H(i, j), column-major lower, ld = dp;
-        // k_newton adds the shift on the diagonal (TrainLoop.h:1449-1454)
+        // k_newton_mc adds the shift on the diagonal (TrainLoop.h:1449-1454)
         const uint32_t j = pos_col(p);
         const uint32_t i = (uint32_t)(p - (size_t)j * (j + 1) / 2);
         W[(size_t)i * dp + j] = h;
@@ -2359,7 +2370,7 @@ __global__ __launch_bounds__(256) void k_master(FEDNL_KARGS)
 }
 
 // ---------------------------------------------------------------------------
-// FedNL-LS (TrainLoop.h:1546-1640), after k_newton has written dx and left x
+// FedNL-LS (TrainLoop.h:1546-1640), after k_newton_mc has written dx and left x
 // at x_k. k_ls_begin keeps x_k, computes d = -(dx, g) and sets x to the first
 // trial x_k - 1 * dx. Each trial is then k_margin<true> (the clients' f_i at
 // the trial point) and k_ls_check: f_req = mean_c f_c is accepted when
@@ -2435,7 +2446,7 @@ __global__ __launch_bounds__(256) void k_ls_check(FEDNL_KARGS)
         st[LS_T] = t;
         st[LS_TRIALS] = (double)trials;
         st[LS_DONE] = done ? 1.0 : 0.0;
-        const int r = *PD.round_ctr - 1;  // k_newton has counted the round
+        const int r = *PD.round_ctr - 1;  // k_newton_mc has counted the round
         if (r >= 0 && r < PD.trace_cap)
             PD.trace[(size_t)r * kTraceStride + 4] = (double)trials;
         t_next = t;
@@ -2503,7 +2514,7 @@ __global__ __launch_bounds__(256) void k_pp_client(FEDNL_KARGS)
     const int d = PD.d, d_pad = PD.d_pad;
     const size_t S = PD.S;
     const double* __restrict__ L = PD.learned + (size_t)c * S;
-    __shared__ double xs[kMaxNewtonD];
+    __shared__ double xs[kMaxD];
     __shared__ double scratch[16];
     __shared__ double s_lk;
     for (int i = threadIdx.x; i < d; i += blockDim.x) xs[i] = PD.x[i];
@@ -2574,7 +2585,8 @@ __global__ __launch_bounds__(256) void k_pp_init_avg(FEDNL_KARGS)
 // reciprocal), the column is eliminated from every other row. The products
 // and sums are rounded separately, as in the reference's loops. A zero
 // tolerance never skips a column, so dx = b at the end. Called by every
-// thread of k_newton's workgroup; kept out of line, off the solve's hot path.
+// thread of the last k_newton_mc workgroup; kept out of line, off the solve's
+// hot path.
 // ---------------------------------------------------------------------------
 __device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double* __restrict__ lds)
 {
@@ -2584,7 +2596,7 @@ __device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double*
     const gmem<double>* __restrict__ H = G(PD.H);
     const gmem<double>* __restrict__ H0 = G(PD.H0);
     const double shift = *PD.shift;
-    // k_newton's LDS: b, the eliminated column, the pivot row, reduction slots
+    // the solve's LDS: b, the eliminated column, the pivot row, reduction slots
     double* bv = lds;
     double* kcol = lds + ld;
     double* prow = lds + 2 * ld;
@@ -2693,47 +2705,12 @@ __device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double*
     }
 }
 
-// ---------------------------------------------------------------------------
-// k_newton: Cholesky of (H + shift I) and the two triangular solves of the
-// master (TrainLoop.h:1449-1466; CholeskyFactorization.h:153-272;
-// ElementarySolvers.h:72-226) in one workgroup of 8 waves. W holds the lower
-// triangle of H + shift I (column-major, ld = dp, dp = d rounded up to 32,
-// padding set to the identity). Right-looking with 32-column panels and a
-// look-ahead of one panel; the right-hand side is carried along, so the
-// forward substitution happens inside the factorisation. Per panel j:
-//   A. every wave solves its 64 rows of L21 = A21 L11^-T in two 16-column
-//      halves (substitution with L11 broadcast by v_readlane, the cross term
-//      on FP64 MFMA), folds L21 y into the right-hand side and writes L21 to W;
-//   B. wave 0 updates the next diagonal block with FP64 MFMA (its tiles
-//      loaded before A) and factors it (rows in lanes, columns broadcast by
-//      v_readlane) while waves 1..7 apply the rest of W22 -= L21 L21^T on
-//      16 x 16 tiles with FP64 MFMA, the next tile group's loads in flight
-//      during the current one;
-//   C. the next panel's rows below its diagonal block are staged in LDS.
-// Then L^T dx = y by 32-row blocks from the bottom (each block's L^T rows
-// staged through LDS by waves 1..7 while wave 0 solves the block), and
-// x -= dx.
-// The factorisation fails, as dopt's, when a pivot is < 0.
-// ---------------------------------------------------------------------------
-constexpr int kNewtonThreads = 512;
-static_assert(kNewtonThreads == 512, "k_newton's panel staging splits 32 columns over two halves of 256 threads");
-static_assert(kMaxNewtonD - 32 < (kNewtonThreads / 64) * 64, "k_newton TRSM assumes one 64-row chunk per wave");
-static_assert(kMaxNewtonD <= kNewtonThreads, "k_newton back solve assumes one row per thread");
 
-// Row stride (doubles) of k_newton's panel rows in LDS. The MFMA operand
+// Row stride (doubles) of k_newton_mc's block-column rows in LDS. The MFMA operand
 // reads take rows l16 = 0..15 and columns kq = 0..3 in one instruction; with
 // a stride of 2 mod 32 the 32 lanes of each half-wave hit 32 distinct bank
 // pairs.
 constexpr int kNewtonPS = kNB + 2;
-
-__host__ __device__ constexpr size_t newton_lds_bytes(int dp)
-{
-    // rows below the diagonal block, row stride kNewtonPS (at least one row);
-    // after a failed factorization the fallback solve reuses it (3 dp + 16)
-    const size_t panel = (size_t)kNewtonPS * (size_t)(dp > kNB ? dp - kNB : 1);
-    const size_t fallback = 3 * (size_t)dp + 16;
-    return sizeof(double) * (panel > fallback ? panel : fallback);
-}
 
 // 1/sqrt(a) and sqrt(a) for a > 0: v_rsq_f64 and two Newton steps, then one
 // correction of the square root (within an ulp or two of the rounded values)
@@ -2776,428 +2753,548 @@ __device__ __forceinline__ void diag_column(double (&row)[kNB], double& yr, doub
         row[c2] = fma(-row[cc], bcast_lane(row[cc], c2), row[c2]);
 }
 
-// Factors the 32 x 32 block held in Dg (row-major, lower part meaningful) in
-// wave 0 and solves its part of L y = b: on return Dg holds L11, invd /
-// invall the reciprocal pivots, yv[jb..jb+31] the solution. Two 16-column
-// halves: the first is factored with rows in lanes, then the lower-right
-// quarter takes A22 -= L21 L21^T on FP64 MFMA (Dq holds the product), then
-// the second half is factored.
-__device__ __forceinline__ void newton_diag_factor(double (*Dg)[kNB + 1], double (*Dq)[17], double* invd,
-                                                   double* invall, double* yv, int jb, int lane, int* s_fail)
+// ---------------------------------------------------------------------------
+// k_newton_mc: the master's Newton solve (TrainLoop.h:1449-1466;
+// CholeskyFactorization.h:153-272; ElementarySolvers.h:72-226) over many
+// workgroups, for d up to kMaxMcD.
+//
+// W holds the lower triangle of H (column-major, ld = dp, dp = d rounded up
+// to 32, padding set to the identity); the shift joins the diagonal as the
+// columns are loaded. Block column J (columns 32J .. 32J+31) is owned by
+// workgroups (J, s), each holding rows 32J + 512 s .. in LDS (left-looking):
+//   1. for every finished panel K < J: Col -= P_K[rows] P_K[block J]^T on
+//      FP64 MFMA (P_K = block column K of L, read from W);
+//   2. (J, 0): its diagonal block is factored by one wave with all 64 lanes:
+//      rows 0..31 hold the block, rows 32..63 an identity, so the same column
+//      operations give L11 and L11^-T together (the right-hand side is
+//      carried along: y_J, the forward substitution);
+//   3. every row below the diagonal block becomes Col L11^-T on FP64 MFMA;
+//      (J, 0) does the 32 rows of block J+1 first and publishes them with
+//      L11, L11^-T and y_J (the next diagonal factor needs only these), then
+//      the rest; (J, s > 0) wait for L11^-T.
+// The workgroup that finishes last (a ticket) runs L^T dx = y by blocks from
+// the bottom (dx_J = L11^-T r_J, then r folded over the rows above) and
+// x -= dx. A workgroup only ever waits for lower-numbered ones, which are
+// dispatched first, so the launch needs no co-residency. Waits are bounded
+// (an expired wait sets the error word). Flags carry an epoch (one per
+// solve), so nothing is reset between solves.
+// A negative pivot fails the factorisation, as dopt's: the last workgroup
+// then solves by Gauss-Jordan elimination (newton_fallback_solve).
+// ---------------------------------------------------------------------------
+constexpr int kSegRows = 512;                 // rows of a block column one workgroup holds
+constexpr int kMcThreads = 512;
+constexpr int kMaxMcD = 2048;
+constexpr long kMcSpinCap = 1L << 22;         // polls of one wait before it gives up
+static_assert(kSegRows == (kMcThreads / 64) * 64, "one 64-row chunk per wave per segment");
+
+__host__ __device__ inline int mc_nseg(int dp, int J) { return (dp - kNB * J + kSegRows - 1) / kSegRows; }
+__host__ __device__ inline int mc_blocks(int dp)
+{
+    int t = 0;
+    for (int J = 0; J < dp / kNB; ++J) t += mc_nseg(dp, J);
+    return t;
+}
+__host__ __device__ constexpr int mc_sync_words(int dp) { return 4 + 2 * (dp / kNB); }
+__host__ __device__ inline size_t mc_lds_bytes(int dp)
+{
+    const size_t col = sizeof(double) * (size_t)kNewtonPS * (size_t)(dp < kSegRows ? dp : kSegRows);
+    const size_t fallback = sizeof(double) * (3 * (size_t)dp + 16);
+    return col > fallback ? col : fallback;
+}
+
+// What one workgroup hands to another goes through device-coherent
+// (sc1) stores and loads: write-through to memory and never served from a
+// non-coherent L2 copy. A flag is stored only after the wave's earlier
+// stores have completed (s_waitcnt), and data are loaded only after the
+// flag has been seen, so no L2 writeback or invalidate is needed.
+__device__ __forceinline__ double ld_dev(const gmem<double>* p)
+{
+    return __hip_atomic_load((gmem<double>*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(gmem<double>* p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// every memory operation of the wave issued so far has completed
+__device__ __forceinline__ void mc_drain()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Waits until *p >= v. Every lane of the calling wave runs the same loop.
+// Returns false (and sets the error word) if the wait expired.
+__device__ inline bool mc_wait_ge(int* p, int v, int* err)
+{
+    long n = 0;
+    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++n > kMcSpinCap) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+    }
+    mc_drain();
+    return n <= kMcSpinCap;
+}
+
+// Intra-workgroup signal between waves through LDS.
+__device__ inline void mc_lds_signal(volatile int* f, int v)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    *f = v;
+}
+__device__ inline void mc_lds_wait(volatile int* f, int v)
+{
+    long n = 0;
+    while (*f < v && ++n <= kMcSpinCap)
+        __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Col rows 16 rt .. 16 rt + 15 (local) for rt = rt0, rt0 + step, .. < rt1, both
+// 16-column tiles (only the lower one where lower_only holds for rt == 0),
+// minus P_K[r0 + row][k] P_K[brow + col][k] summed over k < 32, where
+// P_K[r][k] = W[(32 K + k) dp + r]. The next tile's loads are in flight
+// during the current tile's MFMAs.
+__device__ inline void mc_update_tiles(const gmem<double>* __restrict__ W, int dp, int K, int r0, int brow,
+                                       double* __restrict__ Col, int rt0, int step, int rt1, bool lower_first,
+                                       int lane)
+{
+    if (rt0 >= rt1)
+        return;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const gmem<double>* __restrict__ PK = W + (size_t)(kNB * K + kq) * dp;
+    double av[2][8];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            av[ct][q] = ld_dev(PK + (size_t)(4 * q) * dp + brow + 16 * ct + l16);
+    double bv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        bv[q] = ld_dev(PK + (size_t)(4 * q) * dp + r0 + 16 * rt0 + l16);
+    for (int rt = rt0; rt < rt1; rt += step) {
+        double bn[8];
+        const int rn = (rt + step < rt1) ? rt + step : rt;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            bn[q] = ld_dev(PK + (size_t)(4 * q) * dp + r0 + 16 * rn + l16);
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            if (lower_first && rt == 0 && ct == 1)
+                continue;  // above the diagonal of the diagonal block
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ct][q], bv[q], acc, 0, 0, 0);
+            double* cp = Col + (16 * rt + l16) * kNewtonPS + 16 * ct + kq;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                cp[4 * r] = cp[4 * r] - acc[r];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            bv[q] = bn[q];
+    }
+}
+
+// Col rows 16 rt .. (local) for rt = rt0, rt0 + step, .. < rt1 become
+// Col L11^-T (X = L11^-T, upper, row-major in Xt with stride kNB + 1) on FP64
+// MFMA: the triangular solve of the rows below a diagonal block.
+__device__ inline void mc_solve_tiles(double* __restrict__ Col, const double (*Xt)[kNB + 1], int rt0, int step,
+                                      int rt1, int lane)
+{
+    if (rt0 >= rt1)
+        return;
+    const int l16 = lane & 15, kq = lane >> 4;
+    double av[2][8];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            av[ct][q] = Xt[4 * q + kq][16 * ct + l16];  // A[m = col][k] = X[k][col]
+    for (int rt = rt0; rt < rt1; rt += step) {
+        double* rp = Col + (16 * rt + l16) * kNewtonPS + kq;
+        double bv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            bv[q] = rp[4 * q];  // B[k][n = row]
+        __builtin_amdgcn_wave_barrier();  // the tile's rows are read before they are overwritten
+        dbl4 acc[2];
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ct][q], bv[q], acc[ct], 0, 0, 0);
+        }
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                rp[16 * ct + 4 * r] = acc[ct][r];
+    }
+}
+
+// The diagonal block's factorisation in one wave: lane r < 32 holds row r of
+// the block (Dg), lane 32 + q row q of an identity. The same column
+// operations (the lanes execute them anyway) turn the block rows into L11
+// and the identity rows into L11^-T (the panel rule X L11^T = I); the
+// right-hand side ry is carried along (forward substitution: ry becomes y).
+// Two 16-column halves; between them rows 16..63 take -R L21^T on FP64 MFMA
+// (R: their first 16 columns), one 16-row tile at a time, the rows of L21
+// themselves last. On return Dg holds L11 (lower), Xt L11^-T (upper), invd
+// and invall 1 / L11[i][i]. A negative pivot sets *s_fail.
+__device__ inline void mc_diag_factor(double (*Dg)[kNB + 1], double (*Xt)[kNB + 1], double (*Sq)[17],
+                                      double* invd, double* ry, int lane, int* s_fail)
 {
     constexpr int kHalf = kNB / 2;
-    static_assert(kHalf == 16, "one 16 x 16 MFMA tile per quarter");
     double row[kNB];
-    const bool act = lane < kNB;
-    const int lr = act ? lane : 0;
-    // entries above the diagonal are carried along but never feed the lower
-    // triangle, so the updates below need no per-lane predicates
+    const bool blk = lane < kNB;
+    const int lr = lane & (kNB - 1);
+    double* const src = blk ? &Dg[lr][0] : &Xt[lr][0];  // Xt holds the identity on entry
 #pragma unroll
     for (int c = 0; c < kNB; ++c)
-        row[c] = Dg[lr][c];
-    double yr = yv[jb + lr];
+        row[c] = src[c];
+    double yr = blk ? ry[lr] : 0.0;
     double myinv = 1.0;
     bool fail = false;
 #pragma unroll
-    for (int cc = 0; cc < kHalf; ++cc) {
+    for (int cc = 0; cc < kHalf; ++cc)
         diag_column<kHalf>(row, yr, myinv, fail, cc, lane);
-    }
-    // A22 -= L21 L21^T: L21 (rows 16..31, columns 0..15) through LDS into the
-    // MFMA operands (A[m][k] = B[k][m] = L21[m][k]), the product back through Dq
-    if (lane >= kHalf && lane < kNB) {
+    // rows 16..63 (lanes 16..63), first half of their columns, to LDS
+    if (lane >= kHalf) {
 #pragma unroll
         for (int c = 0; c < kHalf; ++c)
-            Dg[lane][c] = row[c];
+            Sq[lane - kHalf][c] = row[c];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     {
         const int l16 = lane & 15, kq = lane >> 4;
-        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        // U[i][c] = sum_k S[i][k] L21[c][k] (L21 = S rows 0..15), tile by
+        // tile over the rows i, each written over its own rows; tile 0 (the
+        // rows of L21) last
+#pragma unroll 1
+        for (int tt = 0; tt < 3; ++tt) {
+            const int t = (tt + 1) % 3;
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k0 = 0; k0 < kHalf; k0 += 4) {
-            const double av = Dg[kHalf + l16][k0 + kq];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, av, acc, 0, 0, 0);
+            for (int k0 = 0; k0 < kHalf; k0 += 4) {
+                const double av = Sq[l16][k0 + kq];           // A[m = c][k]
+                const double bv = Sq[16 * t + l16][k0 + kq];  // B[k][n = row]
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                Sq[16 * t + l16][4 * r + kq] = acc[r];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            Dq[4 * r + kq][l16] = acc[r];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     {
-        const int m = (lane >= kHalf && lane < kNB) ? lane - kHalf : 0;
+        const int i = lane >= kHalf ? lane - kHalf : 0;
 #pragma unroll
         for (int c = 0; c < kHalf; ++c)
-            row[kHalf + c] = row[kHalf + c] - Dq[m][c];
+            row[kHalf + c] = (lane >= kHalf) ? row[kHalf + c] - Sq[i][c] : row[kHalf + c];
     }
 #pragma unroll
-    for (int cc = kHalf; cc < kNB; ++cc) {
+    for (int cc = kHalf; cc < kNB; ++cc)
         diag_column<kNB>(row, yr, myinv, fail, cc, lane);
-    }
-    if (fail) {
+    if (__ballot(fail)) {
         if (lane == 0)
             *s_fail = 1;
-    } else if (act) {
+    }
 #pragma unroll
-        for (int c = 0; c < kNB; ++c)
-            Dg[lane][c] = row[c];
-        invd[lane] = myinv;
-        invall[jb + lane] = myinv;
-        yv[jb + lane] = yr;
+    for (int c = 0; c < kNB; ++c)
+        src[c] = row[c];
+    if (blk) {
+        invd[lr] = myinv;
+        ry[lr] = yr;
     }
 }
 
-// lower 16 x 16 tile index tt -> (row tile, column tile), column <= row
-__device__ inline void tri_tile(int tt, int& ti, int& tj)
-{
-    int t = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-    while (t * (t + 1) / 2 > tt) --t;
-    while ((t + 1) * (t + 2) / 2 <= tt) ++t;
-    ti = t;
-    tj = tt - t * (t + 1) / 2;
-}
-
-__global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
+__global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
 {
     FEDNL_BLOCK();
-    (void)blk;
-    const int d = PD.d, dp = PD.dp, trace_cap = PD.trace_cap, update_x = PD.update_x;
-    const gmem<double>* __restrict__ Hup = G(PD.hup);
-    const gmem<double>* __restrict__ shift_p = G(PD.shift);
-    const gmem<double>* __restrict__ b = G(PD.nb);
+    const int d = PD.d, dp = PD.dp, nb = dp / kNB;
+    int J = 0, s = blk;
+    while (J < nb - 1 && s >= mc_nseg(dp, J)) {
+        s -= mc_nseg(dp, J);
+        ++J;
+    }
+    const int r0 = kNB * J + kSegRows * s;       // global row of Col row 0
+    const int nrows = min(kSegRows, dp - r0);    // multiple of 32
+    const int ntile = nrows / 16;
+    int* const sync = PD.nsync;
+    int* const cnt = sync + 4;
+    int* const dflag = sync + 4 + nb;
+    int* const err = sync + 3;
     gmem<double>* __restrict__ W = G(PD.W);
-    gmem<double>* __restrict__ x = G(PD.x);
-    gmem<double>* __restrict__ dx_out = G(PD.dx);
-    gmem<double>* __restrict__ trace = G(PD.trace);
-    gmem<int>* __restrict__ round_ctr = G(PD.round_ctr);
+    const gmem<double>* __restrict__ b = G(PD.nb);
+    gmem<double>* __restrict__ ny = G(PD.ny);
+    gmem<double>* __restrict__ ninv = G(PD.ninv);
+    gmem<double>* __restrict__ nxt = G(PD.nxt);
     extern __shared__ __align__(16) double lds[];
-    constexpr int kPS = kNewtonPS;                  // row stride of the panel in LDS
-    double* P21 = lds;                              // rows below the diagonal block: P21[r * kPS + c]
-    __shared__ double Dg[kNB][kNB + 1];             // L11 of the current panel, row-major (lower)
-    __shared__ double Dq[16][17];                   // the diagonal factor's MFMA quarter update
-    __shared__ double invd[kNB];                    // 1 / L11[c][c]
-    __shared__ double invall[kMaxNewtonD];          // 1 / L[i][i] for the back solve
-    __shared__ double yv[kMaxNewtonD];              // right-hand side -> y -> dx
-    __shared__ int s_fail;
+    double* Col = lds;  // nrows x kNewtonPS
+    __shared__ double Dg[kNB][kNB + 1];   // the diagonal block -> L11 (lower)
+    __shared__ double Xt[kNB][kNB + 1];   // L11^-T (upper)
+    __shared__ double Dq[48][17];         // the diagonal factor's quarter products (rows 16..63)
+    __shared__ double invd[kNB], ry[kNB];
+    __shared__ int s_epoch, s_fail, s_last;
+    __shared__ int s_l11, s_rows, s_t32, s_upd;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63, nwaves = nthr >> 6;
-    const int l16 = lane & 15, kq = lane >> 4;
-    const double shift = *shift_p;
-    if (tid == 0)
+    if (tid == 0) {
+        s_epoch = __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
         s_fail = 0;
-    PROBE_T(t_n0);
-    // W(i,j) = H(j,i) (+shift on the diagonal) for i >= j < d, from the packed
-    // upper H in order (eight loads in flight per thread), unless the caller
-    // has written W already (Hup == nullptr). The padding of W (identity) is
-    // set once when W is allocated and survives the factorisation.
-    if (Hup) {
-        const size_t S = (size_t)d * (d + 1) / 2;
-        for (size_t p0 = tid; p0 < S; p0 += (size_t)nthr * 8) {
-            double hvv[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const size_t p = p0 + (size_t)q * nthr;
-                hvv[q] = Hup[p < S ? p : S - 1];
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const size_t p = p0 + (size_t)q * nthr;
-                if (p < S) {
-                    const uint32_t j = pos_col(p);
-                    const uint32_t i = (uint32_t)(p - (size_t)j * (j + 1) / 2);  // i <= j
-                    W[(size_t)i * dp + j] = (i == j) ? hvv[q] + shift : hvv[q];
-                }
-            }
+        s_l11 = 0;
+        s_rows = 0;
+        s_t32 = 0;
+        s_upd = 0;
+    }
+    for (int e = tid; e < kNB * kNB; e += nthr)
+        Xt[e >> 5][e & (kNB - 1)] = ((e >> 5) == (e & (kNB - 1))) ? 1.0 : 0.0;  // the factor's identity rows
+    const double shift = *PD.shift;
+    // the block column's rows (W as the previous kernel left it), shift on
+    // the diagonal. Only the lower d x d part of W belongs to this solve: the
+    // padding rows and columns are formed here as the identity, and the
+    // entries above the diagonal of the diagonal block start at zero (a
+    // failed factorisation or the fallback solve may have left anything
+    // there)
+    for (int c = 0; c < kNB; ++c) {
+        const int col = kNB * J + c;
+        const gmem<double>* wc = W + (size_t)col * dp + r0;
+        for (int i = tid; i < nrows; i += nthr) {
+            const int row = r0 + i;
+            double v;
+            if (row >= d || col >= d)
+                v = (row == col) ? 1.0 : 0.0;
+            else if (row < col)
+                v = 0.0;
+            else
+                v = (row == col) ? wc[i] + shift : wc[i];
+            Col[i * kNewtonPS + c] = v;
         }
-        __threadfence_block();
     }
-    if (!Hup) {
-        // W = H from k_master: the shift joins the diagonal here
-        for (int i = tid; i < d; i += nthr) W[(size_t)i * dp + i] = W[(size_t)i * dp + i] + shift;
-        __threadfence_block();
-    }
-    for (int i = tid; i < dp; i += nthr)
-        yv[i] = (i < d) ? b[i] : 0.0;
-    if (PD.mst) {
-        // an engine round: the record's ||grad f(x_r)|| (TrainLoop.h:1679-1691);
-        // FedNL-PP reports the full gradient
-        __shared__ double gsc[16];
-        const double* __restrict__ gr = PD.pp ? PD.pp_tg : b;
-        double gg = 0.0;
-        for (int i = tid; i < d; i += nthr) gg = fma(gr[i], gr[i], gg);
-        gg = block_sum(gg, gsc);
-        const int r0 = *round_ctr;
-        if (tid == 0 && r0 < trace_cap)
-            trace[(size_t)r0 * kTraceStride + 0] = sqrt(gg);
-    }
+    if (s == 0 && tid < kNB)
+        ry[tid] = (kNB * J + tid < d) ? b[kNB * J + tid] : 0.0;
     __syncthreads();
-    PROBE_ADD(0, t_n0);
+    const int epoch = s_epoch;
+    if (s == 0) PROBE_STAMP_AT(4000 + 16 * J + 0, 0);
 
-    // jb = -kNB is a lead-in: it only stages and factors panel 0's diagonal
-    // block and stages panel 0's rows below it (one copy of each phase)
-    for (int jb = -kNB; jb < dp; jb += kNB) {
-        if (s_fail)
-            break;
-        const int T = dp - jb - kNB;
-        // wave 0's three tiles of the next diagonal block, as the last
-        // trailing update left them (phases A and C do not touch them), are
-        // loaded now so that their latency is hidden behind phase A
-        double wpre[3][4];
-        if (wave == 0 && jb >= 0 && T > 0) {
+    // 1. panels K < J; for (J, 0) the last one (K = J - 1) is on the critical
+    //    path and is split below
+    const int kbulk = (s == 0) ? J - 1 : J;
+    for (int K = 0; K < kbulk; ++K) {
+        if (tid < 64)
+            mc_wait_ge(cnt + K, epoch * mc_nseg(dp, K), err);
+        __syncthreads();
+        mc_update_tiles(W, dp, K, r0, kNB * J, Col, wave, nwaves, ntile, s == 0, lane);
+        if (s == 0 && tid < kNB) {
+            // y_J -= P_K[block J] y_K
+            const gmem<double>* pk = W + (size_t)(kNB * K) * dp + kNB * J + tid;
+            double pv[kNB], yk[kNB];
 #pragma unroll
-            for (int t = 0; t < 3; ++t) {
-                const int ti = (t == 0) ? 0 : 1, tj = (t == 2) ? 1 : 0;
-                const double* wp = W + (size_t)(jb + kNB + tj * 16 + kq) * dp + jb + kNB + ti * 16 + l16;
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    wpre[t][r] = wp[(size_t)4 * r * dp];
+            for (int k = 0; k < kNB; ++k) {
+                pv[k] = ld_dev(pk + (size_t)k * dp);
+                yk[k] = ld_dev(ny + kNB * K + k);
             }
-        }
-        PROBE_T(t_a);
-        if (jb >= 0) {
-            // A. L21 = A21 L11^-T, one 64-row chunk per wave (T < nwaves * 64),
-            // lane l solving row rb + l, in two 16-column halves: the first
-            // half by substitution with L11's entries broadcast by
-            // v_readlane (every wave holds L11 in lanes 0..31, row r in lane
-            // r); then the second half's columns take -X1 L_ba^T on FP64 MFMA
-            // (X1 through the panel rows in LDS); then the second half by
-            // substitution.
-            if (wave * 64 < T) {
-                const int rr = wave * 64 + lane;
-                const bool live = rr < T;
-                double lrow[kNB];
-                const int lr = lane & (kNB - 1);
+            double acc = ry[tid];
 #pragma unroll
-                for (int c = 0; c < kNB; ++c)
-                    lrow[c] = Dg[lr][c];
-                const double invl = invd[lr];
-                const double ypan = yv[jb + lr];
-                double a[kNB];
-                double* pr = P21 + (live ? rr : 0) * kPS;
-#pragma unroll
-                for (int cc = 0; cc < kNB; ++cc)
-                    a[cc] = pr[cc];
-                const int i = jb + kNB + (live ? rr : 0);
-                double bi = yv[i];
-                auto solve_half = [&](auto half) {
-                    constexpr int c0 = decltype(half)::value * 16;
-#pragma unroll
-                    for (int cc = c0; cc < c0 + 16; ++cc) {
-                        a[cc] = a[cc] * bcast_lane(invl, cc);
-                        bi = fma(-a[cc], bcast_lane(ypan, cc), bi);
-                        // ties this column's broadcasts to a[cc] so that they
-                        // are not all hoisted ahead of the solve (SGPR pressure)
-                        double lc = lrow[cc];
-                        asm volatile("" : "+v"(lc) : "v"(a[cc]));
-#pragma unroll
-                        for (int c2 = cc + 1; c2 < c0 + 16; ++c2)
-                            a[c2] = fma(-a[cc], bcast_lane(lc, c2), a[c2]);
-                        __builtin_amdgcn_sched_barrier(0);
-                    }
-                };
-                solve_half(std::integral_constant<int, 0>{});
-                // X1 to this wave's rows of the panel (columns 0..15)
-                if (live) {
-#pragma unroll
-                    for (int cc = 0; cc < 16; ++cc)
-                        pr[cc] = a[cc];
-                }
-                __builtin_amdgcn_wave_barrier();
-                // U = X1 L_ba^T per live 16-row tile: lane (l16, kq), item r
-                // holds U[row l16][column 4r + kq]; written over the tile's
-                // columns 16..31 (already in the owners' registers)
-#pragma unroll
-                for (int rt = 0; rt < 4; ++rt) {
-                    const int rb = wave * 64 + rt * 16;
-                    if (rb >= T)
-                        break;
-                    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                    for (int k0 = 0; k0 < 16; k0 += 4) {
-                        const double av = Dg[16 + l16][k0 + kq];            // A[m = column][k] = L_ba
-                        const double bv = P21[(rb + l16) * kPS + k0 + kq];  // B[k][n = row] = X1
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-                    }
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        P21[(rb + l16) * kPS + 16 + 4 * r + kq] = acc[r];
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (live) {
-#pragma unroll
-                    for (int j = 0; j < 16; ++j)
-                        a[16 + j] = a[16 + j] - pr[16 + j];
-                }
-                solve_half(std::integral_constant<int, 1>{});
-                if (live) {
-                    yv[i] = bi;
-#pragma unroll
-                    for (int cc = 0; cc < kNB; ++cc) {
-                        pr[cc] = a[cc];
-                        W[(size_t)(jb + cc) * dp + jb + kNB + rr] = a[cc];
-                    }
-                }
-            }
-            // L11 back to W (the back solve reads it there)
-            for (int e = tid; e < kNB * kNB; e += nthr) {
-                const int c = e >> 5, r = e & (kNB - 1);
-                if (r >= c)
-                    W[(size_t)(jb + c) * dp + jb + r] = Dg[r][c];
-            }
+            for (int k = 0; k < kNB; ++k)
+                acc = fma(-pv[k], yk[k], acc);
+            ry[tid] = acc;
         }
         __syncthreads();
-        PROBE_ADD(3, t_a);
-        if (T == 0)
-            break;
-        PROBE_T(t_b);
-        // B. the next diagonal block (row/column tiles 0 and 1 of the trailing
-        // matrix) goes to wave 0, which updates and factors it; waves 1..7
-        // apply the rest of the trailing update
+    }
+
+    if (s == 0) {
+        const int K = J - 1;
+        PROBE_STAMP_AT(4000 + 16 * J + 1, 0);
         if (wave == 0) {
-            PROBE_T(t_w0);
-            if (jb >= 0) {
+            // the diagonal block and y_J take panel K's block-J rows, then
+            // the factorisation (32 lanes, rows in lanes)
+            if (K >= 0) {
+                mc_wait_ge(dflag + K, epoch, err);
+                PROBE_STAMP_AT(4000 + 16 * J + 2, 0);
+                mc_update_tiles(W, dp, K, r0, kNB * J, Col, 0, 1, 2, true, lane);
+                if (lane < kNB) {
+                    const gmem<double>* pk = W + (size_t)(kNB * K) * dp + kNB * J + lane;
+                    double pv[kNB], yk[kNB];
 #pragma unroll
-                for (int t = 0; t < 3; ++t) {
-                    const int ti = (t == 0) ? 0 : 1, tj = (t == 2) ? 1 : 0;
-                    const double* wold = wpre[t];
-                    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                    for (int k0 = 0; k0 < kNB; k0 += 4) {
-                        const double av = P21[(tj * 16 + l16) * kPS + k0 + kq];  // A[m = col][k]
-                        const double bv = P21[(ti * 16 + l16) * kPS + k0 + kq];  // B[k][n = row]
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                    for (int k = 0; k < kNB; ++k) {
+                        pv[k] = ld_dev(pk + (size_t)k * dp);
+                        yk[k] = ld_dev(ny + kNB * K + k);
                     }
+                    double acc = ry[lane];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        Dg[ti * 16 + l16][tj * 16 + kq + 4 * r] = wold[r] - acc[r];
+                    for (int k = 0; k < kNB; ++k)
+                        acc = fma(-pv[k], yk[k], acc);
+                    ry[lane] = acc;
                 }
-            } else {
-                // panel 0's diagonal block as it stands in W
-                for (int e = lane; e < kNB * kNB; e += 64) {
-                    const int c = e >> 5, r = e & (kNB - 1);
-                    Dg[r][c] = W[(size_t)c * dp + r];
-                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            PROBE_STAMP_AT(4000 + 16 * J + 3, 0);
+            if (lane < kNB) {
+#pragma unroll
+                for (int c = 0; c < kNB; ++c)
+                    Dg[lane][c] = Col[lane * kNewtonPS + c];
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            PROBE_ADD(7, t_w0);
-            PROBE_T(t_dg);
-            newton_diag_factor(Dg, Dq, invd, invall, yv, jb + kNB, lane, &s_fail);
-            PROBE_ADD(2, t_dg);
-        } else if (jb >= 0) {
-            PROBE_T(t_sy);
-            const int tb = T / 16;
-            const int ntiles = tb * (tb + 1) / 2;
-            constexpr int kGroup = 4;  // tiles per wave per step
-            const int nw = nwaves - 1, wid = wave - 1;
-            const int stride = nw * kGroup;
-            int tcur = 3 + wid * kGroup;
-            int rb[kGroup], cb[kGroup];
-            double wv[kGroup][4];
-            // first group's coordinates and loads
-            int ti = 0, tj = 0;
-            tri_tile(tcur, ti, tj);
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g) {
-                const int tt = tcur + g;
-                if (g > 0 && ++tj > ti) {  // the next tile in row-major lower order
-                    ++ti;
-                    tj = 0;
-                }
-                rb[g] = (tt < ntiles) ? ti * 16 : -1;
-                cb[g] = (tt < ntiles) ? tj * 16 : 0;  // past the last tile: an in-bounds dummy address
-                const double* wp = W + (size_t)(jb + kNB + cb[g] + kq) * dp + jb + kNB + max(rb[g], 0) + l16;
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    wv[g][r] = wp[(size_t)4 * r * dp];
-            }
-            for (; tcur < ntiles; tcur += stride) {
-                // next group's loads in flight during this group's MFMAs
-                int rbn[kGroup], cbn[kGroup];
-                double wn[kGroup][4];
-                int ti = 0, tj = 0;
-                tri_tile(tcur + stride, ti, tj);
-#pragma unroll
-                for (int g = 0; g < kGroup; ++g) {
-                    const int tt = tcur + stride + g;
-                    if (g > 0 && ++tj > ti) {
-                        ++ti;
-                        tj = 0;
-                    }
-                    rbn[g] = (tt < ntiles) ? ti * 16 : -1;
-                    cbn[g] = (tt < ntiles) ? tj * 16 : 0;
-                    const double* wp =
-                        W + (size_t)(jb + kNB + cbn[g] + kq) * dp + jb + kNB + max(rbn[g], 0) + l16;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        wn[g][r] = wp[(size_t)4 * r * dp];
-                }
-#pragma unroll
-                for (int g = 0; g < kGroup; ++g) {
-                    if (rb[g] < 0)
-                        continue;
-                    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                    for (int k0 = 0; k0 < kNB; k0 += 4) {
-                        const double av = P21[(cb[g] + l16) * kPS + k0 + kq];  // A[m = col][k]
-                        const double bv = P21[(rb[g] + l16) * kPS + k0 + kq];  // B[k][n = row]
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-                    }
-                    double* wp = W + (size_t)(jb + kNB + cb[g] + kq) * dp + jb + kNB + rb[g] + l16;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        wp[(size_t)4 * r * dp] = wv[g][r] - acc[r];
-                }
-#pragma unroll
-                for (int g = 0; g < kGroup; ++g) {
-                    rb[g] = rbn[g];
-                    cb[g] = cbn[g];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        wv[g][r] = wn[g][r];
-                }
-            }
-            PROBE_ADD_AT(6, t_sy, 64);
-#ifdef FEDNL_PROBE
+            mc_diag_factor(Dg, Xt, Dq, invd, ry, lane, &s_fail);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            PROBE_STAMP_AT(4000 + 16 * J + 4, 0);
             if (lane == 0)
-                atomicAdd(&g_probe[8 + wave], wall_clock64() - t_sy);
-#endif
-        }
-        __syncthreads();
-        PROBE_ADD(4, t_b);
-        PROBE_T(t_c);
-        // C. the next panel's rows below its diagonal block: thread t holds
-        // row t % 256 and columns 16 (t / 256) .. + 15, all sixteen loads in
-        // flight before the LDS stores
-        {
-            const int jn = jb + kNB;
-            const int rows = T - kNB;
-            const int c0 = 16 * (tid >> 8);
-            for (int r = tid & 255; r < rows; r += 256) {
-                double pv[16];
-#pragma unroll
-                for (int q = 0; q < 16; ++q)
-                    pv[q] = W[(size_t)(jn + c0 + q) * dp + jn + kNB + r];
-#pragma unroll
-                for (int q = 0; q < 16; ++q)
-                    P21[r * kPS + c0 + q] = pv[q];
+                mc_lds_signal(&s_l11, 1);
+            // block J+1's first row tile (wave 2 takes the second), then L11,
+            // y_J and the reciprocal pivots out
+            if (ntile > 2) {
+                mc_lds_wait(&s_rows, 1);
+                mc_solve_tiles(Col, Xt, 2, 1, 3, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
+            for (int c = 0; c < kNB; ++c)
+                if (lane < kNB && lane >= c)
+                    st_dev(W + (size_t)(kNB * J + c) * dp + r0 + lane, Dg[lane][c]);
+            if (lane < kNB) {
+                st_dev(ny + kNB * J + lane, ry[lane]);
+                st_dev(ninv + kNB * J + lane, invd[lane]);
+            }
+            mc_lds_wait(&s_t32, 1);
+            PROBE_STAMP_AT(4000 + 16 * J + 5, 0);
+            // block J+1 (local rows 32..63) and L11^-T, then the flag
+            if (ntile > 2) {
+                for (int c = 0; c < kNB; ++c)
+                    if (lane < kNB)
+                        st_dev(W + (size_t)(kNB * J + c) * dp + r0 + kNB + lane, Col[(kNB + lane) * kNewtonPS + c]);
+            }
+            for (int e = lane; e < kNB * kNB; e += 64)
+                st_dev(nxt + (size_t)kNB * kNB * J + e, Xt[e >> 5][e & (kNB - 1)]);
+            if (s_fail && lane == 0)
+                __hip_atomic_store(sync + 2, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            mc_drain();
+            if (lane == 0)
+                __hip_atomic_store(dflag + J, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            PROBE_STAMP_AT(4000 + 16 * J + 6, 0);
+        } else {
+            // waves 1..7. Wave 1 first brings block J+1's rows (local tiles
+            // 2, 3) up to date; wave 2 solves tile 3 once L11^-T is there;
+            // waves 1, 3..7 update the rest, then all solve it
+            if (K >= 0) {
+                mc_wait_ge(cnt + K, epoch * mc_nseg(dp, K), err);
+                PROBE_STAMP_AT(4000 + 16 * J + 8, 64);
+                if (wave == 1)
+                    mc_update_tiles(W, dp, K, r0, kNB * J, Col, 2, 1, min(4, ntile), false, lane);
+            }
+            if (wave == 1 && lane == 0)
+                mc_lds_signal(&s_rows, 1);
+            // bulk update of tiles 4.. by waves 1, 4, 5, 6, 7
+            const int bw = (wave == 1) ? 0 : wave - 2;  // 0..5 for waves 1, 3..7
+            if (wave != 2) {
+                if (K >= 0)
+                    mc_update_tiles(W, dp, K, r0, kNB * J, Col, 4 + bw, 6, ntile, false, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0)
+                    atomicAdd(&s_upd, 1);
+            }
+            if (wave == 2) {
+                mc_lds_wait(&s_l11, 1);
+                mc_lds_wait(&s_rows, 1);
+                if (ntile > 2)
+                    mc_solve_tiles(Col, Xt, 3, 1, 4, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0)
+                    mc_lds_signal(&s_t32, 1);
+            }
+            // the bulk solve takes tiles in another order than the bulk
+            // update: every updating wave (1, 3..7) has finished first
+            mc_lds_wait(&s_l11, 1);
+            mc_lds_wait(&s_upd, 6);
+            mc_solve_tiles(Col, Xt, 4 + (wave - 1), nwaves - 1, ntile, lane);
+            PROBE_STAMP_AT(4000 + 16 * J + 9, 64);
         }
         __syncthreads();
-        PROBE_ADD(1, t_c);
+        // rows 64.. to W
+        for (int c = 0; c < kNB; ++c) {
+            gmem<double>* wc = W + (size_t)(kNB * J + c) * dp + r0;
+            for (int i = 2 * kNB + tid; i < nrows; i += nthr)
+                st_dev(wc + i, Col[i * kNewtonPS + c]);
+        }
+    } else {
+        // a lower segment: the triangular solve once L11^-T of block column J
+        // is published
+        if (tid < 64)
+            mc_wait_ge(dflag + J, epoch, err);
+        __syncthreads();
+        for (int e = tid; e < kNB * kNB; e += nthr)
+            Xt[e >> 5][e & (kNB - 1)] = ld_dev(nxt + (size_t)kNB * kNB * J + e);
+        __syncthreads();
+        mc_solve_tiles(Col, Xt, wave, nwaves, ntile, lane);
+        __syncthreads();
+        for (int c = 0; c < kNB; ++c) {
+            gmem<double>* wc = W + (size_t)(kNB * J + c) * dp + r0;
+            for (int i = tid; i < nrows; i += nthr)
+                st_dev(wc + i, Col[i * kNewtonPS + c]);
+        }
     }
-    PROBE_T(t_b0);
+    // this segment of block column J is done (every wave's stores complete);
+    // the ticket finds the last one
+    mc_drain();
+    __syncthreads();
+    if (s == 0) PROBE_STAMP_AT(4000 + 16 * J + 7, 0);
+    const int total = mc_blocks(dp);
+    if (tid == 0) {
+        __hip_atomic_fetch_add(cnt + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int t = __hip_atomic_fetch_add(sync + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (t == epoch * total - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last)
+        return;
+    // every other workgroup's results are in memory: drop this CU's and this
+    // XCD's stale cache lines, then plain (cached) loads
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
+    // ---- the last workgroup: record, back solve (or fallback), iterate
+    gmem<double>* __restrict__ x = G(PD.x);
+    gmem<double>* __restrict__ dx_out = G(PD.dx);
+    gmem<double>* __restrict__ trace = G(PD.trace);
+    gmem<int>* __restrict__ round_ctr = G(PD.round_ctr);
+    const int trace_cap = PD.trace_cap, update_x = PD.update_x;
     const int rnd = *round_ctr;
-    if (s_fail) {
-        // the engine's rounds solve by Gauss-Jordan elimination instead; the
-        // standalone solve (no master H) reports dx = 0
-        if (PD.H0) {
-            __syncthreads();  // every wave is past its last use of the panel LDS
+    if (PD.mst) {
+        // an engine round: the record's ||grad f(x_r)|| (TrainLoop.h:1679-1691);
+        // FedNL-PP reports the full gradient
+        __shared__ double gsc[16];
+        const double* __restrict__ gr = PD.pp ? PD.pp_tg : PD.nb;
+        double gg = 0.0;
+        for (int i = tid; i < d; i += nthr) gg = fma(gr[i], gr[i], gg);
+        gg = block_sum(gg, gsc);
+        if (tid == 0 && rnd < trace_cap)
+            trace[(size_t)rnd * kTraceStride + 0] = sqrt(gg);
+    }
+    const bool failed = __hip_atomic_load(sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+    const bool expired = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    if (failed || expired) {
+        if (failed && PD.H0) {
+            __syncthreads();
             newton_fallback_solve(PD, lds);
         } else {
             for (int i = tid; i < d; i += nthr)
@@ -3205,94 +3302,95 @@ __global__ __launch_bounds__(kNewtonThreads) void k_newton(FEDNL_KARGS)
         }
         if (tid == 0) {
             if (rnd < trace_cap)
-                trace[(size_t)rnd * kTraceStride + 3] = 0.0;
+                trace[(size_t)rnd * kTraceStride + 3] = expired ? -1.0 : 0.0;
             *round_ctr = rnd + 1;
+            __hip_atomic_store(sync, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         return;
     }
-
-    // L^T dx = y, 32-row blocks from the bottom. Row rr < jb of the block's
-    // L^T columns is column rr of W at rows jb..jb+31 (256 contiguous bytes):
-    // waves 1-7 stage them into the panel LDS, 32 lanes per column so that
-    // the loads coalesce, while wave 0 solves the block (lane r < 32 keeps
-    // column r of L11); then thread rr folds its row into y.
-    double colv_next[kNB];
-    if (wave == 0) {
-        const double* cp = W + (size_t)(dp - kNB + (lane & (kNB - 1))) * dp + dp - kNB;
-#pragma unroll
-        for (int c = 0; c < kNB; ++c)
-            colv_next[c] = cp[c];
-    }
-    for (int jb = dp - kNB; jb >= 0; jb -= kNB) {
-        const int rr = tid;
-        const bool has = rr < jb;
-        if (wave > 0) {
-            const int tot = jb * kNB;
-            const int t1 = tid - 64, n1 = nthr - 64;
-            for (int e0 = t1; e0 < tot; e0 += n1 * 8) {
-                double v[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int e = min(e0 + u * n1, tot - 1);
-                    v[u] = W[(size_t)(e >> 5) * dp + jb + (e & (kNB - 1))];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int e = e0 + u * n1;
-                    if (e < tot)
-                        P21[(e >> 5) * kPS + (e & (kNB - 1))] = v[u];
-                }
-            }
+    PROBE_STAMP_AT(4000 + 16 * 64 + 0, 0);
+    // L^T dx = y from the bottom block: dx_J = L11^-T r_J (wave 0, lane q:
+    // row q of L11^-T), then every row rr above the block folds
+    // r_rr -= sum_c L[32J + c][rr] dx_J[c] (W[rr dp + 32J + c]). For
+    // dp <= 512 every block's L11^-T is staged in LDS first (row stride 33).
+    const bool staged = dp <= kSegRows;
+    double* Xs = lds;                                           // nb x 32 x 33 when staged
+    double* yv = lds + (staged ? (size_t)nb * kNB * (kNB + 1) : 0);
+    if (staged) {
+        for (int e = tid; e < nb * kNB * kNB; e += nthr) {
+            const int blkx = e >> 10, q = (e >> 5) & (kNB - 1), c = e & (kNB - 1);
+            Xs[(size_t)blkx * kNB * (kNB + 1) + q * (kNB + 1) + c] = nxt[e];
         }
-        if (wave == 0) {
-            const bool act = lane < kNB;
-            const int lr = act ? lane : 0;
-            // column lr of L11 (rows jb..jb+31; entries above the diagonal
-            // unused), loaded one block ahead
-            double colv[kNB];
+    }
+    for (int i = tid; i < dp; i += nthr) yv[i] = ny[i];
+    // row tid's L segment of the bottom block, then each next one a block ahead
+    double lnext[kNB];
+    if (staged && tid < kNB * (nb - 1)) {
+        const double* lp = PD.W + (size_t)tid * dp + kNB * (nb - 1);
 #pragma unroll
-            for (int c = 0; c < kNB; ++c)
-                colv[c] = colv_next[c];
-            if (jb >= kNB) {
-                const double* cp = W + (size_t)(jb - kNB + lr) * dp + jb - kNB;
+        for (int c = 0; c < kNB; ++c) lnext[c] = lp[c];
+    }
+    __syncthreads();
+    for (int Jb = nb - 1; Jb >= 0; --Jb) {
+        const int jb = kNB * Jb;
+        double lrow[kNB];
+        const bool own = staged && tid < jb;
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) lrow[c] = lnext[c];
+        if (staged && Jb > 0 && tid < jb - kNB) {
+            const double* lp = PD.W + (size_t)tid * dp + jb - kNB;
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) lnext[c] = lp[c];
+        }
+        if (wave == 0 && lane < kNB) {
+            double acc = 0.0;
+            if (staged) {
+                const double* xr = Xs + (size_t)Jb * kNB * (kNB + 1) + lane * (kNB + 1);
 #pragma unroll
                 for (int c = 0; c < kNB; ++c)
-                    colv_next[c] = cp[c];
-            }
-            double yr = yv[jb + lr];
+                    acc = fma(xr[c], yv[jb + c], acc);
+            } else {
+                const double* xr = PD.nxt + (size_t)kNB * kNB * Jb + kNB * lane;
 #pragma unroll
-            for (int cc = kNB - 1; cc >= 0; --cc) {
-                const double q = yr * invall[jb + cc];
-                const double xc = bcast_lane(q, cc);
-                const double upd = fma(-colv[cc], xc, yr);
-                yr = (lane == cc) ? xc : ((lane < cc) ? upd : yr);
+                for (int c = 0; c < kNB; ++c)
+                    acc = fma(xr[c], yv[jb + c], acc);
             }
-            if (act)
-                yv[jb + lane] = yr;
+            __builtin_amdgcn_wave_barrier();  // every lane has read r_J before any writes
+            yv[jb + lane] = acc;
         }
         __syncthreads();
-        if (has) {
-            const double* wl = P21 + rr * kPS;
-            double s = yv[rr];
+        if (own) {
+            double sacc = yv[tid];
 #pragma unroll
-            for (int cc = 0; cc < kNB; ++cc)
-                s = fma(-wl[cc], yv[jb + cc], s);
-            yv[rr] = s;
+            for (int c = 0; c < kNB; ++c)
+                sacc = fma(-lrow[c], yv[jb + c], sacc);
+            yv[tid] = sacc;
+        } else if (!staged) {
+            for (int rr = tid; rr < jb; rr += nthr) {
+                const double* lp = PD.W + (size_t)rr * dp + jb;
+                double sacc = yv[rr];
+#pragma unroll 8
+                for (int c = 0; c < kNB; ++c)
+                    sacc = fma(-lp[c], yv[jb + c], sacc);
+                yv[rr] = sacc;
+            }
         }
         __syncthreads();
     }
-    PROBE_ADD(5, t_b0);
+    PROBE_STAMP_AT(4000 + 16 * 64 + 1, 0);
     for (int i = tid; i < d; i += nthr) {
-        dx_out[i] = yv[i];
+        const double v = yv[i];
+        dx_out[i] = v;
         if (update_x == 1)
-            x[i] = x[i] - yv[i];
+            x[i] = x[i] - v;
         else if (update_x == 2)
-            x[i] = yv[i];  // FedNL-PP: the solution is the next iterate
+            x[i] = v;  // FedNL-PP: the solution is the next iterate
     }
     if (tid == 0) {
         if (rnd < trace_cap)
             trace[(size_t)rnd * kTraceStride + 3] = 1.0;
         *round_ctr = rnd + 1;
+        __hip_atomic_store(sync, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -3344,7 +3442,7 @@ enum KernelId {
     KID_MARGIN, KID_HESS, KID_SELECT, KID_EMIT, KID_MASTER, KID_NEWTON, KID_LS, KID_COUNT
 };
 static const char* kKernelNames[KID_COUNT] = {"k_margin", "k_hessian_f64", "k_select", "k_emit",
-                                              "k_master", "k_newton", "k_line_search"};
+                                              "k_master", "k_newton_mc", "k_line_search"};
 
 struct Engine {
     fednl_opts o;
@@ -3361,6 +3459,8 @@ struct Engine {
     int *sel_cnt = nullptr, *cnt = nullptr, *lek_hit = nullptr, *round_ctr = nullptr;
     double *lek_p = nullptr, *vals = nullptr, *H = nullptr, *g_avg = nullptr, *shift = nullptr, *W = nullptr;
     double *dx = nullptr, *trace = nullptr;
+    int* nsync = nullptr;   // multi-workgroup Newton solve: sync words
+    double *ny = nullptr, *ninv = nullptr, *nxt = nullptr;
     double* mst = nullptr;  // master state: MstSlot header + x_k
     double* H0 = nullptr;   // H at creation (S, packed upper)
     // FedNL-PP state (clients_per_round > 0)
@@ -3483,6 +3583,7 @@ static ProblemDev make_desc(const Engine* e)
     P.sel = e->sel; P.pos = e->pos; P.idx = e->idx; P.rng = e->rng; P.lek_state = e->lek_state;
     P.lek_popped = e->lek_popped; P.seq_start = e->seq_start; P.sel_bm = e->sel_bm;
     P.sel_cnt = e->sel_cnt; P.cnt = e->cnt; P.lek_hit = e->lek_hit; P.round_ctr = e->round_ctr;
+    P.nsync = e->nsync; P.ny = e->ny; P.ninv = e->ninv; P.nxt = e->nxt;
     return P;
 }
 
@@ -3652,7 +3753,7 @@ static int launch_round(Engine* e)
     }
     {
         KTime kt(e, KID_NEWTON);
-        hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), e->lds_newton, e->st, pd, nomap);
+        hipLaunchKernelGGL(k_newton_mc, dim3(mc_blocks(P.dp)), dim3(kMcThreads), e->lds_newton, e->st, pd, nomap);
     }
     if (e->o.line_search) {
         // every trial is enqueued; the ones after acceptance return at once
@@ -3683,7 +3784,8 @@ static void free_engine(Engine* e)
                     e->sel, e->pos, e->idx, e->rng, e->lek_state, e->lek_popped, e->seq_start, e->sel_cnt,
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
                     e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev, e->mst, e->H0, e->pp_sel, e->pp_rng,
-                    e->pp_dir, e->pp_lk, e->pp_gmsg, e->pp_lkmsg, e->pp_h, e->pp_tg};
+                    e->pp_dir, e->pp_lk, e->pp_gmsg, e->pp_lkmsg, e->pp_h, e->pp_tg, e->nsync, e->ny, e->ninv,
+                    e->nxt};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -3765,8 +3867,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         set_err("fednl_create: more than 1024 clients");
         return FEDNL_E_UNSUPPORTED;
     }
-    if (d > kMaxNewtonD) {
-        set_err("fednl_create: d > 512 needs the multi-workgroup Newton solve");
+    if (d > kMaxD) {
+        set_err("fednl_create: d > 2048");
         return FEDNL_E_UNSUPPORTED;
     }
     if (opts->hessian_mode < 0 || opts->hessian_mode > 2) {
@@ -3830,7 +3932,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     e->mult = (double)e->S / (double)e->k;
     e->lek_delta = (double)e->k * (1.0 / (double)e->S);
     e->lds_select = select_lds_bytes(std::min<int>(e->k, kMaxHeapK), e->S);
-    e->lds_newton = newton_lds_bytes(e->dp);
+    e->lds_newton = mc_lds_bytes(e->dp);
 
     int rc = FEDNL_OK;
 #define ALLOC(ptr, cnt)                                          \
@@ -3877,6 +3979,15 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     ALLOC(e->g_avg, e->d_pad);
     ALLOC(e->shift, 1);
     ALLOC(e->W, (size_t)e->dp * e->dp);
+    ALLOC(e->nsync, (size_t)mc_sync_words(e->dp));
+    ALLOC(e->ny, (size_t)e->dp);
+    ALLOC(e->ninv, (size_t)e->dp);
+    ALLOC(e->nxt, (size_t)e->dp * kNB);
+    if (hipMemset(e->nsync, 0, sizeof(int) * mc_sync_words(e->dp)) != hipSuccess) {
+        set_err("hipMemset Newton sync words");
+        free_engine(e);
+        return FEDNL_E_DEVICE;
+    }
     {
         // W padding: identity (rows/columns d..dp-1), zero elsewhere; the lower
         // d x d part is rewritten every round
@@ -3917,7 +4028,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
             if (le == hipSuccess)
                 le = raise_lds_limit(f, lds);
         if (le == hipSuccess)
-            le = raise_lds_limit((const void*)k_newton, e->lds_newton);
+            le = raise_lds_limit((const void*)k_newton_mc, e->lds_newton);
         if (le != hipSuccess) {
             set_err("hipFuncSetAttribute (LDS limit)", le);
             free_engine(e);
@@ -4111,7 +4222,7 @@ static int batch_build(Batch* b)
         if (comp == FEDNL_TOPLEK)
             add(BK_EMIT, p, e->n, false);
         add(BK_MASTER, p, blocks_master(P), false);
-        add(BK_NEWTON, p, 1, false);
+        add(BK_NEWTON, p, mc_blocks(P.dp), false);
         if (e->o.line_search) {
             add(BK_LS_MARGIN, p, blocks_margin(P), false);
             add(BK_LS, p, 1, false);
@@ -4193,7 +4304,7 @@ static void batch_launch_round(Batch* b)
     if (b->cnt[BK_EMIT])
         hipLaunchKernelGGL(k_emit, dim3(b->cnt[BK_EMIT]), dim3(256), b->lds_emit, st, pd, map(BK_EMIT));
     hipLaunchKernelGGL(k_master, dim3(b->cnt[BK_MASTER]), dim3(256), 0, st, pd, map(BK_MASTER));
-    hipLaunchKernelGGL(k_newton, dim3(b->cnt[BK_NEWTON]), dim3(kNewtonThreads), b->lds_newton, st, pd,
+    hipLaunchKernelGGL(k_newton_mc, dim3(b->cnt[BK_NEWTON]), dim3(kMcThreads), b->lds_newton, st, pd,
                        map(BK_NEWTON));
     if (b->cnt[BK_LS]) {
         hipLaunchKernelGGL(k_ls_begin, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));
@@ -4293,18 +4404,26 @@ static size_t engine_footprint(const fednl_opts* o, int n, int m, int d)
 // ---------------------------------------------------------------------------
 constexpr int kSolveScratchMaxD = kMaxNewtonD;                 // d <= 512
 constexpr size_t kSolveHdrDoubles = 256;                       // 2 KiB header
-constexpr size_t kSolveSlotShift = 128, kSolveSlotTrace = 136, kSolveSlotCtr = 152;
+constexpr size_t kSolveSlotShift = 128, kSolveSlotTrace = 136, kSolveSlotCtr = 152, kSolveSlotSync = 160;
+constexpr size_t kSolveOffNy = kSolveHdrDoubles;
+constexpr size_t kSolveOffNinv = kSolveOffNy + kSolveScratchMaxD;
+constexpr size_t kSolveOffNxt = kSolveOffNinv + kSolveScratchMaxD;
+constexpr size_t kSolveOffW = kSolveOffNxt + (size_t)kSolveScratchMaxD * kNB;
 constexpr size_t kSolveScratchDoubles =
-    kSolveHdrDoubles + (size_t)kSolveScratchMaxD * (size_t)kSolveScratchMaxD;  // + W: 2 MiB
+    kSolveOffW + (size_t)kSolveScratchMaxD * (size_t)kSolveScratchMaxD;  // 2 MiB + 138 KiB
 static_assert(sizeof(ProblemDev) <= kSolveSlotShift * sizeof(double), "descriptor fits the header");
 static_assert(kSolveSlotTrace + kTraceStride <= kSolveSlotCtr, "status record fits before the counter");
+static_assert(kSolveSlotSync * 2 + mc_sync_words(kSolveScratchMaxD) <= kSolveHdrDoubles * 2, "sync words fit");
+static_assert(kSolveOffW % 32 == 0, "W starts on a 256-byte boundary");
 
 static __device__ __align__(256) double scratch[kSolveScratchDoubles];
 
 // One launch before the solve: the descriptor (by value) and the shift into
-// the header, the status record and round counter cleared, W cleared with
-// the identity on its padding (what the allocating solve uploads).
-__global__ __launch_bounds__(256) void k_solve_scratch_init(ProblemDev hp, double shift, double* base, int d, int dp)
+// the header, the status record, round counter and sync words cleared, and W
+// built from the packed upper H (lower triangle, column-major, identity
+// padding: what the allocating solve uploads).
+__global__ __launch_bounds__(256) void k_solve_scratch_init(ProblemDev hp, double shift, double* base,
+                                                            const double* __restrict__ hup, int d, int dp)
 {
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nthr = (size_t)gridDim.x * blockDim.x;
@@ -4315,11 +4434,17 @@ __global__ __launch_bounds__(256) void k_solve_scratch_init(ProblemDev hp, doubl
     }
     if (tid < (size_t)kTraceStride)
         base[kSolveSlotTrace + tid] = 0.0;
-    double* W = base + kSolveHdrDoubles;
+    int* sync = reinterpret_cast<int*>(base + kSolveSlotSync);
+    for (size_t e = tid; e < (size_t)mc_sync_words(dp); e += nthr)
+        sync[e] = 0;
+    double* W = base + kSolveOffW;
     const size_t total = (size_t)dp * dp;
     for (size_t e = tid; e < total; e += nthr) {
-        const size_t i = e / dp, j = e - i * dp;
-        W[e] = (i == j && (int)i >= d) ? 1.0 : 0.0;
+        const size_t i = e / dp, j = e - i * dp;  // column i, row j
+        double v = 0.0;
+        if (j >= i)
+            v = ((int)j < d) ? hup[j * (j + 1) / 2 + i] : (i == j ? 1.0 : 0.0);
+        W[e] = v;
     }
 }
 
@@ -4355,9 +4480,9 @@ static int solve_scratch(double** out)
             set_err("hipGetSymbolAddress (solve scratch)", e);
             return FEDNL_E_DEVICE;
         }
-        e = raise_lds_limit((const void*)k_newton, newton_lds_bytes(kSolveScratchMaxD));
+        e = raise_lds_limit((const void*)k_newton_mc, mc_lds_bytes(kSolveScratchMaxD));
         if (e != hipSuccess) {
-            set_err("hipFuncSetAttribute (k_newton)", e);
+            set_err("hipFuncSetAttribute (k_newton_mc)", e);
             return FEDNL_E_DEVICE;
         }
         g_scratch_ptr[dev] = (double*)p;
@@ -5070,6 +5195,16 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
 }
 
 #ifdef FEDNL_PROBE
+int fednl_probe_newton(void* handle, double* ny, double* nxt, double* W)
+{
+    Engine* e = (Engine*)handle;
+    HIP_OK(hipStreamSynchronize(e->st));
+    HIP_OK(hipMemcpy(ny, e->ny, sizeof(double) * e->dp, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(nxt, e->nxt, sizeof(double) * e->dp * kNB, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(W, e->W, sizeof(double) * e->dp * e->dp, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int fednl_probe_read(unsigned long long* out, int count, int reset)
 {
     if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_probe), sizeof(unsigned long long) * count) != hipSuccess)
@@ -5177,39 +5312,56 @@ int fednl_newton_solve(int d, const double* H_up, double shift, const double* b,
 {
     if (d <= 0 || !H_up || !b || !dx)
         return FEDNL_E_MISSING_ARGUMENT;
-    if (d > kMaxNewtonD) {
-        set_err("fednl_newton_solve: d > 512");
+    if (d > kMaxMcD) {
+        set_err("fednl_newton_solve: d > 2048");
         return FEDNL_E_UNSUPPORTED;
     }
-    const size_t S = (size_t)d * (d + 1) / 2;
     const int dp = round_up(d, kNB);
-    double *dH = nullptr, *dsh = nullptr, *db = nullptr, *dW = nullptr, *dx_d = nullptr, *dtr = nullptr;
-    int* dctr = nullptr;
-    HIP_OK(dalloc(&dH, S));
-    HIP_OK(dalloc(&dsh, 1));
-    HIP_OK(dalloc(&db, d));
-    HIP_OK(dalloc(&dW, (size_t)dp * dp));
-    HIP_OK(dalloc(&dx_d, d));
-    HIP_OK(dalloc(&dtr, 4));
-    HIP_OK(dalloc(&dctr, 1));
-    HIP_OK(hipMemcpy(dH, H_up, sizeof(double) * S, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(dsh, &shift, sizeof(double), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(db, b, sizeof(double) * d, hipMemcpyHostToDevice));
-    HIP_OK(hipMemset(dctr, 0, sizeof(int)));
-    HIP_OK(hipMemset(dtr, 0, sizeof(double) * 4));
-    {
-        std::vector<double> w0((size_t)dp * dp, 0.0);
-        for (int i = d; i < dp; ++i) w0[(size_t)i * dp + i] = 1.0;
-        HIP_OK(hipMemcpy(dW, w0.data(), sizeof(double) * w0.size(), hipMemcpyHostToDevice));
+    double *dsh = nullptr, *db = nullptr, *dW = nullptr, *dx_d = nullptr, *dtr = nullptr;
+    double *dny = nullptr, *dninv = nullptr, *dnxt = nullptr;
+    int *dctr = nullptr, *dsync = nullptr;
+    ProblemDev* dpd = nullptr;
+    auto cleanup = [&]() {
+        void* ptrs[] = {dsh, db, dW, dx_d, dtr, dny, dninv, dnxt, dctr, dsync, dpd};
+        for (void* q : ptrs)
+            if (q)
+                hipFree(q);
+    };
+    // W: the lower triangle of H (column i, rows j >= i), identity padding
+    std::vector<double> w0((size_t)dp * dp, 0.0);
+    for (int i = 0; i < dp; ++i)
+        for (int j = i; j < dp; ++j)
+            w0[(size_t)i * dp + j] = (j < d) ? H_up[(size_t)j * (j + 1) / 2 + i] : (i == j ? 1.0 : 0.0);
+    hipError_t err = hipSuccess;
+    auto step = [&](hipError_t r) {
+        if (err == hipSuccess && r != hipSuccess)
+            err = r;
+    };
+    step(dalloc(&dsh, 1));
+    step(dalloc(&db, d));
+    step(dalloc(&dW, (size_t)dp * dp));
+    step(dalloc(&dx_d, d));
+    step(dalloc(&dtr, kTraceStride));
+    step(dalloc(&dny, dp));
+    step(dalloc(&dninv, dp));
+    step(dalloc(&dnxt, (size_t)dp * kNB));
+    step(dalloc(&dctr, 1));
+    step(dalloc(&dsync, mc_sync_words(dp)));
+    step(dalloc(&dpd, 1));
+    if (err == hipSuccess) {
+        step(hipMemcpy(dsh, &shift, sizeof(double), hipMemcpyHostToDevice));
+        step(hipMemcpy(db, b, sizeof(double) * d, hipMemcpyHostToDevice));
+        step(hipMemcpy(dW, w0.data(), sizeof(double) * w0.size(), hipMemcpyHostToDevice));
+        step(hipMemset(dctr, 0, sizeof(int)));
+        step(hipMemset(dsync, 0, sizeof(int) * mc_sync_words(dp)));
+        step(hipMemset(dtr, 0, sizeof(double) * kTraceStride));
+        step(raise_lds_limit((const void*)k_newton_mc, mc_lds_bytes(dp)));
     }
-    const size_t lds = newton_lds_bytes(dp);
-    HIP_OK(raise_lds_limit((const void*)k_newton, lds));
     ProblemDev hp{};
     hp.d = d;
     hp.dp = dp;
     hp.trace_cap = 1;
     hp.update_x = 0;
-    hp.hup = dH;
     hp.shift = dsh;
     hp.nb = db;
     hp.W = dW;
@@ -5217,16 +5369,32 @@ int fednl_newton_solve(int d, const double* H_up, double shift, const double* b,
     hp.dx = dx_d;
     hp.trace = dtr;
     hp.round_ctr = dctr;
-    ProblemDev* dpd = nullptr;
-    HIP_OK(hipMalloc((void**)&dpd, sizeof(ProblemDev)));
-    HIP_OK(hipMemcpy(dpd, &hp, sizeof(ProblemDev), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), lds, 0, (const ProblemDev*)dpd, (const int2*)nullptr);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipDeviceSynchronize());
-    double tr[4];
-    HIP_OK(hipMemcpy(tr, dtr, sizeof(tr), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(dx, dx_d, sizeof(double) * d, hipMemcpyDeviceToHost));
-    hipFree(dH); hipFree(dsh); hipFree(db); hipFree(dW); hipFree(dx_d); hipFree(dtr); hipFree(dctr); hipFree(dpd);
+    hp.nsync = dsync;
+    hp.ny = dny;
+    hp.ninv = dninv;
+    hp.nxt = dnxt;
+    if (err == hipSuccess)
+        step(hipMemcpy(dpd, &hp, sizeof(ProblemDev), hipMemcpyHostToDevice));
+    if (err == hipSuccess) {
+        hipLaunchKernelGGL(k_newton_mc, dim3(mc_blocks(dp)), dim3(kMcThreads), mc_lds_bytes(dp), 0,
+                           (const ProblemDev*)dpd, (const int2*)nullptr);
+        step(hipGetLastError());
+        step(hipDeviceSynchronize());
+    }
+    double tr[kTraceStride] = {0};
+    if (err == hipSuccess)
+        step(hipMemcpy(tr, dtr, sizeof(tr), hipMemcpyDeviceToHost));
+    if (err == hipSuccess)
+        step(hipMemcpy(dx, dx_d, sizeof(double) * d, hipMemcpyDeviceToHost));
+    cleanup();
+    if (err != hipSuccess) {
+        set_err("fednl_newton_solve", err);
+        return FEDNL_E_DEVICE;
+    }
+    if (tr[3] < 0.0) {
+        set_err("fednl_newton_solve: a wait between workgroups expired");
+        return FEDNL_E_DEVICE;
+    }
     return tr[3] == 1.0 ? 1 : 0;
 }
 
@@ -5252,18 +5420,21 @@ int fednl_newton_solve_noalloc(int d, const double* H_up, double shift, const do
     hp.dp = dp;
     hp.trace_cap = 1;
     hp.update_x = 0;
-    hp.hup = H_up;
     hp.shift = base + kSolveSlotShift;
     hp.nb = b;
-    hp.W = base + kSolveHdrDoubles;
+    hp.W = base + kSolveOffW;
     hp.x = nullptr;
     hp.dx = dx;
     hp.trace = base + kSolveSlotTrace;
     hp.round_ctr = reinterpret_cast<int*>(base + kSolveSlotCtr);
+    hp.nsync = reinterpret_cast<int*>(base + kSolveSlotSync);
+    hp.ny = base + kSolveOffNy;
+    hp.ninv = base + kSolveOffNinv;
+    hp.nxt = base + kSolveOffNxt;
     const int init_blocks = std::max(1, std::min(1024, (dp * dp + 255) / 256));
-    hipLaunchKernelGGL(k_solve_scratch_init, dim3(init_blocks), dim3(256), 0, st, hp, shift, base, d, dp);
-    hipLaunchKernelGGL(k_newton, dim3(1), dim3(kNewtonThreads), newton_lds_bytes(dp), st, (const ProblemDev*)base,
-                       (const int2*)nullptr);
+    hipLaunchKernelGGL(k_solve_scratch_init, dim3(init_blocks), dim3(256), 0, st, hp, shift, base, H_up, d, dp);
+    hipLaunchKernelGGL(k_newton_mc, dim3(mc_blocks(dp)), dim3(kMcThreads), mc_lds_bytes(dp), st,
+                       (const ProblemDev*)base, (const int2*)nullptr);
     hipLaunchKernelGGL(k_solve_status, dim3(1), dim3(64), 0, st, (const double*)base, ok);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
