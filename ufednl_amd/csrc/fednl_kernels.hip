@@ -96,6 +96,7 @@ constexpr int kKC = 16;            // samples staged per LDS chunk
 constexpr int kPadLds = 4;         // LDS row padding (doubles)
 constexpr int kMaxHeapK = 8192;    // LDS-resident heap limit
 constexpr int kScanGroup = 8;      // TopK scan: chunks of 64 positions per staged group
+constexpr int kMaxScanRing = 16;   // TopK scan: most LDS slots of staged groups
 static_assert(kScanGroup % 2 == 0, "the scan tests chunk pairs");
 constexpr int kMaxClients = 1024;  // master scatter client table
 // per-round trace record: ||grad f||, f, ||x||, factorization flag,
@@ -148,6 +149,23 @@ __device__ inline double natural_round(double v)
 __host__ __device__ inline bool select_bitmap_fits(size_t S, int k)
 {
     return ((S + 31) / 32) * 4 <= (size_t)8 * (k + 4);
+}
+
+// The TopK scan's waves hand groups through LDS stamps: bounded spins (a
+// wave that waits longer than any scan can take gives up rather than hang).
+__device__ inline void scan_lds_wait_eq(int* f, int v)
+{
+    long n = 0;
+    while (*(volatile int*)f != v && ++n < (1L << 24))
+        __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+__device__ inline void scan_lds_wait_ge(int* f, int v)
+{
+    long n = 0;
+    while (*(volatile int*)f < v && ++n < (1L << 24))
+        __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 // Global-memory views of the descriptor's arrays. The pointers in ProblemDev
@@ -266,6 +284,7 @@ struct ProblemDev {
     // L^-T (32 x 32 per block column)
     int* nsync;
     double *ny, *ninv, *nxt;
+    int sel_ring;       // TopK / TopLEK scan: LDS slots of staged groups (2 .. kMaxScanRing)
 };
 
 #define FEDNL_BLOCK()                                                                     \
@@ -1493,7 +1512,13 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
     const bool bm_tail = select_bitmap_fits(S, k);
     double* hv = (double*)smem;                                // k + 4 (padding read by heapify)
     uint32_t* hid = (uint32_t*)(hv + k + 4);                   // k + 4, or P + 4 for the sort
-    double* ring = (double*)(hid + round_up((bm_tail ? k : P) + 4, 2));  // 2 * kScanGroup * 64
+    double* ring = (double*)(hid + round_up((bm_tail ? k : P) + 4, 2));  // sel_ring * kScanGroup * 64
+    __shared__ int s_ready[kMaxScanRing];
+    __shared__ int s_cons;
+    if (tid < kMaxScanRing)
+        s_ready[tid] = 0;
+    if (tid == 0)
+        s_cons = 0;
 
     stage_abs(hv, dc, k, tid, blockDim.x);
     for (int q = tid; q < k; q += blockDim.x)
@@ -1517,14 +1542,19 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
 
     PROBE_ADD(1000 + 8 * c + 0, t_sel0);
     PROBE_T(t_sel1);
-    // scan of list positions k .. S-1 by wave 0 alone, in groups of
-    // kScanGroup chunks of 64 positions. The next group's global loads are in
-    // flight in registers while the current group, staged in LDS, is tested:
-    // one ballot per chunk against the current root, then the admissions in
-    // list order, each later position of the chunk tested again against the
-    // root left by the admission before it. (The quirky heap's root can also
-    // drop after an admission, so no test is ever made against an older
-    // root.)
+    // scan of list positions k .. S-1 in groups of kScanGroup chunks of 64
+    // positions. Waves 1..3 stream the groups' |Delta| (-1 past the end) into
+    // a ring of R slots in LDS, two groups of loads in flight per wave, each
+    // slot stamped with its group when written; wave 0 consumes the groups in
+    // order: a group none of whose values exceeds the current root admits
+    // nothing and is skipped whole, otherwise each chunk is tested with one
+    // ballot against the current root and its admissions run in list order,
+    // each later position of the chunk tested again against the root left by
+    // the admission before it. (The quirky heap's root can also drop after an
+    // admission, so no test is ever made against an older root.)
+    const int rest = (int)(S - (size_t)k);
+    const int ngroups = (rest + kScanGroup * 64 - 1) / (kScanGroup * 64);
+    const int R = PD.sel_ring;
     RegHeap rh;
     if (wave == 0) {
         rh.load(hv, hid, k, lane);
@@ -1532,23 +1562,6 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
         const unsigned long long tscan = clock64();
         unsigned long long adm_cnt = 0, adm_cyc = 0;
 #endif
-        const gdouble* __restrict__ src = (const gdouble*)(dc + k);
-        const int rest = (int)(S - (size_t)k);
-        const int ngroups = (rest + kScanGroup * 64 - 1) / (kScanGroup * 64);
-        double ra[kScanGroup];
-        // group 0 to ring slot 0, group 1's loads in flight
-#pragma unroll
-        for (int j = 0; j < kScanGroup; ++j) {
-            const int e = 64 * j + lane;
-            ra[j] = src[e < rest ? e : rest - 1];  // clamped; masked when tested
-        }
-#pragma unroll
-        for (int j = 0; j < kScanGroup; ++j) ring[64 * j + lane] = ra[j];
-#pragma unroll
-        for (int j = 0; j < kScanGroup; ++j) {
-            const int e = 64 * (kScanGroup + j) + lane;
-            ra[j] = src[e < rest ? e : rest - 1];
-        }
         // the admissions of one chunk (positions base .. base + 63 in v)
         auto scan_chunk = [&](double v, int base) {
             double rv = rh.r0v;
@@ -1577,34 +1590,28 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
             }
         };
         for (int g = 0; g < ngroups; ++g) {
-            const double* cur = ring + (g & 1) * (kScanGroup * 64);
-            const int gbase = g * kScanGroup * 64;
-            // chunk pairs, the next pair's LDS reads in flight
-            double n0 = cur[lane], n1 = cur[64 + lane];
-            for (int j = 0; j < kScanGroup; j += 2) {
-                const int e0 = gbase + 64 * j + lane;
-                const double v0 = (e0 < rest) ? fabs(n0) : -1.0;
-                const double v1 = (e0 + 64 < rest) ? fabs(n1) : -1.0;
-                if (j + 2 < kScanGroup) {
-                    n0 = cur[64 * (j + 2) + lane];
-                    n1 = cur[64 * (j + 3) + lane];
-                }
-                const double rv = rh.r0v;
-                if (__ballot(v0 > rv || v1 > rv) == 0ull)
-                    continue;
-                scan_chunk(v0, gbase + 64 * j);
-                scan_chunk(v1, gbase + 64 * j + 64);
-            }
-            // group g+1 (arrived during the tests) to the other slot, group
-            // g+2's loads in flight
-            double* nxt = ring + ((g + 1) & 1) * (kScanGroup * 64);
-#pragma unroll
-            for (int j = 0; j < kScanGroup; ++j) nxt[64 * j + lane] = ra[j];
-            const int nbase = (g + 2) * kScanGroup * 64;
+            const int slot = g % R;
+            scan_lds_wait_eq(&s_ready[slot], g + 1);
+            const double* cur = ring + (size_t)slot * (kScanGroup * 64);
+            double v[kScanGroup];
+            double mx = -1.0;
 #pragma unroll
             for (int j = 0; j < kScanGroup; ++j) {
-                const int e = nbase + 64 * j + lane;
-                ra[j] = src[e < rest ? e : rest - 1];
+                v[j] = cur[64 * j + lane];
+                mx = fmax(mx, v[j]);
+            }
+            // every lane has its values in registers: the slot is free again
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0)
+                *(volatile int*)&s_cons = g + 1;
+            if (__ballot(mx > rh.r0v) == 0ull)
+                continue;
+            const int gbase = g * kScanGroup * 64;
+#pragma unroll
+            for (int j = 0; j < kScanGroup; ++j) {
+                if (__ballot(v[j] > rh.r0v) != 0ull)
+                    scan_chunk(v[j], gbase + 64 * j);
             }
         }
 #ifdef FEDNL_PROBE
@@ -1614,6 +1621,42 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
             g_probe[1000 + 8 * c + 4] += adm_cyc;
         }
 #endif
+    } else {
+        // producers: wave w takes groups w-1, w+2, ...; the loads of its next
+        // groups are in flight while it waits for a free slot
+        const gdouble* __restrict__ src = (const gdouble*)(dc + k);
+        constexpr int kProd = 3;
+        auto load_group = [&](int g, double (&va)[kScanGroup]) {
+#pragma unroll
+            for (int j = 0; j < kScanGroup; ++j) {
+                const int e = g * kScanGroup * 64 + 64 * j + lane;
+                va[j] = src[e < rest ? e : rest - 1];  // clamped; replaced by -1 below
+            }
+        };
+        double va[kScanGroup], vb[kScanGroup];
+        int g = wave - 1;
+        if (g < ngroups)
+            load_group(g, va);
+        for (; g < ngroups; g += kProd) {
+            if (g + kProd < ngroups)
+                load_group(g + kProd, vb);
+            const int slot = g % R;
+            // the consumer is done with group g - R (the slot's last tenant)
+            scan_lds_wait_ge(&s_cons, g - R + 1);
+            double* dst = ring + (size_t)slot * (kScanGroup * 64);
+#pragma unroll
+            for (int j = 0; j < kScanGroup; ++j) {
+                const int e = g * kScanGroup * 64 + 64 * j + lane;
+                dst[64 * j + lane] = (e < rest) ? fabs(va[j]) : -1.0;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0)
+                *(volatile int*)&s_ready[slot] = g + 1;
+#pragma unroll
+            for (int j = 0; j < kScanGroup; ++j)
+                va[j] = vb[j];
+        }
     }
     if (wave == 0)
         rh.store(hv, hid);
@@ -3559,6 +3602,31 @@ static int ensure_trace(Engine* e, int need)
 }
 
 // the engine's descriptor as the kernels see it
+// The heap selection's LDS: the heap values and ids, then the scan's ring of
+// staged groups (as many slots as keep the selection near 64 KB, so that two
+// selections share a CU; at least 2, at most kMaxScanRing).
+static size_t select_heap_bytes(int k, size_t S)
+{
+    const int P = pow2_at_least(k);
+    const int ids = select_bitmap_fits(S, k) ? k : P;
+    return (size_t)8 * (k + 4) + (size_t)4 * round_up(ids + 4, 2);
+}
+static int scan_ring_slots(int k, size_t S)
+{
+    const size_t slot = (size_t)8 * kScanGroup * 64;
+    const size_t heap = select_heap_bytes(k, S);
+    const size_t target = 64 * 1024;
+    const int r = heap + 2 * slot >= target ? 2 : (int)((target - heap) / slot);
+    return std::max(2, std::min(kMaxScanRing, r));
+}
+static size_t select_lds_bytes(int k, size_t S)
+{
+    const int P = pow2_at_least(k);
+    const size_t heap = select_heap_bytes(k, S) + (size_t)8 * kScanGroup * 64 * scan_ring_slots(k, S);
+    const size_t rk = (size_t)8 * P + (size_t)8 * k;
+    return std::max(heap, rk);
+}
+
 static ProblemDev make_desc(const Engine* e)
 {
     ProblemDev P{};
@@ -3584,6 +3652,7 @@ static ProblemDev make_desc(const Engine* e)
     P.lek_popped = e->lek_popped; P.seq_start = e->seq_start; P.sel_bm = e->sel_bm;
     P.sel_cnt = e->sel_cnt; P.cnt = e->cnt; P.lek_hit = e->lek_hit; P.round_ctr = e->round_ctr;
     P.nsync = e->nsync; P.ny = e->ny; P.ninv = e->ninv; P.nxt = e->nxt;
+    P.sel_ring = scan_ring_slots(std::min<int>(e->k, kMaxHeapK), e->S);
     return P;
 }
 
@@ -3767,14 +3836,6 @@ static int launch_round(Engine* e)
     return FEDNL_OK;
 }
 
-static size_t select_lds_bytes(int k, size_t S)
-{
-    const int P = pow2_at_least(k);
-    const int ids = select_bitmap_fits(S, k) ? k : P;
-    const size_t heap = (size_t)8 * (k + 4) + (size_t)4 * round_up(ids + 4, 2) + (size_t)8 * 2 * kScanGroup * 64;
-    const size_t rk = (size_t)8 * P + (size_t)8 * k;
-    return std::max(heap, rk);
-}
 
 static void free_engine(Engine* e)
 {
@@ -5114,6 +5175,7 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     hp.tiles_up = (compressor == FEDNL_TOPLEK) ? 1 : 0;
     hp.part = parts;
     hp.lek_delta = delta_param;
+    hp.sel_ring = scan_ring_slots(std::min(k, kMaxHeapK), S);
     hp.comp = compressor;
     hp.workers = workers;
     hp.alpha = 1.0;
