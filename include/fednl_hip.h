@@ -174,7 +174,9 @@ int fednl_kernel_times(void *handle, double *ms, int *launches, int max_kernels,
  * updated in place); TopLEK draws from toplek_rng[c % workers] (updated in
  * place), with delta_param = k/S as the TopLEK worker passes it
  * (LocalWorkerForFedNL_TopLEK.h:19,58). out_pos: nclients*k list positions,
- * ascending (RandSeqK: ascending too); out_cnt: counts. */
+ * ascending (RandSeqK: ascending too); out_cnt: counts. TopK, TopLEK and
+ * RandK take k <= 32768 (above 8192, or for S above 2^20, the heap and the
+ * selection bitmap live in global memory); FEDNL_E_UNSUPPORTED beyond. */
 int fednl_select(int compressor, int nclients, int d, int k, const double *delta, uint32_t *rng,
                  uint32_t *toplek_rng, int workers, double delta_param, uint32_t *out_pos,
                  int *out_cnt);
@@ -193,7 +195,8 @@ int fednl_newton_solve(int d, const double *H_up, double shift, const double *b,
  * without waiting. The same kernels as fednl_newton_solve give the same dx;
  * *ok becomes 1 when the factorization succeeded, else 0 (dx = 0). The call
  * allocates nothing: its workspace (descriptor, shift, status and the
- * 512 x 512 Cholesky workspace, 2 MiB + 2 KiB) is a device array of the
+ * 512 x 512 Cholesky workspace with its block-column handoff buffers,
+ * 2 MiB + 138 KiB) is a device array of the
  * library that each GPU holds until the process ends. d <= 512, larger
  * returns FEDNL_E_UNSUPPORTED. Calls on one GPU share that array: calls that
  * overlap on one GPU (other streams, other threads) race. */

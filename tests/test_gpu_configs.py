@@ -202,3 +202,44 @@ def test_config5_d700_lockstep(oracle):
         diag = lockstep_round(eng, orc, "topk", max_flip_frac=1e-3)
         print(f"config5 d=700 lockstep round {r + 1}: {diag}", flush=True)
     eng.close()
+
+
+def test_config5_d2048_lockstep(oracle):
+    """The top of config5's range: d = 2048, TopK k = 16384 (S = 2098176):
+    the heap beyond LDS, the selection bitmap built in global memory, the
+    Newton solve over 64 block columns. Lockstep rounds."""
+    prob = datagen.gaussian_problem("c5_member_n4_m200_d2048", 4, 200, 2048, seed=52)
+    orc = oracle.FedNL(prob, "topk", k_mult=8.0)
+    orc.round()
+    eng = _engine(prob, "topk")
+    assert eng.k == 16384
+    for r in range(2):
+        diag = lockstep_round(eng, orc, "topk", max_flip_frac=1e-3)
+        print(f"config5 d=2048 lockstep round {r + 1}: {diag}", flush=True)
+    eng.close()
+
+
+def test_config5_wide_members_in_batch(oracle):
+    """RandK and RandSeqK members at d = 1500 (k = 12000, S = 1125750: wide
+    selection) in one graph-replayed batch beside a small TopK member:
+    ||grad f|| trajectories against the oracle's."""
+    from ufednl_amd.engine import Batch
+    big = datagen.gaussian_problem("c5_member_n4_m200_d1500", 4, 200, 1500, seed=53)
+    small = datagen.small_problem(n=6, m=45, d_feat=13, density=0.3, seed=5)
+    members = [(big, "randk", 8.0), (big, "seqk", 8.0), (small, "topk", 2.0)]
+    R = 3
+    refs = []
+    for p, c, km in members:
+        orc = oracle.FedNL(p, c, k_mult=km)
+        refs.append(np.array([orc.round(tracking=False)["grad_norm"] for _ in range(R)]))
+    from ufednl_amd.engine import Engine
+    engs = [Engine.from_problem(p, compressor=c, k_mult=km, tracking=False) for p, c, km in members]
+    batch = Batch(engs, use_graph=True)
+    batch.run(R)
+    for (p, c, km), e, g_ref in zip(members, engs, refs):
+        g = e.trace()["grad_norm"]
+        print(f"config5 wide batch member {c} d={e.d}: {g}", flush=True)
+        assert_grad_norms_close(g, g_ref, feature_scale=float(np.abs(p.features).max()))
+    batch.close()
+    for e in engs:
+        e.close()

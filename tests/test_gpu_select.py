@@ -42,6 +42,24 @@ def test_topk_bit_exact(oracle, d, k, kind):
         np.testing.assert_array_equal(pos[c, :cnt[c]], ref)
 
 
+# past the LDS-resident heap (k > 8192) and past the LDS bitmap (S > 2^20):
+# the heap in global scratch, the bitmap built in global memory
+WIDE = [(1100, 9000), (1500, 3000), (1500, 12000), (2048, 16384)]
+
+
+@pytest.mark.parametrize("d,k", WIDE)
+@pytest.mark.parametrize("kind", ["gauss", "ties", "zeros"])
+def test_topk_wide_bit_exact(oracle, d, k, kind):
+    test_topk_bit_exact(oracle, d, k, kind)
+
+
+@pytest.mark.parametrize("d,k", [(1100, 9000), (2048, 16384)])
+@pytest.mark.parametrize("kind", ["gauss", "round1"])
+@pytest.mark.parametrize("workers", [1, 2])
+def test_toplek_wide_bit_exact(oracle, d, k, kind, workers):
+    test_toplek_bit_exact(oracle, d, k, kind, workers)
+
+
 @pytest.mark.parametrize("d,k", [(5, 3), (12, 40), (30, 100), (68, 544), (123, 984)])
 @pytest.mark.parametrize("kind", ["gauss", "ties", "round1"])
 @pytest.mark.parametrize("workers", [1, 2])
@@ -62,7 +80,8 @@ def test_toplek_bit_exact(oracle, d, k, kind, workers):
     np.testing.assert_array_equal(lek_after, np.array(streams, dtype=np.uint32))
 
 
-@pytest.mark.parametrize("d,k", [(5, 3), (5, 15), (12, 40), (68, 544), (123, 984), (300, 2400)])
+@pytest.mark.parametrize("d,k", [(5, 3), (5, 15), (12, 40), (68, 544), (123, 984), (300, 2400), (1500, 3000),
+                                 (2048, 16384)])
 def test_randk_bit_exact(oracle, d, k):
     from ufednl_amd import engine as E
     S = d * (d + 1) // 2
@@ -80,7 +99,7 @@ def test_randk_bit_exact(oracle, d, k):
         states = states_gpu
 
 
-@pytest.mark.parametrize("d,k", [(5, 3), (12, 78), (68, 544), (300, 2400)])
+@pytest.mark.parametrize("d,k", [(5, 3), (12, 78), (68, 544), (300, 2400), (2048, 16384)])
 def test_randseqk_bit_exact(oracle, d, k):
     from ufednl_amd import engine as E
     S = d * (d + 1) // 2

@@ -95,6 +95,8 @@ constexpr int kTile = 64;          // Hessian output tile (rows and columns)
 constexpr int kKC = 16;            // samples staged per LDS chunk
 constexpr int kPadLds = 4;         // LDS row padding (doubles)
 constexpr int kMaxHeapK = 8192;    // LDS-resident heap limit
+constexpr int kMaxSelK = 32768;    // TopK / TopLEK / RandK: largest k (the tail sorts k ids in LDS)
+constexpr int kEmitLdsMax = 128 * 1024;  // largest selection bitmap kept in LDS (bytes)
 constexpr int kScanGroup = 8;      // TopK scan: chunks of 64 positions per staged group
 constexpr int kMaxScanRing = 16;   // TopK scan: most LDS slots of staged groups
 static_assert(kScanGroup % 2 == 0, "the scan tests chunk pairs");
@@ -285,6 +287,12 @@ struct ProblemDev {
     int* nsync;
     double *ny, *ninv, *nxt;
     int sel_ring;       // TopK / TopLEK scan: LDS slots of staged groups (2 .. kMaxScanRing)
+    // wide selection (k above the LDS heap or S above the LDS bitmap): the
+    // heap (or RandK's sort keys) in per-client global scratch, the emit
+    // bitmap written straight to sel_bm
+    int wide;
+    unsigned char* sel_scr;
+    size_t sel_scr_stride;  // bytes per client
 };
 
 #define FEDNL_BLOCK()                                                                     \
@@ -1466,8 +1474,11 @@ __device__ inline double fro2_from_parts(const double* part, int c, int tiles_up
 // tested against the current root with one ballot, and only admitted
 // positions enter the (sequential) heap update.
 // Output: sorted selected list positions, count; TopLEK: hit flag, p, popped.
+// kWide: the heap lives in the client's global scratch (k above what LDS
+// holds); the ring of staged groups stays in LDS and the tail sorts the ids
+// in LDS.
 // ---------------------------------------------------------------------------
-template <bool kLek>
+template <bool kLek, bool kWide>
 __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk)
 {
     const gmem<double>* __restrict__ delta = G(PD.delta);
@@ -1509,10 +1520,19 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
     const size_t words = (S + 31) / 32;
     // the tail orders the selection through a bitmap in the value area, or
     // (when that is too small) by a bitonic sort of P ids
-    const bool bm_tail = select_bitmap_fits(S, k);
-    double* hv = (double*)smem;                                // k + 4 (padding read by heapify)
-    uint32_t* hid = (uint32_t*)(hv + k + 4);                   // k + 4, or P + 4 for the sort
-    double* ring = (double*)(hid + round_up((bm_tail ? k : P) + 4, 2));  // sel_ring * kScanGroup * 64
+    const bool bm_tail = !kWide && select_bitmap_fits(S, k);
+    double* hv;      // k + 4 (padding read by heapify)
+    uint32_t* hid;   // k + 4, or P + 4 for the sort
+    double* ring;    // sel_ring * kScanGroup * 64
+    if (kWide) {
+        hv = (double*)(PD.sel_scr + (size_t)c * PD.sel_scr_stride);
+        hid = (uint32_t*)(hv + k + 4);
+        ring = (double*)smem;
+    } else {
+        hv = (double*)smem;
+        hid = (uint32_t*)(hv + k + 4);
+        ring = (double*)(hid + round_up((bm_tail ? k : P) + 4, 2));
+    }
     __shared__ int s_ready[kMaxScanRing];
     __shared__ int s_cons;
     if (tid < kMaxScanRing)
@@ -1744,12 +1764,16 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
         __syncthreads();
         compact_bitmap(bm, words, sel + (size_t)c * k);
     } else {
-        for (int q = cnt + tid; q < P; q += blockDim.x)
-            hid[q] = 0xFFFFFFFFu;
+        // the ids sorted in LDS (wide: copied there from the global heap,
+        // over the ring, which is no longer read)
+        uint32_t* srt = kWide ? (uint32_t*)smem : hid;
+        for (int q = tid; q < P; q += blockDim.x)
+            if (kWide || q >= cnt)
+                srt[q] = q < cnt ? hid[q] : 0xFFFFFFFFu;
         __syncthreads();
-        bitonic_sort_u32(hid, P);
+        bitonic_sort_u32(srt, P);
         for (int q = tid; q < cnt; q += blockDim.x)
-            sel[(size_t)c * k + q] = hid[q];
+            sel[(size_t)c * k + q] = srt[q];
     }
     if (tid == 0)
         sel_cnt[c] = cnt;
@@ -1762,7 +1786,9 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
 // sequential swaps: every draw u_i comes from an LCG jump-ahead, the swap
 // target t_i = i + round(u_i (S-1-i)); the value left at slot i is found by
 // following the chain of earlier swaps that touched its source slot.
+// kWide: the sort keys and chains in the client's global scratch.
 // ---------------------------------------------------------------------------
+template <bool kWide>
 __device__ __forceinline__ void select_randk_client(const ProblemDev& PD, int blk)
 {
     const size_t S = PD.S;
@@ -1776,7 +1802,7 @@ __device__ __forceinline__ void select_randk_client(const ProblemDev& PD, int bl
     if (pp_idle(PD, c))
         return;
     const int P = pow2_at_least(k);
-    uint64_t* keys = (uint64_t*)smem;            // P
+    uint64_t* keys = kWide ? (uint64_t*)(PD.sel_scr + (size_t)c * PD.sel_scr_stride) : (uint64_t*)smem;  // P
     int* qprev = (int*)(keys + P);                // k
     int* pprev = qprev + k;                       // k
     uint32_t* outk = (uint32_t*)keys;             // reused after the chains are resolved
@@ -1835,8 +1861,9 @@ __device__ __forceinline__ void select_randk_client(const ProblemDev& PD, int bl
         if (src < 0) {
             v = ti;
         } else {
+            // every link goes to a smaller index: at most k steps
             int r = src;
-            while (pprev[r] >= 0) r = pprev[r];
+            for (int hop = 0; hop < k && pprev[r] >= 0; ++hop) r = pprev[r];
             v = (uint64_t)r;
         }
         outk[i] = (uint32_t)v;
@@ -1905,7 +1932,9 @@ __device__ __forceinline__ void select_window_client(const ProblemDev& PD, int b
 // ---------------------------------------------------------------------------
 // the client's message values, positions, flat indices, selection bitmap and
 // learned update (LocalWorkerForFedNL_*.h), after its selection; every
-// thread of the workgroup takes part; dynamic LDS holds the bitmap
+// thread of the workgroup takes part; dynamic LDS holds the bitmap (kWide:
+// the bitmap is built in sel_bm itself)
+template <bool kWide>
 __device__ __forceinline__ void emit_client_body(const ProblemDev& PD, int blk)
 {
     const int comp = PD.comp;
@@ -1925,12 +1954,15 @@ __device__ __forceinline__ void emit_client_body(const ProblemDev& PD, int blk)
     gmem<double>* __restrict__ vals_out = G(PD.vals);
     gmem<int>* __restrict__ cnt_out = G(PD.cnt);
     gmem<uint32_t>* __restrict__ sel_bm = G(PD.sel_bm);
-    extern __shared__ __align__(16) uint32_t bml[];  // bm_words
+    extern __shared__ __align__(16) uint32_t bm_lds[];  // bm_words
     const int c = blk;
     if (pp_idle(PD, c))
         return;
+    uint32_t* const bml = kWide ? (uint32_t*)(sel_bm + (size_t)c * bm_words) : bm_lds;
     for (int w = threadIdx.x; w < bm_words; w += blockDim.x)
         bml[w] = 0u;
+    if (kWide)
+        __threadfence();  // the cleared words reach memory before any atomicOr
     __shared__ int s_cnt, s_ins;
     __shared__ uint32_t s_pop;
     if (threadIdx.x == 0) {
@@ -2009,50 +2041,54 @@ __device__ __forceinline__ void emit_client_body(const ProblemDev& PD, int blk)
         idx_out[(size_t)c * k + t] = 0xFFFFFFFFu;
         vals_out[(size_t)c * k + t] = 0.0;
     }
-    __syncthreads();
-    for (int w = threadIdx.x; w < bm_words; w += blockDim.x)
-        sel_bm[(size_t)c * bm_words + w] = bml[w];
+    if (!kWide) {
+        __syncthreads();
+        for (int w = threadIdx.x; w < bm_words; w += blockDim.x)
+            sel_bm[(size_t)c * bm_words + w] = bml[w];
+    }
 }
 
 
+template <bool kWide>
 __global__ __launch_bounds__(256) void k_emit(FEDNL_KARGS)
 {
     FEDNL_BLOCK();
-    emit_client_body(PD, blk);
+    emit_client_body<kWide>(PD, blk);
 }
 
 // kEmit: the client's messages and learned update follow in the same
-// workgroup (every compressor but TopLEK, whose draw needs the other clients)
-template <bool kLek, bool kEmit>
+// workgroup (every compressor but TopLEK, whose draw needs the other clients).
+// kWide: heap / sort keys in global scratch, bitmap built in global memory.
+template <bool kLek, bool kEmit, bool kWide>
 __global__ __launch_bounds__(256) void k_select_heap(FEDNL_KARGS)
 {
     FEDNL_BLOCK();
-    select_heap_client<kLek>(PD, blk);
+    select_heap_client<kLek, kWide>(PD, blk);
     if (kEmit) {
         __syncthreads();
-        emit_client_body(PD, blk);
+        emit_client_body<kWide>(PD, blk);
     }
 }
 
-template <bool kEmit>
+template <bool kEmit, bool kWide>
 __global__ __launch_bounds__(256) void k_select_randk(FEDNL_KARGS)
 {
     FEDNL_BLOCK();
-    select_randk_client(PD, blk);
+    select_randk_client<kWide>(PD, blk);
     if (kEmit) {
         __syncthreads();
-        emit_client_body(PD, blk);
+        emit_client_body<kWide>(PD, blk);
     }
 }
 
-template <bool kEmit>
+template <bool kEmit, bool kWide>
 __global__ __launch_bounds__(256) void k_select_window(FEDNL_KARGS)
 {
     FEDNL_BLOCK();
     select_window_client(PD, blk);
     if (kEmit) {
         __syncthreads();
-        emit_client_body(PD, blk);
+        emit_client_body<kWide>(PD, blk);
     }
 }
 
@@ -3524,7 +3560,10 @@ struct Engine {
     std::vector<hipEvent_t> ev_pool;
     double kms[KID_COUNT] = {0};
     int klaunch[KID_COUNT] = {0};
-    size_t lds_select = 0, lds_newton = 0;
+    size_t lds_select = 0, lds_emit = 0, lds_newton = 0;
+    bool wide = false;             // selection heap / bitmap in global memory
+    unsigned char* sel_scr = nullptr;  // wide selection: per-client heap or sort keys
+    size_t sel_scr_stride = 0;
     int hmode = 0;                 // 0 FP64, 1 FP32, 2 BF16 split (A^T D A only)
     ProblemDev hdesc{};            // descriptor as last uploaded
     ProblemDev* pdev = nullptr;    // device copy read by every round kernel
@@ -3627,6 +3666,44 @@ static size_t select_lds_bytes(int k, size_t S)
     return std::max(heap, rk);
 }
 
+// the compressor keeps a heap (TopK, TopLEK: unless k covers the whole
+// list) or sorts k keys (RandK: always, k = S included)
+static bool select_keeps_k(int comp, int k, size_t S)
+{
+    return ((comp == FEDNL_TOPK || comp == FEDNL_TOPLEK) && (size_t)k < S) || comp == FEDNL_RANDK;
+}
+// Wide selection: k above the LDS-resident heap, or a selection bitmap of
+// S positions above kEmitLdsMax. The heap (or RandK's keys and chains) then
+// lives in per-client global scratch and the bitmap is built in global memory.
+static bool select_wide(int comp, int k, size_t S)
+{
+    return (select_keeps_k(comp, k, S) && k > kMaxHeapK) || ((S + 31) / 32) * 4 > (size_t)kEmitLdsMax;
+}
+// bytes of wide-selection scratch per client (16-byte multiple)
+static size_t select_scratch_stride(int comp, int k, size_t S)
+{
+    if (!select_keeps_k(comp, k, S))
+        return 0;
+    const size_t P = (size_t)pow2_at_least(k);
+    const size_t heap = (size_t)12 * (k + 4);
+    const size_t rk = (size_t)8 * P + (size_t)8 * k;
+    return (std::max(heap, rk) + 15) / 16 * 16;
+}
+// dynamic LDS of the selection proper (the fused emit adds its bitmap)
+static size_t select_lds(int comp, int k, size_t S, bool wide)
+{
+    if (!select_keeps_k(comp, k, S))
+        return 0;
+    if (!wide)
+        return select_lds_bytes(k, S);
+    if (comp == FEDNL_RANDK)
+        return 0;
+    // the scan's ring, later the tail's sort of the ids
+    const size_t ring = (size_t)8 * kScanGroup * 64 * kMaxScanRing;
+    return std::max(ring, (size_t)4 * pow2_at_least(k));
+}
+static size_t emit_lds(size_t S, bool wide) { return wide ? 0 : sizeof(uint32_t) * ((S + 31) / 32); }
+
 static ProblemDev make_desc(const Engine* e)
 {
     ProblemDev P{};
@@ -3652,7 +3729,10 @@ static ProblemDev make_desc(const Engine* e)
     P.lek_popped = e->lek_popped; P.seq_start = e->seq_start; P.sel_bm = e->sel_bm;
     P.sel_cnt = e->sel_cnt; P.cnt = e->cnt; P.lek_hit = e->lek_hit; P.round_ctr = e->round_ctr;
     P.nsync = e->nsync; P.ny = e->ny; P.ninv = e->ninv; P.nxt = e->nxt;
-    P.sel_ring = scan_ring_slots(std::min<int>(e->k, kMaxHeapK), e->S);
+    P.sel_ring = e->wide ? kMaxScanRing : scan_ring_slots(std::min<int>(e->k, kMaxHeapK), e->S);
+    P.wide = e->wide ? 1 : 0;
+    P.sel_scr = e->sel_scr;
+    P.sel_scr_stride = e->sel_scr_stride;
     return P;
 }
 
@@ -3798,21 +3878,37 @@ static int launch_round(Engine* e)
         KTime kt(e, KID_SELECT);
         // the messages follow the selection in the same workgroup, except
         // for TopLEK (its draw runs over the other clients' outcomes)
-        const size_t lds_bm = sizeof(uint32_t) * e->bm_words;
+        const size_t lds_bm = e->lds_emit;
         const size_t lds_fused = std::max(e->lds_select, lds_bm);
-        if (comp == FEDNL_TOPK)
-            hipLaunchKernelGGL((k_select_heap<false, true>), dim3(n), dim3(256), lds_fused, e->st, pd, nomap);
-        else if (comp == FEDNL_TOPLEK)
-            hipLaunchKernelGGL((k_select_heap<true, false>), dim3(n), dim3(256), e->lds_select, e->st, pd, nomap);
-        else if (comp == FEDNL_RANDK)
-            hipLaunchKernelGGL(k_select_randk<true>, dim3(n), dim3(256), lds_fused, e->st, pd, nomap);
-        else
-            hipLaunchKernelGGL(k_select_window<true>, dim3(n), dim3(256), lds_bm, e->st, pd, nomap);
+        const dim3 g(n), t(256);
+        if (!e->wide) {
+            if (comp == FEDNL_TOPK)
+                hipLaunchKernelGGL((k_select_heap<false, true, false>), g, t, lds_fused, e->st, pd, nomap);
+            else if (comp == FEDNL_TOPLEK)
+                hipLaunchKernelGGL((k_select_heap<true, false, false>), g, t, e->lds_select, e->st, pd, nomap);
+            else if (comp == FEDNL_RANDK)
+                hipLaunchKernelGGL((k_select_randk<true, false>), g, t, lds_fused, e->st, pd, nomap);
+            else
+                hipLaunchKernelGGL((k_select_window<true, false>), g, t, lds_bm, e->st, pd, nomap);
+        } else {
+            if (comp == FEDNL_TOPK)
+                hipLaunchKernelGGL((k_select_heap<false, true, true>), g, t, lds_fused, e->st, pd, nomap);
+            else if (comp == FEDNL_TOPLEK)
+                hipLaunchKernelGGL((k_select_heap<true, false, true>), g, t, e->lds_select, e->st, pd, nomap);
+            else if (comp == FEDNL_RANDK)
+                hipLaunchKernelGGL((k_select_randk<true, true>), g, t, lds_fused, e->st, pd, nomap);
+            else
+                hipLaunchKernelGGL((k_select_window<true, true>), g, t, lds_bm, e->st, pd, nomap);
+        }
     }
     if (comp == FEDNL_TOPLEK || pp) {
         KTime kt(e, KID_EMIT);
-        if (comp == FEDNL_TOPLEK)
-            hipLaunchKernelGGL(k_emit, dim3(n), dim3(256), sizeof(uint32_t) * e->bm_words, e->st, pd, nomap);
+        if (comp == FEDNL_TOPLEK) {
+            if (e->wide)
+                hipLaunchKernelGGL(k_emit<true>, dim3(n), dim3(256), 0, e->st, pd, nomap);
+            else
+                hipLaunchKernelGGL(k_emit<false>, dim3(n), dim3(256), e->lds_emit, e->st, pd, nomap);
+        }
         if (pp)
             hipLaunchKernelGGL(k_pp_client<false>, dim3(n), dim3(256), 0, e->st, pd, nomap);
     }
@@ -3846,7 +3942,7 @@ static void free_engine(Engine* e)
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
                     e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev, e->mst, e->H0, e->pp_sel, e->pp_rng,
                     e->pp_dir, e->pp_lk, e->pp_gmsg, e->pp_lkmsg, e->pp_h, e->pp_tg, e->nsync, e->ny, e->ninv,
-                    e->nxt};
+                    e->nxt, e->sel_scr};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -3975,9 +4071,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         delete e;
         return FEDNL_E_UNSUPPORTED;
     }
-    if ((comp == FEDNL_TOPK || comp == FEDNL_TOPLEK || comp == FEDNL_RANDK) && e->k > kMaxHeapK &&
-        (size_t)e->k < e->S) {
-        set_err("fednl_create: k above the LDS-resident selection limit (8192)");
+    if (select_keeps_k(comp, e->k, e->S) && e->k > kMaxSelK) {
+        set_err("fednl_create: k above the selection limit (32768)");
         delete e;
         return FEDNL_E_UNSUPPORTED;
     }
@@ -3992,7 +4087,10 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     e->prescale = e->inv_n * e->alpha;
     e->mult = (double)e->S / (double)e->k;
     e->lek_delta = (double)e->k * (1.0 / (double)e->S);
-    e->lds_select = select_lds_bytes(std::min<int>(e->k, kMaxHeapK), e->S);
+    e->wide = select_wide(comp, e->k, e->S);
+    e->sel_scr_stride = e->wide ? select_scratch_stride(comp, e->k, e->S) : 0;
+    e->lds_select = select_lds(comp, e->k, e->S, e->wide);
+    e->lds_emit = emit_lds(e->S, e->wide);
     e->lds_newton = mc_lds_bytes(e->dp);
 
     int rc = FEDNL_OK;
@@ -4031,6 +4129,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     ALLOC(e->seq_start, n);
     e->bm_words = (int)((e->S + 31) / 32);
     ALLOC(e->sel_bm, (size_t)n * e->bm_words);
+    if (e->sel_scr_stride)
+        ALLOC(e->sel_scr, (size_t)n * e->sel_scr_stride);
     ALLOC(e->sel_cnt, n);
     ALLOC(e->cnt, n);
     ALLOC(e->lek_hit, n);
@@ -4080,11 +4180,14 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         return rc;
     }
     {
-        const size_t lds = std::max(e->lds_select, sizeof(uint32_t) * e->bm_words);
+        const size_t lds = std::max(e->lds_select, e->lds_emit);
         hipError_t le = hipSuccess;
-        const void* sel_fns[] = {(const void*)k_select_heap<false, true>, (const void*)k_select_heap<false, false>,
-                                 (const void*)k_select_heap<true, false>, (const void*)k_select_randk<true>,
-                                 (const void*)k_select_randk<false>};
+        const void* sel_fns[] = {
+            (const void*)k_select_heap<false, true, false>, (const void*)k_select_heap<false, false, false>,
+            (const void*)k_select_heap<true, false, false>, (const void*)k_select_randk<true, false>,
+            (const void*)k_select_randk<false, false>,      (const void*)k_select_window<true, false>,
+            (const void*)k_emit<false>,                     (const void*)k_select_heap<false, true, true>,
+            (const void*)k_select_heap<false, false, true>, (const void*)k_select_heap<true, false, true>};
         for (const void* f : sel_fns)
             if (le == hipSuccess)
                 le = raise_lds_limit(f, lds);
@@ -4227,7 +4330,8 @@ static int enqueue_rounds(Engine* e, int rounds)
 // ---------------------------------------------------------------------------
 enum BatchKernel {
     BK_MARGIN, BK_HESS64, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
-    BK_EMIT, BK_MASTER, BK_NEWTON, BK_LS_MARGIN, BK_LS, BK_COUNT
+    BK_EMIT, BK_SEL_TOPK_W, BK_SEL_TOPLEK_W, BK_SEL_RANDK_W, BK_SEL_WINDOW_W, BK_EMIT_W, BK_MASTER, BK_NEWTON,
+    BK_LS_MARGIN, BK_LS, BK_COUNT
 };
 
 struct Batch {
@@ -4237,7 +4341,7 @@ struct Batch {
     std::vector<ProblemDev> h_probs;
     int2* d_maps = nullptr;
     int off[BK_COUNT] = {0}, cnt[BK_COUNT] = {0};
-    size_t lds_margin = 0, lds_select = 0, lds_emit = 0, lds_newton = 0;
+    size_t lds[BK_COUNT] = {0};  // dynamic LDS of each launch: the most any member needs
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -4260,7 +4364,9 @@ static void free_batch(Batch* b)
 static int batch_build(Batch* b)
 {
     std::vector<int2> maps[BK_COUNT];
-    auto add = [&](int kind, int p, int nb, bool align8) {
+    for (int q = 0; q < BK_COUNT; ++q) b->lds[q] = 0;
+    auto add = [&](int kind, int p, int nb, bool align8, size_t lds = 0) {
+        b->lds[kind] = std::max(b->lds[kind], lds);
         auto& v = maps[kind];
         if (align8)
             while (v.size() % 8) v.push_back(make_int2(-1, 0));
@@ -4272,27 +4378,31 @@ static int batch_build(Batch* b)
         const ProblemDev P = make_desc(e);
         b->h_probs.push_back(P);
         const int p = (int)i;
-        add(BK_MARGIN, p, blocks_margin(P), false);
+        add(BK_MARGIN, p, blocks_margin(P), false, sizeof(double) * (size_t)e->d_pad);
         add(e->hmode == 0 ? BK_HESS64 : (e->hmode == 1 ? BK_HESS32 : BK_HESS16), p, blocks_hessian(P), true);
         if (e->hmode != 0)
             add(BK_GRAD, p, blocks_grad(P), false);
         const int comp = e->o.compressor;
-        add(comp == FEDNL_TOPK ? BK_SEL_TOPK
-                               : comp == FEDNL_TOPLEK ? BK_SEL_TOPLEK : comp == FEDNL_RANDK ? BK_SEL_RANDK : BK_SEL_WINDOW,
-            p, e->n, false);
+        const bool w = e->wide;
+        // selection (with the messages fused, except for TopLEK)
+        const size_t fused = std::max(e->lds_select, e->lds_emit);
+        if (comp == FEDNL_TOPK)
+            add(w ? BK_SEL_TOPK_W : BK_SEL_TOPK, p, e->n, false, fused);
+        else if (comp == FEDNL_TOPLEK)
+            add(w ? BK_SEL_TOPLEK_W : BK_SEL_TOPLEK, p, e->n, false, e->lds_select);
+        else if (comp == FEDNL_RANDK)
+            add(w ? BK_SEL_RANDK_W : BK_SEL_RANDK, p, e->n, false, fused);
+        else
+            add(w ? BK_SEL_WINDOW_W : BK_SEL_WINDOW, p, e->n, false, e->lds_emit);
         if (comp == FEDNL_TOPLEK)
-            add(BK_EMIT, p, e->n, false);
+            add(w ? BK_EMIT_W : BK_EMIT, p, e->n, false, e->lds_emit);
         add(BK_MASTER, p, blocks_master(P), false);
-        add(BK_NEWTON, p, mc_blocks(P.dp), false);
+        add(BK_NEWTON, p, mc_blocks(P.dp), false, e->lds_newton);
         if (e->o.line_search) {
-            add(BK_LS_MARGIN, p, blocks_margin(P), false);
+            add(BK_LS_MARGIN, p, blocks_margin(P), false, sizeof(double) * (size_t)e->d_pad);
             add(BK_LS, p, 1, false);
             b->ls_steps = std::max(b->ls_steps, e->o.max_line_search_steps);
         }
-        b->lds_margin = std::max(b->lds_margin, sizeof(double) * (size_t)e->d_pad);
-        b->lds_select = std::max(b->lds_select, e->lds_select);
-        b->lds_emit = std::max(b->lds_emit, sizeof(uint32_t) * (size_t)e->bm_words);
-        b->lds_newton = std::max(b->lds_newton, e->lds_newton);
     }
     std::vector<int2> all;
     for (int k = 0; k < BK_COUNT; ++k) {
@@ -4335,7 +4445,7 @@ static void batch_launch_round(Batch* b)
     hipStream_t st = b->st;
     auto map = [&](int k) { return mp + b->off[k]; };
     if (b->cnt[BK_MARGIN])
-        hipLaunchKernelGGL((k_margin<false, 4>), dim3(b->cnt[BK_MARGIN]), dim3(256), b->lds_margin, st, pd,
+        hipLaunchKernelGGL((k_margin<false, 4>), dim3(b->cnt[BK_MARGIN]), dim3(256), b->lds[BK_MARGIN], st, pd,
                            map(BK_MARGIN));
     if (b->cnt[BK_HESS64])
         hipLaunchKernelGGL(k_hessian_f64<false>, dim3(b->cnt[BK_HESS64]), dim3(256), 0, st, pd, map(BK_HESS64),
@@ -4349,29 +4459,28 @@ static void batch_launch_round(Batch* b)
         hipLaunchKernelGGL(k_hessian_bf16<false>, dim3(b->cnt[BK_HESS16]), dim3(256), 0, st, pd, map(BK_HESS16),
                            (double*)nullptr);
     // selection and messages in one workgroup per client (TopLEK: emit after)
-    const size_t lds_fused = std::max(b->lds_select, b->lds_emit);
-    if (b->cnt[BK_SEL_TOPK])
-        hipLaunchKernelGGL((k_select_heap<false, true>), dim3(b->cnt[BK_SEL_TOPK]), dim3(256), lds_fused, st, pd,
-                           map(BK_SEL_TOPK));
-    if (b->cnt[BK_SEL_TOPLEK])
-        hipLaunchKernelGGL((k_select_heap<true, false>), dim3(b->cnt[BK_SEL_TOPLEK]), dim3(256), b->lds_select, st,
-                           pd, map(BK_SEL_TOPLEK));
-    if (b->cnt[BK_SEL_RANDK])
-        hipLaunchKernelGGL(k_select_randk<true>, dim3(b->cnt[BK_SEL_RANDK]), dim3(256), lds_fused, st, pd,
-                           map(BK_SEL_RANDK));
-    if (b->cnt[BK_SEL_WINDOW])
-        hipLaunchKernelGGL(k_select_window<true>, dim3(b->cnt[BK_SEL_WINDOW]), dim3(256), b->lds_emit, st, pd,
-                           map(BK_SEL_WINDOW));
-    if (b->cnt[BK_EMIT])
-        hipLaunchKernelGGL(k_emit, dim3(b->cnt[BK_EMIT]), dim3(256), b->lds_emit, st, pd, map(BK_EMIT));
+#define BK_LAUNCH(kind, ...)                                                                         \
+    if (b->cnt[kind])                                                                                \
+        hipLaunchKernelGGL((__VA_ARGS__), dim3(b->cnt[kind]), dim3(256), b->lds[kind], st, pd, map(kind))
+    BK_LAUNCH(BK_SEL_TOPK, k_select_heap<false, true, false>);
+    BK_LAUNCH(BK_SEL_TOPLEK, k_select_heap<true, false, false>);
+    BK_LAUNCH(BK_SEL_RANDK, k_select_randk<true, false>);
+    BK_LAUNCH(BK_SEL_WINDOW, k_select_window<true, false>);
+    BK_LAUNCH(BK_EMIT, k_emit<false>);
+    BK_LAUNCH(BK_SEL_TOPK_W, k_select_heap<false, true, true>);
+    BK_LAUNCH(BK_SEL_TOPLEK_W, k_select_heap<true, false, true>);
+    BK_LAUNCH(BK_SEL_RANDK_W, k_select_randk<true, true>);
+    BK_LAUNCH(BK_SEL_WINDOW_W, k_select_window<true, true>);
+    BK_LAUNCH(BK_EMIT_W, k_emit<true>);
+#undef BK_LAUNCH
     hipLaunchKernelGGL(k_master, dim3(b->cnt[BK_MASTER]), dim3(256), 0, st, pd, map(BK_MASTER));
-    hipLaunchKernelGGL(k_newton_mc, dim3(b->cnt[BK_NEWTON]), dim3(kMcThreads), b->lds_newton, st, pd,
+    hipLaunchKernelGGL(k_newton_mc, dim3(b->cnt[BK_NEWTON]), dim3(kMcThreads), b->lds[BK_NEWTON], st, pd,
                        map(BK_NEWTON));
     if (b->cnt[BK_LS]) {
         hipLaunchKernelGGL(k_ls_begin, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));
         for (int t = 0; t < b->ls_steps; ++t) {
-            hipLaunchKernelGGL((k_margin<true, 4>), dim3(b->cnt[BK_LS_MARGIN]), dim3(256), b->lds_margin, st, pd,
-                               map(BK_LS_MARGIN));
+            hipLaunchKernelGGL((k_margin<true, 4>), dim3(b->cnt[BK_LS_MARGIN]), dim3(256), b->lds[BK_LS_MARGIN], st,
+                               pd, map(BK_LS_MARGIN));
             hipLaunchKernelGGL(k_ls_check, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));
         }
     }
@@ -5061,8 +5170,8 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     const size_t S = (size_t)d * (d + 1) / 2;
     if ((size_t)k > S)
         return FEDNL_E_WRONG_ARGUMENT;
-    if (k > kMaxHeapK && (size_t)k < S) {
-        set_err("fednl_select: k above the LDS-resident selection limit");
+    if (select_keeps_k(compressor, k, S) && k > kMaxSelK) {
+        set_err("fednl_select: k above the selection limit (32768)");
         return FEDNL_E_UNSUPPORTED;
     }
     workers = std::max(1, std::min(workers, nclients));
@@ -5071,9 +5180,10 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     double *ddelta = nullptr, *learned = nullptr, *lek_p = nullptr, *vals = nullptr;
     uint32_t *drng = nullptr, *dlek = nullptr, *sel = nullptr, *pos = nullptr, *idx = nullptr, *popped = nullptr;
     int *sel_cnt = nullptr, *hit = nullptr, *cnt = nullptr;
+    unsigned char* scr = nullptr;  // wide selection scratch
     int rc = FEDNL_OK;
     auto cleanup = [&]() {
-        void* ps[] = {ddelta, learned, lek_p, vals, drng, dlek, sel, pos, idx, popped, sel_cnt, hit, cnt};
+        void* ps[] = {ddelta, learned, lek_p, vals, drng, dlek, sel, pos, idx, popped, sel_cnt, hit, cnt, scr};
         for (void* p : ps)
             if (p) hipFree(p);
         hipStreamDestroy(st);
@@ -5103,10 +5213,21 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     hipMemsetAsync(hit, 0, sizeof(int) * nclients, st);
     if (rng) hipMemcpyAsync(drng, rng, sizeof(uint32_t) * nclients, hipMemcpyHostToDevice, st);
     if (toplek_rng) hipMemcpyAsync(dlek, toplek_rng, sizeof(uint32_t) * workers, hipMemcpyHostToDevice, st);
-    const size_t lds = select_lds_bytes(std::min(k, kMaxHeapK), S);
-    raise_lds_limit((const void*)k_select_heap<false, false>, lds);
-    raise_lds_limit((const void*)k_select_heap<true, false>, lds);
-    raise_lds_limit((const void*)k_select_randk<false>, lds);
+    const bool wide = select_wide(compressor, k, S);
+    const size_t lds = select_lds(compressor, k, S, wide);
+    const size_t lds_bm = emit_lds(S, wide);
+    raise_lds_limit((const void*)k_select_heap<false, false, false>, lds);
+    raise_lds_limit((const void*)k_select_heap<true, false, false>, lds);
+    raise_lds_limit((const void*)k_select_randk<false, false>, lds);
+    raise_lds_limit((const void*)k_select_heap<false, false, true>, lds);
+    raise_lds_limit((const void*)k_select_heap<true, false, true>, lds);
+    raise_lds_limit((const void*)k_emit<false>, lds_bm);
+    const size_t scr_stride = wide ? select_scratch_stride(compressor, k, S) : 0;
+    if (scr_stride && dalloc(&scr, (size_t)nclients * scr_stride) != hipSuccess) {
+        set_err("hipMalloc (select scratch)");
+        cleanup();
+        return FEDNL_E_DEVICE;
+    }
     // TopLEK needs ||Delta||^2: computed on the host-side copy here as tile partials would be
     // in the round; the standalone path passes one partial per client.
     double* parts = nullptr;
@@ -5175,7 +5296,10 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     hp.tiles_up = (compressor == FEDNL_TOPLEK) ? 1 : 0;
     hp.part = parts;
     hp.lek_delta = delta_param;
-    hp.sel_ring = scan_ring_slots(std::min(k, kMaxHeapK), S);
+    hp.sel_ring = wide ? kMaxScanRing : scan_ring_slots(std::min(k, kMaxHeapK), S);
+    hp.wide = wide ? 1 : 0;
+    hp.sel_scr = scr;
+    hp.sel_scr_stride = scr_stride;
     hp.comp = compressor;
     hp.workers = workers;
     hp.alpha = 1.0;
@@ -5208,21 +5332,31 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     }
     const ProblemDev* pd = dpd;
     const int2* nomap = nullptr;
-    if (compressor == FEDNL_TOPK)
-        hipLaunchKernelGGL((k_select_heap<false, false>), dim3(nclients), dim3(256), lds, st, pd, nomap);
+    const dim3 g(nclients), t(256);
+    if (compressor == FEDNL_TOPK && !wide)
+        hipLaunchKernelGGL((k_select_heap<false, false, false>), g, t, lds, st, pd, nomap);
+    else if (compressor == FEDNL_TOPK)
+        hipLaunchKernelGGL((k_select_heap<false, false, true>), g, t, lds, st, pd, nomap);
+    else if (compressor == FEDNL_TOPLEK && !wide)
+        hipLaunchKernelGGL((k_select_heap<true, false, false>), g, t, lds, st, pd, nomap);
     else if (compressor == FEDNL_TOPLEK)
-        hipLaunchKernelGGL((k_select_heap<true, false>), dim3(nclients), dim3(256), lds, st, pd, nomap);
+        hipLaunchKernelGGL((k_select_heap<true, false, true>), g, t, lds, st, pd, nomap);
+    else if (compressor == FEDNL_RANDK && !wide)
+        hipLaunchKernelGGL((k_select_randk<false, false>), g, t, lds, st, pd, nomap);
     else if (compressor == FEDNL_RANDK)
-        hipLaunchKernelGGL(k_select_randk<false>, dim3(nclients), dim3(256), lds, st, pd, nomap);
+        hipLaunchKernelGGL((k_select_randk<false, true>), g, t, lds, st, pd, nomap);
     else if (compressor == FEDNL_RANDSEQK || compressor == FEDNL_IDENTICAL)
-        hipLaunchKernelGGL(k_select_window<false>, dim3(nclients), dim3(256), 0, st, pd, nomap);
+        hipLaunchKernelGGL((k_select_window<false, false>), g, t, 0, st, pd, nomap);
     else {
         hipFree(dpd);
         hipFree(bm);
         cleanup();
         return FEDNL_E_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(k_emit, dim3(nclients), dim3(256), sizeof(uint32_t) * bm_words, st, pd, nomap);
+    if (wide)
+        hipLaunchKernelGGL(k_emit<true>, g, t, 0, st, pd, nomap);
+    else
+        hipLaunchKernelGGL(k_emit<false>, g, t, lds_bm, st, pd, nomap);
     if (compressor == FEDNL_TOPLEK) {
         // advance the streams as the master kernel does
         std::vector<int> hh(nclients);
