@@ -54,7 +54,7 @@ def algorithmic_work(name, n, m, d, k, S):
     if name == "k_hessian_f64":
         # upper triangle incl. diagonal of A_c^T D_c A_c: m FMAs per entry
         return "mfma", float(n) * m * d * (d + 1)
-    if name == "k_newton":
+    if name in ("k_newton", "k_newton_mc"):
         # Cholesky d^3/3, two triangular solves 2 d^2
         return "mfma", d ** 3 / 3.0 + 2.0 * d * d
     if name == "k_select":

@@ -41,6 +41,7 @@
 #include "fednl_hip.h"
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 namespace fednl {
 
@@ -200,7 +201,8 @@ __device__ inline double bcast_lane(double v, int src)
     return __longlong_as_double(((unsigned long long)hi << 32) | lo);
 }
 
-// quad_perm DPP move of a double (both halves with the same pattern)
+// DPP move of a double (both halves with the same control: quad_perm
+// patterns 0x00-0xFF, row_shl:n = 0x100 + n, ...)
 template <int kPerm>
 __device__ inline double dpp_quad(double v)
 {
@@ -2791,45 +2793,68 @@ __device__ __noinline__ void newton_fallback_solve(const ProblemDev& PD, double*
 // pairs.
 constexpr int kNewtonPS = kNB + 2;
 
-// 1/sqrt(a) and sqrt(a) for a > 0: v_rsq_f64 and two Newton steps, then one
-// correction of the square root (within an ulp or two of the rounded values)
-__device__ inline void sqrt_and_inv(double a, double& root, double& inv)
+// 1/sqrt(a) for a > 0: v_rsq_f64 and two Newton steps
+__device__ __forceinline__ double rsq_nr2(double a)
 {
     double y = __builtin_amdgcn_rsq(a);
     double h = a * y;
     y = fma(0.5 * y, fma(-h, y, 1.0), y);
     h = a * y;
     y = fma(0.5 * y, fma(-h, y, 1.0), y);
-    double r = a * y;
-    r = fma(fma(-r, r, a), 0.5 * y, r);
-    root = r;
-    inv = y;
+    return y;
 }
 
-// One column step of the diagonal-block factorisation (rows in lanes): the
-// pivot of column cc, L[r][cc] for the rows below, the forward-substitution
-// entry, and the rank-1 update of columns cc+1 .. kEnd-1.
-template <int kEnd>
-__device__ __forceinline__ void diag_column(double (&row)[kNB], double& yr, double& myinv, bool& fail, int cc,
-                                            int lane)
+// sqrt(a) from y = rsq_nr2(a): one correction of a y (within an ulp or two
+// of the rounded root)
+__device__ __forceinline__ double root_from_inv(double a, double y)
 {
-    const double piv = bcast_lane(row[cc], cc);
-    fail = fail || (piv < 0.0);  // finish the (now meaningless) block, report after
-    // piv < 0 is reported after the block; piv == 0 gives non-finite
-    // entries as a division by a zero pivot would
-    double l, inv;
-    sqrt_and_inv(piv, l, inv);
-    const double ycc = bcast_lane(yr, cc) * inv;
-    const double scaled = inv * row[cc];
-    const bool below = lane > cc;
-    const bool me = lane == cc;
-    row[cc] = me ? l : (below ? scaled : row[cc]);
-    myinv = me ? inv : myinv;
-    yr = me ? ycc : (below ? fma(-scaled, ycc, yr) : yr);
-    // L[c2][cc] lives in lane c2: broadcast by v_readlane
+    double r = a * y;
+    return fma(fma(-r, r, a), 0.5 * y, r);
+}
+
+// One column step of the diagonal-block factorisation (rows in lanes,
+// column kCc, columns up to kEnd - 1): every lane takes the same path. The
+// scaled column goes to LDS (cb, double-buffered, 64 slots each) for the
+// rank-1 update of columns kCc+2 ..; the entry the next pivot needs,
+// L[kCc+1][kCc], goes by v_readlane, so the pivot chain never waits on LDS.
+// Lanes at or above the pivot compute entries of the block's upper part,
+// which nothing reads. The pivot, its reciprocal root and the column's
+// forward-substitution entry (all uniform) go to meta[4 kCc ..].
+template <int kCc, int kEnd>
+__device__ __forceinline__ void diag_column(double (&row)[kNB], double& yr, bool& fail, double& piv, int lane,
+                                            double* __restrict__ cb, double* __restrict__ meta)
+{
+    // a negative pivot fails the block (reported after it); a zero pivot
+    // gives non-finite entries, as a division by it would
+    fail = fail || (piv < 0.0);
+    meta[4 * kCc + 0] = piv;  // every lane writes the same value
+    const double inv = rsq_nr2(piv);
+    const double scaled = inv * row[kCc];
+    double* b = cb + 64 * (kCc & 1);
+    b[lane] = scaled;
+    if (kCc + 1 < kEnd) {
+        row[kCc + 1] = fma(-scaled, bcast_lane(scaled, kCc + 1), row[kCc + 1]);
+        piv = bcast_lane(row[kCc + 1], kCc + 1);
+    }
+    row[kCc] = scaled;
+    const double ycc = bcast_lane(yr, kCc) * inv;
+    yr = fma(-scaled, ycc, yr);
+    meta[4 * kCc + 1] = inv;
+    meta[4 * kCc + 2] = ycc;
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int c2 = cc + 1; c2 < kEnd; ++c2)
-        row[c2] = fma(-row[cc], bcast_lane(row[cc], c2), row[c2]);
+    for (int c2 = kCc + 2; c2 < kEnd; ++c2)
+        row[c2] = fma(-scaled, b[c2], row[c2]);
+}
+
+template <int kCc, int kEnd>
+__device__ __forceinline__ void diag_columns(double (&row)[kNB], double& yr, bool& fail, double& piv, int lane,
+                                             double* __restrict__ cb, double* __restrict__ meta)
+{
+    if constexpr (kCc < kEnd) {
+        diag_column<kCc, kEnd>(row, yr, fail, piv, lane, cb, meta);
+        diag_columns<kCc + 1, kEnd>(row, yr, fail, piv, lane, cb, meta);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2982,9 +3007,11 @@ __device__ inline void mc_update_tiles(const gmem<double>* __restrict__ W, int d
 
 // Col rows 16 rt .. (local) for rt = rt0, rt0 + step, .. < rt1 become
 // Col L11^-T (X = L11^-T, upper, row-major in Xt with stride kNB + 1) on FP64
-// MFMA: the triangular solve of the rows below a diagonal block.
+// MFMA: the triangular solve of the rows below a diagonal block. With Wc
+// (the block column's first column at the segment's first row, in W) each
+// solved tile also goes to W straight from the accumulators.
 __device__ inline void mc_solve_tiles(double* __restrict__ Col, const double (*Xt)[kNB + 1], int rt0, int step,
-                                      int rt1, int lane)
+                                      int rt1, int lane, gmem<double>* __restrict__ Wc = nullptr, int dp = 0)
 {
     if (rt0 >= rt1)
         return;
@@ -3015,6 +3042,13 @@ __device__ inline void mc_solve_tiles(double* __restrict__ Col, const double (*X
 #pragma unroll
             for (int r = 0; r < 4; ++r)
                 rp[16 * ct + 4 * r] = acc[ct][r];
+        if (Wc) {
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    st_dev(Wc + (size_t)(16 * ct + 4 * r + kq) * dp + 16 * rt + l16, acc[ct][r]);
+        }
     }
 }
 
@@ -3024,11 +3058,13 @@ __device__ inline void mc_solve_tiles(double* __restrict__ Col, const double (*X
 // and the identity rows into L11^-T (the panel rule X L11^T = I); the
 // right-hand side ry is carried along (forward substitution: ry becomes y).
 // Two 16-column halves; between them rows 16..63 take -R L21^T on FP64 MFMA
-// (R: their first 16 columns), one 16-row tile at a time, the rows of L21
-// themselves last. On return Dg holds L11 (lower), Xt L11^-T (upper), invd
-// and invall 1 / L11[i][i]. A negative pivot sets *s_fail.
-__device__ inline void mc_diag_factor(double (*Dg)[kNB + 1], double (*Xt)[kNB + 1], double (*Sq)[17],
-                                      double* invd, double* ry, int lane, int* s_fail)
+// (R: their first 16 columns, L21: the rows 16..31 among them). On return Dg
+// holds L11 (lower), Xt L11^-T (upper), invd 1 / L11[i][i], ry y. A negative
+// pivot sets *s_fail. cb (128 doubles) and meta (128 doubles) are LDS
+// scratch of the calling wave.
+__device__ __forceinline__ void mc_diag_factor(double (*Dg)[kNB + 1], double (*Xt)[kNB + 1], double (*Sq)[17],
+                                               double* invd, double* ry, double* cb, double* meta, int lane,
+                                               int* s_fail)
 {
     constexpr int kHalf = kNB / 2;
     double row[kNB];
@@ -3039,11 +3075,9 @@ __device__ inline void mc_diag_factor(double (*Dg)[kNB + 1], double (*Xt)[kNB + 
     for (int c = 0; c < kNB; ++c)
         row[c] = src[c];
     double yr = blk ? ry[lr] : 0.0;
-    double myinv = 1.0;
     bool fail = false;
-#pragma unroll
-    for (int cc = 0; cc < kHalf; ++cc)
-        diag_column<kHalf>(row, yr, myinv, fail, cc, lane);
+    double piv = bcast_lane(row[0], 0);
+    diag_columns<0, kHalf>(row, yr, fail, piv, lane, cb, meta);
     // rows 16..63 (lanes 16..63), first half of their columns, to LDS
     if (lane >= kHalf) {
 #pragma unroll
@@ -3054,28 +3088,29 @@ __device__ inline void mc_diag_factor(double (*Dg)[kNB + 1], double (*Xt)[kNB + 
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     {
+        // U[i][c] = sum_k S[i][k] L21[c][k] (L21 = S rows 0..15), three
+        // independent 16-row tiles, written back once all are formed
         const int l16 = lane & 15, kq = lane >> 4;
-        // U[i][c] = sum_k S[i][k] L21[c][k] (L21 = S rows 0..15), tile by
-        // tile over the rows i, each written over its own rows; tile 0 (the
-        // rows of L21) last
-#pragma unroll 1
-        for (int tt = 0; tt < 3; ++tt) {
-            const int t = (tt + 1) % 3;
-            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        dbl4 acc[3];
 #pragma unroll
-            for (int k0 = 0; k0 < kHalf; k0 += 4) {
-                const double av = Sq[l16][k0 + kq];           // A[m = c][k]
-                const double bv = Sq[16 * t + l16][k0 + kq];  // B[k][n = row]
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-            }
-            __builtin_amdgcn_wave_barrier();
+        for (int t = 0; t < 3; ++t) {
+            acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k0 = 0; k0 < kHalf; k0 += 4)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Sq[l16][k0 + kq], Sq[16 * t + l16][k0 + kq], acc[t],
+                                                              0, 0, 0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                Sq[16 * t + l16][4 * r + kq] = acc[r];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
+                Sq[16 * t + l16][4 * r + kq] = acc[t][r];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     {
         const int i = lane >= kHalf ? lane - kHalf : 0;
@@ -3083,19 +3118,90 @@ __device__ inline void mc_diag_factor(double (*Dg)[kNB + 1], double (*Xt)[kNB + 
         for (int c = 0; c < kHalf; ++c)
             row[kHalf + c] = (lane >= kHalf) ? row[kHalf + c] - Sq[i][c] : row[kHalf + c];
     }
-#pragma unroll
-    for (int cc = kHalf; cc < kNB; ++cc)
-        diag_column<kNB>(row, yr, myinv, fail, cc, lane);
+    piv = bcast_lane(row[kHalf], kHalf);
+    diag_columns<kHalf, kNB>(row, yr, fail, piv, lane, cb, meta);
     if (__ballot(fail)) {
         if (lane == 0)
             *s_fail = 1;
     }
+    // the per-column records into registers first (meta may alias Dg)
+    double m_piv = 0.0, m_inv = 0.0, m_y = 0.0;
+    if (blk) {
+        m_piv = meta[4 * lr + 0];
+        m_inv = meta[4 * lr + 1];
+        m_y = meta[4 * lr + 2];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
     for (int c = 0; c < kNB; ++c)
         src[c] = row[c];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (blk) {
-        invd[lr] = myinv;
-        ry[lr] = yr;
+        // the diagonal from each column's pivot and reciprocal root
+        Dg[lr][lr] = root_from_inv(m_piv, m_inv);
+        invd[lr] = m_inv;
+        ry[lr] = m_y;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// (J, 0)'s diagonal block (local row tiles 0, 1) and y_J take panel K's
+// block-J rows in one load round trip. Those rows are the MFMA operands of
+// P_K[J] P_K[J]^T, and the same registers give y_J -= P_K[J] y_K: lane
+// (l16, kq) holds P_K[J][16 ct + l16][4 q + kq], so each lane sums its 8
+// products and the four kq lanes of a row are added (kq 0+1, 2+3, then the
+// pairs).
+__device__ inline void mc_update_diag(const gmem<double>* __restrict__ W, const gmem<double>* __restrict__ ny,
+                                      int dp, int K, int brow, double* __restrict__ Col, double* __restrict__ ry,
+                                      int lane)
+{
+    const int l16 = lane & 15, kq = lane >> 4;
+    const gmem<double>* __restrict__ PK = W + (size_t)(kNB * K + kq) * dp + brow;
+    double av[2][8], yk[8];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            av[ct][q] = ld_dev(PK + (size_t)(4 * q) * dp + 16 * ct + l16);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        yk[q] = ld_dev(ny + kNB * K + 4 * q + kq);
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            if (rt == 0 && ct == 1)
+                continue;  // above the diagonal
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ct][q], av[rt][q], acc, 0, 0, 0);
+            double* cp = Col + (16 * rt + l16) * kNewtonPS + 16 * ct + kq;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                cp[4 * r] = cp[4 * r] - acc[r];
+        }
+    double py[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            a = fma(av[ct][q], yk[q], a);
+        a += __shfl_xor(a, 16);
+        a += __shfl_xor(a, 32);
+        py[ct] = a;
+    }
+    if (kq == 0) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+            ry[16 * ct + l16] = ry[16 * ct + l16] - py[ct];
     }
 }
 
@@ -3127,7 +3233,8 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
     __shared__ double Dq[48][17];         // the diagonal factor's quarter products (rows 16..63)
     __shared__ double invd[kNB], ry[kNB];
     __shared__ int s_epoch, s_fail, s_last;
-    __shared__ int s_l11, s_rows, s_t32, s_upd;
+    __shared__ double s_sink;
+    __shared__ int s_l11, s_rows, s_pub, s_upd;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63, nwaves = nthr >> 6;
     if (tid == 0) {
@@ -3135,7 +3242,7 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
         s_fail = 0;
         s_l11 = 0;
         s_rows = 0;
-        s_t32 = 0;
+        s_pub = 0;
         s_upd = 0;
     }
     for (int e = tid; e < kNB * kNB; e += nthr)
@@ -3194,30 +3301,24 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
         __syncthreads();
     }
 
+    // W at this segment's first row of block column J (solved rows go there)
+    gmem<double>* __restrict__ Wcol = W + (size_t)(kNB * J) * dp + r0;
     if (s == 0) {
         const int K = J - 1;
+        // published with the diagonal factor (dflag): L11, y_J, L11^-T and
+        // the local row tiles 2 .. tpub - 1 (row blocks J+1 and J+2), so
+        // that (J+1, 0) needs nothing else for its diagonal block and its
+        // next row block
+        const int tpub = min(ntile, 6);
+        // L11^-T goes with the flag when lower segments wait for it
+        const bool x_first = mc_nseg(dp, J) > 1;
+        const int npub = 1 + (tpub > 3 ? 1 : 0) + (tpub > 4 ? 1 : 0) + (tpub > 5 ? 1 : 0) + (x_first ? 1 : 0);
         PROBE_STAMP_AT(4000 + 16 * J + 1, 0);
         if (wave == 0) {
-            // the diagonal block and y_J take panel K's block-J rows, then
-            // the factorisation (32 lanes, rows in lanes)
             if (K >= 0) {
                 mc_wait_ge(dflag + K, epoch, err);
                 PROBE_STAMP_AT(4000 + 16 * J + 2, 0);
-                mc_update_tiles(W, dp, K, r0, kNB * J, Col, 0, 1, 2, true, lane);
-                if (lane < kNB) {
-                    const gmem<double>* pk = W + (size_t)(kNB * K) * dp + kNB * J + lane;
-                    double pv[kNB], yk[kNB];
-#pragma unroll
-                    for (int k = 0; k < kNB; ++k) {
-                        pv[k] = ld_dev(pk + (size_t)k * dp);
-                        yk[k] = ld_dev(ny + kNB * K + k);
-                    }
-                    double acc = ry[lane];
-#pragma unroll
-                    for (int k = 0; k < kNB; ++k)
-                        acc = fma(-pv[k], yk[k], acc);
-                    ry[lane] = acc;
-                }
+                mc_update_diag(W, ny, dp, K, kNB * J, Col, ry, lane);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -3231,107 +3332,124 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            mc_diag_factor(Dg, Xt, Dq, invd, ry, lane, &s_fail);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // LDS scratch of the factor: the column broadcast in Dq (free
+            // until the halves' MFMA step, and again after it), the per-column
+            // records in Dg (read into registers before Dg is written)
+            mc_diag_factor(Dg, Xt, Dq, invd, ry, &Dq[0][0], &Dg[0][0], lane, &s_fail);
             PROBE_STAMP_AT(4000 + 16 * J + 4, 0);
             if (lane == 0)
                 mc_lds_signal(&s_l11, 1);
-            // block J+1's first row tile (wave 2 takes the second), then L11,
-            // y_J and the reciprocal pivots out
-            if (ntile > 2) {
-                mc_lds_wait(&s_rows, 1);
-                mc_solve_tiles(Col, Xt, 2, 1, 3, lane);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            }
-            for (int c = 0; c < kNB; ++c)
-                if (lane < kNB && lane >= c)
-                    st_dev(W + (size_t)(kNB * J + c) * dp + r0 + lane, Dg[lane][c]);
-            if (lane < kNB) {
+            // y_J out (L11 itself is read by nobody: the rows below use
+            // L11^-T, which wave 4 stores, and the next round rebuilds W)
+            if (lane < kNB)
                 st_dev(ny + kNB * J + lane, ry[lane]);
-                st_dev(ninv + kNB * J + lane, invd[lane]);
+            PROBE_STAMP_AT(4000 + 16 * J + 10, 0);
+            // the first half of block J+1's rows
+            if (tpub > 2) {
+                mc_lds_wait(&s_rows, 1);
+                PROBE_STAMP_AT(4000 + 16 * J + 11, 0);
+                mc_solve_tiles(Col, Xt, 2, 1, 3, lane, Wcol, dp);
+                PROBE_STAMP_AT(4000 + 16 * J + 12, 0);
             }
-            mc_lds_wait(&s_t32, 1);
-            PROBE_STAMP_AT(4000 + 16 * J + 5, 0);
-            // block J+1 (local rows 32..63) and L11^-T, then the flag
-            if (ntile > 2) {
-                for (int c = 0; c < kNB; ++c)
-                    if (lane < kNB)
-                        st_dev(W + (size_t)(kNB * J + c) * dp + r0 + kNB + lane, Col[(kNB + lane) * kNewtonPS + c]);
-            }
-            for (int e = lane; e < kNB * kNB; e += 64)
-                st_dev(nxt + (size_t)kNB * kNB * J + e, Xt[e >> 5][e & (kNB - 1)]);
             if (s_fail && lane == 0)
                 __hip_atomic_store(sync + 2, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            PROBE_STAMP_AT(4000 + 16 * J + 5, 0);
+            // every publishing wave's stores have completed: the flag
             mc_drain();
-            if (lane == 0)
+            PROBE_STAMP_AT(4000 + 16 * J + 13, 0);
+            if (lane == 0) {
+                atomicAdd(&s_pub, 1);
+                mc_lds_wait(&s_pub, npub);
                 __hip_atomic_store(dflag + J, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             PROBE_STAMP_AT(4000 + 16 * J + 6, 0);
         } else {
-            // waves 1..7. Wave 1 first brings block J+1's rows (local tiles
-            // 2, 3) up to date; wave 2 solves tile 3 once L11^-T is there;
-            // waves 1, 3..7 update the rest, then all solve it
-            if (K >= 0) {
-                mc_wait_ge(cnt + K, epoch * mc_nseg(dp, K), err);
-                PROBE_STAMP_AT(4000 + 16 * J + 8, 64);
-                if (wave == 1)
-                    mc_update_tiles(W, dp, K, r0, kNB * J, Col, 2, 1, min(4, ntile), false, lane);
+            // waves 1..7. Wave 1 brings block J+1's rows (tiles 2, 3) up to
+            // date with panel K (published with K's diagonal factor); wave 2
+            // solves tile 3, waves 3 and 5 update and solve tiles 4 and 5
+            // (block J+2: panel K's rows there come with K's whole column);
+            // waves 1, 2, 3, 5, 6, 7 update tiles 6.. (wave 4 shares wave 0's
+            // SIMD and issues no FP64 MFMA while the factor runs)
+            if (wave == 1) {
+                if (K >= 0 && tpub > 2) {
+                    mc_wait_ge(dflag + K, epoch, err);
+                    PROBE_STAMP_AT(4000 + 16 * J + 8, 64);
+                    mc_update_tiles(W, dp, K, r0, kNB * J, Col, 2, 1, 4, false, lane);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0)
+                    mc_lds_signal(&s_rows, 1);
             }
-            if (wave == 1 && lane == 0)
-                mc_lds_signal(&s_rows, 1);
-            // bulk update of tiles 4.. by waves 1, 4, 5, 6, 7
-            const int bw = (wave == 1) ? 0 : wave - 2;  // 0..5 for waves 1, 3..7
-            if (wave != 2) {
+            const bool mine45 = (wave == 3 && tpub > 4) || (wave == 5 && tpub > 5);
+            const int t45 = (wave == 3) ? 4 : 5;
+            const bool bulk = ntile > 6;
+            if ((mine45 || (bulk && wave != 4)) && K >= 0)
+                mc_wait_ge(cnt + K, epoch * mc_nseg(dp, K), err);
+            if (mine45) {
                 if (K >= 0)
-                    mc_update_tiles(W, dp, K, r0, kNB * J, Col, 4 + bw, 6, ntile, false, lane);
+                    mc_update_tiles(W, dp, K, r0, kNB * J, Col, t45, 1, t45 + 1, false, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                mc_lds_wait(&s_l11, 1);
+                mc_solve_tiles(Col, Xt, t45, 1, t45 + 1, lane, Wcol, dp);
+                mc_drain();
+                if (lane == 0)
+                    atomicAdd(&s_pub, 1);
+            }
+            if (wave == 2 && tpub > 3) {
+                mc_lds_wait(&s_l11, 1);
+                mc_lds_wait(&s_rows, 1);
+                mc_solve_tiles(Col, Xt, 3, 1, 4, lane, Wcol, dp);
+                mc_drain();
+                if (lane == 0)
+                    atomicAdd(&s_pub, 1);
+            }
+            if (wave == 4) {
+                // L11^-T by columns (nxt[J][c][q]): lane q + 32 h stores
+                // columns 16 h .. 16 h + 15
+                mc_lds_wait(&s_l11, 1);
+                const int q = lane & (kNB - 1), h = lane >> 5;
+                double v[kNB / 2];
+#pragma unroll
+                for (int c = 0; c < kNB / 2; ++c)
+                    v[c] = Xt[q][16 * h + c];
+#pragma unroll
+                for (int c = 0; c < kNB / 2; ++c)
+                    st_dev(nxt + (size_t)kNB * kNB * J + kNB * (16 * h + c) + q, v[c]);
+                if (x_first) {
+                    mc_drain();
+                    if (lane == 0)
+                        atomicAdd(&s_pub, 1);
+                }
+            }
+            if (wave != 4) {
+                const int bw = (wave <= 3) ? wave - 1 : wave - 2;  // 0..5 for waves 1, 2, 3, 5, 6, 7
+                if (K >= 0 && bulk)
+                    mc_update_tiles(W, dp, K, r0, kNB * J, Col, 6 + bw, 6, ntile, false, lane);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __builtin_amdgcn_wave_barrier();
                 if (lane == 0)
                     atomicAdd(&s_upd, 1);
             }
-            if (wave == 2) {
-                mc_lds_wait(&s_l11, 1);
-                mc_lds_wait(&s_rows, 1);
-                if (ntile > 2)
-                    mc_solve_tiles(Col, Xt, 3, 1, 4, lane);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0)
-                    mc_lds_signal(&s_t32, 1);
-            }
-            // the bulk solve takes tiles in another order than the bulk
-            // update: every updating wave (1, 3..7) has finished first
-            mc_lds_wait(&s_l11, 1);
-            mc_lds_wait(&s_upd, 6);
-            mc_solve_tiles(Col, Xt, 4 + (wave - 1), nwaves - 1, ntile, lane);
             PROBE_STAMP_AT(4000 + 16 * J + 9, 64);
         }
-        __syncthreads();
-        // rows 64.. to W
-        for (int c = 0; c < kNB; ++c) {
-            gmem<double>* wc = W + (size_t)(kNB * J + c) * dp + r0;
-            for (int i = 2 * kNB + tid; i < nrows; i += nthr)
-                st_dev(wc + i, Col[i * kNewtonPS + c]);
+        // tiles 6.. by every wave once L11^-T and their updates are there
+        if (ntile > 6) {
+            mc_lds_wait(&s_l11, 1);
+            mc_lds_wait(&s_upd, 6);
+            mc_solve_tiles(Col, Xt, 6 + wave, nwaves, ntile, lane, Wcol, dp);
         }
     } else {
         // a lower segment: the triangular solve once L11^-T of block column J
-        // is published
+        // is published (stored by columns: nxt[J][c][q])
         if (tid < 64)
             mc_wait_ge(dflag + J, epoch, err);
         __syncthreads();
         for (int e = tid; e < kNB * kNB; e += nthr)
-            Xt[e >> 5][e & (kNB - 1)] = ld_dev(nxt + (size_t)kNB * kNB * J + e);
+            Xt[e & (kNB - 1)][e >> 5] = ld_dev(nxt + (size_t)kNB * kNB * J + e);
         __syncthreads();
-        mc_solve_tiles(Col, Xt, wave, nwaves, ntile, lane);
-        __syncthreads();
-        for (int c = 0; c < kNB; ++c) {
-            gmem<double>* wc = W + (size_t)(kNB * J + c) * dp + r0;
-            for (int i = tid; i < nrows; i += nthr)
-                st_dev(wc + i, Col[i * kNewtonPS + c]);
-        }
+        mc_solve_tiles(Col, Xt, wave, nwaves, ntile, lane, Wcol, dp);
     }
     // this segment of block column J is done (every wave's stores complete);
     // the ticket finds the last one
@@ -3388,39 +3506,93 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
         return;
     }
     PROBE_STAMP_AT(4000 + 16 * 64 + 0, 0);
-    // L^T dx = y from the bottom block: dx_J = L11^-T r_J (wave 0, lane q:
-    // row q of L11^-T), then every row rr above the block folds
-    // r_rr -= sum_c L[32J + c][rr] dx_J[c] (W[rr dp + 32J + c]). For
-    // dp <= 512 every block's L11^-T is staged in LDS first (row stride 33).
+    // L^T dx = y by blocks from the bottom: dx_J = L11^-T r_J (wave 0, lane
+    // q: row q of L11^-T), then every row rr above the block folds
+    // r_rr -= sum_c L[32J + c][rr] dx_J[c], with L[32J + c][rr] =
+    // W[rr dp + 32J + c]. Eight lanes share a row: lane u takes c = 4u ..
+    // 4u + 3 (two 16-byte loads, so one load instruction covers eight rows of
+    // 128 contiguous bytes) and the four products of the eight lanes are added
+    // by a 3-step butterfly. Thread tid takes rows tid / 8 + 64 p, passes p in
+    // chunks of kBsChunk; the next block's first chunk is in flight while the
+    // current block runs. For dp <= 512 every block's L11^-T is staged in LDS
+    // (row stride 33), and the L rows are first touched once per 128-byte
+    // line, all loads in flight, so that the block loop reads them from this
+    // XCD's L2.
     const bool staged = dp <= kSegRows;
-    double* Xs = lds;                                           // nb x 32 x 33 when staged
+    constexpr int kBsChunk = 8;
+    const int rs = tid >> 3, u = tid & 7;
+    const int rpp = nthr >> 3;  // rows per pass
+    const gmem<dbl2>* __restrict__ W2 = (const gmem<dbl2>*)G(PD.W);
+    double* Xs = lds;  // nb x 32 x 33 when staged
     double* yv = lds + (staged ? (size_t)nb * kNB * (kNB + 1) : 0);
+    auto load_chunk = [&](int jb, int p0, dbl2 (&v)[kBsChunk][2]) {
+#pragma unroll
+        for (int p = 0; p < kBsChunk; ++p) {
+            const int rr = rs + rpp * (p0 + p);
+            if (rpp * (p0 + p) >= jb)
+                break;  // no row of this pass (uniform)
+            if (rr < jb) {
+                const gmem<dbl2>* lp = W2 + ((size_t)rr * dp + jb + 4 * u) / 2;
+                v[p][0] = lp[0];
+                v[p][1] = lp[1];
+            } else {
+                v[p][0] = dbl2{0.0, 0.0};
+                v[p][1] = dbl2{0.0, 0.0};
+            }
+        }
+    };
     if (staged) {
-        for (int e = tid; e < nb * kNB * kNB; e += nthr) {
-            const int blkx = e >> 10, q = (e >> 5) & (kNB - 1), c = e & (kNB - 1);
-            Xs[(size_t)blkx * kNB * (kNB + 1) + q * (kNB + 1) + c] = nxt[e];
+        double warm = 0.0;
+        if (tid < d) {
+            const gmem<double>* __restrict__ wc = G(PD.W) + (size_t)tid * dp;
+            const int rw = kNB * (tid / kNB + 1);
+            double t[kSegRows / 16];
+#pragma unroll
+            for (int i = 0; i < kSegRows / 16; ++i) {
+                const int r = rw + 16 * i;
+                t[i] = (r < dp) ? wc[r] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < kSegRows / 16; ++i)
+                warm += t[i];
+        }
+        if (d < 0)  // never: keeps the loads
+            s_sink = warm;
+        // L11^-T pairs (q, q + 1) of column c of block j: nxt[j][c][q]
+        const gmem<dbl2>* __restrict__ x2 = (const gmem<dbl2>*)nxt;
+        dbl2 xv[kSegRows / kNB];
+#pragma unroll
+        for (int k = 0; k < kSegRows / kNB; ++k) {
+            const int e2 = tid + nthr * k;
+            xv[k] = (e2 < nb * kNB * kNB / 2) ? x2[e2] : dbl2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int k = 0; k < kSegRows / kNB; ++k) {
+            const int e2 = tid + nthr * k;
+            if (e2 < nb * kNB * kNB / 2) {
+                const int blkx = e2 >> 9, c = (e2 >> 4) & (kNB - 1), q = 2 * (e2 & 15);
+                double* xb = Xs + (size_t)blkx * kNB * (kNB + 1);
+                xb[q * (kNB + 1) + c] = xv[k].x;
+                xb[(q + 1) * (kNB + 1) + c] = xv[k].y;
+            }
         }
     }
     for (int i = tid; i < dp; i += nthr) yv[i] = ny[i];
-    // row tid's L segment of the bottom block, then each next one a block ahead
-    double lnext[kNB];
-    if (staged && tid < kNB * (nb - 1)) {
-        const double* lp = PD.W + (size_t)tid * dp + kNB * (nb - 1);
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) lnext[c] = lp[c];
-    }
+    dbl2 vn[kBsChunk][2];
+    load_chunk(kNB * (nb - 1), 0, vn);
     __syncthreads();
+    PROBE_STAMP_AT(4000 + 16 * 64 + 2, 0);
     for (int Jb = nb - 1; Jb >= 0; --Jb) {
         const int jb = kNB * Jb;
-        double lrow[kNB];
-        const bool own = staged && tid < jb;
+        PROBE_STAMP_AT(4000 + 16 * 65 + Jb, 0);
+        dbl2 v[kBsChunk][2];
 #pragma unroll
-        for (int c = 0; c < kNB; ++c) lrow[c] = lnext[c];
-        if (staged && Jb > 0 && tid < jb - kNB) {
-            const double* lp = PD.W + (size_t)tid * dp + jb - kNB;
-#pragma unroll
-            for (int c = 0; c < kNB; ++c) lnext[c] = lp[c];
+        for (int p = 0; p < kBsChunk; ++p) {
+            v[p][0] = vn[p][0];
+            v[p][1] = vn[p][1];
         }
+        if (Jb > 0)
+            load_chunk(jb - kNB, 0, vn);
         if (wave == 0 && lane < kNB) {
             double acc = 0.0;
             if (staged) {
@@ -3429,29 +3601,37 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
                 for (int c = 0; c < kNB; ++c)
                     acc = fma(xr[c], yv[jb + c], acc);
             } else {
-                const double* xr = PD.nxt + (size_t)kNB * kNB * Jb + kNB * lane;
+                const gmem<double>* xr = nxt + (size_t)kNB * kNB * Jb + lane;  // X[lane][c] at xr[32 c]
 #pragma unroll
                 for (int c = 0; c < kNB; ++c)
-                    acc = fma(xr[c], yv[jb + c], acc);
+                    acc = fma(xr[kNB * c], yv[jb + c], acc);
             }
             __builtin_amdgcn_wave_barrier();  // every lane has read r_J before any writes
             yv[jb + lane] = acc;
         }
         __syncthreads();
-        if (own) {
-            double sacc = yv[tid];
+        const double dx0 = yv[jb + 4 * u], dx1 = yv[jb + 4 * u + 1];
+        const double dx2 = yv[jb + 4 * u + 2], dx3 = yv[jb + 4 * u + 3];
+        const int np = (jb + rpp - 1) / rpp;
+        for (int p0 = 0; p0 < np; p0 += kBsChunk) {
+            if (p0 > 0)
+                load_chunk(jb, p0, v);
 #pragma unroll
-            for (int c = 0; c < kNB; ++c)
-                sacc = fma(-lrow[c], yv[jb + c], sacc);
-            yv[tid] = sacc;
-        } else if (!staged) {
-            for (int rr = tid; rr < jb; rr += nthr) {
-                const double* lp = PD.W + (size_t)rr * dp + jb;
-                double sacc = yv[rr];
-#pragma unroll 8
-                for (int c = 0; c < kNB; ++c)
-                    sacc = fma(-lp[c], yv[jb + c], sacc);
-                yv[rr] = sacc;
+            for (int p = 0; p < kBsChunk; ++p) {
+                if (p0 + p >= np)
+                    break;
+                const int rr = rs + rpp * (p0 + p);
+                double t = v[p][0].x * dx0;
+                t = fma(v[p][0].y, dx1, t);
+                t = fma(v[p][1].x, dx2, t);
+                t = fma(v[p][1].y, dx3, t);
+                // the row's eight lanes: pairs, quads (DPP quad_perm), then
+                // lane 8k takes lane 8k + 4 (row_shl:4)
+                t += dpp_quad<0xB1>(t);
+                t += dpp_quad<0x4E>(t);
+                t += dpp_quad<0x104>(t);
+                if (u == 0 && rr < jb)
+                    yv[rr] = yv[rr] - t;
             }
         }
         __syncthreads();
