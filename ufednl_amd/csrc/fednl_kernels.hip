@@ -98,8 +98,8 @@ constexpr int kPadLds = 4;         // LDS row padding (doubles)
 constexpr int kMaxHeapK = 8192;    // LDS-resident heap limit
 constexpr int kMaxSelK = 32768;    // TopK / TopLEK / RandK: largest k (the tail sorts k ids in LDS)
 constexpr int kEmitLdsMax = 128 * 1024;  // largest selection bitmap kept in LDS (bytes)
-constexpr int kScanGroup = 8;      // TopK scan: chunks of 64 positions per staged group
-constexpr int kMaxScanRing = 16;   // TopK scan: most LDS slots of staged groups
+constexpr int kScanGroup = 16;     // TopK scan: chunks of 64 positions per staged group
+constexpr int kMaxScanRing = 8;    // TopK scan: most LDS slots of staged groups
 static_assert(kScanGroup % 2 == 0, "the scan tests chunk pairs");
 constexpr int kMaxClients = 1024;  // master scatter client table
 // per-round trace record: ||grad f||, f, ||x||, factorization flag,
@@ -1645,9 +1645,11 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
 #endif
     } else {
         // producers: wave w takes groups w-1, w+2, ...; the loads of its next
-        // groups are in flight while it waits for a free slot
+        // kDepth groups are in flight (in registers, rotated) while it waits
+        // for a free slot and writes the current one
         const gdouble* __restrict__ src = (const gdouble*)(dc + k);
         constexpr int kProd = 3;
+        constexpr int kDepth = 2;
         auto load_group = [&](int g, double (&va)[kScanGroup]) {
 #pragma unroll
             for (int j = 0; j < kScanGroup; ++j) {
@@ -1655,29 +1657,34 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
                 va[j] = src[e < rest ? e : rest - 1];  // clamped; replaced by -1 below
             }
         };
-        double va[kScanGroup], vb[kScanGroup];
-        int g = wave - 1;
-        if (g < ngroups)
-            load_group(g, va);
-        for (; g < ngroups; g += kProd) {
-            if (g + kProd < ngroups)
-                load_group(g + kProd, vb);
-            const int slot = g % R;
-            // the consumer is done with group g - R (the slot's last tenant)
-            scan_lds_wait_ge(&s_cons, g - R + 1);
-            double* dst = ring + (size_t)slot * (kScanGroup * 64);
+        double vq[kDepth][kScanGroup];
+        const int g0 = wave - 1;
 #pragma unroll
-            for (int j = 0; j < kScanGroup; ++j) {
-                const int e = g * kScanGroup * 64 + 64 * j + lane;
-                dst[64 * j + lane] = (e < rest) ? fabs(va[j]) : -1.0;
+        for (int q = 0; q < kDepth; ++q)
+            if (g0 + kProd * q < ngroups)
+                load_group(g0 + kProd * q, vq[q]);
+        for (int gb = g0; gb < ngroups; gb += kProd * kDepth) {
+#pragma unroll
+            for (int q = 0; q < kDepth; ++q) {
+                const int g = gb + kProd * q;
+                if (g >= ngroups)
+                    break;
+                const int slot = g % R;
+                // the consumer is done with group g - R (the slot's last tenant)
+                scan_lds_wait_ge(&s_cons, g - R + 1);
+                double* dst = ring + (size_t)slot * (kScanGroup * 64);
+#pragma unroll
+                for (int j = 0; j < kScanGroup; ++j) {
+                    const int e = g * kScanGroup * 64 + 64 * j + lane;
+                    dst[64 * j + lane] = (e < rest) ? fabs(vq[q][j]) : -1.0;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0)
+                    *(volatile int*)&s_ready[slot] = g + 1;
+                if (g + kProd * kDepth < ngroups)
+                    load_group(g + kProd * kDepth, vq[q]);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            if (lane == 0)
-                *(volatile int*)&s_ready[slot] = g + 1;
-#pragma unroll
-            for (int j = 0; j < kScanGroup; ++j)
-                va[j] = vb[j];
         }
     }
     if (wave == 0)
@@ -3740,7 +3747,7 @@ struct Engine {
     std::vector<hipEvent_t> ev_pool;
     double kms[KID_COUNT] = {0};
     int klaunch[KID_COUNT] = {0};
-    size_t lds_select = 0, lds_emit = 0, lds_newton = 0;
+    size_t lds_select = 0, lds_select_b = 0, lds_emit = 0, lds_newton = 0;  // lds_select_b: as a batch member
     bool wide = false;             // selection heap / bitmap in global memory
     unsigned char* sel_scr = nullptr;  // wide selection: per-client heap or sort keys
     size_t sel_scr_stride = 0;
@@ -3830,18 +3837,21 @@ static size_t select_heap_bytes(int k, size_t S)
     const int ids = select_bitmap_fits(S, k) ? k : P;
     return (size_t)8 * (k + 4) + (size_t)4 * round_up(ids + 4, 2);
 }
-static int scan_ring_slots(int k, size_t S)
+// LDS slots of the TopK scan's ring: up to 96 KB with the heap when every
+// selection of the launch has a CU of its own (big), else up to 64 KB so
+// that two selections share a CU
+static int scan_ring_slots(int k, size_t S, bool big)
 {
     const size_t slot = (size_t)8 * kScanGroup * 64;
     const size_t heap = select_heap_bytes(k, S);
-    const size_t target = 64 * 1024;
+    const size_t target = (big ? 96 : 64) * 1024;
     const int r = heap + 2 * slot >= target ? 2 : (int)((target - heap) / slot);
     return std::max(2, std::min(kMaxScanRing, r));
 }
-static size_t select_lds_bytes(int k, size_t S)
+static size_t select_lds_bytes(int k, size_t S, bool big)
 {
     const int P = pow2_at_least(k);
-    const size_t heap = select_heap_bytes(k, S) + (size_t)8 * kScanGroup * 64 * scan_ring_slots(k, S);
+    const size_t heap = select_heap_bytes(k, S) + (size_t)8 * kScanGroup * 64 * scan_ring_slots(k, S, big);
     const size_t rk = (size_t)8 * P + (size_t)8 * k;
     return std::max(heap, rk);
 }
@@ -3870,12 +3880,12 @@ static size_t select_scratch_stride(int comp, int k, size_t S)
     return (std::max(heap, rk) + 15) / 16 * 16;
 }
 // dynamic LDS of the selection proper (the fused emit adds its bitmap)
-static size_t select_lds(int comp, int k, size_t S, bool wide)
+static size_t select_lds(int comp, int k, size_t S, bool wide, bool big)
 {
     if (!select_keeps_k(comp, k, S))
         return 0;
     if (!wide)
-        return select_lds_bytes(k, S);
+        return select_lds_bytes(k, S, big);
     if (comp == FEDNL_RANDK)
         return 0;
     // the scan's ring, later the tail's sort of the ids
@@ -3884,7 +3894,12 @@ static size_t select_lds(int comp, int k, size_t S, bool wide)
 }
 static size_t emit_lds(size_t S, bool wide) { return wide ? 0 : sizeof(uint32_t) * ((S + 31) / 32); }
 
-static ProblemDev make_desc(const Engine* e)
+// the selection of a solo engine with at most kBigRingClients clients runs
+// one workgroup per CU with the big scan ring; batch members use the small one
+constexpr int kBigRingClients = 256;
+static bool big_ring(const Engine* e, bool batched) { return !batched && e->n <= kBigRingClients; }
+
+static ProblemDev make_desc(const Engine* e, bool batched = false)
 {
     ProblemDev P{};
     P.n = e->n; P.m = e->m; P.d = e->d; P.k = e->k; P.lda = e->lda; P.m_pad = e->m_pad; P.d_pad = e->d_pad;
@@ -3909,7 +3924,7 @@ static ProblemDev make_desc(const Engine* e)
     P.lek_popped = e->lek_popped; P.seq_start = e->seq_start; P.sel_bm = e->sel_bm;
     P.sel_cnt = e->sel_cnt; P.cnt = e->cnt; P.lek_hit = e->lek_hit; P.round_ctr = e->round_ctr;
     P.nsync = e->nsync; P.ny = e->ny; P.ninv = e->ninv; P.nxt = e->nxt;
-    P.sel_ring = e->wide ? kMaxScanRing : scan_ring_slots(std::min<int>(e->k, kMaxHeapK), e->S);
+    P.sel_ring = e->wide ? kMaxScanRing : scan_ring_slots(std::min<int>(e->k, kMaxHeapK), e->S, big_ring(e, batched));
     P.wide = e->wide ? 1 : 0;
     P.sel_scr = e->sel_scr;
     P.sel_scr_stride = e->sel_scr_stride;
@@ -4269,7 +4284,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     e->lek_delta = (double)e->k * (1.0 / (double)e->S);
     e->wide = select_wide(comp, e->k, e->S);
     e->sel_scr_stride = e->wide ? select_scratch_stride(comp, e->k, e->S) : 0;
-    e->lds_select = select_lds(comp, e->k, e->S, e->wide);
+    e->lds_select = select_lds(comp, e->k, e->S, e->wide, big_ring(e, false));
+    e->lds_select_b = select_lds(comp, e->k, e->S, e->wide, big_ring(e, true));
     e->lds_emit = emit_lds(e->S, e->wide);
     e->lds_newton = mc_lds_bytes(e->dp);
 
@@ -4555,7 +4571,7 @@ static int batch_build(Batch* b)
     b->h_probs.clear();
     for (size_t i = 0; i < b->m.size(); ++i) {
         Engine* e = b->m[i];
-        const ProblemDev P = make_desc(e);
+        const ProblemDev P = make_desc(e, true);
         b->h_probs.push_back(P);
         const int p = (int)i;
         add(BK_MARGIN, p, blocks_margin(P), false, sizeof(double) * (size_t)e->d_pad);
@@ -4565,11 +4581,11 @@ static int batch_build(Batch* b)
         const int comp = e->o.compressor;
         const bool w = e->wide;
         // selection (with the messages fused, except for TopLEK)
-        const size_t fused = std::max(e->lds_select, e->lds_emit);
+        const size_t fused = std::max(e->lds_select_b, e->lds_emit);
         if (comp == FEDNL_TOPK)
             add(w ? BK_SEL_TOPK_W : BK_SEL_TOPK, p, e->n, false, fused);
         else if (comp == FEDNL_TOPLEK)
-            add(w ? BK_SEL_TOPLEK_W : BK_SEL_TOPLEK, p, e->n, false, e->lds_select);
+            add(w ? BK_SEL_TOPLEK_W : BK_SEL_TOPLEK, p, e->n, false, e->lds_select_b);
         else if (comp == FEDNL_RANDK)
             add(w ? BK_SEL_RANDK_W : BK_SEL_RANDK, p, e->n, false, fused);
         else
@@ -4604,7 +4620,7 @@ static int batch_sync_desc(Batch* b)
 {
     bool changed = false;
     for (size_t i = 0; i < b->m.size(); ++i) {
-        const ProblemDev P = make_desc(b->m[i]);
+        const ProblemDev P = make_desc(b->m[i], true);
         if (memcmp(&P, &b->h_probs[i], sizeof(P)) != 0) {
             b->h_probs[i] = P;
             changed = true;
@@ -5394,7 +5410,8 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     if (rng) hipMemcpyAsync(drng, rng, sizeof(uint32_t) * nclients, hipMemcpyHostToDevice, st);
     if (toplek_rng) hipMemcpyAsync(dlek, toplek_rng, sizeof(uint32_t) * workers, hipMemcpyHostToDevice, st);
     const bool wide = select_wide(compressor, k, S);
-    const size_t lds = select_lds(compressor, k, S, wide);
+    const bool big = nclients <= kBigRingClients;
+    const size_t lds = select_lds(compressor, k, S, wide, big);
     const size_t lds_bm = emit_lds(S, wide);
     raise_lds_limit((const void*)k_select_heap<false, false, false>, lds);
     raise_lds_limit((const void*)k_select_heap<true, false, false>, lds);
@@ -5476,7 +5493,7 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     hp.tiles_up = (compressor == FEDNL_TOPLEK) ? 1 : 0;
     hp.part = parts;
     hp.lek_delta = delta_param;
-    hp.sel_ring = wide ? kMaxScanRing : scan_ring_slots(std::min(k, kMaxHeapK), S);
+    hp.sel_ring = wide ? kMaxScanRing : scan_ring_slots(std::min(k, kMaxHeapK), S, big);
     hp.wide = wide ? 1 : 0;
     hp.sel_scr = scr;
     hp.sel_scr_stride = scr_stride;
