@@ -289,6 +289,19 @@ typedef void (*ResultCallbackLocalPP)(int xDimension);
 int doptExecuteLocalPP(const char *command, ResultCallbackLocalPP resultCallback);
 int fednl_cli_pp_main(int argc, char **argv);
 
+/* The CLI's LIBSVM reader, as train() reads the training set
+ * (TrainLoop.h:123-170 over dataset_loader_fs.h:381-883): N samples of d
+ * attributes (+1 intercept column, last, when add_intercept), labels mapped
+ * to +-1. With X == NULL only *N and *d are set; else X (N x d, row-major)
+ * and y (N) are filled. Returns 0, FEDNL_E_MISSING_ARGUMENT for a null
+ * pointer, or -4 (the CLI's dataset parse error) as train() would. */
+int fednl_cli_load_libsvm(const char *path, int add_intercept, int *N, int *d, double *X, double *y);
+
+/* The CLI's CRC-32 (dopt/dopt/system/src/digest/Crc.cpp:74-90, seed
+ * 0xFFFFFFFF, no final inversion), as printed for the dataset
+ * (Reports.h:25-33) and the last iterate (TrainLoop.h:1767-1768). */
+uint32_t fednl_cli_crc32(const void *buf, size_t size);
+
 #ifdef __cplusplus
 }
 #endif

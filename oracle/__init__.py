@@ -117,6 +117,13 @@ def ref():
                                     ctypes.c_int, ctypes.c_int, _c_dp, _c_dp, _c_dp, _c_dp, ctypes.c_int,
                                     _c_dp, _c_dp, _c_ip, _c_up, _c_dp, _c_dp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, _c_ip, _c_ip]
+        R.ref_load_libsvm.restype = ctypes.c_int
+        R.ref_load_libsvm.argtypes = [ctypes.c_char_p, ctypes.c_int, _c_ip, _c_ip, _c_dp, _c_dp, _c_up, _c_up]
+        R.ref_crc32.restype = ctypes.c_uint32
+        R.ref_crc32.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32]
+        R.ref_crc32_seed.restype = ctypes.c_uint32
+        R.ref_l_smoothness.restype = ctypes.c_double
+        R.ref_l_smoothness.argtypes = [_c_dp, _c_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_ip]
         R.ref_fednl_pp_run.restype = ctypes.c_int
         R.ref_fednl_pp_run.argtypes = [_c_dp, _c_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -127,6 +134,41 @@ def ref():
 
 
 # ---------------------------------------------------------------- helpers
+
+
+def ref_load_libsvm(path, add_intercept):
+    """dopt's own LIBSVM loading as train() runs it (TrainLoop.h:123-170),
+    from oracle/_ref: returns (X (N x d), y, crc_samples, crc_labels) with the
+    two digests Reports.h:25-33 prints, or raises with the loader's code."""
+    R = ref()
+    N, d = ctypes.c_int(), ctypes.c_int()
+    cx, cy = ctypes.c_uint32(), ctypes.c_uint32()
+    rc = R.ref_load_libsvm(path.encode(), int(add_intercept), ctypes.byref(N), ctypes.byref(d), None, None,
+                           ctypes.byref(cx), ctypes.byref(cy))
+    if rc != 0:
+        raise RuntimeError(f"reference loader failed ({rc}) on {path}")
+    X = np.zeros((N.value, d.value))
+    y = np.zeros(N.value)
+    R.ref_load_libsvm(path.encode(), int(add_intercept), ctypes.byref(N), ctypes.byref(d), X.ctypes.data_as(_c_dp),
+                      y.ctypes.data_as(_c_dp), None, None)
+    return X, y, int(cx.value), int(cy.value)
+
+
+def ref_l_smoothness(features, labels, lam):
+    """(L(f), power iterations) of the reference's computeLSmoothness on the
+    samples (m x d) as train() calls it (TrainLoop.h:306-316)."""
+    f = np.ascontiguousarray(features, dtype=np.float64)
+    b = np.ascontiguousarray(labels, dtype=np.float64)
+    it = np.zeros(1, dtype=np.int32)
+    L = ref().ref_l_smoothness(dp(f), dp(b), f.shape[1], f.shape[0], lam, ip(it))
+    return float(L), int(it[0])
+
+
+def ref_crc32(b):
+    """dopt::crc32(buf, size, crc32Seed()) from oracle/_ref (Crc.cpp:74-90)."""
+    R = ref()
+    buf = ctypes.create_string_buffer(bytes(b), len(b)) if len(b) else ctypes.create_string_buffer(1)
+    return int(R.ref_crc32(buf, len(b), R.ref_crc32_seed()))
 
 def lda(d: int) -> int:
     return (d + 3) // 4 * 4

@@ -25,6 +25,9 @@
 #include "dopt/linalg_matrices/include/factorization/CholeskyFactorization.h"
 #include "dopt/linalg_linsolvers/include/ElementarySolvers.h"
 #include "dopt/linalg_linsolvers/include/GaussEliminationSolvers.h"
+#include "dopt/optimization_problems/include/ml/dataset_loader_fs.h"
+#include "dopt/fs/include/FileSystemHelpers.h"
+#include "dopt/system/include/digest/Crc.h"
 
 #include <stdint.h>
 #include <string.h>
@@ -136,6 +139,59 @@ namespace
 
 extern "C"
 {
+    // dopt's own LIBSVM loading as train() runs it (TrainLoop.h:123-170): the
+    // same separators, the statistics pass, then createDataset with the
+    // intercept flag and the binary label remapping. X (N x d row-major:
+    // sample, attribute) and y; X == NULL: sizes only. crc_x / crc_y (may be
+    // NULL) get the digests Reports.h:25-33 prints (matrixByCols raw data,
+    // LDA padding included, and the labels). Returns 0, -3 (no file), -4.
+    int ref_load_libsvm(const char* path, int add_intercept, int* N, int* d, double* X, double* y,
+                        uint32_t* crc_x, uint32_t* crc_y)
+    {
+        auto line_sep = [](char c) { return c == '\n' || c == '\r'; };
+        auto field_sep = [](char c) { return c == '\t' || c == ' '; };
+        auto kv_sep = [](char c) { return c == ':'; };
+        auto skip = [](char c) { return false; };
+        dopt::FileSystemHelpers::FileMappingResult m = dopt::FileSystemHelpers::mapFileToMemory(path, true);
+        if (!m.isOk)
+            return -3;
+        dopt::Data raw(m.memory, m.memorySizeInBytes, dopt::Data::MemInitializedType::eGiftWholeMemoryPleaseNotFree);
+        dopt::TextFileDataSetLoader loader(raw, line_sep, field_sep, kv_sep, skip);
+        auto stats = loader.computeStatisticsWithOneDataPass<false, true, int32_t, int32_t>();
+        if (!stats.ok) {
+            raw.clear();
+            dopt::FileSystemHelpers::unmapFileFromMemory(m);
+            return -4;
+        }
+        typedef dopt::DataSet<Mat, Vec, dopt::VectorNDRaw_b, Vec> DS;
+        DS ds;
+        const int flags = (add_intercept ? dopt::CreateDatasetFlags::eAddInterceptTerm : dopt::CreateDatasetFlags::eNone) |
+                          dopt::CreateDatasetFlags::eMakeRemappingForBinaryLogisticLoss;
+        loader.createDataset<true, true>(ds, stats, flags);
+        raw.clear();
+        dopt::FileSystemHelpers::unmapFileFromMemory(m);
+        const size_t n = ds.train_samples_tr.columns(), dd = ds.train_samples_tr.rows();
+        *N = (int)n;
+        *d = (int)dd;
+        if (crc_x)
+            *crc_x = dopt::crc32(ds.train_samples_tr.matrixByCols.rawData(),
+                                 ds.train_samples_tr.matrixByCols.sizeInBytes(), dopt::crc32Seed());
+        if (crc_y)
+            *crc_y = dopt::crc32(ds.train_outputs.rawData(), ds.train_outputs.sizeInBytes(), dopt::crc32Seed());
+        if (X) {
+            for (size_t s_ = 0; s_ < n; ++s_) {
+                for (size_t a = 0; a < dd; ++a)
+                    X[s_ * dd + a] = ds.train_samples_tr.get(a, s_);
+                y[s_] = ds.train_outputs[s_];
+            }
+        }
+        return 0;
+    }
+
+    // dopt::crc32 / crc32Seed (Crc.cpp:74-90)
+    uint32_t ref_crc32(const void* buf, size_t size, uint32_t seed) { return dopt::crc32(buf, size, seed); }
+    uint32_t ref_crc32_seed() { return dopt::crc32Seed(); }
+
     int ref_lda(int d)
     {
         Mat m(d, d);
@@ -251,6 +307,20 @@ extern "C"
         g.setSeed(seed);
         dopt::shuffle(v, (size_t)k, g);
         memcpy(out, v.data(), sizeof(uint32_t) * n);
+    }
+
+    // L(f) of logistic_regression.h:707-740 on m samples (features m x d
+    // row-major, labels +-1) with the tolerance train() uses, 1e-3 mu
+    // (TrainLoop.h:306-316); *iters gets the power iterations.
+    double ref_l_smoothness(const double* feat, const double* labels, int d, int m, double lambda, int* iters)
+    {
+        Mat atr = designTr(feat, d, m);
+        Vec b = vecFrom(labels, m);
+        Problem p(atr, b, lambda);
+        size_t it = 0;
+        const double L = p.computeLSmoothness(0.001 * p.computeMuStrongConvexity(), &it);
+        *iters = (int)it;
+        return L;
     }
 
     // Logistic oracle of one client at x (logistic_regression.h:88-547).

@@ -238,10 +238,23 @@ static int parse_args(const CmdLine& cl, Args& a)
 }
 
 // ---------------------------------------------------------------------------
-// LIBSVM text: "label idx:val idx:val ..." per line, fields split on ' ' or '\t',
-// lines on '\n' or '\r'. d = maxIdx - minIdx + 1 (+1 intercept, last column);
-// labels must take exactly two values and map to -1 below their mean, +1 above
-// (dataset_loader_fs.h:381-604, 607-883).
+// LIBSVM text, read as dopt's TextFileDataSetLoader reads it for train()
+// (TrainLoop.h:123-170; dataset_loader_fs.h:381-604 statistics pass,
+// 606-883 createDataset): lines end at '\n' or '\r', fields are split on ' '
+// or '\t', and each field on ':' (empty pieces dropped). A field of one piece
+// is a label, of two pieces a feature index:value pair.
+//  - Statistics pass: labels parse as int32 and indices as int32, both with
+//    dopt's strict decimal parser. On the first example every label resets
+//    the label range (and the class count to 1) and every index resets the
+//    index range; later, a label equal to neither end of the range is a new
+//    class (and widens the range if outside it), and an index outside the
+//    range widens it. Exactly two classes are required.
+//  - Create pass: d = max index - min index + 1 (+ 1 intercept, the last
+//    column); labels parse as double and map to -1 below the mean of the
+//    label range's ends, +1 otherwise; values parse as double with dopt's
+//    decimal parser (integer part + fraction part * 10^-length, scaled by a
+//    power of ten for an exponent), which can differ from strtod in the last
+//    bit.
 // ---------------------------------------------------------------------------
 struct Dataset {
     int N = 0, d = 0;
@@ -249,103 +262,245 @@ struct Dataset {
     std::vector<double> y;  // N
 };
 
-static int load_libsvm(const std::string& path, bool add_intercept, Dataset& ds)
+// StringUtils.h:171-268 (fromStringSignedInteger): optional sign, then
+// decimal digits only; int32 arithmetic (wraps)
+static bool parse_i32(std::string_view s, int32_t& out)
+{
+    if (s.empty())
+        return false;
+    size_t p = 0;
+    int sign = 1;
+    if (s[0] == '+' || s[0] == '-') {
+        sign = s[0] == '-' ? -1 : 1;
+        p = 1;
+    }
+    if (p >= s.size() || s[p] < '0' || s[p] > '9')
+        return false;
+    uint32_t v = (uint32_t)(s[p] - '0');
+    if (sign < 0) v = 0u - v;
+    for (++p; p < s.size(); ++p) {
+        if (s[p] < '0' || s[p] > '9')
+            return false;
+        const uint32_t dgt = (uint32_t)(s[p] - '0');
+        v = sign > 0 ? v * 10u + dgt : v * 10u - dgt;
+    }
+    out = (int32_t)v;
+    return true;
+}
+
+// SimpleMathRoutines.h:179-195: a^n by halving the exponent
+static double power_natural(double a, uint32_t n)
+{
+    if (n == 0)
+        return 1.0;
+    const double t = power_natural(a, n / 2);
+    return (n % 2 == 0) ? t * t : t * t * a;
+}
+
+// StringUtils.h:270-463 (fromStringFloat) for double
+static bool parse_dopt_double(std::string_view s, double& out)
+{
+    if (s.empty())
+        return false;
+    size_t p = 0;
+    int sign = 1;
+    if (s[0] == '+') {
+        p = 1;
+    } else if (s[0] == '-') {
+        sign = -1;
+        p = 1;
+    }
+    uint32_t ip = 0, fp = 0, flen = 0, ep = 0;
+    int esign = 1;
+    for (; p < s.size(); ++p) {
+        const char c = s[p];
+        if (c == '.' || c == 'e' || c == 'E')
+            break;
+        if (c < '0' || c > '9')
+            return false;
+        ip = ip * 10u + (uint32_t)(c - '0');
+    }
+    if (p < s.size() && s[p] == '.') {
+        for (++p; p < s.size(); ++p) {
+            const char c = s[p];
+            if (c == 'e' || c == 'E')
+                break;
+            if (c < '0' || c > '9')
+                return false;
+            fp = fp * 10u + (uint32_t)(c - '0');
+            ++flen;
+        }
+    }
+    if (p < s.size() && (s[p] == 'e' || s[p] == 'E')) {
+        ++p;
+        if (p < s.size() && s[p] == '+') {
+            ++p;
+        } else if (p < s.size() && s[p] == '-') {
+            esign = -1;
+            ++p;
+        }
+        for (; p < s.size(); ++p) {
+            const char c = s[p];
+            if (c < '0' || c > '9')
+                return false;
+            ep = ep * 10u + (uint32_t)(c - '0');
+        }
+    }
+    const double I = (double)(int32_t)ip, F = (double)(int32_t)fp;
+    if (ep == 0) {
+        if (flen == 0) {
+            out = sign > 0 ? I : -I;
+        } else {
+            // the reference's table holds 1 / 1e0 .. 1 / 1e16 as constants
+            static const double kDen[] = {1.0 / 1e0,  1.0 / 1e1,  1.0 / 1e2,  1.0 / 1e3,  1.0 / 1e4,  1.0 / 1e5,
+                                          1.0 / 1e6,  1.0 / 1e7,  1.0 / 1e8,  1.0 / 1e9,  1.0 / 1e10, 1.0 / 1e11,
+                                          1.0 / 1e12, 1.0 / 1e13, 1.0 / 1e14, 1.0 / 1e15, 1.0 / 1e16};
+            const double den = flen < 17 ? kDen[flen] : 1.0 / power_natural(10.0, flen);
+            const double v = I + F * den;
+            out = sign > 0 ? v : -v;
+        }
+    } else {
+        const double den = 1.0 / power_natural(10.0, flen);
+        const double e = power_natural(10.0, ep);
+        const double m = I + F * den;
+        if (esign > 0)
+            out = sign > 0 ? m * e : -m * e;
+        else
+            out = sign > 0 ? m / e : -m / e;
+    }
+    return true;
+}
+
+// splitToSubstrings without empty pieces (StringUtils.h:667-716)
+template <class Pred>
+static void split_pieces(std::string_view s, Pred is_sep, std::vector<std::string_view>& out)
+{
+    out.clear();
+    size_t a = 0;
+    for (size_t i = 0; i <= s.size(); ++i) {
+        if (i == s.size() || is_sep(s[i])) {
+            if (i > a)
+                out.emplace_back(s.substr(a, i - a));
+            a = i + 1;
+        }
+    }
+}
+
+// the text's non-empty lines, in order
+static void text_lines(const std::string& text, std::vector<std::string_view>& lines)
+{
+    split_pieces(std::string_view(text), [](char c) { return c == '\n' || c == '\r'; }, lines);
+}
+
+static int load_libsvm(const std::string& path, bool add_intercept, Dataset& ds, bool print_errors = true)
 {
     std::ifstream f(path, std::ios::binary);
     if (!f)
         return kDatasetParse;
-    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-    struct Row { double label; std::vector<std::pair<long, double>> feats; };
-    std::vector<Row> rows;
-    long lmin = 0, lmax = 0;
-    bool have_label = false;
+    const std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<std::string_view> lines, fields, kv;
+    text_lines(text, lines);
+    auto field_sep = [](char c) { return c == ' ' || c == '\t'; };
+    auto kv_sep = [](char c) { return c == ':'; };
+    // statistics pass
+    int32_t lmin = 0, lmax = 0, imin = 0, imax = 0;
     int unique_labels = 0;
-    long fmin = 0, fmax = 0;
-    bool have_feat = false;
-    size_t pos = 0;
-    const size_t n = text.size();
-    while (pos < n) {
-        size_t e = pos;
-        while (e < n && text[e] != '\n' && text[e] != '\r') ++e;
-        std::string_view line(text.data() + pos, e - pos);
-        pos = e + 1;
-        // fields
-        std::vector<std::string_view> fields;
-        size_t a = 0;
-        while (a < line.size()) {
-            while (a < line.size() && (line[a] == ' ' || line[a] == '\t')) ++a;
-            size_t b = a;
-            while (b < line.size() && line[b] != ' ' && line[b] != '\t') ++b;
-            if (b > a) fields.emplace_back(line.substr(a, b - a));
-            a = b;
-        }
+    size_t examples = 0;
+    for (std::string_view line : lines) {
+        split_pieces(line, field_sep, fields);
         if (fields.empty())
             continue;
-        Row r;
-        {
-            std::string lab(fields[0]);
-            char* endp = nullptr;
-            const double lv = strtod(lab.c_str(), &endp);
-            if (endp == lab.c_str()) {
-                std::cerr << "Error during parsing. Line: " << rows.size() + 1 << ". Can not parse label " << lab << '\n';
-                return kDatasetParse;
-            }
-            r.label = lv;
-            const long li = (long)lv;
-            if (!have_label) {
-                lmin = lmax = li;
-                unique_labels = 1;
-                have_label = true;
-            } else if (li < lmin || li > lmax) {
-                ++unique_labels;
-                if (li < lmin) lmin = li;
-                if (li > lmax) lmax = li;
-            }
-        }
-        for (size_t q = 1; q < fields.size(); ++q) {
-            const size_t colon = fields[q].find(':');
-            if (colon == std::string_view::npos) {
-                std::cerr << "Warning during parsing. Line: " << rows.size() + 1 << ". Can not parse " << fields[q]
+        for (std::string_view fld : fields) {
+            split_pieces(fld, kv_sep, kv);
+            if (kv.size() == 1) {
+                int32_t lv = 0;
+                if (!parse_i32(kv[0], lv)) {
+                    if (print_errors)
+                        std::cerr << "Error during conversion of lable: " << kv[0] << ". Parsing Line: " << examples + 1
+                                  << '\n';
+                } else if (examples == 0) {
+                    lmin = lmax = lv;
+                    unique_labels = 1;
+                } else if (lv != lmin && lv != lmax) {
+                    ++unique_labels;
+                    if (lv < lmin)
+                        lmin = lv;
+                    else if (lv > lmax)
+                        lmax = lv;
+                }
+            } else if (kv.size() == 2) {
+                int32_t idx = 0;
+                if (!parse_i32(kv[0], idx)) {
+                    if (print_errors)
+                        std::cerr << "Error during parsing. Line: " << examples + 1 << ". Can not parse " << kv[0]
+                                  << " for purpose of converting to unsigned integer.\n";
+                    return kDatasetParse;
+                }
+                if (examples == 0) {
+                    imin = imax = idx;
+                } else if (idx < imin) {
+                    imin = idx;
+                } else if (idx > imax) {
+                    imax = idx;
+                }
+            } else if (print_errors) {
+                std::cerr << "Warning during parsing. Line: " << examples + 1 << ". Can not parse " << fld
                           << " for purpose of obtaing feature_index:value of label.\n";
-                continue;
             }
-            std::string is(fields[q].substr(0, colon)), vs(fields[q].substr(colon + 1));
-            char* endp = nullptr;
-            const long idx = strtol(is.c_str(), &endp, 10);
-            if (endp == is.c_str()) {
-                std::cerr << "Error during parsing. Line: " << rows.size() + 1 << ". Can not parse " << is << '\n';
-                return kDatasetParse;
-            }
-            const double v = strtod(vs.c_str(), nullptr);
-            if (!have_feat) {
-                fmin = fmax = idx;
-                have_feat = true;
-            } else if (idx < fmin) {
-                fmin = idx;
-            } else if (idx > fmax) {
-                fmax = idx;
-            }
-            r.feats.emplace_back(idx, v);
         }
-        rows.push_back(std::move(r));
+        ++examples;
     }
-    if (rows.empty() || unique_labels != 2) {
+    if (examples == 0)
+        return kDatasetParse;
+    if (unique_labels != 2) {
         std::cout << "Specified dataset labels for " << unique_labels
                   << " classes. And you need to have a dataset with 2 classes.\n";
         return kDatasetParse;
     }
-    const int dfeat = have_feat ? (int)(fmax - fmin + 1) : 0;
-    ds.N = (int)rows.size();
+    // create pass
+    const int dfeat = (int)((int64_t)imax - (int64_t)imin + 1);
+    ds.N = (int)examples;
     ds.d = dfeat + (add_intercept ? 1 : 0);
     ds.X.assign((size_t)ds.N * ds.d, 0.0);
-    ds.y.resize(ds.N);
+    ds.y.assign(ds.N, 0.0);
     const double avg = ((double)lmin + (double)lmax) / 2.0;
-    for (int s = 0; s < ds.N; ++s) {
-        ds.y[s] = (rows[s].label < avg) ? -1.0 : 1.0;
-        for (auto& fv : rows[s].feats)
-            ds.X[(size_t)s * ds.d + (fv.first - fmin)] = fv.second;
+    size_t s = 0;
+    for (std::string_view line : lines) {
+        split_pieces(line, field_sep, fields);
+        if (fields.empty())
+            continue;
+        double* row = ds.X.data() + s * ds.d;
+        for (std::string_view fld : fields) {
+            split_pieces(fld, kv_sep, kv);
+            if (kv.size() == 1) {
+                double lv = 0.0;
+                if (parse_dopt_double(kv[0], lv))
+                    ds.y[s] = lv;
+                else if (print_errors)
+                    std::cerr << "Error during parsing. Line: " << s + 1 << ". Can not parse " << kv[0]
+                              << " for purpose of converting to response variable.\n";
+                ds.y[s] = (ds.y[s] < avg) ? -1.0 : 1.0;
+            } else if (kv.size() == 2) {
+                int32_t idx = 0;
+                double v = 0.0;
+                const bool ok_i = parse_i32(kv[0], idx);
+                const bool ok_v = parse_dopt_double(kv[1], v);
+                if (!ok_i || !ok_v) {
+                    if (print_errors)
+                        std::cerr << "Error during parsing. Line: " << s + 1 << ". Can not parse: " << fld << '\n';
+                    continue;  // the reference stores an unparsed index or value uninitialised
+                }
+                const int64_t col = (int64_t)idx - imin;
+                if (col >= 0 && col < dfeat)  // outside: the reference writes out of its matrix
+                    row[col] = v;
+            } else if (print_errors) {
+                std::cerr << "Error during parsing. Line: " << s + 1
+                          << ". More then one feature_index:value pair has been found.\n";
+            }
+        }
         if (add_intercept)
-            ds.X[(size_t)s * ds.d + ds.d - 1] = 1.0;
+            row[ds.d - 1] = 1.0;
+        ++s;
     }
     return kOk;
 }
@@ -399,6 +554,115 @@ struct Bytes {
     void f64(double v) { const char* p = (const char*)&v; b.insert(b.end(), p, p + 8); }
 };
 
+// dataset_loader_fs.h:237-313 (checkSplitConsistency) on this split: the
+// clients' sample groups are disjoint, every client row is the training row it
+// was copied from (features and label), and the clients' samples and the
+// skipped ones together cover every sample.
+static bool check_split(const Dataset& ds, const std::vector<int32_t>& order, int n, int mpc,
+                        const std::vector<double>& feats, const std::vector<double>& labs)
+{
+    bool ok = true;
+    const int d = ds.d;
+    std::vector<char> seen(ds.N, 0);
+    for (int c = 0; c < n; ++c) {
+        bool repeat = false;
+        for (int s = 0; s < mpc; ++s) {
+            const int g = order[(size_t)c * mpc + s];
+            if (g < 0 || g >= ds.N || seen[g])
+                repeat = true;
+            else
+                seen[g] = 1;
+        }
+        if (repeat) {
+            std::cerr << "Error in splitting. Some indicies are repeating for group #" << c << '\n';
+            ok = false;
+        }
+    }
+    for (int c = 0; c < n; ++c) {
+        bool bad = false;
+        for (int s = 0; s < mpc; ++s) {
+            const int g = order[(size_t)c * mpc + s];
+            if (g < 0 || g >= ds.N) {
+                bad = true;
+                continue;
+            }
+            const size_t r = (size_t)c * mpc + s;
+            if (labs[r] != ds.y[g] || memcmp(&feats[r * d], &ds.X[(size_t)g * d], sizeof(double) * d) != 0)
+                bad = true;
+        }
+        if (bad) {
+            std::cerr << "Error in splitting. Something has not been copied in a correct way for group #" << c << '\n';
+            ok = false;
+        }
+    }
+    for (size_t q = (size_t)n * mpc; q < order.size(); ++q)
+        if (order[q] >= 0 && order[q] < ds.N)
+            seen[order[q]] = 1;
+    for (int g = 0; g < ds.N; ++g)
+        if (!seen[g]) {
+            std::cerr << "Error in splitting. The record " << g << " has been lost\n";
+            ok = false;
+        }
+    return ok;
+}
+
+// logistic_regression.h:707-740 (computeLSmoothness with tolerance 1e-3 mu,
+// TrainLoop.h:314): power iteration on A^T A / (4 M) over the clients' M
+// samples from a RandomGenRealLinear start (seed 1234, [0, 1],
+// RandomGenRealLinear.h:73-77; VectorND_Raw.h:835-840), normalised by the
+// infinity norm; L = the last norm + lambda. A^T A / (4 M) is the average of
+// the clients' Hessians at x = 0 (sigma(0)(1 - sigma(0)) = 1/4) less lambda I,
+// formed by the engine.
+static bool l_smoothness(void* eng, int n, int d, double lambda, double& L, size_t& iters)
+{
+    const size_t S = (size_t)d * (d + 1) / 2;
+    std::vector<double> hc((size_t)n * S), x0(d, 0.0);
+    if (fednl_hessians_at(eng, x0.data(), hc.data()) != FEDNL_OK)
+        return false;
+    std::vector<double> B((size_t)d * d);
+    for (int j = 0; j < d; ++j)
+        for (int i = 0; i <= j; ++i) {
+            const size_t p = (size_t)j * (j + 1) / 2 + i;
+            double acc = 0.0;
+            for (int c = 0; c < n; ++c) acc += hc[(size_t)c * S + p];
+            double v = acc / n;
+            if (i == j) v -= lambda;
+            B[(size_t)i * d + j] = B[(size_t)j * d + i] = v;
+        }
+    uint32_t xs = 1234u;
+    std::vector<double> v(d), vp(d);
+    for (int i = 0; i < d; ++i) {
+        xs = (uint32_t)((214013ull * xs + 2531011ull) % 4294967296ull) % 4294967295u;
+        v[i] = 0.0 + ((double)xs * (1.0 / 4294967295.0)) * (1.0 - 0.0);
+    }
+    auto linf = [](const std::vector<double>& u) {
+        double r = fabs(u[0]);
+        for (size_t i = 1; i < u.size(); ++i) r = std::max(r, fabs(u[i]));
+        return r;
+    };
+    vp = v;
+    const double v0 = linf(vp);
+    for (auto& t : vp) t /= v0;
+    const double eps = 0.001 * lambda;
+    double maxeig = 0.0;
+    for (iters = 0;; ++iters) {
+        for (int i = 0; i < d; ++i) {
+            double acc = 0.0;
+            for (int j = 0; j < d; ++j) acc += B[(size_t)i * d + j] * vp[j];
+            v[i] = acc;
+        }
+        maxeig = linf(v);
+        for (auto& t : v) t /= maxeig;
+        double diff = 0.0;
+        for (int i = 0; i < d; ++i) diff = std::max(diff, fabs(v[i] - vp[i]));
+        if (diff < eps || iters > 100000)
+            break;
+        std::swap(v, vp);
+    }
+    L = maxeig + lambda;
+    return true;
+}
+
 static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalPP cb_pp, bool pp)
 {
     Args a;
@@ -445,14 +709,35 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
     for (int s = 0; s < ds.N; ++s) order[s] = s;
     if (a.reshuffle)
         shuffle_indices(order, a.reshuffle_seed);
+    const int d = ds.d;
+    // client-major design for the engine: client c takes order[c m .. (c+1) m)
+    std::vector<double> feats((size_t)n * mpc * d), labs((size_t)n * mpc);
+    for (int c = 0; c < n; ++c)
+        for (int s = 0; s < mpc; ++s) {
+            const int src = order[(size_t)c * mpc + s];
+            memcpy(&feats[((size_t)c * mpc + s) * d], &ds.X[(size_t)src * d], sizeof(double) * d);
+            labs[(size_t)c * mpc + s] = ds.y[src];
+        }
     if (print) {
         std::cout << "  Train samples has been reshuffled: " << (a.reshuffle ? "[YES]" : "[NO]") << '\n';
         std::cout << "  Train samples reshuffled seed: " << a.reshuffle_seed << '\n';
         std::cout << "  Number of clients: " << n << '\n';
         std::cout << "  Number of samples per client: " << mpc << '\n';
         std::cout << "  Number of skipped samples during samples distribution: " << (ds.N - mpc * n) << '\n';
-        std::cout << "  Checking data split " << (cl.flag("check_split") ? "[OK]" : "[SKIP]") << '\n';
+        std::cout << "  Train data distribution across clients has been completed. [OK]\n";
+        std::cout << "=========================================================================\n";
     }
+    if (cl.flag("check_split")) {
+        if (!check_split(ds, order, n, mpc, feats, labs)) {
+            if (print) std::cerr << "  Checking data split [FAILED]\n";
+            return kSplitError;
+        }
+        if (print) std::cout << "  Checking data split [OK]\n";
+    } else if (print) {
+        std::cout << "  Checking data split [SKIP] \n";
+    }
+    if (print)
+        std::cout << "=========================================================================\n";
 
     if (a.algorithm != "fednl1") {
         std::cerr << "Algorithm '" << a.algorithm << "' is not provided by this engine (fednl1 only)\n";
@@ -473,7 +758,6 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
         if (print) std::cerr << "You have specified unknown type of compressor: " << a.compressor << '\n';
         return kWrongArgument;
     }
-    const int d = ds.d;
     const size_t S = (size_t)d * (d + 1) / 2;
     const bool whole = (comp == FEDNL_IDENTICAL || comp == FEDNL_NATURAL);  // TrainLoop.h:354-377
     const int k = whole ? (int)S : (int)(a.k_mult * d);
@@ -498,15 +782,6 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
         }
     }
 
-    // client-major design for the engine
-    std::vector<double> feats((size_t)n * mpc * d), labs((size_t)n * mpc);
-    for (int c = 0; c < n; ++c)
-        for (int s = 0; s < mpc; ++s) {
-            const int src = order[(size_t)c * mpc + s];
-            memcpy(&feats[((size_t)c * mpc + s) * d], &ds.X[(size_t)src * d], sizeof(double) * d);
-            labs[(size_t)c * mpc + s] = ds.y[src];
-        }
-
     fednl_opts o;
     fednl_default_opts(&o);
     o.lambda = a.lambda;
@@ -529,6 +804,20 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
         return kInternal;
     }
     const double alpha = fednl_alpha(eng);
+    // TrainLoop.h:306-316: L(f) when asked for (or for a theoretical global
+    // step size, which only GD uses)
+    const bool compute_L = cl.flag("compute-L-smooth") || cl.flag("theoretical_global_lr");
+    double L_f = -1.0;
+    if (compute_L) {
+        size_t iters = 0;
+        if (!l_smoothness(eng, n, d, a.lambda, L_f, iters)) {
+            std::cerr << "engine: " << fednl_last_error() << '\n';
+            fednl_destroy(eng);
+            return kInternal;
+        }
+        if (print)
+            std::cout << "Compute L constants for objective (power iterations: " << iters << ")\n";
+    }
     if (print) {
         std::cout << "=========================================================================\n";
         std::cout << "Main loop has been started: r:" << a.rounds << ",clients:" << n << ",d:" << d << ",ni:" << mpc << '\n';
@@ -659,6 +948,27 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
             std::cout << "  Line Search parameters: " << "c: " << a.c_ls << ", gamma: " << a.gamma_ls << '\n';
         }
         std::cout << "=========================================================================\n";
+        // TrainLoop.h:1741-1764
+        std::cout << '\n';
+        std::cout << "Debug information\n\n";
+        std::cout << "  Transfer indicies for RandK: " << (a.transfer_idx ? "[YES]" : "[NO]") << '\n';
+        std::cout << "  mu(f): " << mu << '\n';
+        if (compute_L) {
+            std::cout << "  L(f): " << L_f << '\n';
+            std::cout << "  cond(f) = (L(f)/mu(f)): " << L_f / mu << '\n';
+        } else {
+            std::cout << "  L(f): " << "unavailable" << '\n';
+            std::cout << "  cond(f) = (L(f)/mu(f)): " << "unavailable" << '\n';
+        }
+        std::cout << "  Maximum allowable k multiplier for RandK and TopK: " << double(((d + 1) * d) / 2) / d << '\n';
+        std::cout << "  Maximum sendable components for RandK and TopK from one client: " << double(((d + 1) * d) / 2)
+                  << '\n';
+        std::cout << '\n';
+        std::cout << "  Print information about CRC32 for datasets: " << (a.crc_data ? 1 : 0) << '\n';
+        std::cout << "  Print information about used memory: " << (a.meminfo ? 1 : 0) << '\n';
+        std::cout << "  Server obtainining updates from clients in order: " << (a.forceseq ? 1 : 0) << '\n';
+        std::cout << '\n';
+        std::cout << "=========================================================================\n";
         std::cout << "Solution Properties\n\n";
         std::cout << "  Last iterate CRC-32: " << crc32_bytes(x.data(), sizeof(double) * d) << '\n';
         std::cout << "  Gradient in last iterate CRC-32: " << crc32_bytes(glast.data(), sizeof(double) * d) << '\n';
@@ -766,5 +1076,27 @@ int doptExecuteLocalPP(const char* command, ResultCallbackLocalPP resultCallback
     fednl_cli::CmdLine cl((int)argv.size(), argv.data());
     return fednl_cli::train(cl, nullptr, resultCallback, true);
 }
+
+
+int fednl_cli_load_libsvm(const char* path, int add_intercept, int* N, int* d, double* X, double* y)
+{
+    if (!path || !N || !d)
+        return FEDNL_E_MISSING_ARGUMENT;
+    fednl_cli::Dataset ds;
+    const int rc = fednl_cli::load_libsvm(path, add_intercept != 0, ds, false);
+    if (rc != 0)
+        return rc;
+    *N = ds.N;
+    *d = ds.d;
+    if (X) {
+        if (!y)
+            return FEDNL_E_MISSING_ARGUMENT;
+        memcpy(X, ds.X.data(), sizeof(double) * ds.X.size());
+        memcpy(y, ds.y.data(), sizeof(double) * ds.y.size());
+    }
+    return 0;
+}
+
+uint32_t fednl_cli_crc32(const void* buf, size_t size) { return fednl_cli::crc32_bytes(buf, size); }
 
 }  // extern "C"
