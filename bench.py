@@ -180,88 +180,142 @@ def mixed_precision_leg(rounds_c2, rounds_c3, warm=None):
     return out
 
 
-def batched_leg(count, rounds, warm=None):
+def _converged_round(eng):
+    """First round index with ||grad f||^2 <= TOL (None if not reached)."""
+    g2 = eng.trace()["grad_norm"] ** 2
+    hit = np.nonzero(g2 <= TOL)[0]
+    return int(hit[0]) if hit.size else None
+
+
+def batched_leg(count, max_rounds=300, chunk=10, modes=("fp64", "fp32", "bf16x2"), warm=None):
     """count independent dense-Gaussian problems (d~U[32,512], n~U[16,256],
-    m~U[100,600], TopK k=8d) stepped as one batch (one launch per kernel,
-    graph replay) against the same problems stepped one after another."""
+    m~U[100,600], TopK k=8d), per Hessian mode: stepped as one batch (one
+    launch per kernel per round, graph replay) in chunks of rounds, members
+    leaving once they reach ||grad f||^2 <= 1e-17, until all have or
+    max_rounds; solves/s counts every problem over the whole wall time.
+    Beside it, the same problems on their own streams: every engine's rounds
+    to tolerance enqueued first, then one wait."""
     from ufednl_amd import datagen
     from ufednl_amd.engine import Batch, Engine
     shapes = datagen.batched_shapes(count)
-    t0 = time.perf_counter()
-    engs = []
-    for i, (d, n, m) in enumerate(shapes):
-        prob = datagen.gaussian_problem(f"b{i}", n, m, d, seed=100 + i)
-        engs.append(Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False))
-        del prob
-    setup = time.perf_counter() - t0
-    # one after another (each engine's own stream, waited for in turn)
-    _warm(warm)
-    t0 = time.perf_counter()
-    for e in engs:
-        e.run(2, check=False)
-    t_seq = (time.perf_counter() - t0) / 2
-    b = Batch(engs, use_graph=True)
-    b.run(2)  # graph capture
-    _warm(warm)
-    t0 = time.perf_counter()
-    b.run(rounds)
-    t_batch = (time.perf_counter() - t0) / rounds
-    b.close()
-    ttol = []
-    for e in engs:
-        tr = e.trace()
-        g2 = tr["grad_norm"] ** 2
-        hit = np.nonzero(g2 <= TOL)[0]  # round index from x0 = 0
-        ttol.append(int(hit[0]) if hit.size else None)
-        e.close()
-    conv = [r for r in ttol if r is not None]
-    return dict(problems=count, shapes="d~U[32,512], n~U[16,256], m~U[100,600], N(0,1) features, TopK k=8d, FP64",
-                setup_s=setup, batch_round_ms=1e3 * t_batch, sequential_round_ms=1e3 * t_seq,
-                batch_speedup=t_seq / t_batch, problem_rounds_per_s=count / t_batch,
-                converged=len(conv), rounds_to_tol_p50=float(np.percentile(conv, 50)) if conv else None,
-                rounds_to_tol_max=max(conv) if conv else None,
-                solves_per_s=(len(conv) / (max(conv) * t_batch)) if conv else None)
+    pool = datagen.gaussian_pool(max(n * m * d for d, n, m in shapes), seed=40)
+    probs = [datagen.pooled_problem(f"b{i}", pool, 0, n, m, d, seed=100 + i) for i, (d, n, m) in enumerate(shapes)]
+    out = dict(problems=count, shapes="d~U[32,512], n~U[16,256], m~U[100,600], N(0,1) features (float32-exact), "
+                                      "TopK k=8d, lambda 1e-3, option b, to ||grad f||^2 <= 1e-17")
+    for mode in modes:
+        engs = [Engine.from_problem(p, compressor="topk", k_mult=8.0, hessian=mode, tracking=False) for p in probs]
+        _warm(warm)
+        # members that reached the tolerance leave the batch after each chunk
+        # of rounds (the batch is rebuilt, and its round graph captured again,
+        # over the rest)
+        live = list(engs)
+        t0 = time.perf_counter()
+        done = 0
+        member_rounds = 0
+        while live and done < max_rounds:
+            b = Batch(live, use_graph=True)
+            b.run(chunk)
+            b.close()
+            done += chunk
+            member_rounds += chunk * len(live)
+            live = [e for e in live if _converged_round(e) is None]
+        wall = time.perf_counter() - t0
+        r_tol = [_converged_round(e) for e in engs]
+        conv = [r for r in r_tol if r is not None]
+        ent = dict(batch_rounds=done, member_rounds=member_rounds, wall_s=wall,
+                   member_round_us=1e6 * wall / max(member_rounds, 1), converged=len(conv),
+                   unconverged=count - len(conv), solves_per_s=len(conv) / wall,
+                   rounds_to_tol_p50=float(np.percentile(conv, 50)) if conv else None,
+                   rounds_to_tol_max=max(conv) if conv else None)
+        for e in engs:
+            e.close()
+        if mode == "fp64" and conv:
+            # the same problems, each on its own stream, rounds to tolerance
+            # enqueued for all before one wait
+            engs = [Engine.from_problem(p, compressor="topk", k_mult=8.0, hessian=mode, tracking=False)
+                    for p in probs]
+            _warm(warm)
+            t0 = time.perf_counter()
+            for e, r in zip(engs, r_tol):
+                e.enqueue((r if r is not None else done) + 1)
+            for e in engs:
+                e.sync()
+            ws = time.perf_counter() - t0
+            ent.update(streams_wall_s=ws, streams_solves_per_s=len(conv) / ws, batch_speedup_vs_streams=ws / wall)
+            for e in engs:
+                e.close()
+        out[mode] = ent
+    return out
 
 
-def serving_leg(count, rate, budget_gb, max_rounds, seed=5, warm=None):
+def serving_leg(count, rate, budget_gb, max_rounds, modes=("fp64", "fp32", "bf16x2"), seed=5, warm=None):
     """Poisson arrivals (rate per second, fixed seed) of FedNL solves with
-    dense N(0,1) features, n in [32,512], m in [200,1000], d log-uniform in
-    [64,2048] clipped to the engine's d <= 512, TopK k=8d, each run to
-    ||grad f||^2 <= 1e-17; continuous batching under a device-memory budget."""
+    dense N(0,1) features (float32-exact), n in [32,512], m in [200,1000],
+    d log-uniform in [64,2048], TopK k=8d, each run to ||grad f||^2 <= 1e-17;
+    continuous batching under a device-memory budget, per Hessian mode.
+    Every request's features are a window of one shared pool of draws."""
     from ufednl_amd import datagen
     from ufednl_amd import engine as E
     rng = np.random.default_rng(seed)
     shapes = datagen.serving_shapes(count, seed=seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
-    reqs = []
     t0 = time.perf_counter()
-    for i, ((d, n, m), t) in enumerate(zip(shapes, arrivals)):
-        d = min(d, 512)
-        prob = datagen.gaussian_problem(f"r{i}", n, m, d, seed=1000 + i)
-        reqs.append(dict(arrival_s=float(t), problem=prob, max_rounds=max_rounds, tol=TOL,
-                         opts=dict(compressor="topk", k_mult=8.0)))
+    size = max(n * m * d for d, n, m in shapes) + (1 << 24)
+    pool = datagen.gaussian_pool(size, seed=seed)
+    probs = []
+    for i, (d, n, m) in enumerate(shapes):
+        off = int(rng.integers(0, size - n * m * d + 1))
+        probs.append(datagen.pooled_problem(f"r{i}", pool, off, n, m, d, seed=1000 + i))
     gen_s = time.perf_counter() - t0
-    _warm(warm)
-    res, _ = E.serve(reqs, hbm_budget_bytes=budget_gb * 1e9, rounds_per_step=4)
-    lat = np.array([r["finished_s"] - rq["arrival_s"] for r, rq in zip(res, reqs)])
-    ok = [r["status"] == 0 for r in res]
-    span = max(r["finished_s"] for r in res) - float(arrivals[0])
-    return dict(requests=count, arrival_rate_per_s=rate, budget_gb=budget_gb, data_gen_s=gen_s,
-                shapes="n~U[32,512], m~U[200,1000], d log-U[64,2048] clipped to 512, N(0,1), TopK k=8d, FP64",
-                converged=int(sum(ok)), solves_per_s=sum(ok) / span if span > 0 else None,
-                latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
-                rounds_to_tol_p50=float(np.median([r["rounds_to_tol"] for r in res if r["status"] == 0]))
-                if any(ok) else None)
+    out = dict(requests=count, arrival_rate_per_s=rate, budget_gb=budget_gb, data_gen_s=gen_s,
+               shapes="n~U[32,512], m~U[200,1000], d log-U[64,2048], N(0,1) (float32-exact), TopK k=8d, "
+                      "lambda 1e-3, to ||grad f||^2 <= 1e-17",
+               d_max=max(d for d, _, _ in shapes))
+    for mode in modes:
+        reqs = [dict(arrival_s=float(t), problem=p, max_rounds=max_rounds, tol=TOL,
+                     opts=dict(compressor="topk", k_mult=8.0, hessian=mode)) for p, t in zip(probs, arrivals)]
+        _warm(warm)
+        res, _ = E.serve(reqs, hbm_budget_bytes=budget_gb * 1e9, rounds_per_step=4)
+        lat = np.array([r["finished_s"] - rq["arrival_s"] for r, rq in zip(res, reqs)])
+        ok = [r["status"] == 0 for r in res]
+        span = max(r["finished_s"] for r in res) - float(arrivals[0])
+        out[mode] = dict(converged=int(sum(ok)), unconverged_d=sorted(p.d for p, o in zip(probs, ok) if not o),
+                         solves_per_s=sum(ok) / span if span > 0 else None, span_s=span,
+                         latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
+                         rounds_to_tol_p50=float(np.median([r["rounds_to_tol"] for r in res if r["status"] == 0]))
+                         if any(ok) else None)
+        print(f"serving {mode}: {out[mode]}", file=sys.stderr, flush=True)
+    base = out.get("fp64", {}).get("solves_per_s")
+    for mode in modes:
+        if mode != "fp64" and base:
+            out[mode]["solves_per_s_vs_fp64"] = out[mode]["solves_per_s"] / base
+    return out
 
 
-def cpu_baseline(prob, rounds, cores):
+CPU_BUILD = ("oracle/_ref: the reference's own sources compiled by oracle/Makefile.ref with g++ -O2 -mavx2 -mfma "
+             "-ffp-contract=off (the reference's Release build adds -O3 -march=native -flto); client work on "
+             "persistent worker threads woken by condition variables (dopt's workers spin and yield); master "
+             "consumes clients in order (--debug forcesequpdate)")
+
+
+def cpu_baseline(prob, rounds, cores, tol_rounds=None):
+    """The reference's building blocks (oracle/_ref) on the host's cores: the
+    round rate over `rounds` rounds and, with tol_rounds, the time to
+    ||grad f||^2 <= 1e-17 from x0 = 0 read from the round timestamps."""
     import oracle as O
     kind = "reference" if O.ref_available() else "port"
     t0 = time.perf_counter()
+    extra = {}
     if kind == "reference":
         out = O.ref_fednl_run(prob, "topk", 8.0, rounds=rounds, workers=cores, tracking=False)
         sec = out["seconds"]
         per_round = (sec[-1] - sec[0]) / max(rounds - 1, 1) if rounds > 1 else sec[0]
+        if tol_rounds:
+            run = O.ref_fednl_run(prob, "topk", 8.0, rounds=tol_rounds, workers=cores, tracking=False)
+            hit = np.nonzero(run["grad_norm"] ** 2 <= TOL)[0]
+            extra = dict(time_to_tol_s=float(run["seconds"][hit[0]]) if hit.size else None,
+                         rounds_to_tol=int(hit[0]) if hit.size else None, tol_rounds_run=tol_rounds)
+        extra["build"] = CPU_BUILD
     else:
         f = O.FedNL(prob, "topk", k_mult=8.0)
         ts = []
@@ -272,9 +326,12 @@ def cpu_baseline(prob, rounds, cores):
         per_round = float(np.mean(ts[1:] if len(ts) > 1 else ts))
         cores = 1
     total = time.perf_counter() - t0
-    return dict(value=1.0 / per_round, unit="rounds/s", cores=cores, kind=kind,
-                sample=f"{rounds} rounds of the same workload (setup included in {total:.1f}s wall), "
-                       f"rate from the round timestamps")
+    res = dict(value=1.0 / per_round, unit="rounds/s", cores=cores, kind=kind,
+               sample=f"{rounds} rounds of the same workload for the rate"
+                      + (f", then {tol_rounds} rounds from x0 for the time to tolerance" if tol_rounds else "")
+                      + f" ({total:.1f}s wall with setup); rates and times from the round timestamps")
+    res.update(extra)
+    return res
 
 
 def main():
@@ -288,11 +345,15 @@ def main():
     ap.add_argument("--tol-rounds", type=int, default=400, help="round budget of the time-to-tolerance solve")
     ap.add_argument("--cpu-rounds", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and batched legs")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline, time-to-tolerance and CPU legs")
+    ap.add_argument("--legs", default="mixed,batched,serving",
+                    help="extra legs to run: any of mixed, batched, serving")
     ap.add_argument("--batch-problems", type=int, default=64)
-    ap.add_argument("--batch-rounds", type=int, default=60)
-    ap.add_argument("--serve-requests", type=int, default=16)
-    ap.add_argument("--serve-rate", type=float, default=8.0)
+    ap.add_argument("--batch-max-rounds", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="requests per second (Poisson)")
+    ap.add_argument("--serve-budget-gb", type=float, default=160.0)
+    ap.add_argument("--serve-max-rounds", type=int, default=400)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -362,17 +423,31 @@ def main():
         pk_ = PEAK_FP64_MFMA_TFLOPS * 1e12 if b_ == "mfma" else PEAK_HBM_GBS * 1e9
         all_roofs[name_] = dict(bound=b_, us=sec_ * 1e6, frac=w_ / sec_ / pk_, traffic=traffic.get(name_))
 
+    # the metric's time to tolerance, whatever --steps is: a fresh engine from
+    # x0 = 0, rounds enqueued at once, the first round with ||grad f||^2 <=
+    # 1e-17 read from the device timestamps
+    _warm(eng_w)
+    eng_t = Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False)
+    r_tol2, t_tol2, rate_tol = time_to_tol(eng_t, args.tol_rounds)
+    eng_t.close()
+    if t_tol is None:
+        r_tol, t_tol = r_tol2, t_tol2
+
+    legs = set() if args.no_extra else {x.strip() for x in args.legs.split(",") if x.strip()}
     extra = {}
-    if not args.no_extra:
+    if "mixed" in legs:
         extra["mixed_precision"] = mixed_precision_leg(args.tol_rounds, 500, warm=eng_w)
-        extra["batched"] = batched_leg(args.batch_problems, args.batch_rounds, warm=eng_w)
-        extra["serving"] = serving_leg(args.serve_requests, args.serve_rate, 64.0, 400, warm=eng_w)
+    if "batched" in legs:
+        extra["batched"] = batched_leg(args.batch_problems, args.batch_max_rounds, warm=eng_w)
+    if "serving" in legs:
+        extra["serving"] = serving_leg(args.serve_requests, args.serve_rate, args.serve_budget_gb,
+                                       args.serve_max_rounds, warm=eng_w)
 
     cpu = None
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
         try:
-            cpu = cpu_baseline(prob, args.cpu_rounds, cores)
+            cpu = cpu_baseline(prob, args.cpu_rounds, cores, (r_tol + 5) if r_tol is not None else None)
         except Exception as ex:  # report, do not fail the GPU measurement
             cpu = dict(value=None, unit="rounds/s", cores=cores, kind="unavailable", sample=str(ex))
 
@@ -394,6 +469,8 @@ def main():
                    "parallelism": "all clients on one GPU"},
         "time_to_tol_s": t_tol,
         "rounds_to_tol": r_tol,
+        "time_to_tol_leg": {"rounds_to_tol": r_tol2, "time_to_tol_s": t_tol2, "rounds_per_s": rate_tol,
+                            "rounds": args.tol_rounds},
         "tol": TOL,
         "final_grad_norm_sq": final_g2,
         "kernel_ms_per_round": {k_: v[0] / max(v[1], 1) for k_, v in calib.items()},

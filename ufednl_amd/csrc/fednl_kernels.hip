@@ -104,7 +104,12 @@ static_assert(kScanGroup % 2 == 0, "the scan tests chunk pairs");
 constexpr int kMaxClients = 1024;  // master scatter client table
 // per-round trace record: ||grad f||, f, ||x||, factorization flag,
 // line-search trials, participating clients, Hessian items they sent
-constexpr int kTraceStride = 7;
+constexpr int kTraceStride = 9;
+// per-round trace record: ||grad f||, f, ||x||, factorization flag,
+// line-search trials, participating clients, items sent, then the device
+// wall clock (ticks) when the round's first kernel started and when its last
+// kernel finished
+constexpr int kTraceT0 = 7, kTraceT1 = 8;
 constexpr int kMaxLineSearchSteps = 64;  // trial launches enqueued per round
 // master state (mst): line-search scalars, FedNL-PP's running Lk average,
 // then x_k (d) from kLsHead on
@@ -324,6 +329,12 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
     FEDNL_BLOCK();
     if (kTrial && (!PD.ls || PD.mst[LS_DONE] != 0.0))
         return;
+    if (!kTrial && blk == 0 && threadIdx.x == 0 && PD.round_ctr) {
+        // the round starts (the first kernel of every round)
+        const int r = *PD.round_ctr;
+        if (r < PD.trace_cap)
+            PD.trace[(size_t)r * kTraceStride + kTraceT0] = (double)wall_clock64();
+    }
     const gmem<double>* __restrict__ A = G(PD.A);
     const gmem<double>* __restrict__ x = G(PD.x);
     const int m = PD.m, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad;
@@ -2503,6 +2514,12 @@ __global__ __launch_bounds__(256) void k_ls_check(FEDNL_KARGS)
     (void)blk;
     if (!PD.ls)
         return;
+    if (threadIdx.x == 0) {
+        // the last trial launch of the round ends it
+        const int r = *PD.round_ctr - 1;
+        if (r >= 0 && r < PD.trace_cap)
+            PD.trace[(size_t)r * kTraceStride + kTraceT1] = (double)wall_clock64();
+    }
     gmem<double>* __restrict__ st = G(PD.mst);
     if (st[LS_DONE] != 0.0)
         return;
@@ -3505,8 +3522,10 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
                 dx_out[i] = 0.0;
         }
         if (tid == 0) {
-            if (rnd < trace_cap)
+            if (rnd < trace_cap) {
                 trace[(size_t)rnd * kTraceStride + 3] = expired ? -1.0 : 0.0;
+                trace[(size_t)rnd * kTraceStride + kTraceT1] = (double)wall_clock64();
+            }
             *round_ctr = rnd + 1;
             __hip_atomic_store(sync, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -3653,8 +3672,10 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
             x[i] = v;  // FedNL-PP: the solution is the next iterate
     }
     if (tid == 0) {
-        if (rnd < trace_cap)
+        if (rnd < trace_cap) {
             trace[(size_t)rnd * kTraceStride + 3] = 1.0;
+            trace[(size_t)rnd * kTraceStride + kTraceT1] = (double)wall_clock64();
+        }
         *round_ctr = rnd + 1;
         __hip_atomic_store(sync, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -3737,10 +3758,15 @@ struct Engine {
     int trace_cap = 0;
     int rounds_enqueued = 0;
 
-    hipEvent_t ev_start = nullptr;
-    std::vector<hipEvent_t> round_ev;
+    double wclk_hz = 1e8;  // device wall clock rate (the trace's round timestamps)
     bool timing = false;
     unsigned timing_mask = ~0u;
+    // FEDNL_ROUND_GRAPH=1: one round captured into a HIP graph and replayed
+    // (unless kernel timing is on). Off by default: on MI355X the replay
+    // measured 3% below plain launches, which already run back to back
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     int margin_rows = 4;           // k_margin rows per wave, picked at creation    // KernelId bits timed when timing is on
     struct Pending { int kid; hipEvent_t a, b; };
     std::vector<Pending> pending;
@@ -4132,6 +4158,10 @@ static void free_engine(Engine* e)
 {
     if (!e)
         return;
+    if (e->gexec)
+        hipGraphExecDestroy(e->gexec);
+    if (e->graph)
+        hipGraphDestroy(e->graph);
     void* ptrs[] = {e->A, e->x, e->Dw, e->gw, e->nlog, e->learned, e->delta, e->grad_c, e->part, e->lk, e->sel_bm,
                     e->sel, e->pos, e->idx, e->rng, e->lek_state, e->lek_popped, e->seq_start, e->sel_cnt,
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
@@ -4146,9 +4176,6 @@ static void free_engine(Engine* e)
         hipEventDestroy(p.b);
     }
     for (auto v : e->ev_pool) hipEventDestroy(v);
-    for (auto v : e->round_ev) hipEventDestroy(v);
-    if (e->ev_start)
-        hipEventDestroy(e->ev_start);
     if (e->st)
         hipStreamDestroy(e->st);
     delete e;
@@ -4303,6 +4330,17 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         set_err("hipStreamCreate");
         delete e;
         return FEDNL_E_DEVICE;
+    }
+    {
+        const char* ge = getenv("FEDNL_ROUND_GRAPH");
+        if (ge && ge[0] == '1')
+            e->use_graph = true;
+    }
+    {
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+            e->wclk_hz = 1e3 * khz;
     }
     const size_t nA = (size_t)n * e->m_pad * e->d_pad;
     ALLOC(e->A, nA);
@@ -4501,16 +4539,31 @@ static int enqueue_rounds(Engine* e, int rounds)
         return rc;
     if ((rc = sync_desc(e)) != FEDNL_OK)
         return rc;
-    if (!e->ev_start) {
-        HIP_OK(hipEventCreate(&e->ev_start));
-        HIP_OK(hipEventRecord(e->ev_start, e->st));
+    const bool graph = e->use_graph && !e->timing;
+    if (graph && !e->gexec) {
+        // every kernel of a round reads the problem through e->pdev, whose
+        // address and the launch shapes never change: one capture serves
+        // all rounds
+        HIP_OK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+        launch_round(e);
+        hipError_t ce = hipStreamEndCapture(e->st, &e->graph);
+        if (ce == hipSuccess)
+            ce = hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) {
+            e->use_graph = false;
+            if (e->graph) {
+                hipGraphDestroy(e->graph);
+                e->graph = nullptr;
+            }
+            e->gexec = nullptr;
+            hipGetLastError();
+        }
     }
     for (int r = 0; r < rounds; ++r) {
-        launch_round(e);
-        hipEvent_t ev = nullptr;
-        HIP_OK(hipEventCreate(&ev));
-        HIP_OK(hipEventRecord(ev, e->st));
-        e->round_ev.push_back(ev);
+        if (graph && e->gexec)
+            HIP_OK(hipGraphLaunch(e->gexec, e->st));
+        else
+            launch_round(e);
         e->rounds_enqueued += 1;
     }
     HIP_OK(hipGetLastError());
@@ -4697,10 +4750,6 @@ static int batch_enqueue(Batch* b, int rounds)
         int rc = ensure_trace(e, e->rounds_enqueued + rounds + 1);
         if (rc != FEDNL_OK)
             return rc;
-        if (!e->ev_start) {
-            HIP_OK(hipEventCreate(&e->ev_start));
-            HIP_OK(hipEventRecord(e->ev_start, b->st));
-        }
     }
     int rc = batch_sync_desc(b);
     if (rc != FEDNL_OK)
@@ -4727,13 +4776,8 @@ static int batch_enqueue(Batch* b, int rounds)
             HIP_OK(hipGraphLaunch(b->gexec, b->st));
         else
             batch_launch_round(b);
-        for (Engine* e : b->m) {
-            hipEvent_t ev = nullptr;
-            HIP_OK(hipEventCreate(&ev));
-            HIP_OK(hipEventRecord(ev, b->st));
-            e->round_ev.push_back(ev);
+        for (Engine* e : b->m)
             e->rounds_enqueued += 1;
-        }
         b->rounds += 1;
     }
     HIP_OK(hipGetLastError());
@@ -5186,12 +5230,8 @@ int fednl_trace(void* handle, double* grad_norm, double* fval, double* x_norm, d
         if (fval) fval[r] = tr[(size_t)kTraceStride * r + 1];
         if (x_norm) x_norm[r] = tr[(size_t)kTraceStride * r + 2];
         if (chol_ok) chol_ok[r] = tr[(size_t)kTraceStride * r + 3] == 1.0 ? 1 : 0;
-        if (seconds) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, e->ev_start, e->round_ev[r]) != hipSuccess)
-                ms = -1.f;
-            seconds[r] = ms * 1e-3;
-        }
+        if (seconds)  // from the start of round 0 to the end of round r (device wall clock)
+            seconds[r] = (tr[(size_t)kTraceStride * r + kTraceT1] - tr[kTraceT0]) / e->wclk_hz;
     }
     return FEDNL_OK;
 }

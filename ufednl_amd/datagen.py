@@ -88,6 +88,30 @@ def gaussian_problem(name, n, m, d, lam=1e-3, seed=0, target_pos=0.5) -> Problem
                    np.ascontiguousarray(y.reshape(n, m)), lam, w)
 
 
+def gaussian_pool(size, seed=0) -> np.ndarray:
+    """``size`` N(0,1) draws rounded to float32, as float64: one buffer many
+    serving requests take their features from (a request is a window of it, so
+    each request's entries are i.i.d. N(0,1); requests may share entries)."""
+    rng = np.random.default_rng(seed)
+    out = np.empty(size, dtype=np.float64)
+    step = 1 << 26
+    for a in range(0, size, step):
+        b = min(size, a + step)
+        out[a:b] = rng.standard_normal(b - a, dtype=np.float32)
+    return out
+
+
+def pooled_problem(name, pool, offset, n, m, d, lam=1e-3, seed=0, target_pos=0.5) -> Problem:
+    """A dense N(0,1) problem whose features are the window
+    pool[offset : offset + n*m*d] (no copy); labels from a planted model."""
+    rng = np.random.default_rng(seed)
+    N = n * m
+    a = pool[offset:offset + N * d].reshape(N, d)
+    w = rng.standard_normal(d) / math.sqrt(d)
+    y, _ = _planted_labels(rng, a, w, target_pos)
+    return Problem(name, a.reshape(n, m, d), np.ascontiguousarray(y.reshape(n, m)), lam, w)
+
+
 # The five workloads the engine is measured on. Seeds are fixed.
 def config1(seed=1):
     """50 clients x 221 samples, 67 Bernoulli(0.5) features + intercept (d=68)."""
