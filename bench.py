@@ -30,6 +30,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-to-‖∇f‖²≤1e-17 & rounds/s vs plain FP64 GPU; solves/s + p99 latency"
 PEAK_FP64_MFMA_TFLOPS = 78.6
+PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_BF16_MFMA_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 TOL = 1e-17
 # HBM bytes per launch of each kernel on this workload (rocprofv3 FETCH_SIZE /
@@ -54,6 +56,15 @@ def algorithmic_work(name, n, m, d, k, S):
     if name == "k_hessian_f64":
         # upper triangle incl. diagonal of A_c^T D_c A_c: m FMAs per entry
         return "mfma", float(n) * m * d * (d + 1)
+    if name == "k_hessian_f32":
+        # the same products in FP32
+        return "mfma", float(n) * m * d * (d + 1)
+    if name == "k_hessian_bf16":
+        # three BF16 products per term (hi*hi, hi*lo, lo*hi)
+        return "mfma", 3.0 * n * m * d * (d + 1)
+    if name == "k_grad_f64":
+        # features once, gradient weights, client gradients written
+        return "hbm", float(n) * m * (8.0 * d + 8.0) + float(n) * d * 8.0
     if name in ("k_newton", "k_newton_mc"):
         # Cholesky d^3/3, two triangular solves 2 d^2
         return "mfma", d ** 3 / 3.0 + 2.0 * d * d
@@ -73,6 +84,14 @@ def algorithmic_work(name, n, m, d, k, S):
         # (W written), client gradients read, the average written
         return "hbm", float(n) * (S / 8.0 + 8.0 * k) + 24.0 * S + float(n) * d * 8.0 + 8.0 * d
     return "hbm", 8.0 * S
+
+
+def peak_of(name, bound):
+    """Published MI355X peak for a kernel's bound, in TFLOP/s or GB/s."""
+    if bound == "hbm":
+        return PEAK_HBM_GBS
+    return {"k_hessian_f32": PEAK_FP32_MFMA_TFLOPS,
+            "k_hessian_bf16": PEAK_BF16_MFMA_TFLOPS}.get(name, PEAK_FP64_MFMA_TFLOPS)
 
 
 def load_traffic(path):
@@ -98,9 +117,10 @@ def kernel_roofline(eng, times, traffic):
     avg_s = ms * 1e-3 / max(launches, 1)
     bound, work = algorithmic_work(name, n, m, d, k, S)
     if bound == "mfma":
-        ach, peak, unit = work / avg_s / 1e12, PEAK_FP64_MFMA_TFLOPS, "TFLOP/s"
+        ach, unit = work / avg_s / 1e12, "TFLOP/s"
     else:
-        ach, peak, unit = work / avg_s / 1e9, PEAK_HBM_GBS, "GB/s"
+        ach, unit = work / avg_s / 1e9, "GB/s"
+    peak = peak_of(name, bound)
     return name, dict(bound=bound, achieved=ach, peak=peak, unit=unit, frac=ach / peak,
                       traffic=traffic.get(name)), avg_s
 
@@ -353,7 +373,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="requests per second (Poisson)")
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
-    ap.add_argument("--serve-max-rounds", type=int, default=400)
+    ap.add_argument("--serve-max-rounds", type=int, default=1000)
+    ap.add_argument("--hessian", default="fp64", choices=("fp64", "fp32", "bf16x2"),
+                    help="Hessian form of the headline engines (fp64: the metric's form; the others for profiling)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -384,8 +406,8 @@ def main():
     # clocks; the same engine then gives the per-kernel table (every kernel
     # timed by HIP events on the engine stream) and stays the warm-up source
     # of the other legs
-    eng_w = Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False)
-    eng = Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False)
+    eng_w = Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False, hessian=args.hessian)
+    eng = Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False, hessian=args.hessian)
     eng_w.run(args.warmup, check=False)
     eng_w.kernel_timing(True)
     eng_w.enqueue(args.calib_rounds)
@@ -420,7 +442,7 @@ def main():
     for name_, (ms_, nl_) in calib.items():
         b_, w_ = algorithmic_work(name_, eng.n, eng.m, eng.d, eng.k, eng.S)
         sec_ = ms_ * 1e-3 / max(nl_, 1)
-        pk_ = PEAK_FP64_MFMA_TFLOPS * 1e12 if b_ == "mfma" else PEAK_HBM_GBS * 1e9
+        pk_ = peak_of(name_, b_) * (1e12 if b_ == "mfma" else 1e9)
         all_roofs[name_] = dict(bound=b_, us=sec_ * 1e6, frac=w_ / sec_ / pk_, traffic=traffic.get(name_))
 
     # the metric's time to tolerance, whatever --steps is: a fresh engine from
@@ -462,7 +484,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.hessian == "fp64" else f"f64 ({args.hessian} Hessian contraction)",
         "data": "synthetic (seeded planted-logistic binary features, generated in-process)",
         "config": {"workload": "fednl_topk8d_fp64_n142_m350_d300", "clients": n, "samples_per_client": m,
                    "d": d, "k": eng.k, "compressor": "topk", "lambda": prob.lam, "option": "b", "alpha": eng.alpha,

@@ -72,16 +72,28 @@ def test_hessian_error_per_client(oracle, mode, tol):
 
 
 @pytest.mark.parametrize("mode", ["fp32", "bf16x2"])
-def test_gradient_path_stays_fp64(oracle, mode):
+@pytest.mark.parametrize("shape", ["config3", "wide"])
+def test_gradient_path_stays_fp64(oracle, mode, shape):
+    """The client gradients come from the margin pass's 16-row partials, added
+    up by the Hessian kernels' diagonal tiles: d = 124 keeps the rows in
+    registers, d = 601 (d_pad 640) reads them again; three rounds, so that the
+    gradient is also checked at x != 0."""
     from ufednl_amd import datagen
     from ufednl_amd.engine import Engine
-    prob = datagen.config3()
+    if shape == "config3":
+        prob = datagen.config3()
+    else:
+        prob = datagen.small_problem(n=3, m=70, d_feat=600, density=0.05, seed=9)
     eng = Engine.from_problem(prob, compressor="seqk", k_mult=8.0, hessian=mode, tracking=False)
-    eng.run(1)
-    g_gpu = eng.last_round()["grad"]
-    f = oracle.FedNL(prob, "seqk", k_mult=8.0)
-    g_orc = f.round(tracking=False)["grad"]
-    np.testing.assert_allclose(g_gpu, g_orc, rtol=1e-12, atol=1e-16)
+    n = prob.features.shape[0]
+    for r in range(3):
+        # the gradient at the engine's own iterate, from the oracle's FP64
+        # client evaluations (logistic_regression.h:148-161), averaged
+        x = eng.x()
+        g_orc = sum(oracle.client_eval(prob.features[c], prob.labels[c], prob.lam, x)["grad"] for c in range(n)) / n
+        eng.run(1)
+        g_gpu = eng.last_round()["grad"]
+        np.testing.assert_allclose(g_gpu, g_orc, rtol=1e-12, atol=1e-16, err_msg=f"round {r}")
     eng.close()
 
 

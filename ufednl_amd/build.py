@@ -3,6 +3,8 @@
   ufednl_amd/libfednl_hip.so   HIP kernels + host engine + C ABI (hipcc, gfx950)
   ufednl_amd/bin_fednl_local   drop-in CLI linked against it
   ufednl_amd/bin_fednl_local_pp  the FedNL-PP CLI, linked the same way
+  ufednl_amd/libfednl_hip_probe.so  (--probe only) the library with device-clock
+      phase stamps, read by python -m ufednl_amd.phases
 
 Run: python -m ufednl_amd.build  (or __graft_entry__.build()).
 """
@@ -20,6 +22,7 @@ INC = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libfednl_hip.so")
 CLI = os.path.join(HERE, "bin_fednl_local")
 CLI_PP = os.path.join(HERE, "bin_fednl_local_pp")
+PROBE_LIB = os.path.join(HERE, "libfednl_hip_probe.so")
 
 ARCH = os.environ.get("FEDNL_OFFLOAD_ARCH", "gfx950")
 
@@ -69,7 +72,18 @@ def build_oracle():
         run(["make", "-s", "-C", od, "-f", "Makefile.ref"])
 
 
-def main():
+def build_probe():
+    """The diagnostic library with device-clock phase stamps (-DFEDNL_PROBE),
+    read by ufednl_amd.phases; never loaded by the product path."""
+    srcs = [os.path.join(CSRC, "fednl_kernels.hip"), os.path.join(CSRC, "fednl_cli.cpp")]
+    run([hipcc(), *HIP_FLAGS, "-DFEDNL_PROBE", "-shared", "-o", PROBE_LIB, *srcs])
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    if "--probe" in argv:
+        build_probe()
+        return
     build_lib()
     build_cli()
     build_oracle()

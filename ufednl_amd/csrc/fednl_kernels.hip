@@ -277,6 +277,10 @@ struct ProblemDev {
     double *x, *Dw, *gw, *nlog, *learned, *delta, *grad_c, *part, *lk, *vals, *H, *g_avg, *shift, *W, *dx, *trace,
         *lek_p;
     double* mst;        // master state (MstSlot header, then x_k)
+    // reduced-precision Hessian modes: per-16-row partial client gradients
+    // from the margin pass (n x m_pad/16 x d_pad), summed by the Hessian
+    // kernels' diagonal tiles; null: k_grad_f64 computes grad_c
+    double* gpart;
     // FedNL-PP: selected flags, client sampler state, per-client direction
     // (n x d_pad) and Lk, this round's messages, H_c(x) (option b), full gradient
     int* pp_sel;
@@ -315,16 +319,25 @@ struct ProblemDev {
 __device__ inline bool pp_idle(const ProblemDev& P, int c) { return P.pp && P.pp_sel[c] == 0; }
 
 // ---------------------------------------------------------------------------
-// k_margin: one wave per sample row. kTrial: a line-search trial point (only
-// the -log sigma partials for f; nothing once the line search has stopped)
+// k_margin: kRows sample rows per wave. kTrial: a line-search trial point
+// (only the -log sigma partials for f; nothing once the line search has
+// stopped). kGrad (reduced-precision Hessian modes): the workgroup's 16 rows
+// also give their part of the client gradient, sum_s A_c[s] g_s over the
+// rows (logistic_regression.h:148-161), from the rows the margins were read
+// from (held in registers while d_pad <= 64 kGradRegs), written to
+// gpart[c][16-row block][d_pad]; the Hessian kernels' diagonal tiles add the
+// blocks up, so A is read once per round in FP64. Dynamic LDS: x, then
+// (kGrad) the four waves' column sums, 8 d_pad doubles each.
 // ---------------------------------------------------------------------------
-template <bool kTrial, int kRows>
+constexpr int kGradRegs = 8;
+template <bool kTrial, int kRows, bool kGrad = false>
 __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
 {
     // kRows rows per wave, 4 kRows per workgroup (m_pad is a multiple of 32);
     // the -log sigma partials stay per 16 rows, whatever kRows is, so that f
     // is summed in the same order
     static_assert(kRows == 4 || kRows == 8, "16 or 32 rows per workgroup");
+    static_assert(!kGrad || (kRows == 4 && !kTrial), "gradient partials per 16-row workgroup");
     constexpr int kRB = 4 * kRows;
     FEDNL_BLOCK();
     if (kTrial && (!PD.ls || PD.mst[LS_DONE] != 0.0))
@@ -358,18 +371,42 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
     double acc[kRows];
 #pragma unroll
     for (int q = 0; q < kRows; ++q) acc[q] = 0.0;
-#pragma unroll 2
-    for (int i = lane; i < d_pad; i += 64) {
-        const double xi = xs[i];
+    // kGrad: the rows' values stay in registers for the gradient part (the
+    // same sums in the same order as the loop below)
+    double av[kRows][kGrad ? kGradRegs : 1];
+    bool in_regs = false;
+    if constexpr (kGrad) {
+        in_regs = d_pad <= 64 * kGradRegs;
+        if (in_regs) {
 #pragma unroll
-        for (int q = 0; q < kRows; ++q)
-            acc[q] = fma(row0[(size_t)q * d_pad + i], xi, acc[q]);
+            for (int j = 0; j < kGradRegs; ++j) {
+                const int i = lane + 64 * j;
+                if (i < d_pad) {
+                    const double xi = xs[i];
+#pragma unroll
+                    for (int q = 0; q < kRows; ++q) {
+                        av[q][j] = row0[(size_t)q * d_pad + i];
+                        acc[q] = fma(av[q][j], xi, acc[q]);
+                    }
+                }
+            }
+        }
+    }
+    if (!in_regs) {
+#pragma unroll 2
+        for (int i = lane; i < d_pad; i += 64) {
+            const double xi = xs[i];
+#pragma unroll
+            for (int q = 0; q < kRows; ++q)
+                acc[q] = fma(row0[(size_t)q * d_pad + i], xi, acc[q]);
+        }
     }
 #pragma unroll
     for (int q = 0; q < kRows; ++q)
         for (int off = 32; off > 0; off >>= 1)
             acc[q] += __shfl_xor(acc[q], off);
     // every lane holds the sums; lane q finishes row s0 + q
+    double gq = 0.0;
     if (lane < kRows) {
         double t = acc[0];
 #pragma unroll
@@ -384,7 +421,8 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
                 const double z1 = 1.0 - sg;
                 const double z2 = z1 * sg;
                 Dw[o] = inv_m * z2;
-                gw[o] = z1 * (-inv_m);
+                gq = z1 * (-inv_m);
+                gw[o] = gq;
             }
             nl[wave * kRows + lane] = -log(sg);
         } else {
@@ -395,6 +433,34 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
             nl[wave * kRows + lane] = 0.0;
         }
     }
+    if constexpr (kGrad) {
+        // this workgroup's 16 rows: per column, each wave's four rows in
+        // order into gl[wave]; after the barrier, the waves in the order
+        // (0 + 1) + (2 + 3)
+        double g[kRows];
+#pragma unroll
+        for (int q = 0; q < kRows; ++q) g[q] = __shfl(gq, q);
+        double* gl = xs + (size_t)(wave + 1) * d_pad;
+        if (in_regs) {
+#pragma unroll
+            for (int j = 0; j < kGradRegs; ++j) {
+                const int i = lane + 64 * j;
+                if (i < d_pad) {
+                    double v = g[0] * av[0][j];
+#pragma unroll
+                    for (int q = 1; q < kRows; ++q) v = fma(g[q], av[q][j], v);
+                    gl[i] = v;
+                }
+            }
+        } else {
+            for (int i = lane; i < d_pad; i += 64) {
+                double v = g[0] * row0[i];
+#pragma unroll
+                for (int q = 1; q < kRows; ++q) v = fma(g[q], row0[(size_t)q * d_pad + i], v);
+                gl[i] = v;
+            }
+        }
+    }
     __syncthreads();
     if (threadIdx.x < kRB / 16) {
         // per-16-row partial of sum_s -log sigma_s (samples in order)
@@ -402,6 +468,12 @@ __global__ __launch_bounds__(256) void k_margin(FEDNL_KARGS)
         double sacc = 0.0;
         for (int q = 0; q < 16; ++q) sacc = sacc + nl[16 * h + q];
         nlog_part[(size_t)c * (m_pad / 16) + bx * (kRB / 16) + h] = sacc;
+    }
+    if constexpr (kGrad) {
+        const double* gl = xs + d_pad;
+        gmem<double>* __restrict__ gp = G(PD.gpart) + ((size_t)c * gxb + bx) * d_pad;
+        for (int i = threadIdx.x; i < d_pad; i += blockDim.x)
+            gp[i] = (gl[i] + gl[d_pad + i]) + (gl[2 * d_pad + i] + gl[3 * d_pad + i]);
     }
 }
 
@@ -512,7 +584,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // k_hessian_f32: the same contraction with v_mfma_f32_16x16x4f32 on an FP32
 // copy of A (fp32 D * A, FP32 accumulation). Only the contraction is FP32:
 // the result is widened to FP64 before lambda, Delta and Lk. The gradient is
-// computed in FP64 by k_grad_f64. Lane layout of the output: lane (l16, kq),
+// FP64: the diagonal tiles add up the margin pass's partials (or, under
+// FedNL-PP, k_grad_f64 computes it). Lane layout of the output: lane (l16, kq),
 // item r holds D[m = 4 kq + r][n = l16] (pinned by the layout probe test).
 // ---------------------------------------------------------------------------
 template <bool kInit>
@@ -606,8 +679,17 @@ __global__ __launch_bounds__(256) void k_hessian_f32(FEDNL_KARGS, double* __rest
                 iv[q] = i0 + wi * 32 + it * 16 + l16;
                 jv[q] = j0 + wj * 32 + jt * 16 + 4 * kq + r;
             }
+    // the client gradient from the margin pass's 16-row partials (diagonal
+    // tiles; wave w adds blocks w, w + 4, ... in order)
+    const bool fin_grad = !kInit && PD.gpart != nullptr && i0 == j0;
+    double gacc = 0.0;
+    if (fin_grad) {
+        const int nb = m_pad / 16;
+        const gmem<double>* gp = G(PD.gpart) + (size_t)c * nb * d_pad + i0 + lane;
+        for (int b = wave; b < nb; b += 4) gacc = gacc + gp[(size_t)b * d_pad];
+    }
     hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, G(PD.pp_h), part,
-                         nullptr, false, 0.0, i0, red, gred);
+                         G(PD.grad_c), fin_grad, gacc, i0, red, gred);
 }
 
 // ---------------------------------------------------------------------------
@@ -736,15 +818,25 @@ __global__ __launch_bounds__(256) void k_hessian_bf16(FEDNL_KARGS, double* __res
                 iv[q] = i0 + wi * 32 + it * 16 + l16;
                 jv[q] = j0 + wj * 32 + jt * 16 + 4 * kq + r;
             }
+    // the client gradient from the margin pass's 16-row partials (diagonal
+    // tiles; wave w adds blocks w, w + 4, ... in order)
+    const bool fin_grad = !kInit && PD.gpart != nullptr && i0 == j0;
+    double gacc = 0.0;
+    if (fin_grad) {
+        const int nb = m_pad / 16;
+        const gmem<double>* gp = G(PD.gpart) + (size_t)c * nb * d_pad + i0 + lane;
+        for (int b = wave; b < nb; b += 4) gacc = gacc + gp[(size_t)b * d_pad];
+    }
     hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, G(PD.pp_h), part,
-                         nullptr, false, 0.0, i0, red, gred);
+                         G(PD.grad_c), fin_grad, gacc, i0, red, gred);
 }
 
 // ---------------------------------------------------------------------------
 // k_grad_f64: client gradients lambda x + sum_s A_c[s] g_s in FP64
-// (logistic_regression.h:148-161) for the reduced-precision Hessian modes.
-// Block (column group of 64, client); four sample phases per column, summed
-// in a fixed order.
+// (logistic_regression.h:148-161) for the reduced-precision Hessian modes
+// under FedNL-PP (every client's gradient, also for clients that sit the
+// round out). Block (column group of 64, client); four sample phases per
+// column, summed in a fixed order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_grad_f64(FEDNL_KARGS)
 {
@@ -782,26 +874,45 @@ __global__ __launch_bounds__(256) void k_grad_f64(FEDNL_KARGS)
     }
 }
 
-// FP32 copy of A (same layout) and the transposed BF16 hi/lo pair
-__global__ void k_convert_a(const double* __restrict__ A, int n, int m_pad, int d_pad, float* __restrict__ A32,
-                            __bf16* __restrict__ AThi, __bf16* __restrict__ ATlo)
+// FP32 copy of A (same layout)
+__global__ void k_convert_a(const double* __restrict__ A, size_t total, float* __restrict__ A32)
 {
-    const size_t total = (size_t)n * m_pad * d_pad;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const float v = (float)A[e];
-        if (A32)
-            A32[e] = v;
-        if (AThi) {
-            const size_t c = e / ((size_t)m_pad * d_pad);
-            const size_t rem = e - c * (size_t)m_pad * d_pad;
-            const size_t s = rem / d_pad, i = rem - s * d_pad;
-            __bf16 h, l;
-            split_bf16(v, h, l);
-            const size_t o = c * (size_t)d_pad * m_pad + i * m_pad + s;
-            AThi[o] = h;
-            ATlo[o] = l;
-        }
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+        A32[e] = (float)A[e];
+}
+
+// BF16 hi / lo of fp32(A), transposed (per client d_pad rows of m_pad
+// samples), through LDS one 32-sample x 64-column tile per block so that the
+// reads (512 B rows) and the writes (16 B per thread, 64 B per row) coalesce.
+// Grid: n * (m_pad / 32) * (d_pad / 64) blocks of 256 threads.
+__global__ __launch_bounds__(256) void k_convert_bf16t(const double* __restrict__ A, int m_pad, int d_pad,
+                                                       __bf16* __restrict__ AThi, __bf16* __restrict__ ATlo)
+{
+    __shared__ float tile[32][65];
+    const int ti = d_pad / 64, ts = m_pad / 32;
+    const int c = blockIdx.x / (ti * ts);
+    const int r = blockIdx.x - c * ti * ts;
+    const int s0 = (r / ti) * 32, i0 = (r % ti) * 64;
+    const double* Ac = A + (size_t)c * m_pad * d_pad;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int sr = (tid >> 6) + 4 * u;
+        tile[sr][tid & 63] = (float)Ac[(size_t)(s0 + sr) * d_pad + i0 + (tid & 63)];
     }
+    __syncthreads();
+    const int i = tid >> 2, sq = (tid & 3) * 8;
+    bf16x8 h, l;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        __bf16 hh, ll;
+        split_bf16(tile[sq + u][i], hh, ll);
+        h[u] = hh;
+        l[u] = ll;
+    }
+    const size_t o = (size_t)c * d_pad * m_pad + (size_t)(i0 + i) * m_pad + s0 + sq;
+    *reinterpret_cast<bf16x8*>(AThi + o) = h;
+    *reinterpret_cast<bf16x8*>(ATlo + o) = l;
 }
 
 // ---------------------------------------------------------------------------
@@ -3767,7 +3878,7 @@ struct Engine {
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int margin_rows = 4;           // k_margin rows per wave, picked at creation    // KernelId bits timed when timing is on
+    int margin_rows = 4;           // k_margin rows per wave, picked at creation
     struct Pending { int kid; hipEvent_t a, b; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
@@ -3783,6 +3894,7 @@ struct Engine {
     float* A32 = nullptr;          // FP32 copy of A (mode 1)
     __bf16* AThi = nullptr;        // transposed BF16 hi / lo of A (mode 2)
     __bf16* ATlo = nullptr;
+    double* gpart = nullptr;       // modes 1 / 2 without FedNL-PP: margin-pass gradient partials
 };
 
 static hipEvent_t pool_event(Engine* e)
@@ -3943,6 +4055,7 @@ static ProblemDev make_desc(const Engine* e, bool batched = false)
     P.lek_delta = e->lek_delta;
     P.A = e->A; P.A32 = e->A32; P.AThi = e->AThi; P.ATlo = e->ATlo;
     P.x = e->x; P.Dw = e->Dw; P.gw = e->gw; P.nlog = e->nlog; P.learned = e->learned; P.delta = e->delta;
+    P.gpart = e->gpart;
     P.grad_c = e->grad_c; P.part = e->part; P.lk = e->lk; P.vals = e->vals; P.H = e->H; P.g_avg = e->g_avg;
     P.shift = e->shift; P.W = e->W; P.dx = e->dx; P.trace = e->trace; P.lek_p = e->lek_p;
     P.hup = nullptr; P.nb = e->g_avg;
@@ -3972,6 +4085,8 @@ static int sync_desc(Engine* e)
 
 // number of blocks each round kernel uses for one problem
 static int blocks_margin(const ProblemDev& P) { return (P.m_pad / 16) * P.n; }
+// dynamic LDS of k_margin<false, 4, true>: x and the four waves' column sums
+static size_t margin_grad_lds(int d_pad) { return 5 * sizeof(double) * (size_t)d_pad; }
 
 // the engine's margin pass with its tuned rows per wave
 static void launch_margin(Engine* e, bool trial)
@@ -3981,6 +4096,12 @@ static void launch_margin(Engine* e, bool trial)
     const size_t lds = sizeof(double) * e->d_pad;
     const ProblemDev* pd = e->pdev;
     const int2* nomap = nullptr;
+    if (e->gpart && !trial) {
+        // with the client gradients' partials (16 rows per workgroup)
+        hipLaunchKernelGGL((k_margin<false, 4, true>), dim3(grid), dim3(256), margin_grad_lds(e->d_pad), e->st, pd,
+                           nomap);
+        return;
+    }
     if (e->margin_rows == 8) {
         if (trial)
             hipLaunchKernelGGL((k_margin<true, 8>), dim3(grid), dim3(256), lds, e->st, pd, nomap);
@@ -4001,6 +4122,11 @@ static void launch_margin(Engine* e, bool trial)
 // FEDNL_MARGIN_ROWS=4 or 8 sets the shape instead.
 static void tune_margin(Engine* e)
 {
+    if (e->gpart) {
+        // the gradient partials are per 16 rows: four rows per wave
+        e->margin_rows = 4;
+        return;
+    }
     const char* env = getenv("FEDNL_MARGIN_ROWS");
     if (env && (env[0] == '4' || env[0] == '8') && env[1] == 0) {
         e->margin_rows = env[0] - '0';
@@ -4042,7 +4168,7 @@ static int blocks_master(const ProblemDev& P) { return master_h_blocks(P.S) + ma
 // out_learned) instead of Delta. pd / map / nblk: a descriptor array and
 // block map (batches); null pd means the engine itself.
 static void launch_hessian_on(hipStream_t st, int hmode, bool init, const ProblemDev* pd, const int2* map,
-                              int nblk_h, int nblk_g, double* out_learned)
+                              int nblk_h, int nblk_g, double* out_learned, bool fused_grad)
 {
     if (hmode == 0) {
         if (init)
@@ -4051,7 +4177,7 @@ static void launch_hessian_on(hipStream_t st, int hmode, bool init, const Proble
             hipLaunchKernelGGL(k_hessian_f64<false>, dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
         return;
     }
-    if (!init)
+    if (!init && !fused_grad)
         hipLaunchKernelGGL(k_grad_f64, dim3(nblk_g), dim3(256), 0, st, pd, map == nullptr ? nullptr : map + nblk_h);
     if (hmode == 1) {
         if (init)
@@ -4070,7 +4196,7 @@ static void launch_hessian(Engine* e, bool init, double* out_learned, const Prob
 {
     const ProblemDev& P = e->hdesc;
     launch_hessian_on(e->st, e->hmode, init, pd ? pd : e->pdev, nullptr, blocks_hessian(P), blocks_grad(P),
-                      out_learned);
+                      out_learned, e->gpart != nullptr);
 }
 
 static int launch_round(Engine* e)
@@ -4167,7 +4293,7 @@ static void free_engine(Engine* e)
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
                     e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev, e->mst, e->H0, e->pp_sel, e->pp_rng,
                     e->pp_dir, e->pp_lk, e->pp_gmsg, e->pp_lkmsg, e->pp_h, e->pp_tg, e->nsync, e->ny, e->ninv,
-                    e->nxt, e->sel_scr};
+                    e->nxt, e->sel_scr, e->gpart};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -4397,6 +4523,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     ALLOC(e->dx, e->d_pad);
     ALLOC(e->mst, kLsHead + (size_t)e->d_pad);
     ALLOC(e->H0, e->S);
+    if (e->hmode != 0 && opts->clients_per_round <= 0)
+        ALLOC(e->gpart, (size_t)n * (e->m_pad / 16) * e->d_pad);
     if (opts->clients_per_round > 0) {
         ALLOC(e->pp_sel, n);
         ALLOC(e->pp_rng, 1);
@@ -4427,6 +4555,8 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
                 le = raise_lds_limit(f, lds);
         if (le == hipSuccess)
             le = raise_lds_limit((const void*)k_newton_mc, e->lds_newton);
+        if (le == hipSuccess && e->gpart)
+            le = raise_lds_limit((const void*)k_margin<false, 4, true>, margin_grad_lds(e->d_pad));
         if (le != hipSuccess) {
             set_err("hipFuncSetAttribute (LDS limit)", le);
             free_engine(e);
@@ -4470,8 +4600,11 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
             free_engine(e);
             return FEDNL_E_DEVICE;
         }
-        hipLaunchKernelGGL(k_convert_a, dim3(2048), dim3(256), 0, e->st, e->A, n, e->m_pad, e->d_pad, e->A32,
-                           e->AThi, e->ATlo);
+        if (e->hmode == 1)
+            hipLaunchKernelGGL(k_convert_a, dim3(2048), dim3(256), 0, e->st, e->A, nA, e->A32);
+        else
+            hipLaunchKernelGGL(k_convert_bf16t, dim3((unsigned)((size_t)n * (e->m_pad / 32) * (e->d_pad / 64))),
+                               dim3(256), 0, e->st, e->A, e->m_pad, e->d_pad, e->AThi, e->ATlo);
     }
     std::vector<uint32_t> seeds(n);
     for (int c = 0; c < n; ++c) seeds[c] = (uint32_t)(123 + c);  // TrainLoop.h:632
@@ -4578,7 +4711,7 @@ static int enqueue_rounds(Engine* e, int rounds)
 // captured once into a HIP graph and replayed.
 // ---------------------------------------------------------------------------
 enum BatchKernel {
-    BK_MARGIN, BK_HESS64, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
+    BK_MARGIN, BK_MARGIN_G, BK_HESS64, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
     BK_EMIT, BK_SEL_TOPK_W, BK_SEL_TOPLEK_W, BK_SEL_RANDK_W, BK_SEL_WINDOW_W, BK_EMIT_W, BK_MASTER, BK_NEWTON,
     BK_LS_MARGIN, BK_LS, BK_COUNT
 };
@@ -4627,9 +4760,12 @@ static int batch_build(Batch* b)
         const ProblemDev P = make_desc(e, true);
         b->h_probs.push_back(P);
         const int p = (int)i;
-        add(BK_MARGIN, p, blocks_margin(P), false, sizeof(double) * (size_t)e->d_pad);
+        if (e->gpart)
+            add(BK_MARGIN_G, p, blocks_margin(P), false, margin_grad_lds(e->d_pad));
+        else
+            add(BK_MARGIN, p, blocks_margin(P), false, sizeof(double) * (size_t)e->d_pad);
         add(e->hmode == 0 ? BK_HESS64 : (e->hmode == 1 ? BK_HESS32 : BK_HESS16), p, blocks_hessian(P), true);
-        if (e->hmode != 0)
+        if (e->hmode != 0 && !e->gpart)
             add(BK_GRAD, p, blocks_grad(P), false);
         const int comp = e->o.compressor;
         const bool w = e->wide;
@@ -4696,6 +4832,9 @@ static void batch_launch_round(Batch* b)
     if (b->cnt[BK_MARGIN])
         hipLaunchKernelGGL((k_margin<false, 4>), dim3(b->cnt[BK_MARGIN]), dim3(256), b->lds[BK_MARGIN], st, pd,
                            map(BK_MARGIN));
+    if (b->cnt[BK_MARGIN_G])
+        hipLaunchKernelGGL((k_margin<false, 4, true>), dim3(b->cnt[BK_MARGIN_G]), dim3(256), b->lds[BK_MARGIN_G], st,
+                           pd, map(BK_MARGIN_G));
     if (b->cnt[BK_HESS64])
         hipLaunchKernelGGL(k_hessian_f64<false>, dim3(b->cnt[BK_HESS64]), dim3(256), 0, st, pd, map(BK_HESS64),
                            (double*)nullptr);
