@@ -158,6 +158,42 @@ def _warm(eng, rounds=300):
         eng.run(rounds, check=False)
 
 
+def small_configs_leg(rounds=1000, warm=None):
+    """Round rates of the small workloads, where launch overhead weighs most:
+    50 x 221, d=68, TopK k=8d and 80 x 407, d=123 with RandSeqK and TopLEK,
+    `rounds` rounds from x0 = 0 enqueued at once, with plain launches and with
+    each round replayed from a captured HIP graph (FEDNL_ROUND_GRAPH=1)."""
+    from ufednl_amd import datagen
+    from ufednl_amd.engine import Engine
+    out = {}
+    for wname, prob, comp in (("n50_m221_d68_topk", datagen.config1(), "topk"),
+                              ("n80_m407_d123_seqk", datagen.config3(), "seqk"),
+                              ("n80_m407_d123_toplek", datagen.config3(), "toplek")):
+        res = {}
+        for form, env in (("plain", "0"), ("graph", "1")):
+            old = os.environ.get("FEDNL_ROUND_GRAPH")
+            os.environ["FEDNL_ROUND_GRAPH"] = env
+            try:
+                eng = Engine.from_problem(prob, compressor=comp, k_mult=8.0, tracking=False)
+            finally:
+                if old is None:
+                    del os.environ["FEDNL_ROUND_GRAPH"]
+                else:
+                    os.environ["FEDNL_ROUND_GRAPH"] = old
+            eng.run(5, check=False)  # graph capture and first replays
+            _warm(warm)
+            _sync_all(eng)
+            t0 = time.perf_counter()
+            eng.enqueue(rounds)
+            _sync_all(eng)
+            res[f"rounds_per_s_{form}"] = rounds / (time.perf_counter() - t0)
+            eng.close()
+        res["rounds"] = rounds
+        out[wname] = res
+        print(f"small configs {wname}: {res}", file=sys.stderr, flush=True)
+    return out
+
+
 def mixed_precision_leg(rounds_c2, rounds_c3, warm=None):
     """FP64 / FP32 / BF16-split A^T D A on the 142 x 350, d=300 TopK workload
     and on the 80 x 407, d=123 RandSeqK / TopLEK workload."""
@@ -177,7 +213,8 @@ def mixed_precision_leg(rounds_c2, rounds_c3, warm=None):
             eng.run(5, check=False)
             kt = eng.kernel_times()
             eng.kernel_timing(False)
-            # the Hessian stage: the contraction of the mode (plus, below FP64, the FP64 gradient pass)
+            # the Hessian stage: the contraction of the mode (below FP64 with the sums of the margin
+            # pass's gradient partials on the diagonal tiles)
             entry = dict(rounds_to_tol=r, time_to_tol_s=t, rounds_per_s=rate,
                          hessian_stage_us=1e3 * kt["k_hessian_f64"][0] / max(kt["k_hessian_f64"][1], 1))
             if mode == "fp64":
@@ -368,8 +405,8 @@ def main():
     ap.add_argument("--cpu-rounds", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline, time-to-tolerance and CPU legs")
-    ap.add_argument("--legs", default="mixed,batched,serving",
-                    help="extra legs to run: any of mixed, batched, serving")
+    ap.add_argument("--legs", default="mixed,small,batched,serving",
+                    help="extra legs to run: any of mixed, small, batched, serving")
     ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--batch-max-rounds", type=int, default=300)
     ap.add_argument("--serve-requests", type=int, default=64)
@@ -464,6 +501,8 @@ def main():
     extra = {}
     if "mixed" in legs:
         extra["mixed_precision"] = mixed_precision_leg(args.tol_rounds, 500, warm=eng_w)
+    if "small" in legs:
+        extra["small_configs"] = small_configs_leg(1000, warm=eng_w)
     if "batched" in legs:
         extra["batched"] = batched_leg(args.batch_problems, args.batch_max_rounds, warm=eng_w)
     if "serving" in legs:

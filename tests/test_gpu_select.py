@@ -61,7 +61,7 @@ def test_toplek_wide_bit_exact(oracle, d, k, kind, workers):
 
 
 @pytest.mark.parametrize("d,k", [(5, 3), (12, 40), (30, 100), (68, 544), (123, 984)])
-@pytest.mark.parametrize("kind", ["gauss", "ties", "round1"])
+@pytest.mark.parametrize("kind", ["gauss", "ties", "round1", "zeros"])
 @pytest.mark.parametrize("workers", [1, 2])
 def test_toplek_bit_exact(oracle, d, k, kind, workers):
     from ufednl_amd import engine as E

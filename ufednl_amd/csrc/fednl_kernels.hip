@@ -1826,8 +1826,13 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
             const double fro2 = s_fro2;
             const double one_minus_delta_xf = (1.0 - lek_delta) * fro2;
             const double eps = 1e-1 * fro2;
+            // fro2 = 0 (or NaN): every energy term is zero (or the sums are
+            // NaN), the acceptance test never holds and the walk extracts the
+            // whole heap without a draw; the result, an empty selection, is
+            // set at once
+            const bool full_walk = !(fro2 > 0.0);
             double numerator = fro2;
-            for (int q0 = 0; q0 < k; q0 += 64) {
+            for (int q0 = 0; q0 < (full_walk ? 0 : k); q0 += 64) {
                 const int q = q0 + lane;
                 double tq = 0.0;
                 if (q < k) {
@@ -1842,7 +1847,14 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
             int hit = 0;
             double pp = 0.0;
             uint32_t popped = 0;
-            do {
+#ifdef FEDNL_PROBE
+            const unsigned long long t_walk = clock64();
+#endif
+            if (full_walk) {
+                nh = 0;
+                rh.n = 0;
+            }
+            while (nh != 0) {
                 const double prev = numerator;
                 const uint32_t pid = rh.r0i;
                 const double pv = rh.r0v;
@@ -1870,7 +1882,13 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
                     popped = pid;
                     break;
                 }
-            } while (nh != 0);
+            }
+#ifdef FEDNL_PROBE
+            if (lane == 0) {
+                g_probe[1000 + 8 * c + 6] += (unsigned long long)(k - nh);
+                g_probe[1000 + 8 * c + 7] += clock64() - t_walk;
+            }
+#endif
             rh.store(hv, hid);
             if (lane == 0) {
                 s_cnt = nh;

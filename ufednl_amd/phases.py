@@ -7,13 +7,15 @@ product library has no stamps. Each client's workgroup adds, per round:
   build  heap construction over the first k positions (MinHeap.h:147-191),
   scan   positions k .. S-1: the streamed |Delta| ring and the admissions
          (MinHeap.h:78-85), with the admission count and their shader cycles,
-  tail   heap read-back, the sorted index list, the message values and the
-         learned update.
-build/scan/tail are wall_clock64 ticks (100 MHz); the scan's and the
-admissions' shader cycles come from clock64.
+  tail   heap read-back, (TopLEK) the energy walk's extractions
+         (SpecialMathRoutinesForMatrix.h:441-499), the sorted index list and,
+         for TopK, the message values and the learned update.
+build/scan/tail are wall_clock64 ticks (100 MHz); the scan's, the
+admissions' and the energy walk's shader cycles come from clock64.
 
-Run: python -m ufednl_amd.phases [--rounds 0,10,40,150,300]
-(142 clients x 350 samples, d = 300, TopK k = 8d, FP64.)
+Run: python -m ufednl_amd.phases [--rounds 0,10,40,150,300] [--workload c2|c3]
+     [--compressor topk|toplek]
+(c2: 142 clients x 350 samples, d = 300; c3: 80 x 407, d = 123; k = 8d, FP64.)
 """
 from __future__ import annotations
 
@@ -32,6 +34,8 @@ BASE = 1000  # per-client records: 8 slots each from here
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--rounds", default="0,10,40,150,300")
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3"))
+    ap.add_argument("--compressor", default="topk", choices=("topk", "toplek"))
     args = ap.parse_args(argv)
     if not os.path.exists(PROBE_LIB):
         raise SystemExit(f"{PROBE_LIB} is missing: python -m ufednl_amd.build --probe")
@@ -41,12 +45,15 @@ def main(argv=None):
     L = E.lib()
     L.fednl_probe_read.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int, ctypes.c_int]
     buf = (ctypes.c_ulonglong * SLOTS)()
-    prob = datagen.config2()
-    eng = E.Engine.from_problem(prob, compressor="topk", k_mult=8.0, tracking=False)
+    prob = datagen.config2() if args.workload == "c2" else datagen.config3()
+    eng = E.Engine.from_problem(prob, compressor=args.compressor, k_mult=8.0, tracking=False)
     done = 0
-    print(f"k_select_heap phases, TopK k={eng.k}, {eng.n} clients x {eng.m} samples, d={eng.d}", flush=True)
+    print(f"k_select_heap phases, {args.compressor} k={eng.k}, {eng.n} clients x {eng.m} samples, d={eng.d}",
+          flush=True)
+    lek = args.compressor == "toplek"
     print("round | build max us | scan max us | tail max us | admissions mean / max | "
-          "cycles per admission | slowest client: admissions, admission cycles, scan cycles")
+          "cycles per admission | slowest client: admissions, admission cycles, scan cycles"
+          + (" | extractions mean, cycles per extraction" if lek else ""))
     for r in sorted(int(x) for x in args.rounds.split(",")):
         if r > done:
             eng.run(r - done, check=False)
@@ -61,7 +68,9 @@ def main(argv=None):
         s = int(np.argmax(sel[:, 1]))
         print(f"{r:5d} | {sel[:, 0].max() / 100:12.1f} | {sel[:, 1].max() / 100:11.1f} | {sel[:, 2].max() / 100:11.1f} | "
               f"{adm.mean():10.0f} / {adm.max():4d} | {cyc.sum() / max(adm.sum(), 1):20.0f} | "
-              f"{s}: {adm[s]}, {cyc[s]}, {scan[s]}", flush=True)
+              f"{s}: {adm[s]}, {cyc[s]}, {scan[s]}"
+              + (f" | {sel[:, 6].mean():.0f}, {sel[:, 7].sum() / max(sel[:, 6].sum(), 1):.0f}" if lek else ""),
+              flush=True)
     eng.close()
 
 
