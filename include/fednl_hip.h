@@ -71,8 +71,19 @@ typedef struct fednl_opts {
     int clients_per_round;      /* 0: FedNL; tau in [1, n]: FedNL-PP with tau clients drawn per round
                                    (bin_fednl_local_pp, --clients-per-round, TrainLoop.h:279-287) */
     unsigned client_selection_seed;  /* FedNL-PP client sampler seed (default 1234, Args.h:244) */
-    int reserved[2];
+    int algorithm;              /* enum fednl_algorithm (--algorithm, TrainLoop.h:297-299) */
+    int reserved;
+    double global_lr;           /* GD step size: --global_lr, or 2 / (mu + L) with
+                                   --theoretical_global_lr, times --global_step_size_multiplier
+                                   (TrainLoop.h:358,403-407,521) */
 } fednl_opts;
+
+/* FEDNL_ALG_FEDNL: FedNL / FedNL-LS / FedNL-PP (--algorithm fednl1);
+ * FEDNL_ALG_GD: gradient descent on the clients' averaged gradient,
+ * x -= global_lr * (sum_c grad f_c(x)) / n (--algorithm gd; LocalWorkerForGD.h,
+ * TrainLoop.h:993-1052). GD uses neither the compressor nor the Hessians and
+ * runs without line search and outside batches. */
+enum fednl_algorithm { FEDNL_ALG_FEDNL = 0, FEDNL_ALG_GD = 1 };
 
 void fednl_default_opts(fednl_opts *o);
 
@@ -90,6 +101,12 @@ int fednl_create(const fednl_opts *opts, int n, int m, int d, const double *feat
                  const double *labels, void **handle);
 
 void fednl_destroy(void *handle);
+
+/* Sets the GD step size of an engine made with FEDNL_ALG_GD, e.g. the
+ * theoretical 2 / (mu + L) once L is known (TrainLoop.h:306-316 computes L
+ * from the clients' data, :403-407,521 the step); takes effect from the next
+ * enqueued round. */
+int fednl_set_global_lr(void *handle, double global_lr);
 
 /* Sizes: k actually used, S = d(d+1)/2, LDA of flat indices. */
 int fednl_sizes(void *handle, int *k, int64_t *S, int *lda);

@@ -164,6 +164,34 @@ def ref_l_smoothness(features, labels, lam):
     return float(L), int(it[0])
 
 
+def gd_run(prob, step, rounds, tracking=True):
+    """The oracle's GD (orc_gd_run: LocalWorkerForGD.h, TrainLoop.h:993-1052)
+    from x = 0: per-round ||g||, recorded f, ||x||, and the last iterate."""
+    n, m, d = prob.features.shape
+    f = np.ascontiguousarray(prob.features, dtype=np.float64)
+    b = np.ascontiguousarray(prob.labels, dtype=np.float64)
+    g = np.zeros(rounds); fv = np.zeros(rounds); xn = np.zeros(rounds); x = np.zeros(d)
+    L = lib()
+    L.orc_gd_run.argtypes = [_c_dp, _c_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                             ctypes.c_double, ctypes.c_int, ctypes.c_int, _c_dp, _c_dp, _c_dp, _c_dp]
+    L.orc_gd_run(dp(f), dp(b), n, m, d, prob.lam, step, rounds, 1 if tracking else 0, dp(g), dp(fv), dp(xn), dp(x))
+    return dict(grad_norm=g, f=fv, x_norm=xn, x=x)
+
+
+def ref_gd_run(prob, step, rounds, tracking=True):
+    """GD from the reference's own client oracle and vector arithmetic
+    (oracle/_ref ref_gd_run); same outputs as gd_run."""
+    n, m, d = prob.features.shape
+    f = np.ascontiguousarray(prob.features, dtype=np.float64)
+    b = np.ascontiguousarray(prob.labels, dtype=np.float64)
+    g = np.zeros(rounds); fv = np.zeros(rounds); xn = np.zeros(rounds); x = np.zeros(d)
+    R = ref()
+    R.ref_gd_run.argtypes = [_c_dp, _c_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                             ctypes.c_double, ctypes.c_int, ctypes.c_int, _c_dp, _c_dp, _c_dp, _c_dp]
+    R.ref_gd_run(dp(f), dp(b), n, m, d, prob.lam, step, rounds, 1 if tracking else 0, dp(g), dp(fv), dp(xn), dp(x))
+    return dict(grad_norm=g, f=fv, x_norm=xn, x=x)
+
+
 def ref_crc32(b):
     """dopt::crc32(buf, size, crc32Seed()) from oracle/_ref (Crc.cpp:74-90)."""
     R = ref()

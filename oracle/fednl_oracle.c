@@ -1175,3 +1175,60 @@ ORC_API int orc_fednl_pp_round(void *h, int tracking, int *sel, int *cnt, double
     free(gr);
     return ok;
 }
+
+/* ------------------------------------------------------------------------ */
+/* GD (--algorithm gd): LocalWorkerForGD.h:8-49, master TrainLoop.h:993-1052 */
+/* ------------------------------------------------------------------------ */
+/* rounds of x -= step * (sum_c grad f_c(x)) / n from x = 0, clients summed in
+ * order. Per round r: gnorm[r] = ||g||, fval[r] = the aggregate the reference
+ * records under tracking ((sum_c (f_c + sizeof(double))) / n,
+ * TrainLoop.h:1023-1030; 0 without tracking), xnorm[r] = ||x|| before the
+ * step (:1002). x_out: the last iterate. */
+ORC_API void orc_gd_run(const double *feat, const double *labels, int n, int m, int d, double lambda,
+                        double step, int rounds, int tracking, double *gnorm, double *fval, double *xnorm,
+                        double *x_out)
+{
+    double *x = (double *)calloc((size_t)d, sizeof(double));
+    double *g = (double *)malloc(sizeof(double) * d);
+    double *gc = (double *)malloc(sizeof(double) * d);
+    double inv_n = 1.0 / (double)n;
+    for (int r = 0; r < rounds; ++r)
+    {
+        double xx = 0.0;
+        for (int i = 0; i < d; ++i)
+            xx = xx + x[i] * x[i];
+        for (int i = 0; i < d; ++i)
+            g[i] = 0.0;
+        double fa = 0.0;
+        for (int c = 0; c < n; ++c)
+        {
+            double fc = 0.0;
+            orc_client_eval(feat + (size_t)c * m * d, labels + (size_t)c * m, d, m, lambda, x, NULL, NULL, gc,
+                            NULL, &fc, NULL);
+            for (int i = 0; i < d; ++i)
+                g[i] = g[i] + gc[i];
+            if (tracking)
+            {
+                fa = fa + fc;
+                fa = fa + (double)sizeof(double);
+            }
+        }
+        double gg = 0.0;
+        for (int i = 0; i < d; ++i)
+        {
+            g[i] = g[i] * inv_n;
+            gg = gg + g[i] * g[i];
+        }
+        gnorm[r] = sqrt(gg);
+        fval[r] = tracking ? fa * inv_n : 0.0;
+        xnorm[r] = sqrt(xx);
+        /* subInPlaceVectorWithMultiple: x_i - step * g_i */
+        for (int i = 0; i < d; ++i)
+            x[i] = x[i] - step * g[i];
+    }
+    if (x_out)
+        memcpy(x_out, x, sizeof(double) * d);
+    free(x);
+    free(g);
+    free(gc);
+}

@@ -380,6 +380,53 @@ extern "C"
         return 1;
     }
 
+    // GD (--algorithm gd) with the reference's own client oracle and vector
+    // arithmetic: the workers' evaluateGradient (LocalWorkerForGD.h:30-37) and
+    // the master's round (TrainLoop.h:993-1052), clients consumed in order.
+    // Per round: ||g||, the recorded f aggregate (each client's f_c and then
+    // sizeof(double) added, divided by n; 0 without tracking), ||x|| before
+    // the step. x_out: the last iterate.
+    void ref_gd_run(const double* feat, const double* labels, int n, int m, int d, double lambda, double step,
+                    int rounds, int tracking, double* grad_norm, double* fval, double* x_norm, double* x_out)
+    {
+        std::vector<Problem*> probs(n);
+        for (int c = 0; c < n; ++c)
+        {
+            Mat atr = designTr(feat + (size_t)c * m * d, d, m);
+            Vec b = vecFrom(labels + (size_t)c * m, m);
+            probs[c] = new Problem(atr, b, lambda);
+        }
+        Vec xCur(d);
+        Vec g(d);
+        const double inv_n = 1.0 / double(n);
+        for (int r = 0; r < rounds; ++r)
+        {
+            const double xnorm = xCur.vectorL2Norm();
+            g.setAllToDefault();
+            double fa = 0.0;
+            for (int c = 0; c < n; ++c)
+            {
+                g += probs[c]->evaluateGradient(xCur);
+                if (tracking)
+                {
+                    fa += probs[c]->evaluateFunction(xCur);
+                    fa += sizeof(fa);
+                }
+            }
+            g *= inv_n;
+            if (tracking)
+                fa *= inv_n;
+            grad_norm[r] = g.vectorL2Norm();
+            fval[r] = fa;
+            x_norm[r] = xnorm;
+            xCur.subInPlaceVectorWithMultiple(step, g);
+        }
+        for (int i = 0; i < d; ++i)
+            x_out[i] = xCur[i];
+        for (int c = 0; c < n; ++c)
+            delete probs[c];
+    }
+
     // compressor codes follow UsedTypes.h:17-25
     enum { C_IDENTICAL = 0, C_RANDK = 1, C_TOPK = 2, C_RANDSEQK = 3, C_TOPLEK = 4, C_NATURAL = 5 };
 

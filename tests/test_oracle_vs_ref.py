@@ -200,3 +200,28 @@ def test_pp_client_sampler_is_the_reference_shuffle(oracle):
         want = np.zeros(n, dtype=np.int32)
         want[v[:tau]] = 1
         np.testing.assert_array_equal(r["sel"][rr], want)
+
+
+@pytest.mark.parametrize("tracking", [True, False])
+def test_gd_matches_reference(oracle, tracking):
+    """GD (--algorithm gd): the oracle's rounds against the reference's own
+    client oracle and vector arithmetic (LocalWorkerForGD.h, TrainLoop.h:993-1052),
+    including the recorded f aggregate (each client adds f_c and then
+    sizeof(double) before the division by n, so f(x) reads as the mean + 8)."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref is not built")
+    from ufednl_amd import datagen
+    prob = datagen.small_problem(n=6, m=41, d_feat=15, density=0.3, seed=12)
+    a = oracle.gd_run(prob, 1.5, 40, tracking)
+    b = oracle.ref_gd_run(prob, 1.5, 40, tracking)
+    for key in ("grad_norm", "x_norm"):
+        np.testing.assert_allclose(a[key], b[key], rtol=1e-13, atol=1e-300, err_msg=key)
+    np.testing.assert_allclose(a["f"], b["f"], rtol=1e-14, atol=0.0)
+    np.testing.assert_allclose(a["x"], b["x"], rtol=1e-12, atol=1e-15)
+    if tracking:
+        # round 0 at x = 0: every sigma is 1/2, f_c = log 2
+        assert abs(a["f"][0] - (np.log(2.0) + 8.0)) < 1e-14
+    else:
+        assert not a["f"].any()
+    # the step: x_1 = -step * g(0), so ||x_1|| = step * ||g_0||
+    assert abs(a["x_norm"][1] - 1.5 * a["grad_norm"][0]) <= 1e-14 * a["x_norm"][1]

@@ -112,6 +112,9 @@ struct Args {
     double c_ls = 0.5, gamma_ls = 0.5;
     uint32_t max_ls = 5;
     bool option_a = false, option_b = false;
+    // GD (Args.h:280-284, 366-379)
+    double global_lr = 0.0, global_mult = 1.0;
+    bool theoretical_global_lr = false;
     // debug (Args.h:74-129)
     bool silent_print = false, meminfo = false, forceseq = false, crc_data = false, print_train = false;
     bool silent = false;
@@ -219,6 +222,14 @@ static int parse_args(const CmdLine& cl, Args& a)
             }
             cl.num("max_line_search_steps", a.max_ls);
         }
+    } else {
+        // GD (Args.h:366-379)
+        a.theoretical_global_lr = cl.flag("theoretical_global_lr");
+        if (!cl.num("global_lr", a.global_lr) && !a.theoretical_global_lr) {
+            std::cerr << "Please specify argument \"global_lr\" with value of global_lr\n";
+            return kMissingArgument;
+        }
+        cl.num("global_step_size_multiplier", a.global_mult);
     }
     cl.num("server_grad_workers", a.server_grad_workers);
     cl.num("server_hessian_workers", a.server_hessian_workers);
@@ -739,27 +750,38 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
     if (print)
         std::cout << "=========================================================================\n";
 
-    if (a.algorithm != "fednl1") {
-        std::cerr << "Algorithm '" << a.algorithm << "' is not provided by this engine (fednl1 only)\n";
-        return kWrongArgument;
-    }
-    if (!a.option_a && !a.option_b) {
+    // algorithm selector: gd || fednl1 (TrainLoop.h:297-330)
+    const bool gd = a.algorithm == "gd";
+    const bool fednl = a.algorithm == "fednl1";
+    if (fednl && !a.option_a && !a.option_b) {
         if (print) std::cerr << "For FedNL you should specify fednl-option-a or fednl-option-b\n";
         return kMissingArgument;
     }
-    int comp;
-    if (a.compressor == "identical") comp = FEDNL_IDENTICAL;
-    else if (a.compressor == "natural") comp = FEDNL_NATURAL;
-    else if (a.compressor == "randk") comp = FEDNL_RANDK;
-    else if (a.compressor == "topk") comp = FEDNL_TOPK;
-    else if (a.compressor == "toplek") comp = FEDNL_TOPLEK;
-    else if (a.compressor == "seqk") comp = FEDNL_RANDSEQK;
-    else {
-        if (print) std::cerr << "You have specified unknown type of compressor: " << a.compressor << '\n';
-        return kWrongArgument;
+    // the compressor (TrainLoop.h:367-400; GD has none)
+    int comp = FEDNL_RANDSEQK;
+    if (!gd) {
+        if (a.compressor == "identical") comp = FEDNL_IDENTICAL;
+        else if (a.compressor == "natural") comp = FEDNL_NATURAL;
+        else if (a.compressor == "randk") comp = FEDNL_RANDK;
+        else if (a.compressor == "topk") comp = FEDNL_TOPK;
+        else if (a.compressor == "toplek") comp = FEDNL_TOPLEK;
+        else if (a.compressor == "seqk") comp = FEDNL_RANDSEQK;
+        else {
+            if (print) std::cerr << "You have specified unknown type of compressor: " << a.compressor << '\n';
+            return kWrongArgument;
+        }
+    }
+    if (!gd && !fednl) {
+        // TrainLoop.h:653-658
+        if (print) std::cerr << " Please specify valid algorithm name.\n";
+        return kInternal;
+    }
+    if (print) {
+        std::cout << "Main Loop preparation for: " << a.algorithm << '\n';
+        std::cout << "=========================================================================\n";
     }
     const size_t S = (size_t)d * (d + 1) / 2;
-    const bool whole = (comp == FEDNL_IDENTICAL || comp == FEDNL_NATURAL);  // TrainLoop.h:354-377
+    const bool whole = !gd && (comp == FEDNL_IDENTICAL || comp == FEDNL_NATURAL);  // TrainLoop.h:354-377
     const int k = whole ? (int)S : (int)(a.k_mult * d);
     if (!whole && (size_t)k > S) {
         if (print)
@@ -768,7 +790,7 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
         return kWrongArgument;
     }
     int alpha_mode = 0;
-    if (a.theoretical_alpha) {
+    if (a.theoretical_alpha && !gd) {
         if (comp == FEDNL_TOPK || comp == FEDNL_TOPLEK) {
             if (a.alpha_opt1) alpha_mode = 1;
             else if (a.alpha_opt2) alpha_mode = 2;
@@ -797,16 +819,21 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
     o.gamma_line_search = a.gamma_ls;
     o.max_line_search_steps = (int)a.max_ls;
     o.clients_per_round = a.pp ? a.clients_per_round : 0;
+    o.algorithm = gd ? FEDNL_ALG_GD : FEDNL_ALG_FEDNL;
+    // TrainLoop.h:358, 521 (the theoretical step is set once L is known)
+    double used_global_step = a.global_lr * a.global_mult;
+    o.global_lr = used_global_step;
     void* eng = nullptr;
     rc = fednl_create(&o, n, mpc, d, feats.data(), labs.data(), &eng);
     if (rc != FEDNL_OK) {
         std::cerr << "engine: " << fednl_last_error() << '\n';
         return kInternal;
     }
-    const double alpha = fednl_alpha(eng);
-    // TrainLoop.h:306-316: L(f) when asked for (or for a theoretical global
-    // step size, which only GD uses)
-    const bool compute_L = cl.flag("compute-L-smooth") || cl.flag("theoretical_global_lr");
+    // GD has no alpha: the reported one is --alpha_step_size's default
+    const double alpha = gd ? a.alpha : fednl_alpha(eng);
+    // TrainLoop.h:306-316: L(f) when asked for, or for GD's theoretical
+    // global step size (a flag that only GD parses)
+    const bool compute_L = cl.flag("compute-L-smooth") || (gd && a.theoretical_global_lr);
     double L_f = -1.0;
     if (compute_L) {
         size_t iters = 0;
@@ -817,6 +844,16 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
         }
         if (print)
             std::cout << "Compute L constants for objective (power iterations: " << iters << ")\n";
+    }
+    if (gd && a.theoretical_global_lr) {
+        // TrainLoop.h:403-407, 521: 2 / (mu + L) times the multiplier
+        used_global_step = 2.0 / (a.lambda + L_f);
+        used_global_step *= a.global_mult;
+        if (fednl_set_global_lr(eng, used_global_step) != FEDNL_OK) {
+            std::cerr << "engine: " << fednl_last_error() << '\n';
+            fednl_destroy(eng);
+            return kInternal;
+        }
     }
     if (print) {
         std::cout << "=========================================================================\n";
@@ -854,6 +891,12 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
     // (LocalWorkerForFedNL_Natural.h:80-120)
     const bool send_fi = a.tracking || a.line_search;  // TrainLoop.h:620
     auto round_bytes = [&](int r, double& scal, double& idxb) {
+        if (gd) {
+            // every client's gradient (TrainLoop.h:1022)
+            scal = (double)n * 8.0 * d;
+            idxb = 0.0;
+            return;
+        }
         const double np = (double)parts[r];
         const double item_bytes = (comp == FEDNL_NATURAL) ? np * 3.0 * (double)(S / 2 + 1) : 8.0 * items[r];
         scal = np * (8.0 * d + (a.option_b ? 8.0 : 0.0) + ((send_fi && a.option_b) ? 8.0 : 0.0) +
@@ -870,6 +913,7 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
         if (a.line_search) alg += "-LS";
         if (a.option_a) alg += " (a)";
         if (a.option_b) alg += " (b)";
+        if (gd) alg = "GD";  // TrainLoop.h:767-770
         tr.str(alg, true);
         tr.str(a.compressor, true);
         tr.i32(mpc);
@@ -879,7 +923,7 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
         tr.i32(d);
         tr.i32(k);
         tr.f64(a.k_mult);
-        tr.f64(0.0);
+        tr.f64(used_global_step);
         tr.f64(alpha);
         tr.f64(a.lambda);
     }
@@ -926,6 +970,10 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
     const double f_gap = gg / (2.0 * mu);   // ConvexOptimizationRoutines.h:12-27
     const double x_gap = 2.0 * gg / mu;
     if (print) {
+        // TrainLoop.h:1700-1712
+        std::cout << "=========================================================================\n";
+        std::cout << "Main loop has been finished: r:" << a.rounds << ",clients:" << n << ",d:" << d << ",ni:" << mpc
+                  << '\n';
         std::cout << "=========================================================================\n";
         std::cout << "Statistics\n\n";
         std::cout << "  Average time per round: " << (R ? elapsed_ms / R : 0.0) << " milliseconds\n";
@@ -933,10 +981,29 @@ static int train(const CmdLine& cl, ResultCallbackLocal cb, ResultCallbackLocalP
         std::cout << "  Total number of clients: " << n << '\n';
         if (a.pp)
             std::cout << "  Number of clients per round selected u.a.r: " << a.clients_per_round << '\n';
+        if (!a.pp) {
+            // TrainLoop.h:1716-1720
+            std::cout << '\n';
+            std::cout << "  Total number of workers for clients: " << a.workers << '\n';
+            std::cout << "  Total number of workers for master [for gradient updates]: " << a.server_grad_workers
+                      << '\n';
+            std::cout << "  Total number of workers for master [for hessian updates]: " << a.server_hessian_workers
+                      << '\n';
+            std::cout << '\n';
+        }
         std::cout << "  Algorithm: " << a.algorithm << '\n';
         std::cout << "  Data set: " << a.path << '\n';
         std::cout << "  d: " << d << '\n';
+        if (!a.pp) {
+            // TrainLoop.h:1724-1729
+            std::cout << '\n';
+            std::cout << "  Global step size (with multiplier): " << used_global_step << '\n';
+            std::cout << "  Global step size multiplier: " << a.global_mult << '\n';
+            std::cout << "  Is global step size theoretical: " << (a.theoretical_global_lr ? "[YES]" : "[NO]") << '\n';
+        }
         std::cout << "  Alpha step size: " << alpha << '\n';
+        if (!a.pp)
+            std::cout << "  Is alpha step size theoretical: " << (a.theoretical_alpha && !gd ? "[YES]" : "[NO]") << '\n';
         std::cout << "  Lambda for regularization: " << a.lambda << '\n';
         std::cout << "  Received MBytes from the clients: " << (acc_s + acc_i) / (1024.0 * 1024.0) << '\n';
         std::cout << "  Clients compressor: " << a.compressor << '\n';

@@ -270,6 +270,7 @@ struct ProblemDev {
         trace_cap, update_x, ls, ls_max, pp, pp_tau;
     size_t S;
     double lambda, alpha, prescale, inv_n, mult, lek_delta, ls_c, ls_gamma;
+    double gd_step;     // GD: the global step size
     const double* A;
     const float* A32;
     const __bf16* AThi;
@@ -2598,6 +2599,93 @@ __global__ __launch_bounds__(256) void k_master(FEDNL_KARGS)
 }
 
 // ---------------------------------------------------------------------------
+// GD (--algorithm gd; LocalWorkerForGD.h, TrainLoop.h:993-1052): every client
+// sends grad f_c(x) (k_grad_f64), the master averages them in client order
+// and steps x -= global_lr * g. k_gd_master: blocks [0, ng) the gradient
+// average, the last block f and ||x|| for the record. With tracking the
+// reference adds each client's f_c and then sizeof(double) to the aggregate
+// before dividing by n (TrainLoop.h:1023-1030), so the recorded f(x) is the
+// mean f_c + 8: reproduced, in client order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void gd_vec_block(const ProblemDev& PD)
+{
+    const int n = PD.n, d = PD.d, m = PD.m, nlog_blocks = PD.m_pad / 16;
+    const gmem<double>* __restrict__ nlog_part = G(PD.nlog);
+    const gmem<double>* __restrict__ x = G(PD.x);
+    gmem<double>* __restrict__ trace = G(PD.trace);
+    __shared__ double scratch[16];
+    __shared__ double fc_s[kMaxClients];
+    double xx = 0.0;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) xx = fma(x[i], x[i], xx);
+    xx = block_sum(xx, scratch);
+    if (PD.tracking)
+        for (int c = threadIdx.x; c < n; c += blockDim.x)
+            fc_s[c] = client_f(nlog_part, c, nlog_blocks, m, PD.lambda, xx);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double fa = 0.0;
+        if (PD.tracking) {
+            for (int c = 0; c < n; ++c) {
+                fa = fa + fc_s[c];
+                fa = fa + (double)sizeof(double);
+            }
+            fa = fa * PD.inv_n;
+        }
+        const int r = *PD.round_ctr;
+        if (r < PD.trace_cap) {
+            trace[(size_t)r * kTraceStride + 1] = fa;
+            trace[(size_t)r * kTraceStride + 2] = sqrt(xx);
+            trace[(size_t)r * kTraceStride + 5] = (double)n;
+            trace[(size_t)r * kTraceStride + 6] = 0.0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gd_master(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    if (blk < master_grad_blocks(PD.d))
+        master_grad_block(PD, blk);
+    else
+        gd_vec_block(PD);
+}
+
+// the record's ||g|| (TrainLoop.h:1679-1691), x -= global_lr * g
+// (subInPlaceVectorWithMultiple, TrainLoop.h:1050-1051), the round's end
+__global__ __launch_bounds__(256) void k_gd_step(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    (void)blk;
+    const int d = PD.d;
+    const gmem<double>* __restrict__ g = G(PD.g_avg);
+    gmem<double>* __restrict__ x = G(PD.x);
+    gmem<double>* __restrict__ dx = G(PD.dx);
+    gmem<double>* __restrict__ trace = G(PD.trace);
+    gmem<int>* __restrict__ round_ctr = G(PD.round_ctr);
+    __shared__ double scratch[16];
+    const double step = PD.gd_step;
+    double gg = 0.0;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) {
+        const double gi = g[i];
+        gg = fma(gi, gi, gg);
+        const double s = step * gi;
+        dx[i] = s;
+        x[i] = x[i] - s;
+    }
+    gg = block_sum(gg, scratch);
+    if (threadIdx.x == 0) {
+        const int r = *round_ctr;
+        if (r < PD.trace_cap) {
+            trace[(size_t)r * kTraceStride + 0] = sqrt(gg);
+            trace[(size_t)r * kTraceStride + 3] = 1.0;
+            trace[(size_t)r * kTraceStride + kTraceT1] = (double)wall_clock64();
+        }
+        *round_ctr = r + 1;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
 // FedNL-LS (TrainLoop.h:1546-1640), after k_newton_mc has written dx and left x
 // at x_k. k_ls_begin keeps x_k, computes d = -(dx, g) and sets x to the first
 // trial x_k - 1 * dx. Each trial is then k_margin<true> (the clients' f_i at
@@ -3855,10 +3943,10 @@ __global__ __launch_bounds__(64) void k_mfma_probe_mixed(const float* __restrict
 // host engine
 // ---------------------------------------------------------------------------
 enum KernelId {
-    KID_MARGIN, KID_HESS, KID_SELECT, KID_EMIT, KID_MASTER, KID_NEWTON, KID_LS, KID_COUNT
+    KID_MARGIN, KID_HESS, KID_SELECT, KID_EMIT, KID_MASTER, KID_NEWTON, KID_LS, KID_GRAD, KID_GD, KID_COUNT
 };
-static const char* kKernelNames[KID_COUNT] = {"k_margin", "k_hessian_f64", "k_select", "k_emit",
-                                              "k_master", "k_newton_mc", "k_line_search"};
+static const char* kKernelNames[KID_COUNT] = {"k_margin", "k_hessian_f64", "k_select", "k_emit", "k_master",
+                                              "k_newton_mc", "k_line_search", "k_grad_f64", "k_gd"};
 
 struct Engine {
     fednl_opts o;
@@ -4063,6 +4151,7 @@ static ProblemDev make_desc(const Engine* e, bool batched = false)
     P.option_b = e->o.option_b; P.tracking = e->o.tracking; P.bm_words = e->bm_words;
     P.trace_cap = e->trace_cap; P.S = e->S;
     P.pp = e->o.clients_per_round > 0 ? 1 : 0; P.pp_tau = e->o.clients_per_round;
+    P.gd_step = e->o.global_lr;
     // x -= dx (FedNL), x = solution (FedNL-PP), x left to the line search (FedNL-LS)
     P.update_x = P.pp ? 2 : (e->o.line_search ? 0 : 1);
     P.ls = e->o.line_search ? 1 : 0; P.ls_max = e->o.max_line_search_steps;
@@ -4217,8 +4306,33 @@ static void launch_hessian(Engine* e, bool init, double* out_learned, const Prob
                       out_learned, e->gpart != nullptr);
 }
 
+// one GD round: margins (gradient weights, f partials), client gradients,
+// their average with f and ||x||, the step
+static int launch_round_gd(Engine* e)
+{
+    const ProblemDev& P = e->hdesc;
+    const ProblemDev* pd = e->pdev;
+    const int2* nomap = nullptr;
+    {
+        KTime kt(e, KID_MARGIN);
+        launch_margin(e, false);
+    }
+    {
+        KTime kt(e, KID_GRAD);
+        hipLaunchKernelGGL(k_grad_f64, dim3(blocks_grad(P)), dim3(256), 0, e->st, pd, nomap);
+    }
+    {
+        KTime kt(e, KID_GD);
+        hipLaunchKernelGGL(k_gd_master, dim3(master_grad_blocks(P.d) + 1), dim3(256), 0, e->st, pd, nomap);
+        hipLaunchKernelGGL(k_gd_step, dim3(1), dim3(256), 0, e->st, pd, nomap);
+    }
+    return FEDNL_OK;
+}
+
 static int launch_round(Engine* e)
 {
+    if (e->o.algorithm == FEDNL_ALG_GD)
+        return launch_round_gd(e);
     const ProblemDev& P = e->hdesc;
     const ProblemDev* pd = e->pdev;
     const int2* nomap = nullptr;
@@ -4408,6 +4522,15 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         set_err("fednl_create: clients_per_round must be in [1, n] (FedNL-PP) or 0 (FedNL)");
         return FEDNL_E_WRONG_ARGUMENT;
     }
+    if (opts->algorithm != FEDNL_ALG_FEDNL && opts->algorithm != FEDNL_ALG_GD) {
+        set_err("fednl_create: algorithm must be FEDNL_ALG_FEDNL or FEDNL_ALG_GD");
+        return FEDNL_E_WRONG_ARGUMENT;
+    }
+    if (opts->algorithm == FEDNL_ALG_GD && (opts->line_search || opts->clients_per_round > 0)) {
+        // the reference parses the line-search flags only for FedNL (Args.h:321-380)
+        set_err("fednl_create: GD runs without line search and without FedNL-PP");
+        return FEDNL_E_UNSUPPORTED;
+    }
     if (opts->clients_per_round > 0 && opts->line_search) {
         // bin_fednl_local_pp has no line search (its Args.h drops the flags)
         set_err("fednl_create: FedNL-PP runs without line search");
@@ -4416,13 +4539,23 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     Engine* e = new Engine();
     e->hmode = opts->hessian_mode;
     e->o = *opts;
+    const bool gd = opts->algorithm == FEDNL_ALG_GD;
+    if (gd) {
+        // GD sends gradients only: no Hessians, no selection (the smallest
+        // selection buffers, never launched)
+        e->hmode = 0;
+        e->o.hessian_mode = 0;
+        e->o.compressor = FEDNL_RANDSEQK;
+    }
     e->n = n;
     e->m = m;
     e->d = d;
     e->lda = round_up(d, 4);
     e->S = (size_t)d * (d + 1) / 2;
-    const int comp = opts->compressor;
-    if (comp == FEDNL_IDENTICAL || comp == FEDNL_NATURAL)
+    const int comp = e->o.compressor;
+    if (gd)
+        e->k = 1;
+    else if (comp == FEDNL_IDENTICAL || comp == FEDNL_NATURAL)
         e->k = (int)e->S;  // TrainLoop.h:354-377
     else
         e->k = (int)(opts->k_mult * d);  // TrainLoop.h:362
@@ -4643,11 +4776,13 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     }
     {
         tune_margin(e);
-        launch_margin(e, false);
-        launch_hessian(e, true, nullptr);
-        hipLaunchKernelGGL(k_init_H, dim3((unsigned)((e->S + 255) / 256)), dim3(256), 0, e->st, e->H, e->learned,
-                           e->S, n, e->inv_n);
-        hipMemcpyAsync(e->H0, e->H, sizeof(double) * e->S, hipMemcpyDeviceToDevice, e->st);
+        if (!gd) {
+            launch_margin(e, false);
+            launch_hessian(e, true, nullptr);
+            hipLaunchKernelGGL(k_init_H, dim3((unsigned)((e->S + 255) / 256)), dim3(256), 0, e->st, e->H,
+                               e->learned, e->S, n, e->inv_n);
+            hipMemcpyAsync(e->H0, e->H, sizeof(double) * e->S, hipMemcpyDeviceToDevice, e->st);
+        }
         if (e->pp_sel) {
             // FedNL-PP: every client's gradient and direction at x0, their
             // average, l_c = 0, the client sampler's seed (Args.h:242-244)
@@ -5089,6 +5224,21 @@ void fednl_default_opts(fednl_opts* o)
 
 const char* fednl_last_error(void) { return g_err.c_str(); }
 
+int fednl_set_global_lr(void* handle, double global_lr)
+{
+    Engine* e = (Engine*)handle;
+    if (!e) {
+        set_err("fednl_set_global_lr: null handle");
+        return FEDNL_E_MISSING_ARGUMENT;
+    }
+    if (e->o.algorithm != FEDNL_ALG_GD) {
+        set_err("fednl_set_global_lr: the engine does not run GD");
+        return FEDNL_E_WRONG_ARGUMENT;
+    }
+    e->o.global_lr = global_lr;
+    return sync_desc(e);
+}
+
 int fednl_create(const fednl_opts* opts, int n, int m, int d, const double* features, const double* labels,
                  void** handle)
 {
@@ -5152,8 +5302,8 @@ int fednl_batch_create(void* const* engines, int count, int use_graph, void** ba
         return FEDNL_E_MISSING_ARGUMENT;
     }
     for (int i = 0; i < count; ++i)
-        if (engines[i] && ((Engine*)engines[i])->pp_sel) {
-            set_err("fednl_batch_create: FedNL-PP engines run on their own");
+        if (engines[i] && (((Engine*)engines[i])->pp_sel || ((Engine*)engines[i])->o.algorithm == FEDNL_ALG_GD)) {
+            set_err("fednl_batch_create: FedNL-PP and GD engines run on their own");
             return FEDNL_E_UNSUPPORTED;
         }
     Batch* b = new Batch();
@@ -5238,8 +5388,8 @@ int fednl_serve(const fednl_request* reqs, int count, double hbm_budget_bytes, i
         return FEDNL_E_MISSING_ARGUMENT;
     }
     for (int i = 0; i < count; ++i)
-        if (reqs[i].opts.clients_per_round > 0) {
-            set_err("fednl_serve: FedNL-PP requests are not batched");
+        if (reqs[i].opts.clients_per_round > 0 || reqs[i].opts.algorithm == FEDNL_ALG_GD) {
+            set_err("fednl_serve: FedNL-PP and GD requests are not batched");
             return FEDNL_E_UNSUPPORTED;
         }
     using clk = std::chrono::steady_clock;

@@ -35,7 +35,9 @@ class FednlOpts(ctypes.Structure):
         ("gamma_line_search", ctypes.c_double),
         ("clients_per_round", ctypes.c_int),
         ("client_selection_seed", ctypes.c_uint),
-        ("reserved", ctypes.c_int * 2),
+        ("algorithm", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("global_lr", ctypes.c_double),
     ]
 
 
@@ -151,13 +153,14 @@ class Engine:
     """One FedNL problem resident on the GPU."""
 
     def __init__(self, features, labels, lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0,
-                 option_b=True, workers=1, tracking=True, hessian="fp64", line_search=None, pp=None):
+                 option_b=True, workers=1, tracking=True, hessian="fp64", line_search=None, pp=None,
+                 algorithm="fednl1", global_lr=0.0):
         L = lib()
         feats = np.ascontiguousarray(features, dtype=np.float64)
         labs = np.ascontiguousarray(labels, dtype=np.float64)
         n, m, d = feats.shape
         o = make_opts(lam, compressor, k_mult, alpha_mode, alpha, option_b, workers, tracking, hessian, line_search,
-                      pp)
+                      pp, algorithm, global_lr)
         h = ctypes.c_void_p()
         _err(L.fednl_create(ctypes.byref(o), n, m, d, dp(feats), dp(labs), ctypes.byref(h)), "fednl_create")
         self.h = h
@@ -345,10 +348,14 @@ class Batch:
             pass
 
 
+ALGORITHMS = {"fednl1": 0, "gd": 1}
+
+
 def make_opts(lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0, option_b=True, workers=1,
-              tracking=False, hessian="fp64", line_search=None, pp=None):
+              tracking=False, hessian="fp64", line_search=None, pp=None, algorithm="fednl1", global_lr=0.0):
     """fednl_opts; line_search: None (FedNL) or (c, gamma, max_steps) for
-    FedNL-LS; pp: None or (clients per round, sampler seed) for FedNL-PP."""
+    FedNL-LS; pp: None or (clients per round, sampler seed) for FedNL-PP;
+    algorithm "gd" with its step size global_lr for gradient descent."""
     o = FednlOpts()
     lib().fednl_default_opts(ctypes.byref(o))
     o.lambda_ = lam
@@ -370,6 +377,8 @@ def make_opts(lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0, 
         tau, seed = pp
         o.clients_per_round = int(tau)
         o.client_selection_seed = int(seed)
+    o.algorithm = ALGORITHMS[algorithm]
+    o.global_lr = global_lr
     return o
 
 
