@@ -3495,19 +3495,25 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
     // entries above the diagonal of the diagonal block start at zero (a
     // failed factorisation or the fallback solve may have left anything
     // there)
-    for (int c = 0; c < kNB; ++c) {
-        const int col = kNB * J + c;
-        const gmem<double>* wc = W + (size_t)col * dp + r0;
-        for (int i = tid; i < nrows; i += nthr) {
-            const int row = r0 + i;
-            double v;
+    // one row per thread (nrows <= kSegRows = blockDim.x); the row's 32
+    // loads are all in flight before the first LDS store
+    if (tid < nrows) {
+        const int row = r0 + tid;
+        double v[kNB];
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+            const int col = kNB * J + c;
+            v[c] = (row < d && col < d && row >= col) ? W[(size_t)col * dp + row] : 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+            const int col = kNB * J + c;
+            double t;
             if (row >= d || col >= d)
-                v = (row == col) ? 1.0 : 0.0;
-            else if (row < col)
-                v = 0.0;
+                t = (row == col) ? 1.0 : 0.0;
             else
-                v = (row == col) ? wc[i] + shift : wc[i];
-            Col[i * kNewtonPS + c] = v;
+                t = (row == col) ? v[c] + shift : v[c];
+            Col[tid * kNewtonPS + c] = t;
         }
     }
     if (s == 0 && tid < kNB)
@@ -3837,18 +3843,20 @@ __global__ __launch_bounds__(kMcThreads) void k_newton_mc(FEDNL_KARGS)
         if (Jb > 0)
             load_chunk(jb - kNB, 0, vn);
         if (wave == 0 && lane < kNB) {
-            double acc = 0.0;
+            // four interleaved partial sums: a quarter of the dependent chain
+            double a4[4] = {0.0, 0.0, 0.0, 0.0};
             if (staged) {
                 const double* xr = Xs + (size_t)Jb * kNB * (kNB + 1) + lane * (kNB + 1);
 #pragma unroll
                 for (int c = 0; c < kNB; ++c)
-                    acc = fma(xr[c], yv[jb + c], acc);
+                    a4[c & 3] = fma(xr[c], yv[jb + c], a4[c & 3]);
             } else {
                 const gmem<double>* xr = nxt + (size_t)kNB * kNB * Jb + lane;  // X[lane][c] at xr[32 c]
 #pragma unroll
                 for (int c = 0; c < kNB; ++c)
-                    acc = fma(xr[kNB * c], yv[jb + c], acc);
+                    a4[c & 3] = fma(xr[kNB * c], yv[jb + c], a4[c & 3]);
             }
+            const double acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
             __builtin_amdgcn_wave_barrier();  // every lane has read r_J before any writes
             yv[jb + lane] = acc;
         }
