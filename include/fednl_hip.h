@@ -172,9 +172,11 @@ int fednl_set_state(void *handle, const double *x, const double *learned, const 
 
 /* The tile shapes the engine picked when it was created, by timing each
  * candidate on its own data: rows per wave of the margin pass (4 or 8; the
- * environment variable FEDNL_MARGIN_ROWS=4|8 sets it instead). Both shapes
- * compute the same bits. No counterpart in the reference. */
-int fednl_tuning(void *handle, int *margin_rows);
+ * environment variable FEDNL_MARGIN_ROWS=4|8 sets it instead) and, in the
+ * FP64 Hessian mode, the A^T D A tile's sample depth per LDS stage
+ * (16 or 32; FEDNL_HESS_CHUNK=16|32; hessian_chunk may be NULL). The
+ * candidates of each compute the same bits. No counterpart in the reference. */
+int fednl_tuning(void *handle, int *margin_rows, int *hessian_chunk);
 
 /* Per-kernel device time (milliseconds, summed over rounds since the last
  * reset) measured with HIP events on the engine stream when enabled. */

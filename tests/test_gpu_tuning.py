@@ -47,3 +47,45 @@ def test_tuner_picks_a_candidate():
     assert eng.tuning()["margin_rows"] in (4, 8)
     eng.run(3)
     eng.close()
+
+
+def _engine_chunk(prob, chunk, **kw):
+    from ufednl_amd.engine import Engine
+    os.environ["FEDNL_HESS_CHUNK"] = str(chunk)
+    try:
+        return Engine.from_problem(prob, **kw)
+    finally:
+        del os.environ["FEDNL_HESS_CHUNK"]
+
+
+@pytest.mark.parametrize("n,m,dfeat", [(7, 45, 20), (5, 130, 70), (3, 96, 150)])
+def test_hessian_chunks_give_identical_rounds(n, m, dfeat):
+    """The FP64 A^T D A tile staged 16 or 32 samples at a time: the same MFMA
+    sequence, so the same Delta, Lk, gradient and trajectory bits."""
+    from ufednl_amd import datagen
+    prob = datagen.small_problem(n=n, m=m, d_feat=dfeat, seed=n * m)
+    kw = dict(compressor="topk", k_mult=4.0, tracking=True)
+    e16 = _engine_chunk(prob, 16, **kw)
+    e32 = _engine_chunk(prob, 32, **kw)
+    assert e16.tuning()["hessian_chunk"] == 16 and e32.tuning()["hessian_chunk"] == 32
+    for r in range(6):
+        e16.run(1)
+        e32.run(1)
+        d16, d32 = e16.last_delta(), e32.last_delta()
+        np.testing.assert_array_equal(d16.view(np.uint64), d32.view(np.uint64), err_msg=f"round {r} Delta")
+    t16, t32 = e16.trace(), e32.trace()
+    for key in ("grad_norm", "f", "x_norm"):
+        np.testing.assert_array_equal(t16[key], t32[key], err_msg=key)
+    np.testing.assert_array_equal(e16.x(), e32.x())
+    e16.close()
+    e32.close()
+
+
+def test_hessian_tuner_picks_a_candidate():
+    from ufednl_amd import datagen
+    from ufednl_amd.engine import Engine
+    os.environ.pop("FEDNL_HESS_CHUNK", None)
+    eng = Engine.from_problem(datagen.small_problem(n=9, m=64, d_feat=30, seed=4), compressor="topk", k_mult=2.0)
+    assert eng.tuning()["hessian_chunk"] in (16, 32)
+    eng.run(3)
+    eng.close()

@@ -921,10 +921,15 @@ __global__ __launch_bounds__(256) void k_convert_bf16t(const double* __restrict_
 // block = 4 waves (2 x 2), each wave a 32 x 32 sub-tile of 2 x 2 MFMA 16x16 blocks.
 // Blocks are mapped so that all tiles of one client run on one XCD
 // (blockIdx % 8 selects the XCD), keeping the client's rows of A in that L2.
+// kCh: samples staged per LDS chunk (the tile's K extent), 16 (four
+// workgroups per CU) or 32 (two, half the barriers); the engine times both at
+// creation. Both issue the same MFMAs in the same order: the same bits.
 // ---------------------------------------------------------------------------
-template <bool kInit>
-__global__ __launch_bounds__(256, 4) void k_hessian_f64(FEDNL_KARGS, double* __restrict__ learned_out)
+template <bool kInit, int kCh>
+__global__ __launch_bounds__(256, kCh == 16 ? 4 : 2) void k_hessian_f64(FEDNL_KARGS, double* __restrict__ learned_out)
 {
+    static_assert(kCh == 16 || kCh == 32, "16 or 32 samples per chunk (m_pad is a multiple of 32)");
+    constexpr int kRG = kCh / 16;
     FEDNL_BLOCK();
     const gmem<double>* __restrict__ A = G(PD.A);
     const gmem<double>* __restrict__ Dw = G(PD.Dw);
@@ -947,9 +952,9 @@ __global__ __launch_bounds__(256, 4) void k_hessian_f64(FEDNL_KARGS, double* __r
     // two LDS buffers: the next chunk's rows are loaded into registers while
     // the current chunk is multiplied, then stored (scaled by D) into the
     // other buffer; one barrier per chunk
-    __shared__ __align__(16) double sI[2][kKC][kTile + kPadLds];
-    __shared__ __align__(16) double sJ[2][kKC][kTile + kPadLds];
-    __shared__ double sG[2][kKC];
+    __shared__ __align__(16) double sI[2][kCh][kTile + kPadLds];
+    __shared__ __align__(16) double sJ[2][kCh][kTile + kPadLds];
+    __shared__ double sG[2][kCh];
     __shared__ double red[2][4];
     __shared__ double gred[4][kTile];
 
@@ -969,55 +974,67 @@ __global__ __launch_bounds__(256, 4) void k_hessian_f64(FEDNL_KARGS, double* __r
     const gmem<double>* Dc = Dw + (size_t)c * m_pad;
     const gmem<double>* Gc = gw + (size_t)c * m_pad;
 
-    // load assignment: rows lr and lr + 8 of a chunk, columns lc, lc + 1 of
-    // both column blocks (diagonal tiles read the same block twice, from cache)
+    // load assignment: rows lr and lr + 8 of each 16-row group, columns lc,
+    // lc + 1 of both column blocks (diagonal tiles read the same block
+    // twice, from cache)
     const int lr = tid >> 5, lc = (tid & 31) * 2;
-    // named registers (no arrays: an array captured by the lambdas below
-    // would be placed in scratch memory)
-    double2 ri0, ri1, rj0, rj1;
-    double rd0, rd1, rg0, rg1;
-    auto gload = [&](int s0) {
-        const gmem<double>* row0 = Ac + (size_t)(s0 + lr) * d_pad;
-        const gmem<double>* row1 = Ac + (size_t)(s0 + lr + 8) * d_pad;
-        ri0 = *reinterpret_cast<const gmem<double2>*>(row0 + i0 + lc);
-        rj0 = *reinterpret_cast<const gmem<double2>*>(row0 + j0 + lc);
-        ri1 = *reinterpret_cast<const gmem<double2>*>(row1 + i0 + lc);
-        rj1 = *reinterpret_cast<const gmem<double2>*>(row1 + j0 + lc);
-        rd0 = Dc[s0 + lr];
-        rd1 = Dc[s0 + lr + 8];
-        rg0 = Gc[s0 + lr];
-        rg1 = Gc[s0 + lr + 8];
-    };
-    auto lstore = [&](int buf) {
-        *reinterpret_cast<double2*>(&sI[buf][lr][lc]) = ri0;
-        *reinterpret_cast<double2*>(&sI[buf][lr + 8][lc]) = ri1;
-        double2 sj0, sj1;
-        sj0.x = rd0 * rj0.x;
-        sj0.y = rd0 * rj0.y;
-        sj1.x = rd1 * rj1.x;
-        sj1.y = rd1 * rj1.y;
-        *reinterpret_cast<double2*>(&sJ[buf][lr][lc]) = sj0;
-        *reinterpret_cast<double2*>(&sJ[buf][lr + 8][lc]) = sj1;
-        if (lc == 0) {
-            sG[buf][lr] = rg0;
-            sG[buf][lr + 8] = rg1;
-        }
-    };
-
-    const int nch = m_pad / kKC;
-    gload(0);
-    lstore(0);
+    // one chunk's loads in registers: for each 16-row group q, rows lr and
+    // lr + 8, columns lc, lc + 1 of both column blocks, their D and g weights
+    double2 ri0[kRG], ri1[kRG], rj0[kRG], rj1[kRG];
+    double rd0[kRG], rd1[kRG], rg0[kRG], rg1[kRG];
+#define HESS_GLOAD_Q(q, s0_)                                                             \
+    {                                                                                    \
+        const int r_ = (s0_) + 16 * (q) + lr;                                            \
+        const gmem<double>* row0_ = Ac + (size_t)r_ * d_pad;                             \
+        const gmem<double>* row1_ = Ac + (size_t)(r_ + 8) * d_pad;                       \
+        ri0[q] = *reinterpret_cast<const gmem<double2>*>(row0_ + i0 + lc);               \
+        rj0[q] = *reinterpret_cast<const gmem<double2>*>(row0_ + j0 + lc);               \
+        ri1[q] = *reinterpret_cast<const gmem<double2>*>(row1_ + i0 + lc);               \
+        rj1[q] = *reinterpret_cast<const gmem<double2>*>(row1_ + j0 + lc);               \
+        rd0[q] = Dc[r_];                                                                 \
+        rd1[q] = Dc[r_ + 8];                                                             \
+        rg0[q] = Gc[r_];                                                                 \
+        rg1[q] = Gc[r_ + 8];                                                             \
+    }
+    // the chunk into LDS buffer b_, the J block scaled by D
+#define HESS_LSTORE_Q(q, b_)                                                             \
+    {                                                                                    \
+        const int r_ = 16 * (q) + lr;                                                    \
+        *reinterpret_cast<double2*>(&sI[b_][r_][lc]) = ri0[q];                           \
+        *reinterpret_cast<double2*>(&sI[b_][r_ + 8][lc]) = ri1[q];                       \
+        double2 sj0_, sj1_;                                                              \
+        sj0_.x = rd0[q] * rj0[q].x;                                                      \
+        sj0_.y = rd0[q] * rj0[q].y;                                                      \
+        sj1_.x = rd1[q] * rj1[q].x;                                                      \
+        sj1_.y = rd1[q] * rj1[q].y;                                                      \
+        *reinterpret_cast<double2*>(&sJ[b_][r_][lc]) = sj0_;                             \
+        *reinterpret_cast<double2*>(&sJ[b_][r_ + 8][lc]) = sj1_;                         \
+        if (lc == 0) {                                                                   \
+            sG[b_][r_] = rg0[q];                                                         \
+            sG[b_][r_ + 8] = rg1[q];                                                     \
+        }                                                                                \
+    }
+#define HESS_GLOAD(s0_)                                                                  \
+    HESS_GLOAD_Q(0, s0_)                                                                 \
+    if constexpr (kRG > 1) HESS_GLOAD_Q(kRG - 1, s0_)
+#define HESS_LSTORE(b_)                                                                  \
+    HESS_LSTORE_Q(0, b_)                                                                 \
+    if constexpr (kRG > 1) HESS_LSTORE_Q(kRG - 1, b_)
+    const int nch = m_pad / kCh;
+    HESS_GLOAD(0)
+    HESS_LSTORE(0)
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
         const int buf = ch & 1;
         const bool more = ch + 1 < nch;
-        if (more)
-            gload((ch + 1) * kKC);
+        if (more) {
+            HESS_GLOAD((ch + 1) * kCh)
+        }
         // on a diagonal tile the (wi = 1, wj = 0) quadrant lies below the
         // diagonal: that wave stages and reduces but multiplies nothing
         if (!below)
 #pragma unroll
-        for (int ks = 0; ks < kKC; ks += 4) {
+        for (int ks = 0; ks < kCh; ks += 4) {
             const int kk = ks + kq;
             const double a0 = sJ[buf][kk][wj * 32 + l16];
             const double a1 = sJ[buf][kk][wj * 32 + 16 + l16];
@@ -1029,15 +1046,24 @@ __global__ __launch_bounds__(256, 4) void k_hessian_f64(FEDNL_KARGS, double* __r
             acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
         if (!kInit && dg) {
+            // wave g4 takes samples 4 g4 .. 4 g4 + 3 of each 16-row group
             const int ii = tid & 63, g4 = tid >> 6;
 #pragma unroll
-            for (int kk = g4 * 4; kk < g4 * 4 + 4; ++kk)
-                gacc = fma(sI[buf][kk][ii], sG[buf][kk], gacc);
+            for (int q = 0; q < kRG; ++q)
+#pragma unroll
+                for (int kk = 16 * q + g4 * 4; kk < 16 * q + g4 * 4 + 4; ++kk)
+                    gacc = fma(sI[buf][kk][ii], sG[buf][kk], gacc);
         }
-        if (more)
-            lstore(buf ^ 1);
+        if (more) {
+            HESS_LSTORE(buf ^ 1)
+        }
         __syncthreads();
     }
+
+#undef HESS_GLOAD
+#undef HESS_LSTORE
+#undef HESS_GLOAD_Q
+#undef HESS_LSTORE_Q
 
     // epilogue: lane (l16, kq), item r of v_mfma_f64_16x16x4f64 holds
     // D[m = 4r + kq][n = l16] (pinned by the layout probe test), i.e.
@@ -3993,6 +4019,7 @@ struct Engine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int margin_rows = 4;           // k_margin rows per wave, picked at creation
+    int hess_chunk = 16;           // k_hessian_f64 samples per LDS chunk, picked at creation
     struct Pending { int kid; hipEvent_t a, b; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
@@ -4283,13 +4310,15 @@ static int blocks_master(const ProblemDev& P) { return master_h_blocks(P.S) + ma
 // out_learned) instead of Delta. pd / map / nblk: a descriptor array and
 // block map (batches); null pd means the engine itself.
 static void launch_hessian_on(hipStream_t st, int hmode, bool init, const ProblemDev* pd, const int2* map,
-                              int nblk_h, int nblk_g, double* out_learned, bool fused_grad)
+                              int nblk_h, int nblk_g, double* out_learned, bool fused_grad, int hchunk = 16)
 {
     if (hmode == 0) {
         if (init)
-            hipLaunchKernelGGL(k_hessian_f64<true>, dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
+            hipLaunchKernelGGL((k_hessian_f64<true, 16>), dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
+        else if (hchunk == 32)
+            hipLaunchKernelGGL((k_hessian_f64<false, 32>), dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
         else
-            hipLaunchKernelGGL(k_hessian_f64<false>, dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
+            hipLaunchKernelGGL((k_hessian_f64<false, 16>), dim3(nblk_h), dim3(256), 0, st, pd, map, out_learned);
         return;
     }
     if (!init && !fused_grad)
@@ -4311,7 +4340,52 @@ static void launch_hessian(Engine* e, bool init, double* out_learned, const Prob
 {
     const ProblemDev& P = e->hdesc;
     launch_hessian_on(e->st, e->hmode, init, pd ? pd : e->pdev, nullptr, blocks_hessian(P), blocks_grad(P),
-                      out_learned, e->gpart != nullptr);
+                      out_learned, e->gpart != nullptr, e->hess_chunk);
+}
+
+// The FP64 Hessian's chunk depth (16 or 32 samples per LDS stage), timed at
+// creation like the margin pass: each candidate on the engine's own data
+// (after the learned Hessians are set; a round's Hessian pass at x0 only
+// rewrites Delta, the Lk partials and the client gradients, which round 0
+// recomputes), five launches each after a warm-up. Both give the same bits.
+// FEDNL_HESS_CHUNK=16 or 32 sets the depth instead. (Measured on MI355X the
+// 16-deep stage wins at every shape tried: the 32-deep one holds two
+// workgroups per CU for its 80 KB of LDS.)
+static void tune_hessian(Engine* e)
+{
+    if (e->hmode != 0)
+        return;
+    const char* env = getenv("FEDNL_HESS_CHUNK");
+    if (env && (!strcmp(env, "16") || !strcmp(env, "32"))) {
+        e->hess_chunk = atoi(env);
+        return;
+    }
+    static const int cand[2] = {16, 32};
+    float best = 0.f;
+    int pick = 16;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        if (a) hipEventDestroy(a);
+        e->hess_chunk = 16;
+        return;
+    }
+    for (int q = 0; q < 2; ++q) {
+        e->hess_chunk = cand[q];
+        for (int w = 0; w < 2; ++w) launch_hessian(e, false, nullptr);
+        hipEventRecord(a, e->st);
+        for (int r = 0; r < 5; ++r) launch_hessian(e, false, nullptr);
+        hipEventRecord(b, e->st);
+        float ms = 0.f;
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
+            ms = 1e30f;
+        if (q == 0 || ms < best) {
+            best = ms;
+            pick = cand[q];
+        }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    e->hess_chunk = pick;
 }
 
 // one GD round: margins (gradient weights, f partials), client gradients,
@@ -4790,6 +4864,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
             hipLaunchKernelGGL(k_init_H, dim3((unsigned)((e->S + 255) / 256)), dim3(256), 0, e->st, e->H,
                                e->learned, e->S, n, e->inv_n);
             hipMemcpyAsync(e->H0, e->H, sizeof(double) * e->S, hipMemcpyDeviceToDevice, e->st);
+            tune_hessian(e);
         }
         if (e->pp_sel) {
             // FedNL-PP: every client's gradient and direction at x0, their
@@ -4997,7 +5072,7 @@ static void batch_launch_round(Batch* b)
         hipLaunchKernelGGL((k_margin<false, 4, true>), dim3(b->cnt[BK_MARGIN_G]), dim3(256), b->lds[BK_MARGIN_G], st,
                            pd, map(BK_MARGIN_G));
     if (b->cnt[BK_HESS64])
-        hipLaunchKernelGGL(k_hessian_f64<false>, dim3(b->cnt[BK_HESS64]), dim3(256), 0, st, pd, map(BK_HESS64),
+        hipLaunchKernelGGL((k_hessian_f64<false, 16>), dim3(b->cnt[BK_HESS64]), dim3(256), 0, st, pd, map(BK_HESS64),
                            (double*)nullptr);
     if (b->cnt[BK_GRAD])
         hipLaunchKernelGGL(k_grad_f64, dim3(b->cnt[BK_GRAD]), dim3(256), 0, st, pd, map(BK_GRAD));
@@ -5672,12 +5747,14 @@ int fednl_kernel_timing(void* handle, int enable)
     return fednl_kernel_timing_mask(handle, enable ? ~0u : 0u);
 }
 
-int fednl_tuning(void* handle, int* margin_rows)
+int fednl_tuning(void* handle, int* margin_rows, int* hessian_chunk)
 {
     Engine* e = (Engine*)handle;
     if (!e || !margin_rows)
         return FEDNL_E_MISSING_ARGUMENT;
     *margin_rows = e->margin_rows;
+    if (hessian_chunk)
+        *hessian_chunk = e->hess_chunk;
     return FEDNL_OK;
 }
 

@@ -101,7 +101,7 @@ def lib():
         L.fednl_set_state.argtypes = [ctypes.c_void_p, _c_dp, _c_dp, _c_dp, _c_up, _c_up]
         L.fednl_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.fednl_kernel_timing_mask.argtypes = [ctypes.c_void_p, ctypes.c_uint]
-        L.fednl_tuning.argtypes = [ctypes.c_void_p, _c_ip]
+        L.fednl_tuning.argtypes = [ctypes.c_void_p, _c_ip, _c_ip]
         L.fednl_kernel_times.argtypes = [ctypes.c_void_p, _c_dp, _c_ip, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_char_p)]
         L.fednl_select.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dp, _c_up, _c_up,
@@ -258,10 +258,11 @@ class Engine:
                                    up(c(st.get("lek"), np.uint32))), "fednl_set_state")
 
     def tuning(self):
-        """Tile shapes picked at creation: {"margin_rows": 4 or 8}."""
-        r = np.zeros(1, dtype=np.int32)
-        _err(lib().fednl_tuning(self.h, ip(r)), "fednl_tuning")
-        return {"margin_rows": int(r[0])}
+        """Tile shapes picked at creation: {"margin_rows": 4 or 8,
+        "hessian_chunk": 16 or 32 (samples per LDS stage of the FP64 A^T D A)}."""
+        r = np.zeros(2, dtype=np.int32)
+        _err(lib().fednl_tuning(self.h, ip(r[:1]), ip(r[1:])), "fednl_tuning")
+        return {"margin_rows": int(r[0]), "hessian_chunk": int(r[1])}
 
     def kernel_timing(self, enable=True, only=None):
         """Per-kernel device timing; only: a kernel name to time alone."""
