@@ -136,11 +136,14 @@ __device__ inline uint32_t pos_flat(uint64_t p, int lda)
     return (uint32_t)(j * (uint64_t)lda + i);
 }
 
-// list position p is a diagonal entry (j, j) iff p = j(j+3)/2, i.e. 8p+9 = (2j+3)^2
+// list position p is a diagonal entry (j, j) iff p = j(j+3)/2, i.e. 8p+9 = (2j+3)^2.
+// The root's estimate comes from the hardware's single-precision square root
+// (one instruction) and is corrected in integers, so the test is exact for
+// every p (8p + 9 below 2^62) at a fraction of a double sqrt's cost.
 __device__ inline bool pos_is_diag(uint64_t p)
 {
-    uint64_t v = 8 * p + 9;
-    uint64_t r = (uint64_t)sqrt((double)v);
+    const uint64_t v = 8 * p + 9;
+    uint64_t r = (uint64_t)__builtin_amdgcn_sqrtf((float)v);
     while (r * r > v) --r;
     while ((r + 1) * (r + 1) <= v) ++r;
     return r * r == v;
@@ -1881,7 +1884,13 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
                 nh = 0;
                 rh.n = 0;
             }
+#ifdef FEDNL_PROBE
+            unsigned long long pr_get = 0, pr_admit = 0, pr_rest = 0;
+#endif
             while (nh != 0) {
+#ifdef FEDNL_PROBE
+                const unsigned long long pt0 = clock64();
+#endif
                 const double prev = numerator;
                 const uint32_t pid = rh.r0i;
                 const double pv = rh.r0v;
@@ -1893,17 +1902,31 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
                     rh.get(hv, hid, last, lv, lid);
                     rh.kill(last);
                     rh.n = last;
+#ifdef FEDNL_PROBE
+                    const unsigned long long pt1 = clock64();
+                    pr_get += pt1 - pt0;
+#endif
                     rh.admit(hv, hid, lv, lid);
+#ifdef FEDNL_PROBE
+                    pr_admit += clock64() - pt1;
+#endif
                 } else {
                     rh.n = 0;
                 }
                 nh = last;
+#ifdef FEDNL_PROBE
+                const unsigned long long pt2 = clock64();
+#endif
                 if (pos_is_diag(pid))
                     numerator += pv * pv;
                 else
                     numerator += 2.0 * pv * pv;
                 const double next = numerator;
-                if (one_minus_delta_xf + eps > prev && one_minus_delta_xf < next + eps) {
+                const bool stop = one_minus_delta_xf + eps > prev && one_minus_delta_xf < next + eps;
+#ifdef FEDNL_PROBE
+                pr_rest += clock64() - pt2;
+#endif
+                if (stop) {
                     pp = (one_minus_delta_xf - prev + eps) / (next - prev + eps);
                     hit = 1;
                     popped = pid;
@@ -1912,6 +1935,9 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
             }
 #ifdef FEDNL_PROBE
             if (lane == 0) {
+                g_probe[2000 + 4 * c + 0] += pr_get;
+                g_probe[2000 + 4 * c + 1] += pr_admit;
+                g_probe[2000 + 4 * c + 2] += pr_rest;
                 g_probe[1000 + 8 * c + 6] += (unsigned long long)(k - nh);
                 g_probe[1000 + 8 * c + 7] += clock64() - t_walk;
             }

@@ -71,6 +71,11 @@ def main(argv=None):
               f"{s}: {adm[s]}, {cyc[s]}, {scan[s]}"
               + (f" | {sel[:, 6].mean():.0f}, {sel[:, 7].sum() / max(sel[:, 6].sum(), 1):.0f}" if lek else ""),
               flush=True)
+        if lek:
+            w = a[2000:2000 + 4 * eng.n].reshape(eng.n, 4)
+            ne = max(sel[:, 6].sum(), 1)
+            print(f"      walk cycles per extraction: last element out {w[:, 0].sum() / ne:.0f}, "
+                  f"sift {w[:, 1].sum() / ne:.0f}, energy test {w[:, 2].sum() / ne:.0f}", flush=True)
     eng.close()
 
 
