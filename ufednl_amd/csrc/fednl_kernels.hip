@@ -1176,12 +1176,18 @@ __device__ inline void heapify_cont(double* hv, uint32_t* hid, int i, int n, dou
         }
         if (lowest < 0)
             break;
-        hv[i] = lowv;
-        hid[i] = ci[lowest];
+        // one lane stores (every lane holds the same values; the wave's
+        // later LDS reads see the store in order)
+        if ((threadIdx.x & 63) == 0) {
+            hv[i] = lowv;
+            hid[i] = ci[lowest];
+        }
         i = first + lowest;
     }
-    hv[i] = xval;
-    hid[i] = xid;
+    if ((threadIdx.x & 63) == 0) {
+        hv[i] = xval;
+        hid[i] = xid;
+    }
 }
 
 // ---------------------------------------------------------------------------
