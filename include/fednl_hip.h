@@ -72,7 +72,7 @@ typedef struct fednl_opts {
                                    (bin_fednl_local_pp, --clients-per-round, TrainLoop.h:279-287) */
     unsigned client_selection_seed;  /* FedNL-PP client sampler seed (default 1234, Args.h:244) */
     int algorithm;              /* enum fednl_algorithm (--algorithm, TrainLoop.h:297-299) */
-    int reserved;
+    int hessian_form;           /* enum fednl_hessian_form: how the FP64 A^T D A is computed */
     double global_lr;           /* GD step size: --global_lr, or 2 / (mu + L) with
                                    --theoretical_global_lr, times --global_step_size_multiplier
                                    (TrainLoop.h:358,403-407,521) */
@@ -84,6 +84,19 @@ typedef struct fednl_opts {
  * TrainLoop.h:993-1052). GD uses neither the compressor nor the Hessians and
  * runs without line search and outside batches. */
 enum fednl_algorithm { FEDNL_ALG_FEDNL = 0, FEDNL_ALG_GD = 1 };
+
+/* FEDNL_HESS_DENSE: every client's A^T D A on FP64 MFMA over the padded
+ * design. FEDNL_HESS_SPARSE (FP64 mode, binary designs: every feature 0 or
+ * 1): each upper entry H_c[i,j] is the sum of D_s over the samples holding
+ * both features, in ascending sample order; with 0/1 features and +-1 labels
+ * every nonzero term of the reference's sum (logistic_regression.h:365-547)
+ * is exactly D_s, so this is the bits of its sequential sum, at a cost that
+ * follows the number of co-occurring feature pairs instead of m d^2.
+ * FEDNL_HESS_AUTO (0, the default) takes the sparse form in FP64 mode when
+ * the design is binary, those pairs number at most a tenth of m d(d+1)/2
+ * and the per-segment lists fit in LDS; FEDNL_HESS_FORM=dense|sparse in the
+ * environment sets it instead. No counterpart in the reference. */
+enum fednl_hessian_form { FEDNL_HESS_AUTO = 0, FEDNL_HESS_DENSE = 1, FEDNL_HESS_SPARSE = 2 };
 
 void fednl_default_opts(fednl_opts *o);
 
@@ -177,6 +190,10 @@ int fednl_set_state(void *handle, const double *x, const double *learned, const 
  * (16 or 32; FEDNL_HESS_CHUNK=16|32; hessian_chunk may be NULL). The
  * candidates of each compute the same bits. No counterpart in the reference. */
 int fednl_tuning(void *handle, int *margin_rows, int *hessian_chunk);
+
+/* The Hessian form the engine runs (FEDNL_HESS_DENSE or FEDNL_HESS_SPARSE),
+ * or a negative status. No counterpart in the reference. */
+int fednl_hessian_form(void *handle);
 
 /* Per-kernel device time (milliseconds, summed over rounds since the last
  * reset) measured with HIP events on the engine stream when enabled. */

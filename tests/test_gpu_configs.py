@@ -48,13 +48,15 @@ def _engine(prob, comp, **kw):
     return Engine.from_problem(prob, compressor=comp, k_mult=8.0, **kw)
 
 
-def _run_to_tolerance(case, work, comp, workers=1):
+def _run_to_tolerance(case, work, comp, workers=1, form="auto", expect_form=None):
     prob = _problem(work, case)
     g_ref = FIX[f"{case}_grad_norm"]
     R = len(g_ref)
     r_ref = int(FIX[f"{case}_r_star"])
-    print(f"{case}: {R} rounds", flush=True)
-    eng = _engine(prob, comp, workers=workers, tracking=False)
+    print(f"{case}: {R} rounds, Hessian form {form}", flush=True)
+    eng = _engine(prob, comp, workers=workers, tracking=False, hessian_form=form)
+    if expect_form is not None:
+        assert eng.hessian_form == expect_form
     eng.run(R)
     g = eng.trace()["grad_norm"]
     x = eng.x()
@@ -86,17 +88,20 @@ def test_config2_topk_lockstep(oracle):
     eng.close()
 
 
-def test_config2_topk_reaches_tolerance():
-    r_gpu, r_ref = _run_to_tolerance("c2_topk", "config2", "topk")
+@pytest.mark.parametrize("form,expect", [("auto", "sparse"), ("dense", "dense")])
+def test_config2_topk_reaches_tolerance(form, expect):
+    """Both FP64 Hessian forms: the binary 4%-dense design takes the sparse
+    one by default; the MFMA form is forced."""
+    r_gpu, r_ref = _run_to_tolerance("c2_topk", "config2", "topk", form=form, expect_form=expect)
     assert abs(r_gpu - r_ref) <= 10, (r_gpu, r_ref)
 
 
-@pytest.mark.parametrize("comp", ["randk", "seqk"])
-def test_config2_value_independent_trajectory(comp):
+@pytest.mark.parametrize("comp,form", [("randk", "auto"), ("seqk", "auto"), ("randk", "dense")])
+def test_config2_value_independent_trajectory(comp, form):
     case = f"c2_{comp}"
     prob = _problem("config2", case)
     g_ref = FIX[f"{case}_grad_norm"]
-    eng = _engine(prob, comp, tracking=False)
+    eng = _engine(prob, comp, tracking=False, hessian_form=form)
     eng.run(len(g_ref))
     g = eng.trace()["grad_norm"]
     eng.close()

@@ -64,7 +64,7 @@ def test_hessian_chunks_give_identical_rounds(n, m, dfeat):
     sequence, so the same Delta, Lk, gradient and trajectory bits."""
     from ufednl_amd import datagen
     prob = datagen.small_problem(n=n, m=m, d_feat=dfeat, seed=n * m)
-    kw = dict(compressor="topk", k_mult=4.0, tracking=True)
+    kw = dict(compressor="topk", k_mult=4.0, tracking=True, hessian_form="dense")
     e16 = _engine_chunk(prob, 16, **kw)
     e32 = _engine_chunk(prob, 32, **kw)
     assert e16.tuning()["hessian_chunk"] == 16 and e32.tuning()["hessian_chunk"] == 32
@@ -85,7 +85,8 @@ def test_hessian_tuner_picks_a_candidate():
     from ufednl_amd import datagen
     from ufednl_amd.engine import Engine
     os.environ.pop("FEDNL_HESS_CHUNK", None)
-    eng = Engine.from_problem(datagen.small_problem(n=9, m=64, d_feat=30, seed=4), compressor="topk", k_mult=2.0)
+    eng = Engine.from_problem(datagen.small_problem(n=9, m=64, d_feat=30, seed=4), compressor="topk", k_mult=2.0,
+                              hessian_form="dense")
     assert eng.tuning()["hessian_chunk"] in (16, 32)
     eng.run(3)
     eng.close()

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <chrono>
 #include <mutex>
+#include <queue>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -271,6 +272,7 @@ __device__ inline uint64_t round_nearest(double v)
 struct ProblemDev {
     int n, m, d, k, lda, m_pad, d_pad, tiles_up, dp, workers, comp, hmode, option_b, tracking, bm_words,
         trace_cap, update_x, ls, ls_max, pp, pp_tau;
+    int nparts;         // Lk partial slots per client: Hessian tiles (dense), segments (sparse)
     size_t S;
     double lambda, alpha, prescale, inv_n, mult, lek_delta, ls_c, ls_gamma;
     double gd_step;     // GD: the global step size
@@ -285,6 +287,14 @@ struct ProblemDev {
     // from the margin pass (n x m_pad/16 x d_pad), summed by the Hessian
     // kernels' diagonal tiles; null: k_grad_f64 computes grad_c
     double* gpart;
+    // sparse Hessian form (binary designs): per (client, dispatch rank) the
+    // segment record (segment index, stream length, stream offset in 16-bit
+    // tokens), the lane offsets inside its stream (kSpLanes + 1), the
+    // streams, each sample's label sign
+    const uint4* sp_seg;
+    const uint32_t* sp_lane;
+    const uint16_t* sp_stream;
+    const double* sp_b;
     // FedNL-PP: selected flags, client sampler state, per-client direction
     // (n x d_pad) and Lk, this round's messages, H_c(x) (option b), full gradient
     int* pp_sel;
@@ -1090,6 +1100,342 @@ __global__ __launch_bounds__(256, kCh == 16 ? 4 : 2) void k_hessian_f64(FEDNL_KA
 }
 
 // ---------------------------------------------------------------------------
+// k_hessian_sparse: the FP64 client Hessians of a binary design (every feature
+// 0 or 1). With labels b_s = +-1 the sign-folded entries are 0 or b_s, so every
+// nonzero term A[s,i] (D_s A[s,j]) of the sum over samples
+// (logistic_regression.h:365-547) is b_s (D_s b_s) = D_s exactly: H_c[i,j] is
+// the sum of D_s over the samples that hold both features, added in ascending
+// sample order, which is the bits of the sequential sum over all samples (its
+// zero terms change nothing). Each entry's samples are listed at creation.
+//
+// One workgroup per segment of kSpSeg consecutive packed upper entries of a
+// client (nparts segments per client, dispatched heaviest first; the blocks
+// of a client on one XCD). The segment's token stream holds, lane after lane,
+// the items each lane sums (balanced over the lanes at creation):
+//   header  bits 0-11 the entry's offset in the segment, bits 12-15 a code:
+//           1..12  an off-diagonal entry with that many samples;
+//           13     an off-diagonal entry, its sample count in the next token;
+//           14     a diagonal entry, its sample count in the next token;
+//           15     the client gradient at a column whose diagonal entry lies
+//                  in the segment: count, then the column, in the next two;
+//   then the item's sample indices, ascending.
+// Off-diagonal entries without a common sample are not listed (their H is 0);
+// every diagonal entry is, and so is every column's gradient
+// lambda x_i + sum_s b_s g_s over the same samples (logistic_regression.h:
+// 148-161, in the order of that sequential sum). The lanes add into LDS; then
+// the segment's Delta = H - learned (or, at creation, learned = H), its Lk
+// partials and (FedNL-PP option b) H_c are written in list order.
+// Dynamic LDS: H (kSpSeg), D, b g (m_pad each), x (d_pad), the lane offsets,
+// a diagonal flag per entry, the stream.
+// ---------------------------------------------------------------------------
+
+// Workgroup barrier for LDS data only: the wave's LDS operations complete,
+// then s_barrier. Unlike __syncthreads(), whose release fence also waits for
+// the wave's global stores, it leaves stores issued before it in flight.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+constexpr int kSpSeg = 2048;
+constexpr int kSpThreads = 256;             // threads per workgroup, one token lane each
+constexpr int kSpLanes = kSpThreads;
+constexpr int kSpLaneWords = kSpLanes + 4;  // lane offsets in LDS (8-byte aligned)
+__host__ __device__ inline size_t sparse_lds_bytes(int m_pad, int d_pad, size_t max_tokens)
+{
+    // (the stream gets eight tokens of slack: a lane reads up to four past its last header)
+    return sizeof(double) * ((size_t)kSpSeg + 2 * (size_t)m_pad + (size_t)d_pad) + sizeof(uint32_t) * kSpLaneWords +
+           (size_t)kSpSeg + sizeof(uint16_t) * ((max_tokens + 3) / 4 * 4 + 8);
+}
+
+template <bool kInit>
+__global__ __launch_bounds__(kSpThreads) void k_hessian_sparse(FEDNL_KARGS, double* __restrict__ learned_out)
+{
+    FEDNL_BLOCK();
+    const int n = PD.n, m_pad = PD.m_pad, d_pad = PD.d_pad, nparts = PD.nparts;
+    const double lambda = PD.lambda;
+    const size_t S = PD.S;
+    // block -> (client, dispatch rank), all segments of a client on one XCD;
+    // the rank's record names the segment
+    const int xcd = blk & 7, slot = blk >> 3;
+    const int q8 = slot / nparts;
+    const int rank = slot - q8 * nparts;
+    const int c = xcd + 8 * q8;
+    if (c >= n)
+        return;
+    if (!kInit && pp_idle(PD, c))
+        return;
+    constexpr int kWaves = kSpThreads / 64;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const size_t seg = (size_t)c * nparts + rank;  // record, lane table and Lk partial slot
+    // diagnostic build: per-segment stamps (start, staged, summed, end)
+    PROBE_STAMP_AT(6000 + 4 * (seg % 2000), 0);
+    const uint4 rec = G(PD.sp_seg)[seg];
+    const int t = (int)rec.x;
+    const size_t e0 = (size_t)t * kSpSeg;
+    const int nent = (int)((S - e0) < (size_t)kSpSeg ? (S - e0) : (size_t)kSpSeg);
+    const bool with_grad = !kInit && !PD.pp;
+    gmem<double>* __restrict__ learned = G(learned_out ? learned_out : PD.learned);
+    gmem<double>* __restrict__ delta = G(PD.delta);
+    gmem<double>* __restrict__ hout = G(PD.pp_h);
+    gmem<double>* __restrict__ part = G(PD.part);
+    constexpr int kPer = kSpSeg / kSpThreads;
+    const size_t base = (size_t)c * S + e0;
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* Hs = reinterpret_cast<double*>(smem);
+    double* Dl = Hs + kSpSeg;
+    double* Gl = Dl + m_pad;
+    double* Xl = Gl + m_pad;
+    uint32_t* Lo = reinterpret_cast<uint32_t*>(Xl + d_pad);
+    unsigned char* Df = reinterpret_cast<unsigned char*>(Lo + kSpLaneWords);  // diagonal entry flags
+    uint16_t* St = reinterpret_cast<uint16_t*>(Df + kSpSeg);
+    __shared__ double red[2][kWaves];
+
+    // staging: every load first (the weights D and b g, x, the lane offsets,
+    // the stream as 8-byte words), then the learned values this thread
+    // updates at the end, then the LDS writes, which wait for the staging
+    // loads only; the learned loads stay in flight through the sums. Sizes
+    // beyond the unrolled counts are staged by the loops after.
+    constexpr int kStM = 4, kStX = 2, kStL = 2, kStW = 8;
+    const gmem<uint32_t>* lane_g = G(PD.sp_lane) + seg * (kSpLanes + 1);
+    const gmem<double>* Dc = G(PD.Dw) + (size_t)c * m_pad;
+    const gmem<double>* Gc = G(PD.gw) + (size_t)c * m_pad;
+    const gmem<double>* Bc = G(PD.sp_b) + (size_t)c * m_pad;
+    const gmem<double>* __restrict__ xg = G(PD.x);
+    const uint32_t len = rec.y;
+    const uint32_t nwords = (len + 3) / 4;  // the stream starts 8-byte aligned, padded to whole words
+    const gmem<uint64_t>* sw =
+        reinterpret_cast<const gmem<uint64_t>*>(G(PD.sp_stream) + ((uint64_t)rec.w << 32 | rec.z));
+    double rd[kStM], rg[kStM], rb[kStM], rx[kStX];
+    uint32_t rl[kStL];
+    uint64_t rw[kStW];
+#pragma unroll
+    for (int u = 0; u < kStM; ++u) {
+        const int i = tid + kSpThreads * u;
+        const int ii = i < m_pad ? i : 0;
+        rd[u] = Dc[ii];
+        if (with_grad) {
+            rg[u] = Gc[ii];
+            rb[u] = Bc[ii];
+        }
+    }
+    if (with_grad) {
+#pragma unroll
+        for (int u = 0; u < kStX; ++u) {
+            const int i = tid + kSpThreads * u;
+            rx[u] = xg[i < d_pad ? i : 0];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kStL; ++u) {
+        const int i = tid + kSpThreads * u;
+        rl[u] = lane_g[i <= kSpLanes ? i : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < kStW; ++u) {
+        const uint32_t w = tid + kSpThreads * u;
+        rw[u] = sw[w < nwords ? w : 0];
+    }
+    double lv[kPer];
+    if (!kInit) {
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const int idx = tid + kSpThreads * r;
+            lv[r] = learned[base + (idx < nent ? idx : 0)];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kStM; ++u) {
+        const int i = tid + kSpThreads * u;
+        if (i < m_pad) {
+            Dl[i] = rd[u];
+            if (with_grad)
+                Gl[i] = rg[u] * rb[u];  // b_s g_s, exact
+        }
+    }
+    if (with_grad) {
+#pragma unroll
+        for (int u = 0; u < kStX; ++u) {
+            const int i = tid + kSpThreads * u;
+            if (i < d_pad)
+                Xl[i] = rx[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kStL; ++u) {
+        const int i = tid + kSpThreads * u;
+        if (i <= kSpLanes)
+            Lo[i] = rl[u];
+    }
+    uint64_t* Sw = reinterpret_cast<uint64_t*>(St);
+#pragma unroll
+    for (int u = 0; u < kStW; ++u) {
+        const uint32_t w = tid + kSpThreads * u;
+        if (w < nwords)
+            Sw[w] = rw[u];
+    }
+    for (int i = tid + kSpThreads * kStM; i < m_pad; i += kSpThreads) {
+        Dl[i] = Dc[i];
+        if (with_grad)
+            Gl[i] = Gc[i] * Bc[i];
+    }
+    if (with_grad)
+        for (int i = tid + kSpThreads * kStX; i < d_pad; i += kSpThreads) Xl[i] = xg[i];
+    for (uint32_t w = tid + kSpThreads * kStW; w < nwords; w += kSpThreads) Sw[w] = sw[w];
+    for (int i = tid; i < kSpSeg; i += kSpThreads) Hs[i] = 0.0;
+    for (int i = tid; i < kSpSeg / 4; i += kSpThreads) reinterpret_cast<uint32_t*>(Df)[i] = 0u;
+    lds_barrier();
+    PROBE_STAMP_AT(6000 + 4 * (seg % 2000) + 1, 0);
+
+    {
+#ifdef FEDNL_PROBE
+        const unsigned long long tw0 = wall_clock64();
+#endif
+        gmem<double>* __restrict__ grad_c = G(PD.grad_c) + (size_t)c * d_pad;
+        uint32_t p = Lo[tid];
+        const uint32_t pe = Lo[tid + 1];
+        while (p < pe) {
+            // the header with the next four tokens: an off-diagonal entry of at
+            // most four samples needs no further index reads
+            uint32_t tk[5];
+#pragma unroll
+            for (int u = 0; u < 5; ++u) tk[u] = St[p + u];
+            const uint32_t off = tk[0] & 0xFFFu, code = tk[0] >> 12;
+            if (code - 1u < 4u) {
+                double dv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) dv[u] = Dl[(uint32_t)u < code ? tk[1 + u] : tk[1]];
+                double acc = 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if ((uint32_t)u < code)
+                        acc = acc + dv[u];
+                Hs[off] = acc;
+                p += 1 + code;
+                continue;
+            }
+            ++p;
+            uint32_t cnt = code;
+            int col = -1;
+            if (code >= 13) {
+                cnt = tk[1];
+                ++p;
+                if (code == 15) {
+                    col = tk[2];
+                    ++p;
+                }
+            }
+            if (code == 15 && !with_grad) {
+                p += cnt;  // no gradient at creation or under FedNL-PP
+                continue;
+            }
+            // one sum per item: H (codes up to 14) or the gradient (15), in
+            // the same accumulator
+            const bool gr = code == 15;
+            double acc = gr ? lambda * Xl[col] : 0.0;
+            const double* W = gr ? Gl : Dl;
+            const uint16_t* sp = St + p;
+            uint32_t k = 0;
+            const uint32_t full = cnt & ~7u;
+            if (full) {
+                // batches of eight: the indices of the next batch are read
+                // with the weights of this one, then the ordered adds
+                int sn[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) sn[u] = sp[u];
+                for (; k < full; k += 8) {
+                    int sx[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) sx[u] = sn[u];
+                    if (k + 8 < full) {
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) sn[u] = sp[k + 8 + u];
+                    }
+                    double wv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) wv[u] = W[sx[u]];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc = acc + wv[u];
+                }
+            }
+            // the rest (under eight) in batches of four, past the end masked
+            for (; k < cnt; k += 4) {
+                int sx[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) sx[u] = sp[k + u < cnt ? k + u : k];
+                double wv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) wv[u] = W[sx[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (k + u < cnt)
+                        acc = acc + wv[u];
+            }
+            p += cnt;
+            if (gr) {
+                grad_c[col] = acc;
+            } else {
+                Hs[off] = acc;
+                if (code == 14)
+                    Df[off] = 1;
+            }
+        }
+#ifdef FEDNL_PROBE
+        // diagnostic build: each wave's time in the sums (first 200 segments)
+        if (lane == 0 && seg < 200)
+            g_probe[14000 + kWaves * seg + wave] = wall_clock64() - tw0;
+#endif
+    }
+    // (the gradient stores stay in flight)
+    lds_barrier();
+    PROBE_STAMP_AT(6000 + 4 * (seg % 2000) + 2, 0);
+
+    // the segment in list order: Delta (or learned), Lk partials
+    double soff = 0.0, sdiag = 0.0;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int idx = tid + kSpThreads * r;
+        if (idx < nent) {
+            double hv = Hs[idx];
+            const bool dgl = Df[idx] != 0;
+            if (dgl)
+                hv = hv + lambda;
+            if (kInit) {
+                learned[base + idx] = hv;
+            } else {
+                const double dv = hv - lv[r];
+                delta[base + idx] = dv;
+                if (hout)
+                    hout[base + idx] = hv;
+                if (dgl)
+                    sdiag = fma(dv, dv, sdiag);
+                else
+                    soff = fma(dv, dv, soff);
+            }
+        }
+    }
+    if (kInit)
+        return;
+    for (int off = 32; off > 0; off >>= 1) {
+        soff += __shfl_xor(soff, off);
+        sdiag += __shfl_xor(sdiag, off);
+    }
+    if (lane == 0) {
+        red[0][wave] = soff;
+        red[1][wave] = sdiag;
+    }
+    lds_barrier();  // (the Delta stores stay in flight)
+    if (tid == 0) {
+        double o = 0.0, dd = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w += 2) {
+            o = o + (red[0][w] + red[0][w + 1]);
+            dd = dd + (red[1][w] + red[1][w + 1]);
+        }
+        part[seg * 2 + 0] = o;
+        part[seg * 2 + 1] = dd;
+    }
+    PROBE_STAMP_AT(6000 + 4 * (seg % 2000) + 3, 0);
+}
+
+// ---------------------------------------------------------------------------
 // dopt's MinHeap<4>::minHeapify (MinHeap.h:255-290), values and ids in LDS.
 // After a swap it keeps comparing against the value that moved up.
 // ---------------------------------------------------------------------------
@@ -1603,20 +1949,20 @@ __device__ inline void compact_bitmap(const uint32_t* bm, size_t words, uint32_t
 // Lk (TopK/TopLEK worker: frobeniusNormForSymmetricMatrixFromUpPart of Delta,
 // LocalWorkerForFedNL_TopK.h:74-78) from the Hessian kernel's tile partials,
 // summed in tile order.
-__device__ inline double fro2_from_parts(const double* part, int c, int tiles_up)
+__device__ inline double fro2_from_parts(const double* part, int c, int nparts)
 {
-    // (off-diagonal, diagonal) pairs, eight tiles' loads in flight; summed in
-    // tile order
-    const gmem<double2>* pc = G(reinterpret_cast<const double2*>(part) + (size_t)c * tiles_up);
+    // (off-diagonal, diagonal) pairs, eight slots' loads in flight; summed in
+    // slot order
+    const gmem<double2>* pc = G(reinterpret_cast<const double2*>(part) + (size_t)c * nparts);
     double o = 0.0, dd = 0.0;
-    for (int t0 = 0; t0 < tiles_up; t0 += 8) {
+    for (int t0 = 0; t0 < nparts; t0 += 8) {
         double2 v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            v[u] = pc[t0 + u < tiles_up ? t0 + u : 0];
+            v[u] = pc[t0 + u < nparts ? t0 + u : 0];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (t0 + u < tiles_up) {
+            if (t0 + u < nparts) {
                 o = o + v[u].x;
                 dd = dd + v[u].y;
             }
@@ -1665,7 +2011,7 @@ __device__ __forceinline__ void select_heap_client(const ProblemDev& PD, int blk
     // TopLEK's energy target needs ||Delta||_F^2 (from the Hessian kernel's
     // tile partials)
     if (kLek && tid == 0)
-        s_fro2 = part ? fro2_from_parts(part, c, tiles_up) : 0.0;
+        s_fro2 = part ? fro2_from_parts(part, c, PD.nparts) : 0.0;
     if ((size_t)k >= S) {
         for (size_t p = tid; p < S; p += blockDim.x)
             sel[(size_t)c * k + p] = (uint32_t)p;
@@ -2575,7 +2921,7 @@ __device__ __forceinline__ void master_vec_block(const ProblemDev& PD)
     }
     __syncthreads();
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        const double lkc = sqrt(fro2_from_parts(part, c, tiles_up));
+        const double lkc = sqrt(fro2_from_parts(part, c, PD.nparts));
         lk[c] = lkc;
         // this round's participants and message items (integer counts)
         if (!pp || PD.pp_sel[c] != 0) {
@@ -4017,6 +4363,7 @@ static const char* kKernelNames[KID_COUNT] = {"k_margin", "k_hessian_f64", "k_se
 struct Engine {
     fednl_opts o;
     int n = 0, m = 0, d = 0, k = 0, lda = 0, m_pad = 0, d_pad = 0, nt = 0, tiles_up = 0, dp = 0, workers = 1;
+    int nparts = 0;  // Lk partial slots per client (tiles_up, or the sparse form's segments)
     size_t S = 0;
     double alpha = 1.0, prescale = 0.0, inv_n = 0.0, mult = 1.0, lek_delta = 0.0;
     hipStream_t st = nullptr;
@@ -4068,6 +4415,12 @@ struct Engine {
     __bf16* AThi = nullptr;        // transposed BF16 hi / lo of A (mode 2)
     __bf16* ATlo = nullptr;
     double* gpart = nullptr;       // modes 1 / 2 without FedNL-PP: margin-pass gradient partials
+    int hform = FEDNL_HESS_DENSE;  // the FP64 Hessian form picked at creation
+    uint4* sp_seg = nullptr;       // sparse form: segment records, lane tables, streams (k_hessian_sparse)
+    uint32_t* sp_lane = nullptr;
+    uint16_t* sp_stream = nullptr;
+    double* sp_b = nullptr;
+    size_t lds_sparse = 0;
 };
 
 static hipEvent_t pool_event(Engine* e)
@@ -4214,7 +4567,7 @@ static ProblemDev make_desc(const Engine* e, bool batched = false)
 {
     ProblemDev P{};
     P.n = e->n; P.m = e->m; P.d = e->d; P.k = e->k; P.lda = e->lda; P.m_pad = e->m_pad; P.d_pad = e->d_pad;
-    P.tiles_up = e->tiles_up; P.dp = e->dp; P.workers = e->workers; P.comp = e->o.compressor; P.hmode = e->hmode;
+    P.tiles_up = e->tiles_up; P.nparts = e->nparts; P.dp = e->dp; P.workers = e->workers; P.comp = e->o.compressor; P.hmode = e->hmode;
     P.option_b = e->o.option_b; P.tracking = e->o.tracking; P.bm_words = e->bm_words;
     P.trace_cap = e->trace_cap; P.S = e->S;
     P.pp = e->o.clients_per_round > 0 ? 1 : 0; P.pp_tau = e->o.clients_per_round;
@@ -4230,6 +4583,7 @@ static ProblemDev make_desc(const Engine* e, bool batched = false)
     P.A = e->A; P.A32 = e->A32; P.AThi = e->AThi; P.ATlo = e->ATlo;
     P.x = e->x; P.Dw = e->Dw; P.gw = e->gw; P.nlog = e->nlog; P.learned = e->learned; P.delta = e->delta;
     P.gpart = e->gpart;
+    P.sp_seg = e->sp_seg; P.sp_lane = e->sp_lane; P.sp_stream = e->sp_stream; P.sp_b = e->sp_b;
     P.grad_c = e->grad_c; P.part = e->part; P.lk = e->lk; P.vals = e->vals; P.H = e->H; P.g_avg = e->g_avg;
     P.shift = e->shift; P.W = e->W; P.dx = e->dx; P.trace = e->trace; P.lek_p = e->lek_p;
     P.hup = nullptr; P.nb = e->g_avg;
@@ -4334,6 +4688,7 @@ static void tune_margin(Engine* e)
     e->margin_rows = pick;
 }
 static int blocks_hessian(const ProblemDev& P) { return 8 * ((P.n + 7) / 8) * P.tiles_up; }
+static int blocks_sparse(const ProblemDev& P) { return 8 * ((P.n + 7) / 8) * P.nparts; }
 static int blocks_grad(const ProblemDev& P) { return (P.d_pad / kTile) * P.n; }
 static int blocks_master(const ProblemDev& P) { return master_h_blocks(P.S) + master_grad_blocks(P.d) + 1; }
 
@@ -4371,6 +4726,17 @@ static void launch_hessian_on(hipStream_t st, int hmode, bool init, const Proble
 static void launch_hessian(Engine* e, bool init, double* out_learned, const ProblemDev* pd = nullptr)
 {
     const ProblemDev& P = e->hdesc;
+    if (e->hform == FEDNL_HESS_SPARSE) {
+        const ProblemDev* q = pd ? pd : e->pdev;
+        const int2* nomap = nullptr;
+        if (init)
+            hipLaunchKernelGGL(k_hessian_sparse<true>, dim3(blocks_sparse(P)), dim3(kSpThreads), e->lds_sparse, e->st, q,
+                               nomap, out_learned);
+        else
+            hipLaunchKernelGGL(k_hessian_sparse<false>, dim3(blocks_sparse(P)), dim3(kSpThreads), e->lds_sparse, e->st, q,
+                               nomap, out_learned);
+        return;
+    }
     launch_hessian_on(e->st, e->hmode, init, pd ? pd : e->pdev, nullptr, blocks_hessian(P), blocks_grad(P),
                       out_learned, e->gpart != nullptr, e->hess_chunk);
 }
@@ -4385,7 +4751,7 @@ static void launch_hessian(Engine* e, bool init, double* out_learned, const Prob
 // workgroups per CU for its 80 KB of LDS.)
 static void tune_hessian(Engine* e)
 {
-    if (e->hmode != 0)
+    if (e->hmode != 0 || e->hform != FEDNL_HESS_DENSE)
         return;
     const char* env = getenv("FEDNL_HESS_CHUNK");
     if (env && (!strcmp(env, "16") || !strcmp(env, "32"))) {
@@ -4539,7 +4905,7 @@ static void free_engine(Engine* e)
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
                     e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev, e->mst, e->H0, e->pp_sel, e->pp_rng,
                     e->pp_dir, e->pp_lk, e->pp_gmsg, e->pp_lkmsg, e->pp_h, e->pp_tg, e->nsync, e->ny, e->ninv,
-                    e->nxt, e->sel_scr, e->gpart};
+                    e->nxt, e->sel_scr, e->gpart, e->sp_seg, e->sp_lane, e->sp_stream, e->sp_b};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -4607,6 +4973,198 @@ static double theory_alpha(int comp, int alpha_mode, double alpha, int k, int d,
     return 1.0;
 }
 
+// ---------------------------------------------------------------------------
+// Sparse Hessian form (k_hessian_sparse): eligibility and the token streams
+// ---------------------------------------------------------------------------
+static bool design_is_binary(const double* f, size_t count)
+{
+    for (size_t i = 0; i < count; ++i)
+        if (f[i] != 0.0 && f[i] != 1.0)
+            return false;
+    return true;
+}
+
+// sum over samples of z (z + 1) / 2, z the sample's nonzero features: the
+// terms the sparse form adds
+static double sparse_pair_count(const double* f, size_t rows, int d)
+{
+    double tot = 0.0;
+    for (size_t r = 0; r < rows; ++r) {
+        const double* a = f + r * d;
+        size_t z = 0;
+        for (int i = 0; i < d; ++i) z += (a[i] != 0.0);
+        tot += 0.5 * (double)z * (double)(z + 1);
+    }
+    return tot;
+}
+
+// an item's cost to its lane in k_hessian_sparse, in dependent LDS round
+// trips: header (with four samples for a short off-diagonal entry) and
+// weights; else header, batches of eight with the next batch's indices read
+// ahead, and the rest in masked batches of four
+static uint32_t sparse_item_cost(uint32_t cnt, bool short_entry)
+{
+    if (short_entry)
+        return 2;
+    const uint32_t rest = cnt & 7u;
+    return 2 + (cnt >= 8 ? 1 + (cnt >> 3) : 0) + (rest ? (rest > 4 ? 4 : 2) : 0);
+}
+
+struct SparsePlan {
+    std::vector<uint32_t> seg;     // per (client, dispatch rank): segment, length, stream offset (lo, hi)
+    std::vector<uint32_t> lane;    // per (client, dispatch rank): kSpLanes + 1 lane offsets
+    std::vector<uint16_t> stream;  // all token streams
+    std::vector<double> b;         // per (client, padded sample): label sign, 0 on padding
+    size_t max_tokens = 0;         // longest segment stream
+};
+
+// For every client: the samples of each upper entry (ascending); per segment
+// of kSpSeg entries the items (listed entries, and the gradient of each
+// column whose diagonal entry the segment holds) spread over kSpLanes lanes,
+// costliest first, each to the lane with the least cost so far (ties in an
+// order that alternates the waves), and the lanes' tokens in lane order;
+// the client's segments dispatched in decreasing order of their costliest
+// lane.
+static void build_sparse_plan(const double* features, const double* labels, int n, int m, int d, int m_pad,
+                              SparsePlan& P)
+{
+    const size_t S = (size_t)d * (d + 1) / 2;
+    const int nseg = (int)((S + kSpSeg - 1) / kSpSeg);
+    constexpr int kW = kSpLanes / 64;
+    P.seg.assign((size_t)n * nseg * 4, 0);
+    P.lane.assign((size_t)n * nseg * (kSpLanes + 1), 0);
+    P.stream.clear();
+    P.b.assign((size_t)n * m_pad, 0.0);
+    P.max_tokens = 0;
+    std::vector<int> diag_col(S, -1);
+    for (int j = 0; j < d; ++j) diag_col[(size_t)j * (j + 3) / 2] = j;
+    std::vector<uint32_t> cnt(S), start(S + 1), cur(S);
+    std::vector<uint16_t> hits;
+    std::vector<uint32_t> rowptr(m + 1);
+    std::vector<int> cols;
+    struct Item {
+        uint32_t cost, e;
+        bool grad;
+    };
+    std::vector<Item> items;
+    std::vector<std::vector<Item>> lane_items(kSpLanes);
+    std::vector<std::vector<uint16_t>> seg_tok(nseg);
+    std::vector<std::vector<uint32_t>> seg_lo(nseg);
+    std::vector<uint64_t> seg_max(nseg);
+    std::vector<int> order(nseg);
+    using Load = std::pair<uint64_t, int>;
+    for (int c = 0; c < n; ++c) {
+        cols.clear();
+        for (int s = 0; s < m; ++s) {
+            rowptr[s] = (uint32_t)cols.size();
+            const double* a = features + ((size_t)c * m + s) * d;
+            for (int i = 0; i < d; ++i)
+                if (a[i] != 0.0)
+                    cols.push_back(i);
+            P.b[(size_t)c * m_pad + s] = labels[(size_t)c * m + s] == 1.0 ? 1.0 : -1.0;
+        }
+        rowptr[m] = (uint32_t)cols.size();
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (int s = 0; s < m; ++s)
+            for (uint32_t bq = rowptr[s]; bq < rowptr[s + 1]; ++bq) {
+                const size_t col0 = (size_t)cols[bq] * (cols[bq] + 1) / 2;
+                for (uint32_t aq = rowptr[s]; aq <= bq; ++aq) cnt[col0 + cols[aq]]++;
+            }
+        start[0] = 0;
+        for (size_t e = 0; e < S; ++e) start[e + 1] = start[e] + cnt[e];
+        hits.resize(start[S]);
+        std::copy(start.begin(), start.begin() + S, cur.begin());
+        for (int s = 0; s < m; ++s)
+            for (uint32_t bq = rowptr[s]; bq < rowptr[s + 1]; ++bq) {
+                const size_t col0 = (size_t)cols[bq] * (cols[bq] + 1) / 2;
+                for (uint32_t aq = rowptr[s]; aq <= bq; ++aq) hits[cur[col0 + cols[aq]]++] = (uint16_t)s;
+            }
+        for (int t = 0; t < nseg; ++t) {
+            const size_t e0 = (size_t)t * kSpSeg, e1 = std::min(S, e0 + kSpSeg);
+            items.clear();
+            for (size_t e = e0; e < e1; ++e) {
+                const bool dg = diag_col[e] >= 0;
+                if (dg) {
+                    items.push_back({sparse_item_cost(cnt[e], false), (uint32_t)e, false});
+                    items.push_back({sparse_item_cost(cnt[e], false), (uint32_t)e, true});
+                } else if (cnt[e] > 0) {
+                    items.push_back({sparse_item_cost(cnt[e], cnt[e] <= 4), (uint32_t)e, false});
+                }
+            }
+            std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+                if (a.cost != b.cost)
+                    return a.cost > b.cost;
+                if (a.e != b.e)
+                    return a.e < b.e;
+                return a.grad < b.grad;
+            });
+            // heap over (cost so far, rank); rank r is lane (r % kW) * 64 + r / kW
+            std::priority_queue<Load, std::vector<Load>, std::greater<Load>> pq;
+            for (int r = 0; r < kSpLanes; ++r) {
+                pq.push({0, r});
+                lane_items[r].clear();
+            }
+            for (const Item& it : items) {
+                Load top = pq.top();
+                pq.pop();
+                lane_items[(top.second % kW) * 64 + top.second / kW].push_back(it);
+                top.first += it.cost;
+                pq.push(top);
+            }
+            uint64_t mx = 0;
+            auto& tok = seg_tok[t];
+            auto& lo = seg_lo[t];
+            tok.clear();
+            lo.assign(kSpLanes + 1, 0);
+            for (int l = 0; l < kSpLanes; ++l) {
+                lo[l] = (uint32_t)tok.size();
+                uint64_t lc = 0;
+                for (const Item& it : lane_items[l]) {
+                    const uint32_t e = it.e, off = (uint32_t)(e - e0), k = cnt[e];
+                    lc += it.cost;
+                    if (it.grad) {
+                        tok.push_back((uint16_t)(off | (15u << 12)));
+                        tok.push_back((uint16_t)k);
+                        tok.push_back((uint16_t)diag_col[e]);
+                    } else if (diag_col[e] >= 0) {
+                        tok.push_back((uint16_t)(off | (14u << 12)));
+                        tok.push_back((uint16_t)k);
+                    } else if (k <= 12) {
+                        tok.push_back((uint16_t)(off | (k << 12)));
+                    } else {
+                        tok.push_back((uint16_t)(off | (13u << 12)));
+                        tok.push_back((uint16_t)k);
+                    }
+                    tok.insert(tok.end(), hits.begin() + start[e], hits.begin() + start[e] + k);
+                }
+                mx = std::max(mx, lc);
+            }
+            lo[kSpLanes] = (uint32_t)tok.size();
+            seg_max[t] = mx;
+            P.max_tokens = std::max<size_t>(P.max_tokens, tok.size());
+        }
+        // dispatch order: costliest lane first (ties: lower segment)
+        for (int t = 0; t < nseg; ++t) order[t] = t;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return seg_max[a] > seg_max[b]; });
+        for (int r = 0; r < nseg; ++r) {
+            const int t = order[r];
+            while (P.stream.size() % 4) P.stream.push_back(0);
+            const uint64_t off = P.stream.size();
+            uint32_t* rec = &P.seg[((size_t)c * nseg + r) * 4];
+            rec[0] = (uint32_t)t;
+            rec[1] = (uint32_t)seg_tok[t].size();
+            rec[2] = (uint32_t)(off & 0xFFFFFFFFull);
+            rec[3] = (uint32_t)(off >> 32);
+            std::copy(seg_lo[t].begin(), seg_lo[t].end(), P.lane.begin() + ((size_t)c * nseg + r) * (kSpLanes + 1));
+            P.stream.insert(P.stream.end(), seg_tok[t].begin(), seg_tok[t].end());
+        }
+    }
+    P.stream.resize(P.stream.size() + 8, 0);  // the kernel copies whole 8-byte words
+}
+
+// LDS the sparse form may take per workgroup
+constexpr size_t kSpMaxLds = 160 * 1024 - 1024;
+
 static int create_engine(const fednl_opts* opts, int n, int m, int d, const double* features, const double* labels,
                          Engine** out)
 {
@@ -4624,6 +5182,10 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     }
     if (opts->hessian_mode < 0 || opts->hessian_mode > 2) {
         set_err("fednl_create: hessian_mode must be 0 (FP64), 1 (FP32) or 2 (BF16 split)");
+        return FEDNL_E_WRONG_ARGUMENT;
+    }
+    if (opts->hessian_form < FEDNL_HESS_AUTO || opts->hessian_form > FEDNL_HESS_SPARSE) {
+        set_err("fednl_create: hessian_form must be FEDNL_HESS_AUTO, FEDNL_HESS_DENSE or FEDNL_HESS_SPARSE");
         return FEDNL_E_WRONG_ARGUMENT;
     }
     if (opts->line_search &&
@@ -4693,6 +5255,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     e->d_pad = round_up(d, kTile);
     e->nt = e->d_pad / kTile;
     e->tiles_up = e->nt * (e->nt + 1) / 2;
+    e->nparts = e->tiles_up;
     e->dp = round_up(d, kNB);
     e->workers = std::max(1, std::min(opts->workers, n));
     e->alpha = theory_alpha(comp, opts->alpha_mode, opts->alpha, e->k, d, e->S);
@@ -4742,7 +5305,7 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
     ALLOC(e->learned, (size_t)n * e->S);
     ALLOC(e->delta, (size_t)n * e->S);
     ALLOC(e->grad_c, (size_t)n * e->d_pad);
-    ALLOC(e->part, (size_t)n * e->tiles_up * 2);
+    ALLOC(e->part, (size_t)n * std::max<size_t>(e->tiles_up, (e->S + kSpSeg - 1) / kSpSeg) * 2);
     ALLOC(e->lk, n);
     ALLOC(e->sel, (size_t)n * e->k);
     ALLOC(e->pos, (size_t)n * e->k);
@@ -4846,6 +5409,72 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
                 set_err("hipMemcpy A");
                 free_engine(e);
                 return FEDNL_E_DEVICE;
+            }
+        }
+    }
+    // the FP64 Hessian's form: sparse for binary designs with few
+    // co-occurring feature pairs (at most a tenth of the dense m S terms)
+    if (!gd) {
+        int want = opts->hessian_form;
+        const char* fe = getenv("FEDNL_HESS_FORM");
+        if (fe && !strcmp(fe, "dense"))
+            want = FEDNL_HESS_DENSE;
+        else if (fe && !strcmp(fe, "sparse"))
+            want = FEDNL_HESS_SPARSE;
+        const size_t cells = (size_t)n * m * d;
+        const bool binary = e->hmode == 0 && m <= 65535 && want != FEDNL_HESS_DENSE && design_is_binary(features, cells);
+        bool sparse = false;
+        if (binary) {
+            const double pairs = sparse_pair_count(features, (size_t)n * m, d);
+            sparse = want == FEDNL_HESS_SPARSE || pairs <= 0.1 * (double)n * (double)m * (double)e->S;
+        }
+        if (want == FEDNL_HESS_SPARSE && !sparse) {
+            set_err("fednl_create: the sparse Hessian form needs FP64 mode, m <= 65535 and 0/1 features");
+            free_engine(e);
+            return FEDNL_E_UNSUPPORTED;
+        }
+        if (sparse) {
+            SparsePlan plan;
+            build_sparse_plan(features, labels, n, m, d, e->m_pad, plan);
+            const size_t lds = sparse_lds_bytes(e->m_pad, e->d_pad, plan.max_tokens);
+            if (lds > kSpMaxLds) {
+                if (want == FEDNL_HESS_SPARSE) {
+                    set_err("fednl_create: a segment's sample lists exceed the LDS of the sparse Hessian form");
+                    free_engine(e);
+                    return FEDNL_E_UNSUPPORTED;
+                }
+                sparse = false;
+            } else {
+                hipError_t se = hipMalloc((void**)&e->sp_seg, sizeof(uint32_t) * plan.seg.size());
+                if (se == hipSuccess)
+                    se = hipMalloc((void**)&e->sp_lane, sizeof(uint32_t) * plan.lane.size());
+                if (se == hipSuccess)
+                    se = hipMalloc((void**)&e->sp_stream, sizeof(uint16_t) * plan.stream.size());
+                if (se == hipSuccess)
+                    se = hipMalloc((void**)&e->sp_b, sizeof(double) * plan.b.size());
+                if (se == hipSuccess)
+                    se = hipMemcpy(e->sp_seg, plan.seg.data(), sizeof(uint32_t) * plan.seg.size(),
+                                   hipMemcpyHostToDevice);
+                if (se == hipSuccess)
+                    se = hipMemcpy(e->sp_lane, plan.lane.data(), sizeof(uint32_t) * plan.lane.size(),
+                                   hipMemcpyHostToDevice);
+                if (se == hipSuccess)
+                    se = hipMemcpy(e->sp_stream, plan.stream.data(), sizeof(uint16_t) * plan.stream.size(),
+                                   hipMemcpyHostToDevice);
+                if (se == hipSuccess)
+                    se = hipMemcpy(e->sp_b, plan.b.data(), sizeof(double) * plan.b.size(), hipMemcpyHostToDevice);
+                if (se == hipSuccess)
+                    se = raise_lds_limit((const void*)k_hessian_sparse<true>, lds);
+                if (se == hipSuccess)
+                    se = raise_lds_limit((const void*)k_hessian_sparse<false>, lds);
+                if (se != hipSuccess) {
+                    set_err("sparse Hessian form upload", se);
+                    free_engine(e);
+                    return FEDNL_E_DEVICE;
+                }
+                e->lds_sparse = lds;
+                e->hform = FEDNL_HESS_SPARSE;
+                e->nparts = (int)((e->S + kSpSeg - 1) / kSpSeg);
             }
         }
     }
@@ -4979,7 +5608,7 @@ static int enqueue_rounds(Engine* e, int rounds)
 // captured once into a HIP graph and replayed.
 // ---------------------------------------------------------------------------
 enum BatchKernel {
-    BK_MARGIN, BK_MARGIN_G, BK_HESS64, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
+    BK_MARGIN, BK_MARGIN_G, BK_HESS64, BK_HESS_SP, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
     BK_EMIT, BK_SEL_TOPK_W, BK_SEL_TOPLEK_W, BK_SEL_RANDK_W, BK_SEL_WINDOW_W, BK_EMIT_W, BK_MASTER, BK_NEWTON,
     BK_LS_MARGIN, BK_LS, BK_COUNT
 };
@@ -5032,7 +5661,10 @@ static int batch_build(Batch* b)
             add(BK_MARGIN_G, p, blocks_margin(P), false, margin_grad_lds(e->d_pad));
         else
             add(BK_MARGIN, p, blocks_margin(P), false, sizeof(double) * (size_t)e->d_pad);
-        add(e->hmode == 0 ? BK_HESS64 : (e->hmode == 1 ? BK_HESS32 : BK_HESS16), p, blocks_hessian(P), true);
+        if (e->hform == FEDNL_HESS_SPARSE)
+            add(BK_HESS_SP, p, blocks_sparse(P), true, e->lds_sparse);
+        else
+            add(e->hmode == 0 ? BK_HESS64 : (e->hmode == 1 ? BK_HESS32 : BK_HESS16), p, blocks_hessian(P), true);
         if (e->hmode != 0 && !e->gpart)
             add(BK_GRAD, p, blocks_grad(P), false);
         const int comp = e->o.compressor;
@@ -5106,6 +5738,9 @@ static void batch_launch_round(Batch* b)
     if (b->cnt[BK_HESS64])
         hipLaunchKernelGGL((k_hessian_f64<false, 16>), dim3(b->cnt[BK_HESS64]), dim3(256), 0, st, pd, map(BK_HESS64),
                            (double*)nullptr);
+    if (b->cnt[BK_HESS_SP])
+        hipLaunchKernelGGL(k_hessian_sparse<false>, dim3(b->cnt[BK_HESS_SP]), dim3(kSpThreads), b->lds[BK_HESS_SP], st, pd,
+                           map(BK_HESS_SP), (double*)nullptr);
     if (b->cnt[BK_GRAD])
         hipLaunchKernelGGL(k_grad_f64, dim3(b->cnt[BK_GRAD]), dim3(256), 0, st, pd, map(BK_GRAD));
     if (b->cnt[BK_HESS32])
@@ -5335,6 +5970,7 @@ void fednl_default_opts(fednl_opts* o)
     o->gamma_line_search = 0.5;
     o->clients_per_round = 0;            // FedNL; > 0: FedNL-PP
     o->client_selection_seed = 1234u;    // bin_fednl_local_pp/include/Args.h:244
+    o->hessian_form = FEDNL_HESS_AUTO;
 }
 
 const char* fednl_last_error(void) { return g_err.c_str(); }
@@ -5790,6 +6426,14 @@ int fednl_tuning(void* handle, int* margin_rows, int* hessian_chunk)
     return FEDNL_OK;
 }
 
+int fednl_hessian_form(void* handle)
+{
+    Engine* e = (Engine*)handle;
+    if (!e)
+        return FEDNL_E_MISSING_ARGUMENT;
+    return e->hform;
+}
+
 int fednl_kernel_timing_mask(void* handle, unsigned mask)
 {
     Engine* e = (Engine*)handle;
@@ -5955,6 +6599,7 @@ int fednl_select(int compressor, int nclients, int d, int k, const double* delta
     hp.S = S;
     hp.lda = round_up(d, 4);
     hp.tiles_up = (compressor == FEDNL_TOPLEK) ? 1 : 0;
+    hp.nparts = hp.tiles_up;
     hp.part = parts;
     hp.lek_delta = delta_param;
     hp.sel_ring = wide ? kMaxScanRing : scan_ring_slots(std::min(k, kMaxHeapK), S, big);

@@ -36,7 +36,7 @@ class FednlOpts(ctypes.Structure):
         ("clients_per_round", ctypes.c_int),
         ("client_selection_seed", ctypes.c_uint),
         ("algorithm", ctypes.c_int),
-        ("reserved", ctypes.c_int),
+        ("hessian_form", ctypes.c_int),
         ("global_lr", ctypes.c_double),
     ]
 
@@ -102,6 +102,7 @@ def lib():
         L.fednl_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.fednl_kernel_timing_mask.argtypes = [ctypes.c_void_p, ctypes.c_uint]
         L.fednl_tuning.argtypes = [ctypes.c_void_p, _c_ip, _c_ip]
+        L.fednl_hessian_form.argtypes = [ctypes.c_void_p]
         L.fednl_kernel_times.argtypes = [ctypes.c_void_p, _c_dp, _c_ip, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_char_p)]
         L.fednl_select.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dp, _c_up, _c_up,
@@ -154,13 +155,13 @@ class Engine:
 
     def __init__(self, features, labels, lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0,
                  option_b=True, workers=1, tracking=True, hessian="fp64", line_search=None, pp=None,
-                 algorithm="fednl1", global_lr=0.0):
+                 algorithm="fednl1", global_lr=0.0, hessian_form="auto"):
         L = lib()
         feats = np.ascontiguousarray(features, dtype=np.float64)
         labs = np.ascontiguousarray(labels, dtype=np.float64)
         n, m, d = feats.shape
         o = make_opts(lam, compressor, k_mult, alpha_mode, alpha, option_b, workers, tracking, hessian, line_search,
-                      pp, algorithm, global_lr)
+                      pp, algorithm, global_lr, hessian_form)
         h = ctypes.c_void_p()
         _err(L.fednl_create(ctypes.byref(o), n, m, d, dp(feats), dp(labs), ctypes.byref(h)), "fednl_create")
         self.h = h
@@ -264,6 +265,15 @@ class Engine:
         _err(lib().fednl_tuning(self.h, ip(r[:1]), ip(r[1:])), "fednl_tuning")
         return {"margin_rows": int(r[0]), "hessian_chunk": int(r[1])}
 
+    @property
+    def hessian_form(self):
+        """The FP64 Hessian form picked at creation: "dense" (MFMA over the
+        padded design) or "sparse" (binary designs: sums over the samples
+        holding both features)."""
+        v = lib().fednl_hessian_form(self.h)
+        _err(min(v, 0), "fednl_hessian_form")
+        return {v_: k_ for k_, v_ in HESSIAN_FORMS.items()}[v]
+
     def kernel_timing(self, enable=True, only=None):
         """Per-kernel device timing; only: a kernel name to time alone."""
         if only is None:
@@ -350,13 +360,16 @@ class Batch:
 
 
 ALGORITHMS = {"fednl1": 0, "gd": 1}
+HESSIAN_FORMS = {"auto": 0, "dense": 1, "sparse": 2}
 
 
 def make_opts(lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0, option_b=True, workers=1,
-              tracking=False, hessian="fp64", line_search=None, pp=None, algorithm="fednl1", global_lr=0.0):
+              tracking=False, hessian="fp64", line_search=None, pp=None, algorithm="fednl1", global_lr=0.0,
+              hessian_form="auto"):
     """fednl_opts; line_search: None (FedNL) or (c, gamma, max_steps) for
     FedNL-LS; pp: None or (clients per round, sampler seed) for FedNL-PP;
-    algorithm "gd" with its step size global_lr for gradient descent."""
+    algorithm "gd" with its step size global_lr for gradient descent;
+    hessian_form "auto", "dense" or "sparse" (the FP64 A^T D A)."""
     o = FednlOpts()
     lib().fednl_default_opts(ctypes.byref(o))
     o.lambda_ = lam
@@ -380,6 +393,7 @@ def make_opts(lam=1e-3, compressor="topk", k_mult=8.0, alpha_mode=2, alpha=1.0, 
         o.client_selection_seed = int(seed)
     o.algorithm = ALGORITHMS[algorithm]
     o.global_lr = global_lr
+    o.hessian_form = HESSIAN_FORMS[hessian_form]
     return o
 
 
