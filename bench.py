@@ -56,6 +56,12 @@ def algorithmic_work(name, n, m, d, k, S):
     if name == "k_hessian_f64":
         # upper triangle incl. diagonal of A_c^T D_c A_c: m FMAs per entry
         return "mfma", float(n) * m * d * (d + 1)
+    if name == "k_hessian_sparse":
+        # binary designs, sums over co-occurring samples: learned read and
+        # Delta written per upper entry, the sample weights D and g read, the
+        # client gradient written (the per-entry sample lists, a fixed
+        # encoding of the design, are the kernel's own overhead)
+        return "hbm", float(n) * (16.0 * S + 16.0 * m + 8.0 * d)
     if name == "k_hessian_f32":
         # the same products in FP32
         return "mfma", float(n) * m * d * (d + 1)
@@ -187,6 +193,7 @@ def small_configs_leg(rounds=1000, warm=None):
             eng.enqueue(rounds)
             _sync_all(eng)
             res[f"rounds_per_s_{form}"] = rounds / (time.perf_counter() - t0)
+            res["hessian_form"] = eng.hessian_form
             eng.close()
         res["rounds"] = rounds
         out[wname] = res
@@ -194,9 +201,18 @@ def small_configs_leg(rounds=1000, warm=None):
     return out
 
 
+def _hessian_stage_us(kt):
+    """Per-launch time of the Hessian stage, whichever kernel runs it."""
+    name = next(k for k in kt if k.startswith("k_hessian"))
+    return 1e3 * kt[name][0] / max(kt[name][1], 1)
+
+
 def mixed_precision_leg(rounds_c2, rounds_c3, warm=None):
     """FP64 / FP32 / BF16-split A^T D A on the 142 x 350, d=300 TopK workload
-    and on the 80 x 407, d=123 RandSeqK / TopLEK workload."""
+    and on the 80 x 407, d=123 RandSeqK / TopLEK workload. "fp64" is the
+    engine's default FP64 form (on these binary designs the sparse one),
+    "fp64_dense" the FP64 MFMA contraction; the FP32 and BF16 modes run the
+    dense contraction."""
     from ufednl_amd import datagen
     from ufednl_amd.engine import Engine
     out = {}
@@ -205,8 +221,9 @@ def mixed_precision_leg(rounds_c2, rounds_c3, warm=None):
                                       ("n80_m407_d123_toplek", datagen.config3(), "toplek", rounds_c3)):
         res = {}
         ref = None
-        for mode in ("fp64", "fp32", "bf16x2"):
-            eng = Engine.from_problem(prob, compressor=comp, k_mult=8.0, hessian=mode, tracking=False)
+        for mode in ("fp64", "fp64_dense", "fp32", "bf16x2"):
+            kw = dict(hessian="fp64", hessian_form="dense") if mode == "fp64_dense" else dict(hessian=mode)
+            eng = Engine.from_problem(prob, compressor=comp, k_mult=8.0, tracking=False, **kw)
             _warm(warm)
             r, t, rate = time_to_tol(eng, rounds)
             eng.kernel_timing(True)
@@ -215,8 +232,8 @@ def mixed_precision_leg(rounds_c2, rounds_c3, warm=None):
             eng.kernel_timing(False)
             # the Hessian stage: the contraction of the mode (below FP64 with the sums of the margin
             # pass's gradient partials on the diagonal tiles)
-            entry = dict(rounds_to_tol=r, time_to_tol_s=t, rounds_per_s=rate,
-                         hessian_stage_us=1e3 * kt["k_hessian_f64"][0] / max(kt["k_hessian_f64"][1], 1))
+            entry = dict(rounds_to_tol=r, time_to_tol_s=t, rounds_per_s=rate, hessian_form=eng.hessian_form,
+                         hessian_stage_us=_hessian_stage_us(kt))
             if mode == "fp64":
                 ref = eng
                 entry["x_final"] = eng.x()
@@ -230,9 +247,12 @@ def mixed_precision_leg(rounds_c2, rounds_c3, warm=None):
         for v in res.values():
             v.pop("x_final", None)
         base = res["fp64"]["time_to_tol_s"]
-        for mode in ("fp32", "bf16x2"):
+        base_d = res["fp64_dense"]["time_to_tol_s"]
+        for mode in ("fp64_dense", "fp32", "bf16x2"):
             t = res[mode]["time_to_tol_s"]
             res[mode]["time_to_tol_speedup_vs_fp64"] = (base / t) if (base and t) else None
+            if mode != "fp64_dense":
+                res[mode]["time_to_tol_speedup_vs_fp64_dense"] = (base_d / t) if (base_d and t) else None
         out[wname] = res
     return out
 
@@ -532,7 +552,7 @@ def main():
         "data": "synthetic (seeded planted-logistic binary features, generated in-process)",
         "config": {"workload": "fednl_topk8d_fp64_n142_m350_d300", "clients": n, "samples_per_client": m,
                    "d": d, "k": eng.k, "compressor": "topk", "lambda": prob.lam, "option": "b", "alpha": eng.alpha,
-                   "parallelism": "all clients on one GPU"},
+                   "hessian_form": eng.hessian_form, "parallelism": "all clients on one GPU"},
         "time_to_tol_s": t_tol,
         "rounds_to_tol": r_tol,
         "time_to_tol_leg": {"rounds_to_tol": r_tol2, "time_to_tol_s": t_tol2, "rounds_per_s": rate_tol,

@@ -295,6 +295,10 @@ struct ProblemDev {
     const uint32_t* sp_lane;
     const uint16_t* sp_stream;
     const double* sp_b;
+    // and for its margin pass: per client m + 1 row offsets into the column
+    // lists, each sample's nonzero columns (ascending)
+    const uint32_t* sp_rowptr;
+    const uint16_t* sp_cols;
     // FedNL-PP: selected flags, client sampler state, per-client direction
     // (n x d_pad) and Lk, this round's messages, H_c(x) (option b), full gradient
     int* pp_sel;
@@ -1097,6 +1101,95 @@ __global__ __launch_bounds__(256, kCh == 16 ? 4 : 2) void k_hessian_f64(FEDNL_KA
     // FedNL-PP takes every client's gradient from k_grad_f64
     hess_epilogue<kInit>(hval, iv, jv, c, d, d_pad, lambda, S, tiles_up, t, x, learned, delta, G(PD.pp_h), part,
                          grad_c, !kInit && dg && !PD.pp, gacc, i0, red, gred);
+}
+
+// ---------------------------------------------------------------------------
+// k_margin_sparse: the margin pass of a binary design (every feature 0 or 1)
+// from each sample's nonzero columns. The sign-folded row is b_s on those
+// columns, so each nonzero term of t_s = sum_i A[s,i] x_i
+// (logistic_regression.h:98-101) is b_s x_i; added in column order they are
+// the bits of the sequential sum over all columns (its zero terms change
+// nothing). Then sigma, the sample weights D and g and the -log sigma
+// partials per 16 samples as k_margin; kTrial as there. One lane per sample,
+// kSpMarginRows samples per workgroup; dynamic LDS: x.
+// ---------------------------------------------------------------------------
+constexpr int kSpMarginRows = 256;
+template <bool kTrial>
+__global__ __launch_bounds__(kSpMarginRows) void k_margin_sparse(FEDNL_KARGS)
+{
+    FEDNL_BLOCK();
+    if (kTrial && (!PD.ls || PD.mst[LS_DONE] != 0.0))
+        return;
+    if (!kTrial && blk == 0 && threadIdx.x == 0 && PD.round_ctr) {
+        // the round starts (the first kernel of every round)
+        const int r = *PD.round_ctr;
+        if (r < PD.trace_cap)
+            PD.trace[(size_t)r * kTraceStride + kTraceT0] = (double)wall_clock64();
+    }
+    const int m = PD.m, m_pad = PD.m_pad, d = PD.d, d_pad = PD.d_pad;
+    const double inv_m = 1.0 / (double)m;
+    const int wpc = (m_pad + kSpMarginRows - 1) / kSpMarginRows;
+    const int c = blk / wpc, bx = blk - c * wpc;
+    const int tid = threadIdx.x;
+    const int s = bx * kSpMarginRows + tid;
+    extern __shared__ double xs[];
+    __shared__ double nl[kSpMarginRows];
+    // the row's column range and its label sign, loaded with x
+    const gmem<uint32_t>* rp = G(PD.sp_rowptr) + (size_t)c * (m + 1);
+    uint32_t k0 = 0, k1 = 0;
+    double bs = 0.0;
+    if (s < m) {
+        k0 = rp[s];
+        k1 = rp[s + 1];
+        bs = G(PD.sp_b)[(size_t)c * m_pad + s];
+    }
+    {
+        const gmem<double>* __restrict__ x = G(PD.x);
+        for (int i = tid; i < d_pad; i += kSpMarginRows) xs[i] = (i < d) ? x[i] : 0.0;
+    }
+    __syncthreads();
+    const gmem<uint16_t>* cols = G(PD.sp_cols);
+    double acc = 0.0;
+    // eight columns at a time: indices, then x, then the ordered adds
+    for (uint32_t k = k0; k < k1; k += 8) {
+        int ci[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ci[u] = cols[k + u < k1 ? k + u : k];
+        double xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = xs[ci[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k + u < k1)
+                acc = acc + bs * xv[u];
+    }
+    if (s < m) {
+        // logistic_regression.h:98-101,148-161,398-404
+        const double sg = 1.0 / (1.0 + exp(-acc));
+        if (!kTrial) {
+            const double z1 = 1.0 - sg;
+            const double z2 = z1 * sg;
+            G(PD.Dw)[(size_t)c * m_pad + s] = inv_m * z2;
+            G(PD.gw)[(size_t)c * m_pad + s] = z1 * (-inv_m);
+        }
+        nl[tid] = -log(sg);
+    } else {
+        if (!kTrial && s < m_pad) {
+            G(PD.Dw)[(size_t)c * m_pad + s] = 0.0;
+            G(PD.gw)[(size_t)c * m_pad + s] = 0.0;
+        }
+        nl[tid] = 0.0;
+    }
+    __syncthreads();
+    if (tid < kSpMarginRows / 16) {
+        // per-16-row partial of sum_s -log sigma_s (samples in order)
+        const int h = tid;
+        if (bx * kSpMarginRows + 16 * h < m_pad) {
+            double sacc = 0.0;
+            for (int q = 0; q < 16; ++q) sacc = sacc + nl[16 * h + q];
+            G(PD.nlog)[(size_t)c * (m_pad / 16) + bx * (kSpMarginRows / 16) + h] = sacc;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -4420,6 +4513,8 @@ struct Engine {
     uint32_t* sp_lane = nullptr;
     uint16_t* sp_stream = nullptr;
     double* sp_b = nullptr;
+    uint32_t* sp_rowptr = nullptr;  // sparse form: the margin pass's row lists (k_margin_sparse)
+    uint16_t* sp_cols = nullptr;
     size_t lds_sparse = 0;
 };
 
@@ -4584,6 +4679,7 @@ static ProblemDev make_desc(const Engine* e, bool batched = false)
     P.x = e->x; P.Dw = e->Dw; P.gw = e->gw; P.nlog = e->nlog; P.learned = e->learned; P.delta = e->delta;
     P.gpart = e->gpart;
     P.sp_seg = e->sp_seg; P.sp_lane = e->sp_lane; P.sp_stream = e->sp_stream; P.sp_b = e->sp_b;
+    P.sp_rowptr = e->sp_rowptr; P.sp_cols = e->sp_cols;
     P.grad_c = e->grad_c; P.part = e->part; P.lk = e->lk; P.vals = e->vals; P.H = e->H; P.g_avg = e->g_avg;
     P.shift = e->shift; P.W = e->W; P.dx = e->dx; P.trace = e->trace; P.lek_p = e->lek_p;
     P.hup = nullptr; P.nb = e->g_avg;
@@ -4617,9 +4713,23 @@ static int blocks_margin(const ProblemDev& P) { return (P.m_pad / 16) * P.n; }
 static size_t margin_grad_lds(int d_pad) { return 5 * sizeof(double) * (size_t)d_pad; }
 
 // the engine's margin pass with its tuned rows per wave
-static void launch_margin(Engine* e, bool trial)
+static int blocks_margin_sparse(const ProblemDev& P) { return ((P.m_pad + kSpMarginRows - 1) / kSpMarginRows) * P.n; }
+
+static void launch_margin(Engine* e, bool trial, const ProblemDev* pd_over = nullptr)
 {
     const ProblemDev& P = e->hdesc;
+    if (e->hform == FEDNL_HESS_SPARSE) {
+        const ProblemDev* q = pd_over ? pd_over : e->pdev;
+        const int2* nomap = nullptr;
+        const size_t lds = sizeof(double) * e->d_pad;
+        if (trial)
+            hipLaunchKernelGGL(k_margin_sparse<true>, dim3(blocks_margin_sparse(P)), dim3(kSpMarginRows), lds, e->st,
+                               q, nomap);
+        else
+            hipLaunchKernelGGL(k_margin_sparse<false>, dim3(blocks_margin_sparse(P)), dim3(kSpMarginRows), lds, e->st,
+                               q, nomap);
+        return;
+    }
     const unsigned grid = (unsigned)((P.m_pad / (4 * e->margin_rows)) * P.n);
     const size_t lds = sizeof(double) * e->d_pad;
     const ProblemDev* pd = e->pdev;
@@ -4650,6 +4760,8 @@ static void launch_margin(Engine* e, bool trial)
 // FEDNL_MARGIN_ROWS=4 or 8 sets the shape instead.
 static void tune_margin(Engine* e)
 {
+    if (e->hform == FEDNL_HESS_SPARSE)
+        return;  // the sparse form's margin pass has one shape
     if (e->gpart) {
         // the gradient partials are per 16 rows: four rows per wave
         e->margin_rows = 4;
@@ -4905,7 +5017,8 @@ static void free_engine(Engine* e)
                     e->cnt, e->lek_hit, e->round_ctr, e->lek_p, e->vals, e->H, e->g_avg, e->shift, e->W,
                     e->dx, e->trace, e->A32, e->AThi, e->ATlo, e->pdev, e->mst, e->H0, e->pp_sel, e->pp_rng,
                     e->pp_dir, e->pp_lk, e->pp_gmsg, e->pp_lkmsg, e->pp_h, e->pp_tg, e->nsync, e->ny, e->ninv,
-                    e->nxt, e->sel_scr, e->gpart, e->sp_seg, e->sp_lane, e->sp_stream, e->sp_b};
+                    e->nxt, e->sel_scr, e->gpart, e->sp_seg, e->sp_lane, e->sp_stream, e->sp_b,
+                    e->sp_rowptr, e->sp_cols};
     for (void* p : ptrs)
         if (p)
             hipFree(p);
@@ -5015,6 +5128,8 @@ struct SparsePlan {
     std::vector<uint32_t> lane;    // per (client, dispatch rank): kSpLanes + 1 lane offsets
     std::vector<uint16_t> stream;  // all token streams
     std::vector<double> b;         // per (client, padded sample): label sign, 0 on padding
+    std::vector<uint32_t> rowptr;  // per client m + 1 offsets into cols
+    std::vector<uint16_t> cols;    // every sample's nonzero columns, ascending
     size_t max_tokens = 0;         // longest segment stream
 };
 
@@ -5035,6 +5150,8 @@ static void build_sparse_plan(const double* features, const double* labels, int 
     P.lane.assign((size_t)n * nseg * (kSpLanes + 1), 0);
     P.stream.clear();
     P.b.assign((size_t)n * m_pad, 0.0);
+    P.rowptr.assign((size_t)n * (m + 1), 0);
+    P.cols.clear();
     P.max_tokens = 0;
     std::vector<int> diag_col(S, -1);
     for (int j = 0; j < d; ++j) diag_col[(size_t)j * (j + 3) / 2] = j;
@@ -5064,6 +5181,11 @@ static void build_sparse_plan(const double* features, const double* labels, int 
             P.b[(size_t)c * m_pad + s] = labels[(size_t)c * m + s] == 1.0 ? 1.0 : -1.0;
         }
         rowptr[m] = (uint32_t)cols.size();
+        {
+            const uint32_t cbase = (uint32_t)P.cols.size();
+            for (int s = 0; s <= m; ++s) P.rowptr[(size_t)c * (m + 1) + s] = cbase + rowptr[s];
+            for (int i : cols) P.cols.push_back((uint16_t)i);
+        }
         std::fill(cnt.begin(), cnt.end(), 0u);
         for (int s = 0; s < m; ++s)
             for (uint32_t bq = rowptr[s]; bq < rowptr[s + 1]; ++bq) {
@@ -5160,6 +5282,7 @@ static void build_sparse_plan(const double* features, const double* labels, int 
         }
     }
     P.stream.resize(P.stream.size() + 8, 0);  // the kernel copies whole 8-byte words
+    P.cols.push_back(0);                        // never empty
 }
 
 // LDS the sparse form may take per workgroup
@@ -5464,6 +5587,16 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
                 if (se == hipSuccess)
                     se = hipMemcpy(e->sp_b, plan.b.data(), sizeof(double) * plan.b.size(), hipMemcpyHostToDevice);
                 if (se == hipSuccess)
+                    se = hipMalloc((void**)&e->sp_rowptr, sizeof(uint32_t) * plan.rowptr.size());
+                if (se == hipSuccess)
+                    se = hipMalloc((void**)&e->sp_cols, sizeof(uint16_t) * plan.cols.size());
+                if (se == hipSuccess)
+                    se = hipMemcpy(e->sp_rowptr, plan.rowptr.data(), sizeof(uint32_t) * plan.rowptr.size(),
+                                   hipMemcpyHostToDevice);
+                if (se == hipSuccess)
+                    se = hipMemcpy(e->sp_cols, plan.cols.data(), sizeof(uint16_t) * plan.cols.size(),
+                                   hipMemcpyHostToDevice);
+                if (se == hipSuccess)
                     se = raise_lds_limit((const void*)k_hessian_sparse<true>, lds);
                 if (se == hipSuccess)
                     se = raise_lds_limit((const void*)k_hessian_sparse<false>, lds);
@@ -5608,9 +5741,9 @@ static int enqueue_rounds(Engine* e, int rounds)
 // captured once into a HIP graph and replayed.
 // ---------------------------------------------------------------------------
 enum BatchKernel {
-    BK_MARGIN, BK_MARGIN_G, BK_HESS64, BK_HESS_SP, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
+    BK_MARGIN, BK_MARGIN_G, BK_MARGIN_SP, BK_HESS64, BK_HESS_SP, BK_HESS32, BK_HESS16, BK_GRAD, BK_SEL_TOPK, BK_SEL_TOPLEK, BK_SEL_RANDK, BK_SEL_WINDOW,
     BK_EMIT, BK_SEL_TOPK_W, BK_SEL_TOPLEK_W, BK_SEL_RANDK_W, BK_SEL_WINDOW_W, BK_EMIT_W, BK_MASTER, BK_NEWTON,
-    BK_LS_MARGIN, BK_LS, BK_COUNT
+    BK_LS_MARGIN, BK_LS_MARGIN_SP, BK_LS, BK_COUNT
 };
 
 struct Batch {
@@ -5657,7 +5790,9 @@ static int batch_build(Batch* b)
         const ProblemDev P = make_desc(e, true);
         b->h_probs.push_back(P);
         const int p = (int)i;
-        if (e->gpart)
+        if (e->hform == FEDNL_HESS_SPARSE)
+            add(BK_MARGIN_SP, p, blocks_margin_sparse(P), false, sizeof(double) * (size_t)e->d_pad);
+        else if (e->gpart)
             add(BK_MARGIN_G, p, blocks_margin(P), false, margin_grad_lds(e->d_pad));
         else
             add(BK_MARGIN, p, blocks_margin(P), false, sizeof(double) * (size_t)e->d_pad);
@@ -5684,7 +5819,10 @@ static int batch_build(Batch* b)
         add(BK_MASTER, p, blocks_master(P), false);
         add(BK_NEWTON, p, mc_blocks(P.dp), false, e->lds_newton);
         if (e->o.line_search) {
-            add(BK_LS_MARGIN, p, blocks_margin(P), false, sizeof(double) * (size_t)e->d_pad);
+            if (e->hform == FEDNL_HESS_SPARSE)
+                add(BK_LS_MARGIN_SP, p, blocks_margin_sparse(P), false, sizeof(double) * (size_t)e->d_pad);
+            else
+                add(BK_LS_MARGIN, p, blocks_margin(P), false, sizeof(double) * (size_t)e->d_pad);
             add(BK_LS, p, 1, false);
             b->ls_steps = std::max(b->ls_steps, e->o.max_line_search_steps);
         }
@@ -5732,6 +5870,9 @@ static void batch_launch_round(Batch* b)
     if (b->cnt[BK_MARGIN])
         hipLaunchKernelGGL((k_margin<false, 4>), dim3(b->cnt[BK_MARGIN]), dim3(256), b->lds[BK_MARGIN], st, pd,
                            map(BK_MARGIN));
+    if (b->cnt[BK_MARGIN_SP])
+        hipLaunchKernelGGL(k_margin_sparse<false>, dim3(b->cnt[BK_MARGIN_SP]), dim3(kSpMarginRows),
+                           b->lds[BK_MARGIN_SP], st, pd, map(BK_MARGIN_SP));
     if (b->cnt[BK_MARGIN_G])
         hipLaunchKernelGGL((k_margin<false, 4, true>), dim3(b->cnt[BK_MARGIN_G]), dim3(256), b->lds[BK_MARGIN_G], st,
                            pd, map(BK_MARGIN_G));
@@ -5770,8 +5911,12 @@ static void batch_launch_round(Batch* b)
     if (b->cnt[BK_LS]) {
         hipLaunchKernelGGL(k_ls_begin, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));
         for (int t = 0; t < b->ls_steps; ++t) {
-            hipLaunchKernelGGL((k_margin<true, 4>), dim3(b->cnt[BK_LS_MARGIN]), dim3(256), b->lds[BK_LS_MARGIN], st,
-                               pd, map(BK_LS_MARGIN));
+            if (b->cnt[BK_LS_MARGIN])
+                hipLaunchKernelGGL((k_margin<true, 4>), dim3(b->cnt[BK_LS_MARGIN]), dim3(256), b->lds[BK_LS_MARGIN],
+                                   st, pd, map(BK_LS_MARGIN));
+            if (b->cnt[BK_LS_MARGIN_SP])
+                hipLaunchKernelGGL(k_margin_sparse<true>, dim3(b->cnt[BK_LS_MARGIN_SP]), dim3(kSpMarginRows),
+                                   b->lds[BK_LS_MARGIN_SP], st, pd, map(BK_LS_MARGIN_SP));
             hipLaunchKernelGGL(k_ls_check, dim3(b->cnt[BK_LS]), dim3(256), 0, st, pd, map(BK_LS));
         }
     }
@@ -6461,7 +6606,14 @@ int fednl_kernel_times(void* handle, double* ms, int* launches, int max_kernels,
     for (int q = 0; q < nk; ++q) {
         if (ms) ms[q] = e->kms[q];
         if (launches) launches[q] = e->klaunch[q];
-        if (names) names[q] = kKernelNames[q];
+        if (names) {
+            // the Hessian stage is named for the kernel that runs it
+            const char* hn = e->hform == FEDNL_HESS_SPARSE ? "k_hessian_sparse"
+                             : e->hmode == 1               ? "k_hessian_f32"
+                             : e->hmode == 2               ? "k_hessian_bf16"
+                                                           : "k_hessian_f64";
+            names[q] = q == KID_HESS ? hn : kKernelNames[q];
+        }
     }
     return KID_COUNT;
 }
@@ -6790,8 +6942,11 @@ int fednl_hessians_at(void* handle, const double* x, double* out)
         err = hipMemcpy(dpd, &hp, sizeof(ProblemDev), hipMemcpyHostToDevice);
     if (err == hipSuccess) {
         // the margin weights of x, then the Hessians in the engine's mode
-        hipLaunchKernelGGL((k_margin<false, 4>), dim3(blocks_margin(hp)), dim3(256), sizeof(double) * e->d_pad, e->st,
-                           (const ProblemDev*)dpd, (const int2*)nullptr);
+        if (e->hform == FEDNL_HESS_SPARSE)
+            launch_margin(e, false, dpd);
+        else
+            hipLaunchKernelGGL((k_margin<false, 4>), dim3(blocks_margin(hp)), dim3(256), sizeof(double) * e->d_pad,
+                               e->st, (const ProblemDev*)dpd, (const int2*)nullptr);
         launch_hessian(e, true, dout, dpd);
         err = hipGetLastError();
     }
