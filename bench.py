@@ -49,10 +49,11 @@ def _sync_all(eng):
         pass
 
 
-def algorithmic_work(name, n, m, d, k, S):
+def algorithmic_work(name, n, m, d, k, S, nnz=None):
     """Algorithmic work of one launch of each round kernel: ("mfma", flops)
     for the FP64 arithmetic kernels, ("hbm", bytes) for the data-movement
-    ones. Bytes count each input read once and each output written once."""
+    ones. Bytes count each input read once and each output written once.
+    nnz: the design's nonzero count (the sparse forms read only those)."""
     if name == "k_hessian_f64":
         # upper triangle incl. diagonal of A_c^T D_c A_c: m FMAs per entry
         return "mfma", float(n) * m * d * (d + 1)
@@ -81,6 +82,11 @@ def algorithmic_work(name, n, m, d, k, S):
     if name == "k_margin":
         # features once, labels, x; sigmoid weights, gradient weights, partials
         return "hbm", float(n) * m * (8.0 * d + 8.0 + 16.0) + 8.0 * d
+    if name == "k_margin_sparse":
+        # binary designs: the samples' nonzero column ids (2 B each), row
+        # offsets and label signs, x; D, g and the -log sigma partials written
+        z = float(nnz) if nnz is not None else float(n) * m * d
+        return "hbm", 2.0 * z + float(n) * m * (4.0 + 8.0 + 16.0 + 0.5) + 8.0 * d
     if name == "k_emit":
         # Delta and learned at the k positions, learned written back, message
         # (value, flat index, position) and the selection bitmap written
@@ -116,12 +122,12 @@ def load_traffic(path):
     return out
 
 
-def kernel_roofline(eng, times, traffic):
+def kernel_roofline(eng, times, traffic, nnz=None):
     """Roofline record of the kernel with the largest device time."""
     n, m, d, k, S = eng.n, eng.m, eng.d, eng.k, eng.S
     name, (ms, launches) = max(times.items(), key=lambda kv: kv[1][0])
     avg_s = ms * 1e-3 / max(launches, 1)
-    bound, work = algorithmic_work(name, n, m, d, k, S)
+    bound, work = algorithmic_work(name, n, m, d, k, S, nnz)
     if bound == "mfma":
         ach, unit = work / avg_s / 1e12, "TFLOP/s"
     else:
@@ -499,10 +505,11 @@ def main():
     t_tol = float(tr["seconds"][r_tol]) if r_tol is not None else None
     final_g2 = float(g2[-1])
     traffic = load_traffic(os.path.join(ROOT, "profiles", TRAFFIC_FILE))
-    kname, roof, kavg = kernel_roofline(eng, times, traffic)
+    nnz = int(np.count_nonzero(prob.features))
+    kname, roof, kavg = kernel_roofline(eng, times, traffic, nnz)
     all_roofs = {}
     for name_, (ms_, nl_) in calib.items():
-        b_, w_ = algorithmic_work(name_, eng.n, eng.m, eng.d, eng.k, eng.S)
+        b_, w_ = algorithmic_work(name_, eng.n, eng.m, eng.d, eng.k, eng.S, nnz)
         sec_ = ms_ * 1e-3 / max(nl_, 1)
         pk_ = peak_of(name_, b_) * (1e12 if b_ == "mfma" else 1e9)
         all_roofs[name_] = dict(bound=b_, us=sec_ * 1e6, frac=w_ / sec_ / pk_, traffic=traffic.get(name_))

@@ -6607,12 +6607,14 @@ int fednl_kernel_times(void* handle, double* ms, int* launches, int max_kernels,
         if (ms) ms[q] = e->kms[q];
         if (launches) launches[q] = e->klaunch[q];
         if (names) {
-            // the Hessian stage is named for the kernel that runs it
+            // the Hessian and margin stages are named for the kernels that run them
             const char* hn = e->hform == FEDNL_HESS_SPARSE ? "k_hessian_sparse"
                              : e->hmode == 1               ? "k_hessian_f32"
                              : e->hmode == 2               ? "k_hessian_bf16"
                                                            : "k_hessian_f64";
-            names[q] = q == KID_HESS ? hn : kKernelNames[q];
+            names[q] = q == KID_HESS ? hn
+                       : (q == KID_MARGIN && e->hform == FEDNL_HESS_SPARSE) ? "k_margin_sparse"
+                                                                           : kKernelNames[q];
         }
     }
     return KID_COUNT;
