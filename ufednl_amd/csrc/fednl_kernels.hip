@@ -1238,6 +1238,24 @@ __host__ __device__ inline size_t sparse_lds_bytes(int m_pad, int d_pad, size_t 
            (size_t)kSpSeg + sizeof(uint16_t) * ((max_tokens + 3) / 4 * 4 + 8);
 }
 
+// acc plus the weights W of the first rem (< kN) samples listed at sp, in
+// order: the indices, then the weights, then the adds (past rem masked)
+template <int kN>
+__device__ __forceinline__ double sparse_sum_masked(double acc, const uint16_t* sp, uint32_t rem, const double* W)
+{
+    int sx[kN];
+#pragma unroll
+    for (int u = 0; u < kN; ++u) sx[u] = sp[(uint32_t)u < rem ? u : 0];
+    double wv[kN];
+#pragma unroll
+    for (int u = 0; u < kN; ++u) wv[u] = W[sx[u]];
+#pragma unroll
+    for (int u = 0; u < kN; ++u)
+        if ((uint32_t)u < rem)
+            acc = acc + wv[u];
+    return acc;
+}
+
 template <bool kInit>
 __global__ __launch_bounds__(kSpThreads) void k_hessian_sparse(FEDNL_KARGS, double* __restrict__ learned_out)
 {
@@ -1426,41 +1444,32 @@ __global__ __launch_bounds__(kSpThreads) void k_hessian_sparse(FEDNL_KARGS, doub
             const double* W = gr ? Gl : Dl;
             const uint16_t* sp = St + p;
             uint32_t k = 0;
-            const uint32_t full = cnt & ~7u;
-            if (full) {
-                // batches of eight: the indices of the next batch are read
-                // with the weights of this one, then the ordered adds
-                int sn[8];
+            if (cnt >= 16) {
+                // batches of sixteen: the next batch's indices are read with
+                // this batch's weights, then the ordered adds
+                int sn[16];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) sn[u] = sp[u];
-                for (; k < full; k += 8) {
-                    int sx[8];
+                for (int u = 0; u < 16; ++u) sn[u] = sp[u];
+                for (; k + 16 <= cnt; k += 16) {
+                    double wv[16];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) sx[u] = sn[u];
-                    if (k + 8 < full) {
+                    for (int u = 0; u < 16; ++u) wv[u] = W[sn[u]];
+                    if (k + 32 <= cnt) {
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) sn[u] = sp[k + 8 + u];
+                        for (int u = 0; u < 16; ++u) sn[u] = sp[k + 16 + u];
                     }
-                    double wv[8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) wv[u] = W[sx[u]];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) acc = acc + wv[u];
+                    for (int u = 0; u < 16; ++u) acc = acc + wv[u];
                 }
             }
-            // the rest (under eight) in batches of four, past the end masked
-            for (; k < cnt; k += 4) {
-                int sx[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) sx[u] = sp[k + u < cnt ? k + u : k];
-                double wv[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) wv[u] = W[sx[u]];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (k + u < cnt)
-                        acc = acc + wv[u];
-            }
+            // the rest (under sixteen): one batch of 16, 8 or 4, past the end masked
+            const uint32_t rem = cnt - k;
+            if (rem > 8)
+                acc = sparse_sum_masked<16>(acc, sp + k, rem, W);
+            else if (rem > 4)
+                acc = sparse_sum_masked<8>(acc, sp + k, rem, W);
+            else if (rem > 0)
+                acc = sparse_sum_masked<4>(acc, sp + k, rem, W);
             p += cnt;
             if (gr) {
                 grad_c[col] = acc;
@@ -5113,14 +5122,13 @@ static double sparse_pair_count(const double* f, size_t rows, int d)
 
 // an item's cost to its lane in k_hessian_sparse, in dependent LDS round
 // trips: header (with four samples for a short off-diagonal entry) and
-// weights; else header, batches of eight with the next batch's indices read
-// ahead, and the rest in masked batches of four
+// weights; else header, batches of sixteen with the next batch's indices read
+// ahead, and the rest in one masked batch
 static uint32_t sparse_item_cost(uint32_t cnt, bool short_entry)
 {
     if (short_entry)
         return 2;
-    const uint32_t rest = cnt & 7u;
-    return 2 + (cnt >= 8 ? 1 + (cnt >> 3) : 0) + (rest ? (rest > 4 ? 4 : 2) : 0);
+    return 2 + (cnt >= 16 ? 1 + (cnt >> 4) : 0) + ((cnt & 15u) ? 2 : 0);
 }
 
 struct SparsePlan {
