@@ -1211,7 +1211,10 @@ __global__ __launch_bounds__(kSpMarginRows) void k_margin_sparse(FEDNL_KARGS)
 //           14     a diagonal entry, its sample count in the next token;
 //           15     the client gradient at a column whose diagonal entry lies
 //                  in the segment: count, then the column, in the next two;
-//   then the item's sample indices, ascending.
+//   then the item's sample indices, ascending; or
+//           0      an item whose samples are consecutive (a dense column's):
+//                  its kind (0 off-diagonal, 1 diagonal, 2 gradient), count,
+//                  first sample and, for a gradient, the column.
 // Off-diagonal entries without a common sample are not listed (their H is 0);
 // every diagonal entry is, and so is every column's gradient
 // lambda x_i + sum_s b_s g_s over the same samples (logistic_regression.h:
@@ -1420,6 +1423,44 @@ __global__ __launch_bounds__(kSpThreads) void k_hessian_sparse(FEDNL_KARGS, doub
                         acc = acc + dv[u];
                 Hs[off] = acc;
                 p += 1 + code;
+                continue;
+            }
+            if (code == 0) {
+                // a run: kind (0 off-diagonal, 1 diagonal, 2 gradient), count,
+                // first sample (then the column, for a gradient); the weights
+                // of consecutive samples, no index reads
+                const uint32_t kind = tk[1], rc = tk[2], s0 = tk[3];
+                const bool gr = kind == 2;
+                p += gr ? 5 : 4;
+                if (gr && !with_grad)
+                    continue;
+                const int col = gr ? (int)tk[4] : -1;
+                const double* W = gr ? Gl : Dl;
+                double acc = gr ? lambda * Xl[col] : 0.0;
+                uint32_t k = 0;
+                for (; k + 16 <= rc; k += 16) {
+                    double wv[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) wv[u] = W[s0 + k + u];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) acc = acc + wv[u];
+                }
+                if (k < rc) {
+                    double wv[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) wv[u] = W[s0 + (k + u < rc ? k + u : k)];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        if (k + u < rc)
+                            acc = acc + wv[u];
+                }
+                if (gr) {
+                    grad_c[col] = acc;
+                } else {
+                    Hs[off] = acc;
+                    if (kind == 1)
+                        Df[off] = 1;
+                }
                 continue;
             }
             ++p;
@@ -5131,6 +5172,14 @@ static uint32_t sparse_item_cost(uint32_t cnt, bool short_entry)
     return 2 + (cnt >= 16 ? 1 + (cnt >> 4) : 0) + ((cnt & 15u) ? 2 : 0);
 }
 
+// an item of at least 32 samples that are consecutive goes as a run
+static bool sparse_is_run(const uint16_t* h, uint32_t k)
+{
+    if (k < 32)
+        return false;
+    return (uint32_t)h[k - 1] - h[0] == k - 1;  // ascending and distinct
+}
+
 struct SparsePlan {
     std::vector<uint32_t> seg;     // per (client, dispatch rank): segment, length, stream offset (lo, hi)
     std::vector<uint32_t> lane;    // per (client, dispatch rank): kSpLanes + 1 lane offsets
@@ -5214,11 +5263,13 @@ static void build_sparse_plan(const double* features, const double* labels, int 
             items.clear();
             for (size_t e = e0; e < e1; ++e) {
                 const bool dg = diag_col[e] >= 0;
+                const bool run = sparse_is_run(hits.data() + start[e], cnt[e]);
+                const uint32_t cost = run ? 2 + (cnt[e] >> 4) : sparse_item_cost(cnt[e], !dg && cnt[e] <= 4);
                 if (dg) {
-                    items.push_back({sparse_item_cost(cnt[e], false), (uint32_t)e, false});
-                    items.push_back({sparse_item_cost(cnt[e], false), (uint32_t)e, true});
+                    items.push_back({cost, (uint32_t)e, false});
+                    items.push_back({cost, (uint32_t)e, true});
                 } else if (cnt[e] > 0) {
-                    items.push_back({sparse_item_cost(cnt[e], cnt[e] <= 4), (uint32_t)e, false});
+                    items.push_back({cost, (uint32_t)e, false});
                 }
             }
             std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
@@ -5252,6 +5303,16 @@ static void build_sparse_plan(const double* features, const double* labels, int 
                 for (const Item& it : lane_items[l]) {
                     const uint32_t e = it.e, off = (uint32_t)(e - e0), k = cnt[e];
                     lc += it.cost;
+                    if (sparse_is_run(hits.data() + start[e], k)) {
+                        const uint32_t kind = it.grad ? 2u : (diag_col[e] >= 0 ? 1u : 0u);
+                        tok.push_back((uint16_t)off);  // code 0
+                        tok.push_back((uint16_t)kind);
+                        tok.push_back((uint16_t)k);
+                        tok.push_back(hits[start[e]]);
+                        if (it.grad)
+                            tok.push_back((uint16_t)diag_col[e]);
+                        continue;
+                    }
                     if (it.grad) {
                         tok.push_back((uint16_t)(off | (15u << 12)));
                         tok.push_back((uint16_t)k);
