@@ -5190,19 +5190,162 @@ struct SparsePlan {
     size_t max_tokens = 0;         // longest segment stream
 };
 
-// For every client: the samples of each upper entry (ascending); per segment
-// of kSpSeg entries the items (listed entries, and the gradient of each
-// column whose diagonal entry the segment holds) spread over kSpLanes lanes,
+// One client's part of the plan (its segments in segment order).
+struct SparseClient {
+    std::vector<std::vector<uint16_t>> tok;  // per segment: the lanes' tokens
+    std::vector<std::vector<uint32_t>> lo;   // per segment: kSpLanes + 1 lane offsets
+    std::vector<uint64_t> mx;                // per segment: its costliest lane
+    std::vector<uint32_t> rowptr;            // m + 1 offsets into cols
+    std::vector<uint16_t> cols;              // the samples' nonzero columns
+};
+
+// For client c: the samples of each upper entry (ascending); per segment of
+// kSpSeg entries the items (listed entries, and the gradient of each column
+// whose diagonal entry the segment holds) spread over kSpLanes lanes,
 // costliest first, each to the lane with the least cost so far (ties in an
-// order that alternates the waves), and the lanes' tokens in lane order;
-// the client's segments dispatched in decreasing order of their costliest
-// lane.
+// order that alternates the waves), and the lanes' tokens in lane order.
+static void build_sparse_client(const double* features, int c, int m, int d, const std::vector<int>& diag_col,
+                                SparseClient& out)
+{
+    const size_t S = (size_t)d * (d + 1) / 2;
+    const int nseg = (int)((S + kSpSeg - 1) / kSpSeg);
+    constexpr int kW = kSpLanes / 64;
+    std::vector<uint32_t> cnt(S, 0u), start(S + 1), cur(S);
+    std::vector<uint16_t> hits;
+    std::vector<uint32_t>& rowptr = out.rowptr;
+    std::vector<uint16_t>& cols = out.cols;
+    rowptr.assign(m + 1, 0);
+    cols.clear();
+    struct Item {
+        uint32_t cost, e;
+        bool grad;
+    };
+    std::vector<Item> items;
+    std::vector<std::vector<Item>> lane_items(kSpLanes);
+    using Load = std::pair<uint64_t, int>;
+    for (int s = 0; s < m; ++s) {
+        rowptr[s] = (uint32_t)cols.size();
+        const double* a = features + ((size_t)c * m + s) * d;
+        for (int i = 0; i < d; ++i)
+            if (a[i] != 0.0)
+                cols.push_back((uint16_t)i);
+    }
+    rowptr[m] = (uint32_t)cols.size();
+    for (int s = 0; s < m; ++s)
+        for (uint32_t bq = rowptr[s]; bq < rowptr[s + 1]; ++bq) {
+            const size_t col0 = (size_t)cols[bq] * (cols[bq] + 1) / 2;
+            for (uint32_t aq = rowptr[s]; aq <= bq; ++aq) cnt[col0 + cols[aq]]++;
+        }
+    start[0] = 0;
+    for (size_t e = 0; e < S; ++e) start[e + 1] = start[e] + cnt[e];
+    hits.resize(start[S]);
+    std::copy(start.begin(), start.begin() + S, cur.begin());
+    for (int s = 0; s < m; ++s)
+        for (uint32_t bq = rowptr[s]; bq < rowptr[s + 1]; ++bq) {
+            const size_t col0 = (size_t)cols[bq] * (cols[bq] + 1) / 2;
+            for (uint32_t aq = rowptr[s]; aq <= bq; ++aq) hits[cur[col0 + cols[aq]]++] = (uint16_t)s;
+        }
+    out.tok.assign(nseg, {});
+    out.lo.assign(nseg, {});
+    out.mx.assign(nseg, 0);
+    for (int t = 0; t < nseg; ++t) {
+        const size_t e0 = (size_t)t * kSpSeg, e1 = std::min(S, e0 + kSpSeg);
+        items.clear();
+        for (size_t e = e0; e < e1; ++e) {
+            const bool dg = diag_col[e] >= 0;
+            const bool run = sparse_is_run(hits.data() + start[e], cnt[e]);
+            const uint32_t cost = run ? 2 + (cnt[e] >> 4) : sparse_item_cost(cnt[e], !dg && cnt[e] <= 4);
+            if (dg) {
+                items.push_back({cost, (uint32_t)e, false});
+                items.push_back({cost, (uint32_t)e, true});
+            } else if (cnt[e] > 0) {
+                items.push_back({cost, (uint32_t)e, false});
+            }
+        }
+        std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+            if (a.cost != b.cost)
+                return a.cost > b.cost;
+            if (a.e != b.e)
+                return a.e < b.e;
+            return a.grad < b.grad;
+        });
+        // heap over (cost so far, rank); rank r is lane (r % kW) * 64 + r / kW
+        std::priority_queue<Load, std::vector<Load>, std::greater<Load>> pq;
+        for (int r = 0; r < kSpLanes; ++r) {
+            pq.push({0, r});
+            lane_items[r].clear();
+        }
+        for (const Item& it : items) {
+            Load top = pq.top();
+            pq.pop();
+            lane_items[(top.second % kW) * 64 + top.second / kW].push_back(it);
+            top.first += it.cost;
+            pq.push(top);
+        }
+        uint64_t mx = 0;
+        auto& tok = out.tok[t];
+        auto& lo = out.lo[t];
+        lo.assign(kSpLanes + 1, 0);
+        for (int l = 0; l < kSpLanes; ++l) {
+            lo[l] = (uint32_t)tok.size();
+            uint64_t lc = 0;
+            for (const Item& it : lane_items[l]) {
+                const uint32_t e = it.e, off = (uint32_t)(e - e0), k = cnt[e];
+                lc += it.cost;
+                if (sparse_is_run(hits.data() + start[e], k)) {
+                    const uint32_t kind = it.grad ? 2u : (diag_col[e] >= 0 ? 1u : 0u);
+                    tok.push_back((uint16_t)off);  // code 0
+                    tok.push_back((uint16_t)kind);
+                    tok.push_back((uint16_t)k);
+                    tok.push_back(hits[start[e]]);
+                    if (it.grad)
+                        tok.push_back((uint16_t)diag_col[e]);
+                    continue;
+                }
+                if (it.grad) {
+                    tok.push_back((uint16_t)(off | (15u << 12)));
+                    tok.push_back((uint16_t)k);
+                    tok.push_back((uint16_t)diag_col[e]);
+                } else if (diag_col[e] >= 0) {
+                    tok.push_back((uint16_t)(off | (14u << 12)));
+                    tok.push_back((uint16_t)k);
+                } else if (k <= 12) {
+                    tok.push_back((uint16_t)(off | (k << 12)));
+                } else {
+                    tok.push_back((uint16_t)(off | (13u << 12)));
+                    tok.push_back((uint16_t)k);
+                }
+                tok.insert(tok.end(), hits.begin() + start[e], hits.begin() + start[e] + k);
+            }
+            mx = std::max(mx, lc);
+        }
+        lo[kSpLanes] = (uint32_t)tok.size();
+        out.mx[t] = mx;
+    }
+}
+
+// The clients' parts are built on host threads (each client on its own, so
+// the plan does not depend on the thread count), then laid out in client
+// order; a client's segments are dispatched in decreasing order of their
+// costliest lane (ties: lower segment).
 static void build_sparse_plan(const double* features, const double* labels, int n, int m, int d, int m_pad,
                               SparsePlan& P)
 {
     const size_t S = (size_t)d * (d + 1) / 2;
     const int nseg = (int)((S + kSpSeg - 1) / kSpSeg);
-    constexpr int kW = kSpLanes / 64;
+    std::vector<int> diag_col(S, -1);
+    for (int j = 0; j < d; ++j) diag_col[(size_t)j * (j + 3) / 2] = j;
+    std::vector<SparseClient> cl(n);
+    {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const int nt = (int)std::min<unsigned>(std::min<unsigned>(hw, 16u), (unsigned)n);
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt; ++t)
+            pool.emplace_back([&, t]() {
+                for (int c = t; c < n; c += nt) build_sparse_client(features, c, m, d, diag_col, cl[c]);
+            });
+        for (auto& th : pool) th.join();
+    }
     P.seg.assign((size_t)n * nseg * 4, 0);
     P.lane.assign((size_t)n * nseg * (kSpLanes + 1), 0);
     P.stream.clear();
@@ -5210,145 +5353,29 @@ static void build_sparse_plan(const double* features, const double* labels, int 
     P.rowptr.assign((size_t)n * (m + 1), 0);
     P.cols.clear();
     P.max_tokens = 0;
-    std::vector<int> diag_col(S, -1);
-    for (int j = 0; j < d; ++j) diag_col[(size_t)j * (j + 3) / 2] = j;
-    std::vector<uint32_t> cnt(S), start(S + 1), cur(S);
-    std::vector<uint16_t> hits;
-    std::vector<uint32_t> rowptr(m + 1);
-    std::vector<int> cols;
-    struct Item {
-        uint32_t cost, e;
-        bool grad;
-    };
-    std::vector<Item> items;
-    std::vector<std::vector<Item>> lane_items(kSpLanes);
-    std::vector<std::vector<uint16_t>> seg_tok(nseg);
-    std::vector<std::vector<uint32_t>> seg_lo(nseg);
-    std::vector<uint64_t> seg_max(nseg);
     std::vector<int> order(nseg);
-    using Load = std::pair<uint64_t, int>;
     for (int c = 0; c < n; ++c) {
-        cols.clear();
-        for (int s = 0; s < m; ++s) {
-            rowptr[s] = (uint32_t)cols.size();
-            const double* a = features + ((size_t)c * m + s) * d;
-            for (int i = 0; i < d; ++i)
-                if (a[i] != 0.0)
-                    cols.push_back(i);
-            P.b[(size_t)c * m_pad + s] = labels[(size_t)c * m + s] == 1.0 ? 1.0 : -1.0;
-        }
-        rowptr[m] = (uint32_t)cols.size();
-        {
-            const uint32_t cbase = (uint32_t)P.cols.size();
-            for (int s = 0; s <= m; ++s) P.rowptr[(size_t)c * (m + 1) + s] = cbase + rowptr[s];
-            for (int i : cols) P.cols.push_back((uint16_t)i);
-        }
-        std::fill(cnt.begin(), cnt.end(), 0u);
-        for (int s = 0; s < m; ++s)
-            for (uint32_t bq = rowptr[s]; bq < rowptr[s + 1]; ++bq) {
-                const size_t col0 = (size_t)cols[bq] * (cols[bq] + 1) / 2;
-                for (uint32_t aq = rowptr[s]; aq <= bq; ++aq) cnt[col0 + cols[aq]]++;
-            }
-        start[0] = 0;
-        for (size_t e = 0; e < S; ++e) start[e + 1] = start[e] + cnt[e];
-        hits.resize(start[S]);
-        std::copy(start.begin(), start.begin() + S, cur.begin());
-        for (int s = 0; s < m; ++s)
-            for (uint32_t bq = rowptr[s]; bq < rowptr[s + 1]; ++bq) {
-                const size_t col0 = (size_t)cols[bq] * (cols[bq] + 1) / 2;
-                for (uint32_t aq = rowptr[s]; aq <= bq; ++aq) hits[cur[col0 + cols[aq]]++] = (uint16_t)s;
-            }
-        for (int t = 0; t < nseg; ++t) {
-            const size_t e0 = (size_t)t * kSpSeg, e1 = std::min(S, e0 + kSpSeg);
-            items.clear();
-            for (size_t e = e0; e < e1; ++e) {
-                const bool dg = diag_col[e] >= 0;
-                const bool run = sparse_is_run(hits.data() + start[e], cnt[e]);
-                const uint32_t cost = run ? 2 + (cnt[e] >> 4) : sparse_item_cost(cnt[e], !dg && cnt[e] <= 4);
-                if (dg) {
-                    items.push_back({cost, (uint32_t)e, false});
-                    items.push_back({cost, (uint32_t)e, true});
-                } else if (cnt[e] > 0) {
-                    items.push_back({cost, (uint32_t)e, false});
-                }
-            }
-            std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
-                if (a.cost != b.cost)
-                    return a.cost > b.cost;
-                if (a.e != b.e)
-                    return a.e < b.e;
-                return a.grad < b.grad;
-            });
-            // heap over (cost so far, rank); rank r is lane (r % kW) * 64 + r / kW
-            std::priority_queue<Load, std::vector<Load>, std::greater<Load>> pq;
-            for (int r = 0; r < kSpLanes; ++r) {
-                pq.push({0, r});
-                lane_items[r].clear();
-            }
-            for (const Item& it : items) {
-                Load top = pq.top();
-                pq.pop();
-                lane_items[(top.second % kW) * 64 + top.second / kW].push_back(it);
-                top.first += it.cost;
-                pq.push(top);
-            }
-            uint64_t mx = 0;
-            auto& tok = seg_tok[t];
-            auto& lo = seg_lo[t];
-            tok.clear();
-            lo.assign(kSpLanes + 1, 0);
-            for (int l = 0; l < kSpLanes; ++l) {
-                lo[l] = (uint32_t)tok.size();
-                uint64_t lc = 0;
-                for (const Item& it : lane_items[l]) {
-                    const uint32_t e = it.e, off = (uint32_t)(e - e0), k = cnt[e];
-                    lc += it.cost;
-                    if (sparse_is_run(hits.data() + start[e], k)) {
-                        const uint32_t kind = it.grad ? 2u : (diag_col[e] >= 0 ? 1u : 0u);
-                        tok.push_back((uint16_t)off);  // code 0
-                        tok.push_back((uint16_t)kind);
-                        tok.push_back((uint16_t)k);
-                        tok.push_back(hits[start[e]]);
-                        if (it.grad)
-                            tok.push_back((uint16_t)diag_col[e]);
-                        continue;
-                    }
-                    if (it.grad) {
-                        tok.push_back((uint16_t)(off | (15u << 12)));
-                        tok.push_back((uint16_t)k);
-                        tok.push_back((uint16_t)diag_col[e]);
-                    } else if (diag_col[e] >= 0) {
-                        tok.push_back((uint16_t)(off | (14u << 12)));
-                        tok.push_back((uint16_t)k);
-                    } else if (k <= 12) {
-                        tok.push_back((uint16_t)(off | (k << 12)));
-                    } else {
-                        tok.push_back((uint16_t)(off | (13u << 12)));
-                        tok.push_back((uint16_t)k);
-                    }
-                    tok.insert(tok.end(), hits.begin() + start[e], hits.begin() + start[e] + k);
-                }
-                mx = std::max(mx, lc);
-            }
-            lo[kSpLanes] = (uint32_t)tok.size();
-            seg_max[t] = mx;
-            P.max_tokens = std::max<size_t>(P.max_tokens, tok.size());
-        }
-        // dispatch order: costliest lane first (ties: lower segment)
+        SparseClient& sc = cl[c];
+        for (int s = 0; s < m; ++s) P.b[(size_t)c * m_pad + s] = labels[(size_t)c * m + s] == 1.0 ? 1.0 : -1.0;
+        const uint32_t cbase = (uint32_t)P.cols.size();
+        for (int s = 0; s <= m; ++s) P.rowptr[(size_t)c * (m + 1) + s] = cbase + sc.rowptr[s];
+        P.cols.insert(P.cols.end(), sc.cols.begin(), sc.cols.end());
         for (int t = 0; t < nseg; ++t) order[t] = t;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return seg_max[a] > seg_max[b]; });
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sc.mx[a] > sc.mx[b]; });
         for (int r = 0; r < nseg; ++r) {
             const int t = order[r];
             while (P.stream.size() % 4) P.stream.push_back(0);
             const uint64_t off = P.stream.size();
             uint32_t* rec = &P.seg[((size_t)c * nseg + r) * 4];
             rec[0] = (uint32_t)t;
-            rec[1] = (uint32_t)seg_tok[t].size();
+            rec[1] = (uint32_t)sc.tok[t].size();
             rec[2] = (uint32_t)(off & 0xFFFFFFFFull);
             rec[3] = (uint32_t)(off >> 32);
-            std::copy(seg_lo[t].begin(), seg_lo[t].end(), P.lane.begin() + ((size_t)c * nseg + r) * (kSpLanes + 1));
-            P.stream.insert(P.stream.end(), seg_tok[t].begin(), seg_tok[t].end());
+            std::copy(sc.lo[t].begin(), sc.lo[t].end(), P.lane.begin() + ((size_t)c * nseg + r) * (kSpLanes + 1));
+            P.stream.insert(P.stream.end(), sc.tok[t].begin(), sc.tok[t].end());
+            P.max_tokens = std::max<size_t>(P.max_tokens, sc.tok[t].size());
         }
+        sc = SparseClient();  // release as we go
     }
     P.stream.resize(P.stream.size() + 8, 0);  // the kernel copies whole 8-byte words
     P.cols.push_back(0);                        // never empty
