@@ -262,7 +262,9 @@ int fednl_batch_run(void *batch, int rounds);
  * ||grad f||^2 <= tol (or max_rounds). fednl_serve runs them in real time:
  * a request is admitted at or after arrival_s (seconds from the call), in
  * arrival order, while the device bytes of the admitted problems stay within
- * hbm_budget_bytes (fednl_footprint); admitted problems are stepped together
+ * hbm_budget_bytes (fednl_footprint, plus, for a binary design in the sparse
+ * Hessian form, its sample lists once read: the budget can be passed by one
+ * request's lists); admitted problems are stepped together
  * as one batch, rounds_per_step rounds at a time, members joining and leaving
  * between steps (continuous batching). Results per request, in input order;
  * x_out (d doubles, optional) receives the final iterate. Returns FEDNL_OK or

@@ -4566,6 +4566,7 @@ struct Engine {
     uint32_t* sp_rowptr = nullptr;  // sparse form: the margin pass's row lists (k_margin_sparse)
     uint16_t* sp_cols = nullptr;
     size_t lds_sparse = 0;
+    size_t sp_bytes = 0;           // device bytes of the sparse form's lists
 };
 
 static hipEvent_t pool_event(Engine* e)
@@ -5702,6 +5703,9 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
                     return FEDNL_E_DEVICE;
                 }
                 e->lds_sparse = lds;
+                e->sp_bytes = sizeof(uint32_t) * (plan.seg.size() + plan.lane.size() + plan.rowptr.size()) +
+                              sizeof(uint16_t) * (plan.stream.size() + plan.cols.size()) +
+                              sizeof(double) * plan.b.size();
                 e->hform = FEDNL_HESS_SPARSE;
                 e->nparts = (int)((e->S + kSpSeg - 1) / kSpSeg);
             }
@@ -6430,8 +6434,10 @@ int fednl_serve(const fednl_request* reqs, int count, double hbm_budget_bytes, i
                 waiting.erase(waiting.begin());
                 continue;
             }
-            active.push_back({q, e, need, 0});
-            used += need;
+            // a binary design's sparse lists are known once its data are read:
+            // counted from here on (the budget can be passed by one request's lists)
+            active.push_back({q, e, need + e->sp_bytes, 0});
+            used += need + e->sp_bytes;
             waiting.erase(waiting.begin());
             dirty = true;
         }
