@@ -184,3 +184,48 @@ def test_forced_sparse_rejected_where_it_does_not_apply():
     sp = _binary((3, 50, 20, 0.1), seed=6)
     with pytest.raises(FednlError, match="sparse Hessian form"):
         _engine(sp, "sparse", hessian="bf16x2")
+
+
+def _with_dense_column(prob, col):
+    feats = prob.features.copy()
+    feats[:, :, col] = 1.0
+    return datagen.Problem(prob.name + f"_dense{col}", feats, prob.labels, prob.lam, prob.w_true)
+
+
+@pytest.mark.parametrize("case", ["second_dense_column", "intercept_only", "empty_features"])
+def test_runs_and_edge_designs_bit_exact(oracle, case):
+    """Runs (consecutive samples: a dense column's diagonal and gradient, and
+    the off-diagonal entry of two dense columns), a design of the intercept
+    alone (d = 1) and one whose features are all zero: H(0) against the
+    oracle bit for bit."""
+    if case == "second_dense_column":
+        prob = _with_dense_column(_binary((4, 120, 60, 0.06), seed=41), 10)
+        kw = {}
+    elif case == "intercept_only":
+        prob = _binary((3, 50, 0, 0.0), seed=42)
+        kw = dict(k_mult=1.0)
+    else:
+        prob = _binary((3, 70, 20, 0.0), seed=43)
+        kw = dict(k_mult=2.0)
+    eng = _engine(prob, "sparse", **kw)
+    assert eng.hessian_form == "sparse"
+    x0 = np.zeros(eng.d)
+    H = eng.hessians_at(x0)
+    for c in range(eng.n):
+        ref = oracle.client_eval(prob.features[c], prob.labels[c], prob.lam, x0)["hess_up"]
+        assert np.array_equal(H[c], ref), (case, c)
+    eng.close()
+
+
+def test_runs_in_rounds_match_dense():
+    """A design with two dense columns: runs in every round (Delta, the
+    gradient through them) against the dense form."""
+    prob = _with_dense_column(_binary((5, 160, 80, 0.05), seed=44), 7)
+    R = 40
+    out = []
+    for form in ("sparse", "dense"):
+        eng = _engine(prob, form, compressor="randk", k_mult=2.0, tracking=True)
+        eng.run(R)
+        out.append(eng.trace()["grad_norm"])
+        eng.close()
+    assert_grad_norms_close(out[0], out[1])
