@@ -229,3 +229,30 @@ def test_runs_in_rounds_match_dense():
         out.append(eng.trace()["grad_norm"])
         eng.close()
     assert_grad_norms_close(out[0], out[1])
+
+
+def test_line_search_batch_member_bit_identical_to_solo():
+    """FedNL-LS members (a sparse and a dense one) in one batch: the trial
+    margins of the sparse member run in the batch's sparse margin launch;
+    both give their solo runs' bits, trial counts included."""
+    from ufednl_amd.engine import Batch
+    pa = _binary((6, 150, 90, 0.05), seed=51)
+    pb = datagen.gaussian_problem("g_n4_m70_d30", 4, 70, 30, seed=52)
+    ls = (0.5, 0.5, 5)
+    R = 10
+    solo = []
+    for prob, form in ((pa, "sparse"), (pb, "dense")):
+        eng = _engine(prob, form, line_search=ls)
+        eng.run(R)
+        solo.append((eng.trace()["grad_norm"], eng.x(), eng.trace()["ls_trials"]))
+        eng.close()
+    ea, eb = _engine(pa, "sparse", line_search=ls), _engine(pb, "dense", line_search=ls)
+    b = Batch([ea, eb])
+    b.run(R)
+    for eng, (g0, x0, t0) in zip((ea, eb), solo):
+        assert np.array_equal(eng.trace()["grad_norm"], g0)
+        assert np.array_equal(eng.x(), x0)
+        assert np.array_equal(eng.trace()["ls_trials"], t0)
+    b.close()
+    ea.close()
+    eb.close()
