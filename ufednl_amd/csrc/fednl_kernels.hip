@@ -3629,7 +3629,9 @@ __device__ __forceinline__ void diag_column(double (&row)[kNB], double& yr, bool
     yr = fma(-scaled, ycc, yr);
     meta[4 * kCc + 1] = inv;
     meta[4 * kCc + 2] = ycc;
-    __builtin_amdgcn_wave_barrier();
+    // (the wave's LDS operations complete in order, so the reads below see
+    // every lane's scaled entry; no scheduling barrier, so that the next
+    // column's pivot chain can overlap these updates)
 #pragma unroll
     for (int c2 = kCc + 2; c2 < kEnd; ++c2)
         row[c2] = fma(-scaled, b[c2], row[c2]);
