@@ -31,8 +31,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <mutex>
+#include <new>
 #include <queue>
 #include <string>
 #include <type_traits>
@@ -5340,14 +5342,30 @@ static void build_sparse_plan(const double* features, const double* labels, int 
     for (int j = 0; j < d; ++j) diag_col[(size_t)j * (j + 3) / 2] = j;
     std::vector<SparseClient> cl(n);
     {
+        // clients taken from a counter by the calling thread and up to 15
+        // more; a thread that cannot be started is simply not there. A
+        // worker's failure (host memory) is raised here after the join.
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
         const int nt = (int)std::min<unsigned>(std::min<unsigned>(hw, 16u), (unsigned)n);
+        std::atomic<int> next{0};
+        std::atomic<bool> failed{false};
+        auto work = [&]() {
+            try {
+                for (int c; !failed.load() && (c = next.fetch_add(1)) < n;)
+                    build_sparse_client(features, c, m, d, diag_col, cl[c]);
+            } catch (...) {
+                failed.store(true);
+            }
+        };
         std::vector<std::thread> pool;
-        for (int t = 0; t < nt; ++t)
-            pool.emplace_back([&, t]() {
-                for (int c = t; c < n; c += nt) build_sparse_client(features, c, m, d, diag_col, cl[c]);
-            });
+        try {
+            for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+        } catch (...) {
+        }
+        work();
         for (auto& th : pool) th.join();
+        if (failed.load())
+            throw std::bad_alloc();
     }
     P.seg.assign((size_t)n * nseg * 4, 0);
     P.lane.assign((size_t)n * nseg * (kSpLanes + 1), 0);
@@ -5657,7 +5675,13 @@ static int create_engine(const fednl_opts* opts, int n, int m, int d, const doub
         }
         if (sparse) {
             SparsePlan plan;
-            build_sparse_plan(features, labels, n, m, d, e->m_pad, plan);
+            try {
+                build_sparse_plan(features, labels, n, m, d, e->m_pad, plan);
+            } catch (...) {
+                set_err("fednl_create: out of host memory building the sparse Hessian form's sample lists");
+                free_engine(e);
+                return FEDNL_E_INTERNAL;
+            }
             const size_t lds = sparse_lds_bytes(e->m_pad, e->d_pad, plan.max_tokens);
             if (lds > kSpMaxLds) {
                 if (want == FEDNL_HESS_SPARSE) {
